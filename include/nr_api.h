@@ -1,0 +1,164 @@
+/*
+ * nr_api.h - C-ABI of the MI355X two-terminal reliability engine
+ * (path decomposition / factoring, hand-written HIP kernels for gfx950).
+ *
+ * Plain C types only: no HIP, no torch, no C++ in the signatures.  Every
+ * entry point returns an nr_status (0 = NR_OK); none calls exit() or prints.
+ * Calls are blocking; the library owns every device allocation it makes and
+ * the caller owns its input arrays for the duration of a call.  Calls on one
+ * nr_graph are not reentrant.
+ *
+ * Reference surface each entry point replaces (paths relative to the
+ * reference repository):
+ *   nr_graph_load      workingPad/graph.h:46-145   graph_load()
+ *   nr_graph_create    workingPad/graph.h:104-145  CSR build + probabilities
+ *   nr_graph_destroy   workingPad/graph.h:181-188  graph_free_device()
+ *   nr_solve           workingPad/main.cu:795-1056 main(): initial mask
+ *                      (717-748), queue setup (635-711, 847-880), level loop
+ *                      (963-1008), factoring_kernel (465-629), stats readback
+ *                      (1017-1029)
+ *   nr_solve_batch     parallel/CUDA_Reliability.h:95-122 (the reference's
+ *                      only extern "C" host API: batched GPU work over many
+ *                      masks) generalised to many graphs in one launch
+ *   nr_expand_items    workingPad/main.cu:114-283 path_edge_sweep_warp() on a
+ *                      caller-supplied batch of items (one expansion each)
+ *   nr_config_init     workingPad/common.h:33-36 defaults, main.cu:753-790
+ */
+#ifndef NR_API_H
+#define NR_API_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum nr_status {
+  NR_OK = 0,
+  NR_ERR_IO = 1,          /* graph file cannot be opened */
+  NR_ERR_PARSE = 2,       /* malformed graph file */
+  NR_ERR_RANGE = 3,       /* node/edge index or terminal out of range */
+  NR_ERR_ARG = 4,         /* null pointer or invalid config value */
+  NR_ERR_UNSUPPORTED = 5, /* graph shape beyond the engine's width classes */
+  NR_ERR_DEVICE = 6,      /* HIP runtime error */
+  NR_ERR_NOMEM = 7        /* device allocation failed */
+} nr_status;
+
+/* Work order.  Any order gives an exact decomposition; order only changes
+ * which subproblems are resolved before a capacity/time limit. */
+typedef enum nr_order {
+  NR_ORDER_LEVEL = 0, /* FIFO by tree depth: the reference's order
+                         (main.cu:963-1008); reports levels as iterations */
+  NR_ORDER_MASS = 1,  /* max-mass first (bucketed device priority queue);
+                         the reference's CPU heap order (seq/seq.cpp:49-197) */
+  NR_ORDER_LEARNED = 2 /* learned scorer orders the queue */
+} nr_order;
+
+typedef struct nr_config {
+  long long queue_capacity; /* items per queue buffer (--queue_capacity,
+                               main.cu:781-782); default 1<<20 */
+  double truncation_eps;    /* items with mass < eps go to ESP (main.cu:537);
+                               FP64; 0 disables; default 1e-15 */
+  double time_budget_s;     /* 0 = none; when it expires every queued item is
+                               folded into ESP, keeping R <= R_true <= R+ESP */
+  int order;                /* nr_order */
+  int device;               /* HIP device ordinal */
+  int block_size;           /* threads per block; 0 = auto (256) */
+  int grid_blocks;          /* blocks per launch; 0 = auto from occupancy */
+  int chunk_items;          /* persistent modes: items claimed per wave refill;
+                               0 = auto */
+  int reserved[7];
+} nr_config;
+
+typedef struct nr_result {
+  double R;                     /* reliability lower bound (exact when ESP=0) */
+  double ESP;                   /* certified gap: truncated + overflowed +
+                                   folded mass */
+  double D;                     /* mass of dead subproblems; R+D+ESP = 1 */
+  unsigned long long paths;     /* successes + overflowed children
+                                   (main.cu:251,262) */
+  unsigned long long nodes;     /* dequeued items incl. dead and truncated
+                                   (main.cu:533-534) */
+  unsigned long long dead, truncated, overflowed, folded;
+  long long iterations;         /* levels (level order) or refill rounds */
+  long long max_queue_items;    /* widest level / largest queue occupancy */
+  long long item_bytes;         /* bytes per queued item in HBM */
+  double elapsed_ms;            /* first launch to final sync, excludes load
+                                   and allocation (main.cu:951-1014) */
+  double kernel_ms;             /* summed expand-kernel time (HIP events) */
+  long long kernel_launches;    /* expand-kernel launches */
+  int deadline_hit;
+  int overflowed_flag;
+  int grid_blocks;              /* launch geometry actually used */
+  int block_size;
+  int lds_bytes;                /* dynamic LDS per workgroup */
+  int blocks_per_cu;            /* occupancy reported by the runtime */
+  int compute_units;
+  int reserved;
+} nr_result;
+
+typedef struct nr_graph_info {
+  int n, e, nnz;
+  int header_e;   /* E from the header (halved for type U, graph.h:93) */
+  int type;       /* 'U', 'D', or 0 when the header has no type token */
+  int src, dst;
+  int mask_words; /* 64-bit words per edge-state plane */
+  int symmetric;  /* every CSR entry u->v has a matching v->u entry */
+  int precontracted, predeleted; /* p>=1 / p<=0 edges (main.cu:729-745) */
+} nr_graph_info;
+
+typedef struct nr_graph nr_graph;
+
+/* Defaults of workingPad/common.h:33-34: capacity 2^20, eps 1e-15. */
+int nr_config_init(nr_config* cfg);
+
+/* Parse the reference's adjacency-list file: header "N E [T]" (the type
+ * token is optional), then per node "deg (nbr p)*"; CRLF accepted; p parsed
+ * as FP64.  Edge ids follow first appearance of (min,max), CSR rows follow
+ * the file. */
+int nr_graph_load(const char* path, int src, int dst, nr_graph** out);
+
+/* Build from CSR arrays (row_ptr[n+1], col_idx[nnz], edge_id[nnz], p[e]). */
+int nr_graph_create(int n, int e, const int* row_ptr, const int* col_idx,
+                    const int* edge_id, const double* p, int src, int dst,
+                    nr_graph** out);
+
+int nr_graph_get_info(const nr_graph* g, nr_graph_info* info);
+
+/* Copy out the host CSR (any pointer may be null). */
+int nr_graph_get_csr(const nr_graph* g, int* row_ptr, int* col_idx,
+                     int* edge_id, double* p);
+
+int nr_graph_set_terminals(nr_graph* g, int src, int dst);
+
+void nr_graph_destroy(nr_graph* g);
+
+/* Solve one instance on the GPU. */
+int nr_solve(const nr_graph* g, const nr_config* cfg, nr_result* out);
+
+/* Solve `count` independent instances (graphs and/or terminal pairs) in one
+ * persistent launch sequence; results[i] belongs to graphs[i].  Counts are
+ * per instance; elapsed_ms/kernel_ms are the whole batch's. */
+int nr_solve_batch(const nr_graph* const* graphs, int count,
+                   const nr_config* cfg, nr_result* results);
+
+/* Expand `count` caller-supplied items once (one subproblem each) on the
+ * GPU.  Item i has planes known[i*W..], up[i*W..] (W = mask_words) and mass
+ * mass[i].  Outputs: nchild[i] (-1 dead), succ[i], and children in slots
+ * i*max_children.. of child_known/child_up (W words each) and child_mass,
+ * ordered by their failed hop from src to dst. */
+int nr_expand_items(const nr_graph* g, int device, int count,
+                    const unsigned long long* known,
+                    const unsigned long long* up, const double* mass,
+                    int max_children, int* nchild, double* succ,
+                    unsigned long long* child_known,
+                    unsigned long long* child_up, double* child_mass);
+
+const char* nr_status_str(int status);
+
+/* Device name / compute units / wavefront size of a HIP device. */
+int nr_device_query(int device, char* name, int name_len, int* compute_units,
+                    int* wave_size, long long* total_mem);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NR_API_H */

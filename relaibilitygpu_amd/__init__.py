@@ -1,0 +1,311 @@
+"""relaibilitygpu_amd - MI355X (gfx950) engine for two-terminal network
+reliability by path decomposition.
+
+Python mirror of the C-ABI in ``include/nr_api.h`` (ctypes over the in-tree
+``libnrgpu.so``).  The product path is the HIP library: if it has not been
+built, importing the bindings raises instead of falling back to anything.
+
+Reference surface mirrored here:
+  * ``load_graph``   - workingPad/graph.h:46-145 ``graph_load``
+  * ``solve``        - workingPad/main.cu:795-1056 (the whole run, report
+                       fields of main.cu:1037-1056)
+  * ``expand_items`` - workingPad/main.cu:114-283 on a batch of items
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnrgpu.so")
+CLI_PATH = os.path.join(_HERE, "bin", "network_reliability")
+
+NR_ORDER_LEVEL = 0
+NR_ORDER_MASS = 1
+NR_ORDER_LEARNED = 2
+ORDERS = {"level": NR_ORDER_LEVEL, "mass": NR_ORDER_MASS, "learned": NR_ORDER_LEARNED}
+
+STATUS = {
+    0: "NR_OK",
+    1: "NR_ERR_IO",
+    2: "NR_ERR_PARSE",
+    3: "NR_ERR_RANGE",
+    4: "NR_ERR_ARG",
+    5: "NR_ERR_UNSUPPORTED",
+    6: "NR_ERR_DEVICE",
+    7: "NR_ERR_NOMEM",
+}
+
+
+class NRError(RuntimeError):
+    """A non-zero nr_status returned by the library."""
+
+    def __init__(self, status: int, what: str):
+        self.status = status
+        super().__init__(f"{what}: {STATUS.get(status, status)}")
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("queue_capacity", ctypes.c_longlong),
+        ("truncation_eps", ctypes.c_double),
+        ("time_budget_s", ctypes.c_double),
+        ("order", ctypes.c_int),
+        ("device", ctypes.c_int),
+        ("block_size", ctypes.c_int),
+        ("grid_blocks", ctypes.c_int),
+        ("chunk_items", ctypes.c_int),
+        ("reserved", ctypes.c_int * 7),
+    ]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [
+        ("R", ctypes.c_double),
+        ("ESP", ctypes.c_double),
+        ("D", ctypes.c_double),
+        ("paths", ctypes.c_ulonglong),
+        ("nodes", ctypes.c_ulonglong),
+        ("dead", ctypes.c_ulonglong),
+        ("truncated", ctypes.c_ulonglong),
+        ("overflowed", ctypes.c_ulonglong),
+        ("folded", ctypes.c_ulonglong),
+        ("iterations", ctypes.c_longlong),
+        ("max_queue_items", ctypes.c_longlong),
+        ("item_bytes", ctypes.c_longlong),
+        ("elapsed_ms", ctypes.c_double),
+        ("kernel_ms", ctypes.c_double),
+        ("kernel_launches", ctypes.c_longlong),
+        ("deadline_hit", ctypes.c_int),
+        ("overflowed_flag", ctypes.c_int),
+        ("grid_blocks", ctypes.c_int),
+        ("block_size", ctypes.c_int),
+        ("lds_bytes", ctypes.c_int),
+        ("blocks_per_cu", ctypes.c_int),
+        ("compute_units", ctypes.c_int),
+        ("reserved", ctypes.c_int),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+class GraphInfo(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int),
+        ("e", ctypes.c_int),
+        ("nnz", ctypes.c_int),
+        ("header_e", ctypes.c_int),
+        ("type", ctypes.c_int),
+        ("src", ctypes.c_int),
+        ("dst", ctypes.c_int),
+        ("mask_words", ctypes.c_int),
+        ("symmetric", ctypes.c_int),
+        ("precontracted", ctypes.c_int),
+        ("predeleted", ctypes.c_int),
+    ]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libnrgpu.so (built in-tree by ``make -C relaibilitygpu_amd``)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build the HIP engine first "
+            "(python -c 'import __graft_entry__ as g; g.build()')"
+        )
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    vp = ctypes.c_void_p
+    L.nr_config_init.argtypes = [P(Config)]
+    L.nr_graph_load.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, P(vp)]
+    L.nr_graph_create.argtypes = [
+        ctypes.c_int, ctypes.c_int, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int),
+        P(ctypes.c_double), ctypes.c_int, ctypes.c_int, P(vp),
+    ]
+    L.nr_graph_get_info.argtypes = [vp, P(GraphInfo)]
+    L.nr_graph_get_csr.argtypes = [vp, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int),
+                                   P(ctypes.c_double)]
+    L.nr_graph_set_terminals.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.nr_graph_destroy.argtypes = [vp]
+    L.nr_graph_destroy.restype = None
+    L.nr_solve.argtypes = [vp, P(Config), P(Result)]
+    L.nr_solve_batch.argtypes = [P(vp), ctypes.c_int, P(Config), P(Result)]
+    L.nr_expand_items.argtypes = [
+        vp, ctypes.c_int, ctypes.c_int, P(ctypes.c_ulonglong), P(ctypes.c_ulonglong),
+        P(ctypes.c_double), ctypes.c_int, P(ctypes.c_int), P(ctypes.c_double),
+        P(ctypes.c_ulonglong), P(ctypes.c_ulonglong), P(ctypes.c_double),
+    ]
+    L.nr_status_str.argtypes = [ctypes.c_int]
+    L.nr_status_str.restype = ctypes.c_char_p
+    L.nr_device_query.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                                  P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_longlong)]
+    _lib = L
+    return L
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise NRError(rc, what)
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def make_config(queue_capacity: int = 1 << 20, truncation_eps: float = 1e-15,
+                time_budget_s: float = 0.0, order: str | int = "level", device: int = 0,
+                block_size: int = 0, grid_blocks: int = 0, chunk_items: int = 0) -> Config:
+    c = Config()
+    lib().nr_config_init(ctypes.byref(c))
+    c.queue_capacity = int(queue_capacity)
+    c.truncation_eps = float(truncation_eps)
+    c.time_budget_s = float(time_budget_s)
+    c.order = ORDERS[order] if isinstance(order, str) else int(order)
+    c.device = int(device)
+    c.block_size = int(block_size)
+    c.grid_blocks = int(grid_blocks)
+    c.chunk_items = int(chunk_items)
+    return c
+
+
+class Graph:
+    """Owned handle to an nr_graph (host CSR + FP64 probabilities)."""
+
+    def __init__(self, handle: int):
+        self._h = ctypes.c_void_p(handle)
+
+    @classmethod
+    def load(cls, path: str, src: int, dst: int) -> "Graph":
+        h = ctypes.c_void_p()
+        _check(lib().nr_graph_load(os.fsencode(path), int(src), int(dst), ctypes.byref(h)),
+               f"nr_graph_load({path})")
+        return cls(h.value)
+
+    @classmethod
+    def from_csr(cls, row_ptr, col_idx, edge_id, p, src: int, dst: int) -> "Graph":
+        rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+        ci = np.ascontiguousarray(col_idx, dtype=np.int32)
+        ei = np.ascontiguousarray(edge_id, dtype=np.int32)
+        pp = np.ascontiguousarray(p, dtype=np.float64)
+        h = ctypes.c_void_p()
+        _check(lib().nr_graph_create(len(rp) - 1, len(pp), _ptr(rp, ctypes.c_int),
+                                     _ptr(ci, ctypes.c_int), _ptr(ei, ctypes.c_int),
+                                     _ptr(pp, ctypes.c_double), int(src), int(dst),
+                                     ctypes.byref(h)), "nr_graph_create")
+        return cls(h.value)
+
+    @classmethod
+    def from_edges(cls, n: int, edges, probs, src: int, dst: int) -> "Graph":
+        """Undirected edge list -> CSR with both directions, rows sorted by
+        neighbour, ids in the order given (first appearance)."""
+        rows = [[] for _ in range(n)]
+        for i, (a, b) in enumerate(edges):
+            rows[a].append((b, i))
+            if a != b:
+                rows[b].append((a, i))
+        rp, ci, ei = [0], [], []
+        for u in range(n):
+            for v, i in sorted(rows[u]):
+                ci.append(v)
+                ei.append(i)
+            rp.append(len(ci))
+        return cls.from_csr(rp, ci, ei, probs, src, dst)
+
+    def info(self) -> GraphInfo:
+        gi = GraphInfo()
+        _check(lib().nr_graph_get_info(self._h, ctypes.byref(gi)), "nr_graph_get_info")
+        return gi
+
+    def csr(self):
+        gi = self.info()
+        rp = np.zeros(gi.n + 1, np.int32)
+        ci = np.zeros(max(gi.nnz, 1), np.int32)
+        ei = np.zeros(max(gi.nnz, 1), np.int32)
+        p = np.zeros(max(gi.e, 1), np.float64)
+        _check(lib().nr_graph_get_csr(self._h, _ptr(rp, ctypes.c_int), _ptr(ci, ctypes.c_int),
+                                      _ptr(ei, ctypes.c_int), _ptr(p, ctypes.c_double)),
+               "nr_graph_get_csr")
+        return rp, ci[: gi.nnz], ei[: gi.nnz], p[: gi.e]
+
+    def set_terminals(self, src: int, dst: int) -> None:
+        _check(lib().nr_graph_set_terminals(self._h, int(src), int(dst)), "set_terminals")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self) -> None:
+        if self._h:
+            lib().nr_graph_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve(graph: Graph, config: Config | None = None, **kw) -> dict:
+    """Run one solve on the GPU and return the report fields as a dict."""
+    cfg = config if config is not None else make_config(**kw)
+    r = Result()
+    _check(lib().nr_solve(graph.handle, ctypes.byref(cfg), ctypes.byref(r)), "nr_solve")
+    return r.as_dict()
+
+
+def solve_batch(graphs, config: Config | None = None, **kw) -> list[dict]:
+    cfg = config if config is not None else make_config(**kw)
+    arr = (ctypes.c_void_p * len(graphs))(*[g.handle.value for g in graphs])
+    res = (Result * len(graphs))()
+    _check(lib().nr_solve_batch(arr, len(graphs), ctypes.byref(cfg), res), "nr_solve_batch")
+    return [r.as_dict() for r in res]
+
+
+@dataclass
+class Expansion:
+    nchild: np.ndarray  # [count] (-1 dead)
+    succ: np.ndarray  # [count]
+    child_known: np.ndarray  # [count, max_children, W]
+    child_up: np.ndarray
+    child_mass: np.ndarray  # [count, max_children]
+
+
+def expand_items(graph: Graph, known: np.ndarray, up: np.ndarray, mass: np.ndarray,
+                 max_children: int, device: int = 0) -> Expansion:
+    """One GPU expansion of each item (nr_expand_items)."""
+    W = graph.info().mask_words
+    known = np.ascontiguousarray(known, dtype=np.uint64).reshape(-1, W)
+    up = np.ascontiguousarray(up, dtype=np.uint64).reshape(-1, W)
+    mass = np.ascontiguousarray(mass, dtype=np.float64).reshape(-1)
+    n = len(mass)
+    nchild = np.zeros(n, np.int32)
+    succ = np.zeros(n, np.float64)
+    ck = np.zeros((n, max_children, W), np.uint64)
+    cu = np.zeros((n, max_children, W), np.uint64)
+    cm = np.zeros((n, max_children), np.float64)
+    U = ctypes.c_ulonglong
+    _check(lib().nr_expand_items(graph.handle, int(device), n, _ptr(known, U), _ptr(up, U),
+                                 _ptr(mass, ctypes.c_double), int(max_children),
+                                 _ptr(nchild, ctypes.c_int), _ptr(succ, ctypes.c_double),
+                                 _ptr(ck, U), _ptr(cu, U), _ptr(cm, ctypes.c_double)),
+           "nr_expand_items")
+    return Expansion(nchild, succ, ck, cu, cm)
+
+
+def device_query(device: int = 0) -> dict:
+    name = ctypes.create_string_buffer(256)
+    cus, ws, mem = ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong()
+    _check(lib().nr_device_query(device, name, 256, ctypes.byref(cus), ctypes.byref(ws),
+                                 ctypes.byref(mem)), "nr_device_query")
+    return {"name": name.value.decode(), "compute_units": cus.value,
+            "wave_size": ws.value, "total_mem": mem.value}

@@ -1,0 +1,303 @@
+/*
+ * nr_device.h - device-side building blocks of the expand kernels (gfx950,
+ * wave64).  One lane owns one subproblem ("item"): its two edge-state bit
+ * planes live in VGPRs, the graph's adjacency/incidence tables are staged in
+ * LDS per workgroup, and the lane's BFS level sets and path bookkeeping live in
+ * an LDS slice laid out [slot][lane] so a wave's accesses fall in distinct
+ * banks.
+ *
+ * Edge states: known/up bit planes.  UNKNOWN (0,0), WORKING (1,1),
+ * FAILED (1,0); the reference's 2-bit codes 0b11/0b01/0b00
+ * (workingPad/common.h:22-24).
+ */
+#ifndef NR_DEVICE_H
+#define NR_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nr {
+
+typedef unsigned long long u64;
+typedef unsigned int u32;
+typedef unsigned short u16;
+
+/* Double-double partial (hi, lo) for compensated sums. */
+struct DD {
+  double hi, lo;
+};
+
+__host__ __device__ inline void two_sum(double a, double b, double& s, double& e) {
+  s = a + b;
+  double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+/* Neumaier accumulator held per lane. */
+struct Acc2 {
+  double s, c;
+  __device__ void init() { s = 0.0; c = 0.0; }
+  __device__ void add(double x) {
+    double t = s + x;
+    if (fabs(s) >= fabs(x))
+      c += (s - t) + x;
+    else
+      c += (x - t) + s;
+    s = t;
+  }
+};
+
+/* LDS layout of the per-workgroup graph tables followed by the per-lane
+ * scratch slices.  Offsets in bytes; computed identically on host and
+ * device. */
+struct Layout {
+  int n, e, w;
+  int off_sadj, off_inc, off_ep, off_eidm, off_deg, off_p, off_q;
+  int off_lvl, off_uh;
+  int table_bytes, total_bytes;
+  int block, ns_bytes;
+};
+
+inline Layout make_layout(int n, int e, int w, int block, int ns_bytes) {
+  Layout L;
+  L.n = n;
+  L.e = e;
+  L.w = w;
+  L.block = block;
+  L.ns_bytes = ns_bytes;
+  int o = 0;
+  auto take = [&](int bytes, int align) {
+    o = (o + align - 1) / align * align;
+    int at = o;
+    o += bytes;
+    return at;
+  };
+  int ee = e > 0 ? e : 1;
+  L.off_p = take(8 * ee, 16);
+  L.off_q = take(8 * ee, 16);
+  L.off_sadj = take(8 * n, 16);
+  L.off_inc = take(8 * n * w, 16);
+  L.off_ep = take(4 * ee, 16);
+  L.off_deg = take(4 * n, 16);
+  L.off_eidm = take(2 * n * n, 16);
+  L.table_bytes = (o + 15) / 16 * 16;
+  L.off_lvl = take(ns_bytes * n * block, 16);
+  L.off_uh = take(2 * n * block, 16);
+  L.total_bytes = (o + 15) / 16 * 16;
+  return L;
+}
+
+/* Global-memory graph tables (built on the host, see nr_engine.hip). */
+struct DevGraph {
+  const double* p;
+  const double* q;
+  const u64* sadj;  /* [n] static neighbour set */
+  const u64* inc;   /* [n*w] incident edge set */
+  const u32* ep;    /* [e] endpoints a | b << 16 */
+  const u32* deg;   /* [n] incident edge count */
+  const u16* eidm;  /* [n*n] edge id of (u,v), 0xffff if none */
+  int n, e, w, src, dst;
+};
+
+/* Cooperative copy of the tables into LDS (all threads of the block). */
+__device__ inline void stage_tables(const DevGraph& g, const Layout& L,
+                                    unsigned char* smem) {
+  auto cp = [&](const void* src, int off, int bytes) {
+    const u32* s = (const u32*)src;
+    u32* d = (u32*)(smem + off);
+    for (int i = threadIdx.x; i < bytes / 4; i += blockDim.x) d[i] = s[i];
+  };
+  cp(g.p, L.off_p, 8 * g.e);
+  cp(g.q, L.off_q, 8 * g.e);
+  cp(g.sadj, L.off_sadj, 8 * g.n);
+  cp(g.inc, L.off_inc, 8 * g.n * g.w);
+  cp(g.ep, L.off_ep, 4 * g.e);
+  cp(g.deg, L.off_deg, 4 * g.n);
+  /* eidm is 2-byte; copy as 2-byte to stay in bounds for odd n*n */
+  const u16* s = g.eidm;
+  u16* d = (u16*)(smem + L.off_eidm);
+  for (int i = threadIdx.x; i < g.n * g.n; i += blockDim.x) d[i] = s[i];
+}
+
+template <typename NS>
+__device__ inline int ctzs(NS x);
+template <>
+__device__ inline int ctzs<u32>(u32 x) {
+  return __builtin_ctz(x);
+}
+template <>
+__device__ inline int ctzs<u64>(u64 x) {
+  return __builtin_ctzll(x);
+}
+
+template <int W>
+__device__ inline u64 pick(const u64 (&a)[W], int wi) {
+  u64 r = a[0];
+#pragma unroll
+  for (int i = 1; i < W; ++i)
+    if (wi == i) r = a[i];
+  return r;
+}
+
+/* The per-lane expander: BFS + lowest-index parent path + unknown hops. */
+template <int W, typename NS>
+struct Expander {
+  const double* p;
+  const double* q;
+  const u64* sadj;
+  const u64* inc;
+  const u32* ep;
+  const u32* deg;
+  const u16* eidm;
+  NS* lvl;  /* this lane's level sets, stride `stride` */
+  u16* uh;  /* this lane's unknown-hop stack, stride `stride` */
+  int stride, n, src, dst;
+
+  __device__ void bind(unsigned char* smem, const Layout& L, int slot, int src_,
+                       int dst_) {
+    p = (const double*)(smem + L.off_p);
+    q = (const double*)(smem + L.off_q);
+    sadj = (const u64*)(smem + L.off_sadj);
+    inc = (const u64*)(smem + L.off_inc);
+    ep = (const u32*)(smem + L.off_ep);
+    deg = (const u32*)(smem + L.off_deg);
+    eidm = (const u16*)(smem + L.off_eidm);
+    lvl = (NS*)(smem + L.off_lvl) + slot;
+    uh = (u16*)(smem + L.off_uh) + slot;
+    stride = L.block;
+    n = L.n;
+    src = src_;
+    dst = dst_;
+  }
+
+  /* Allowed neighbours of u (edges not FAILED): the static neighbour set
+   * minus the ends of u's failed edges, or the union of the ends of u's
+   * allowed edges, whichever set of edges is smaller. */
+  __device__ NS adj_of(int u, const u64 (&Dn)[W]) const {
+    NS a = (NS)sadj[u];
+    u64 dn[W];
+    int cnt = 0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+      dn[i] = inc[u * W + i] & Dn[i];
+      cnt += __popcll(dn[i]);
+    }
+    if (cnt == 0) return a;
+    if (2 * cnt <= (int)deg[u]) {
+#pragma unroll
+      for (int i = 0; i < W; ++i) {
+        u64 m = dn[i];
+        while (m) {
+          int e = i * 64 + __builtin_ctzll(m);
+          m &= m - 1;
+          u32 x = ep[e];
+          int o = (int)((x & 0xffffu) ^ (x >> 16)) ^ u;
+          a &= ~((NS)1 << o);
+        }
+      }
+    } else {
+      a = 0;
+#pragma unroll
+      for (int i = 0; i < W; ++i) {
+        u64 m = inc[u * W + i] & ~Dn[i];
+        while (m) {
+          int e = i * 64 + __builtin_ctzll(m);
+          m &= m - 1;
+          u32 x = ep[e];
+          int o = (int)((x & 0xffffu) ^ (x >> 16)) ^ u;
+          a |= (NS)1 << o;
+        }
+      }
+    }
+    return a;
+  }
+
+  /* Optimistic BFS from src (device_code.cuh:80-194) and the path to dst
+   * under the lowest-index parent rule: the parent of v is the lowest-index
+   * node of the previous BFS level joined to v by an allowed edge.  Pushes
+   * the path's UNKNOWN hops on uh[] in dst->src order.  Returns the number
+   * of UNKNOWN hops, -1 if dst is unreachable, -2 on an internal
+   * inconsistency (never expected). */
+  __device__ int find(const u64 (&known)[W], const u64 (&Dn)[W]) const {
+    if (src == dst) return 0;
+    const NS dbit = (NS)1 << dst;
+    NS vis = (NS)1 << src;
+    NS F = vis;
+    lvl[0] = F;
+    int L = 0;
+    for (;;) {
+      NS nx = 0;
+      NS f = F;
+      while (f) {
+        int u = ctzs<NS>(f);
+        f &= f - 1;
+        nx |= adj_of(u, Dn);
+        if (nx & dbit) break;
+      }
+      nx &= ~vis;
+      if (!nx) return -1;
+      ++L;
+      if (nx & dbit) break;
+      lvl[L * stride] = nx;
+      vis |= nx;
+      F = nx;
+    }
+    int v = dst, k = 0;
+    for (int l = L; l >= 1; --l) {
+      NS cand = adj_of(v, Dn) & lvl[(l - 1) * stride];
+      if (!cand) return -2;
+      int u = ctzs<NS>(cand);
+      int e = eidm[u * n + v];
+      if (!((pick<W>(known, e >> 6) >> (e & 63)) & 1ull)) uh[(k++) * stride] = (u16)e;
+      v = u;
+    }
+    return k;
+  }
+};
+
+/* Wave-wide inclusive prefix sum of a small non-negative int. */
+__device__ inline int wave_incl_scan(int x) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    int t = __shfl_up(x, off, 64);
+    if (lane >= off) x += t;
+  }
+  return x;
+}
+
+__device__ inline void wave_sum_dd(double& s, double& c) {
+  /* combine (s, c) pairs across the wave with error-free additions */
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    double os = __shfl_xor(s, off, 64);
+    double oc = __shfl_xor(c, off, 64);
+    double t, e;
+    two_sum(s, os, t, e);
+    s = t;
+    c = c + oc + e;
+  }
+}
+
+__device__ inline u64 wave_sum_u64(u64 x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  return x;
+}
+
+/* Merge a wave's (s, c) into a global double-double slot (slot owned by this
+ * wave for the duration of the launch). */
+__device__ inline void dd_merge_slot(DD* slot, double s, double c) {
+  DD o = *slot;
+  double t, e;
+  two_sum(o.hi, s, t, e);
+  double lo = o.lo + c + e;
+  double hi2, e2;
+  two_sum(t, lo, hi2, e2);
+  slot->hi = hi2;
+  slot->lo = e2;
+}
+
+}  // namespace nr
+
+#endif

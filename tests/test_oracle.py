@@ -1,0 +1,223 @@
+"""The CPU oracle against known answers (CPU only).
+
+Pins, each chosen so that a plausible mistake in the oracle fails one:
+  * closed form R(K_n) (exact rationals)            -> wrong mass products
+  * brute force over all 2^E states                 -> wrong child construction
+  * reference-logged paths/nodes/levels/width       -> wrong parent rule / counting
+  * R + D + ESP = 1 in every mode                   -> lost or double-counted mass
+  * counts equal across orders without limits       -> order-dependent tree
+  * series / parallel / bridge graphs               -> textbook formulas
+"""
+import math
+import random
+
+import pytest
+
+from conftest import closed_form, graph_path
+from oracle_ffi import (MODE_PATH, MODE_PIVOT, ORDER_DFS, ORDER_HEAP, ORDER_LEVEL,
+                        OracleGraph)
+
+
+def load(name):
+    return OracleGraph.load(graph_path(name))
+
+
+@pytest.mark.parametrize("idx", range(7))
+def test_brute_force_small_graphs(expected, idx):
+    bf = expected["brute_force"][idx]
+    g = load(bf["graph"])
+    for order in (ORDER_LEVEL, ORDER_HEAP, ORDER_DFS):
+        r = g.solve(bf["src"], bf["dst"], order=order)
+        assert abs(r["R"] - bf["R"]) <= 1e-15, (bf, order, r)
+        assert r["ESP"] == 0
+        assert abs(r["R"] + r["D"] - 1.0) <= 1e-15
+
+
+@pytest.mark.parametrize("idx", range(5))
+def test_small_graph_counts(expected, idx):
+    c = expected["derived_counts"][idx]
+    r = load(c["graph"]).solve(c["src"], c["dst"])
+    assert (r["paths"], r["nodes"], r["iterations"], r["max_width"]) == (
+        c["paths"], c["nodes"], c["iterations"], c["max_width"])
+
+
+@pytest.mark.parametrize("name,n,p", [("COMP8.txt", 8, "0.5"), ("COMP8-03.txt", 8, "0.3"),
+                                      ("COMP8-09.txt", 8, "0.9"), ("temp4x4.txt", 4, "0.5")])
+def test_closed_form_small(expected, name, n, p):
+    r = load(name).solve(0, n - 1)
+    R = closed_form(expected, n, p)
+    assert abs(r["R"] - R) <= 1e-14 * R
+    assert abs(r["R"] + r["D"] - 1.0) <= 1e-14
+
+
+@pytest.mark.parametrize("name", ["COMP8.txt", "COMP9.txt"])
+def test_logged_counts_level_order(expected, name):
+    c = next(x for x in expected["logged_counts"] if x["graph"] == name)
+    r = load(name).solve(c["src"], c["dst"])
+    assert (r["paths"], r["nodes"], r["iterations"], r["max_width"]) == (
+        c["paths"], c["nodes"], c["iterations"], c["max_width"]), c["source"]
+
+
+def test_k10_counts_and_closed_form(expected):
+    """K10 p=0.5: the logged tree (7433032 paths / 14866064 nodes) and the
+    exact R, via the multithreaded depth-first order (order-independent)."""
+    c = next(x for x in expected["logged_counts"] if x["graph"] == "COMP10-05.txt")
+    r = load("COMP10-05.txt").solve(0, 9, order=ORDER_DFS, threads=8)
+    assert (r["paths"], r["nodes"]) == (c["paths"], c["nodes"])
+    R = closed_form(expected, 10, "0.5")
+    assert abs(r["R"] - R) <= 1e-13 * R
+    assert r["dead"] == r["nodes"] - r["paths"]
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_counts_independent_of_order_and_threads(threads):
+    g = load("COMP8-03.txt")
+    lv = g.solve(0, 7, order=ORDER_LEVEL)
+    hp = g.solve(0, 7, order=ORDER_HEAP)
+    df = g.solve(0, 7, order=ORDER_DFS, threads=threads)
+    for r in (hp, df):
+        assert (r["paths"], r["nodes"], r["dead"]) == (lv["paths"], lv["nodes"], lv["dead"])
+        assert abs(r["R"] - lv["R"]) <= 1e-15
+
+
+@pytest.mark.parametrize("eps", [1e-3, 1e-5, 1e-8])
+def test_truncation_is_order_independent_and_brackets(expected, eps):
+    g = load("COMP9.txt")
+    Rt = closed_form(expected, 9, "0.5")
+    rs = [g.solve(0, 8, order=o, trunc_eps=eps, threads=4)
+          for o in (ORDER_LEVEL, ORDER_HEAP, ORDER_DFS)]
+    for r in rs:
+        assert (r["paths"], r["nodes"], r["truncated"]) == (
+            rs[0]["paths"], rs[0]["nodes"], rs[0]["truncated"])
+        assert r["R"] <= Rt + 1e-15 and Rt <= r["R"] + r["ESP"] + 1e-15
+        assert abs(r["R"] + r["D"] + r["ESP"] - 1.0) <= 1e-14
+    assert rs[0]["truncated"] > 0
+
+
+@pytest.mark.parametrize("order", [ORDER_LEVEL, ORDER_HEAP])
+@pytest.mark.parametrize("cap", [1, 7, 100, 1000])
+def test_capacity_overflow_brackets(expected, order, cap):
+    g = load("COMP9.txt")
+    Rt = closed_form(expected, 9, "0.5")
+    r = g.solve(0, 8, order=order, capacity=cap)
+    assert r["R"] <= Rt + 1e-15 <= r["R"] + r["ESP"] + 2e-15
+    assert abs(r["R"] + r["D"] + r["ESP"] - 1.0) <= 1e-14
+    assert r["overflowed"] > 0
+
+
+def test_heap_order_beats_level_order_under_capacity(expected):
+    """Max-mass order resolves more mass than FIFO for the same capacity."""
+    g = load("COMP10-05.txt")
+    lv = g.solve(0, 9, order=ORDER_LEVEL, capacity=2000)
+    hp = g.solve(0, 9, order=ORDER_HEAP, capacity=2000)
+    assert hp["ESP"] < lv["ESP"]
+
+
+@pytest.mark.parametrize("order", [ORDER_LEVEL, ORDER_HEAP, ORDER_DFS])
+def test_time_budget_fold_keeps_bracket(expected, order):
+    g = load("COMP15-05.txt")
+    Rt = closed_form(expected, 15, "0.5")
+    r = g.solve(0, 14, order=order, time_budget_s=0.2, threads=2)
+    assert r["deadline_hit"] == 1
+    assert r["R"] <= Rt <= r["R"] + r["ESP"] + 1e-14
+    assert abs(r["R"] + r["D"] + r["ESP"] - 1.0) <= 1e-13
+
+
+def test_src_equals_dst():
+    r = load("test_kite.txt").solve(2, 2)
+    assert r["R"] == 1.0 and r["paths"] == 1 and r["nodes"] == 1
+
+
+def _csr(n, edges):
+    rows = [[] for _ in range(n)]
+    for i, (a, b) in enumerate(edges):
+        rows[a].append((b, i))
+        rows[b].append((a, i))
+    rp, ci, ei = [0], [], []
+    for u in range(n):
+        for v, i in sorted(rows[u]):
+            ci.append(v)
+            ei.append(i)
+        rp.append(len(ci))
+    return rp, ci, ei
+
+
+def test_series_graph_is_product():
+    rnd = random.Random(1)
+    n = 12
+    edges = [(i, i + 1) for i in range(n - 1)]
+    p = [rnd.uniform(0.5, 0.99) for _ in edges]
+    g = OracleGraph.from_csr(*_csr(n, edges), p)
+    r = g.solve(0, n - 1)
+    assert abs(r["R"] - math.prod(p)) <= 1e-15
+    assert r["nodes"] == n and r["paths"] == 1  # root + (n-1) dead children
+
+
+def test_parallel_bundle():
+    """Paths of length 2 through k middle nodes: 1 - prod(1 - p_a p_b)."""
+    rnd = random.Random(2)
+    k = 9
+    edges = []
+    for m in range(1, k + 1):
+        edges += [(0, m), (m, k + 1)]
+    p = [rnd.uniform(0.3, 0.9) for _ in edges]
+    g = OracleGraph.from_csr(*_csr(k + 2, edges), p)
+    r = g.solve(0, k + 1)
+    want = 1.0 - math.prod(1 - p[2 * i] * p[2 * i + 1] for i in range(k))
+    assert abs(r["R"] - want) <= 1e-14
+
+
+def test_bridge_factors_on_bridge():
+    """Two K4 blocks joined by one bridge edge: R = R1 * p_bridge * R2."""
+    edges = [(a, b) for a in range(4) for b in range(a + 1, 4)]
+    edges += [(a + 4, b + 4) for a in range(4) for b in range(a + 1, 4)]
+    edges.append((3, 4))
+    p = [0.5] * 12 + [0.8]
+    g = OracleGraph.from_csr(*_csr(8, edges), p)
+    r = g.solve(0, 7)
+    # R(K4, 0->3) = 0.75 (closed form); inside the second block 4->7 also 0.75
+    assert abs(r["R"] - 0.75 * 0.8 * 0.75) <= 1e-15
+
+
+def test_pivot_mode_matches_closed_form(expected):
+    """Edge-pivot factoring (workingPad/cpu_reference.cpp semantics)."""
+    for name, n, p in [("COMP8.txt", 8, "0.5"), ("temp4x4.txt", 4, "0.5")]:
+        r = load(name).solve(0, n - 1, mode=MODE_PIVOT, order=ORDER_DFS)
+        R = closed_form(expected, n, p)
+        assert abs(r["R"] - R) <= 1e-14
+
+
+def test_pivot_mode_brute_force(expected):
+    for bf in expected["brute_force"]:
+        r = load(bf["graph"]).solve(bf["src"], bf["dst"], mode=MODE_PIVOT)
+        assert abs(r["R"] - bf["R"]) <= 1e-15
+
+
+def test_expand_children_partition_mass():
+    """One expansion: children + success mass = item mass (telescoping sum),
+    each child differs from the item on the path edges only."""
+    g = load("COMP10-05.txt")
+    W = g.mask_words
+    rnd = random.Random(3)
+    for _ in range(200):
+        known = up = 0
+        for e in range(45):
+            s = rnd.random()
+            if s < 0.3:
+                known |= 1 << e  # down
+            elif s < 0.5:
+                known |= 1 << e
+                up |= 1 << e
+        m = rnd.uniform(0.1, 1.0)
+        k, succ, ck, cu, cm, path = g.expand(0, 9, [known], [up], m)
+        if k < 0:
+            continue
+        assert abs(sum(cm) + succ - m) <= 1e-15 * (k + 1)
+        for i in range(k):
+            assert int(ck[i][0]) & known == known and int(cu[i][0]) & up == up
+        # failed hop of child i is the i-th unknown hop on the path
+        unknown_hops = [int(e) for e in path if not (known >> int(e)) & 1]
+        assert len(unknown_hops) == k
+        for i, e in enumerate(unknown_hops):
+            assert (int(ck[i][0]) >> e) & 1 and not (int(cu[i][0]) >> e) & 1
+        assert W == 1
