@@ -65,8 +65,14 @@ typedef struct nr_config {
   int grid_blocks;          /* blocks per launch; 0 = auto from occupancy */
   int chunk_items;          /* persistent modes: items claimed per wave refill;
                                0 = auto */
-  int reserved[7];
+  int flags;                /* NR_FLAG_* */
+  int reserved[6];
 } nr_config;
+
+/* nr_config.flags */
+#define NR_FLAG_TIME_LAUNCHES 1 /* time every expand launch with HIP events
+                                   (fills kernel_ms / kernel_launches; adds a
+                                   few microseconds between launches) */
 
 typedef struct nr_result {
   double R;                     /* reliability lower bound (exact when ESP=0) */
@@ -153,6 +159,12 @@ int nr_expand_items(const nr_graph* g, int device, int count,
                     unsigned long long* child_up, double* child_mass);
 
 const char* nr_status_str(int status);
+
+/* Free the per-device workspace (queue buffers, graph tables, stream,
+ * events) that solves on `device` keep between calls; the next call
+ * reallocates it.  Replaces the end-of-run frees of workingPad/main.cu:
+ * 1058-1069 and graph_free_device (graph.h:181-188). */
+int nr_device_release(int device);
 
 /* Device name / compute units / wavefront size of a HIP device. */
 int nr_device_query(int device, char* name, int name_len, int* compute_units,

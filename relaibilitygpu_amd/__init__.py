@@ -23,6 +23,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libnrgpu.so")
 CLI_PATH = os.path.join(_HERE, "bin", "network_reliability")
 
+NR_FLAG_TIME_LAUNCHES = 1
+
 NR_ORDER_LEVEL = 0
 NR_ORDER_MASS = 1
 NR_ORDER_LEARNED = 2
@@ -58,7 +60,8 @@ class Config(ctypes.Structure):
         ("block_size", ctypes.c_int),
         ("grid_blocks", ctypes.c_int),
         ("chunk_items", ctypes.c_int),
-        ("reserved", ctypes.c_int * 7),
+        ("flags", ctypes.c_int),
+        ("reserved", ctypes.c_int * 6),
     ]
 
 
@@ -163,7 +166,8 @@ def _ptr(a: np.ndarray, ctype):
 
 def make_config(queue_capacity: int = 1 << 20, truncation_eps: float = 1e-15,
                 time_budget_s: float = 0.0, order: str | int = "level", device: int = 0,
-                block_size: int = 0, grid_blocks: int = 0, chunk_items: int = 0) -> Config:
+                block_size: int = 0, grid_blocks: int = 0, chunk_items: int = 0,
+                time_launches: bool = False) -> Config:
     c = Config()
     lib().nr_config_init(ctypes.byref(c))
     c.queue_capacity = int(queue_capacity)
@@ -174,6 +178,7 @@ def make_config(queue_capacity: int = 1 << 20, truncation_eps: float = 1e-15,
     c.block_size = int(block_size)
     c.grid_blocks = int(grid_blocks)
     c.chunk_items = int(chunk_items)
+    c.flags = NR_FLAG_TIME_LAUNCHES if time_launches else 0
     return c
 
 

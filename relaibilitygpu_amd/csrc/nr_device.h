@@ -1,10 +1,10 @@
 /*
  * nr_device.h - device-side building blocks of the expand kernels (gfx950,
  * wave64).  One lane owns one subproblem ("item"): its two edge-state bit
- * planes live in VGPRs, the graph's adjacency/incidence tables are staged in
- * LDS per workgroup, and the lane's BFS level sets and path bookkeeping live in
- * an LDS slice laid out [slot][lane] so a wave's accesses fall in distinct
- * banks.
+ * planes live in VGPRs; its allowed-adjacency rows (row u = the neighbours of
+ * u over edges that are not FAILED), BFS level sets and path bookkeeping live
+ * in an LDS slice laid out [slot][lane] so a wave's accesses fall in distinct
+ * banks; the graph's static tables are staged in LDS per workgroup.
  *
  * Edge states: known/up bit planes.  UNKNOWN (0,0), WORKING (1,1),
  * FAILED (1,0); the reference's 2-bit codes 0b11/0b01/0b00
@@ -53,12 +53,14 @@ struct Acc2 {
 struct Layout {
   int n, e, w;
   int off_sadj, off_inc, off_ep, off_eidm, off_deg, off_p, off_q;
-  int off_lvl, off_uh;
+  int off_lvl, off_uh, off_rows;
+  int off_extra; /* kernel-specific region (segment prefix table, counters) */
   int table_bytes, total_bytes;
   int block, ns_bytes;
 };
 
-inline Layout make_layout(int n, int e, int w, int block, int ns_bytes) {
+inline Layout make_layout(int n, int e, int w, int block, int ns_bytes,
+                          int extra_bytes = 0) {
   Layout L;
   L.n = n;
   L.e = e;
@@ -81,6 +83,8 @@ inline Layout make_layout(int n, int e, int w, int block, int ns_bytes) {
   L.off_deg = take(4 * n, 16);
   L.off_eidm = take(2 * n * n, 16);
   L.table_bytes = (o + 15) / 16 * 16;
+  L.off_extra = take(extra_bytes, 16);
+  L.off_rows = take(ns_bytes * n * block, 16);
   L.off_lvl = take(ns_bytes * n * block, 16);
   L.off_uh = take(2 * n * block, 16);
   L.total_bytes = (o + 15) / 16 * 16;
@@ -149,6 +153,7 @@ struct Expander {
   const u32* ep;
   const u32* deg;
   const u16* eidm;
+  NS* rows; /* this lane's allowed-adjacency rows, stride `stride` */
   NS* lvl;  /* this lane's level sets, stride `stride` */
   u16* uh;  /* this lane's unknown-hop stack, stride `stride` */
   int stride, n, src, dst;
@@ -162,6 +167,7 @@ struct Expander {
     ep = (const u32*)(smem + L.off_ep);
     deg = (const u32*)(smem + L.off_deg);
     eidm = (const u16*)(smem + L.off_eidm);
+    rows = (NS*)(smem + L.off_rows) + slot;
     lvl = (NS*)(smem + L.off_lvl) + slot;
     uh = (u16*)(smem + L.off_uh) + slot;
     stride = L.block;
@@ -212,13 +218,19 @@ struct Expander {
     return a;
   }
 
-  /* Optimistic BFS from src (device_code.cuh:80-194) and the path to dst
-   * under the lowest-index parent rule: the parent of v is the lowest-index
-   * node of the previous BFS level joined to v by an allowed edge.  Pushes
-   * the path's UNKNOWN hops on uh[] in dst->src order.  Returns the number
-   * of UNKNOWN hops, -1 if dst is unreachable, -2 on an internal
-   * inconsistency (never expected). */
-  __device__ int find(const u64 (&known)[W], const u64 (&Dn)[W]) const {
+  /* rows[u] = adj_of(u) for every node: derives an item's rows from its
+   * planes (root item, caller-supplied items). */
+  __device__ void build_rows(const u64 (&Dn)[W]) const {
+    for (int u = 0; u < n; ++u) rows[u * stride] = adj_of(u, Dn);
+  }
+
+  /* Optimistic BFS from src (device_code.cuh:80-194) over the item's rows,
+   * and the path to dst under the lowest-index parent rule: the parent of v
+   * is the lowest-index node of the previous BFS level joined to v by an
+   * allowed edge.  Pushes the path's UNKNOWN hops on uh[] in dst->src order.
+   * Returns the number of UNKNOWN hops, -1 if dst is unreachable, -2 on an
+   * internal inconsistency (never expected). */
+  __device__ int find(const u64 (&known)[W]) const {
     if (src == dst) return 0;
     const NS dbit = (NS)1 << dst;
     NS vis = (NS)1 << src;
@@ -229,9 +241,9 @@ struct Expander {
       NS nx = 0;
       NS f = F;
       while (f) {
-        int u = ctzs<NS>(f);
+        const int u = ctzs<NS>(f);
         f &= f - 1;
-        nx |= adj_of(u, Dn);
+        nx |= rows[u * stride];
         if (nx & dbit) break;
       }
       nx &= ~vis;
@@ -244,14 +256,25 @@ struct Expander {
     }
     int v = dst, k = 0;
     for (int l = L; l >= 1; --l) {
-      NS cand = adj_of(v, Dn) & lvl[(l - 1) * stride];
+      const NS cand = rows[v * stride] & lvl[(l - 1) * stride];
       if (!cand) return -2;
-      int u = ctzs<NS>(cand);
-      int e = eidm[u * n + v];
+      const int u = ctzs<NS>(cand);
+      const int e = eidm[u * n + v];
       if (!((pick<W>(known, e >> 6) >> (e & 63)) & 1ull)) uh[(k++) * stride] = (u16)e;
       v = u;
     }
     return k;
+  }
+
+  /* Row u of a child whose FAILED hop is edge e: the parent's row with the
+   * ends of e removed from each other's rows. */
+  __device__ NS child_row(int u, int e) const {
+    const u32 x = ep[e];
+    const int a = (int)(x & 0xffffu), b = (int)(x >> 16);
+    NS r = rows[u * stride];
+    if (u == a) r &= ~((NS)1 << b);
+    if (u == b) r &= ~((NS)1 << a);
+    return r;
   }
 };
 

@@ -3,21 +3,24 @@
  * two-terminal reliability engine; implements nr_solve / nr_expand_items of
  * include/nr_api.h.
  *
- * Level order (the reference's order, workingPad/main.cu:963-1008): the queue
- * is a pair of HBM buffers in structure-of-arrays form (known planes, up
- * planes, FP64 mass).  One launch of k_expand_level consumes one tree level:
- * each lane takes one item (coalesced SoA loads), runs the BFS/path search
- * of nr_device.h, and the wave reserves room for all of its children with a
- * single atomic (prefix scan over the lanes' child counts), then every lane
- * writes its children at its offset.  The last workgroup to finish clamps
- * the next level to the capacity, counts the level and resets the counters,
- * so consecutive levels are launched back to back with no host round trip;
- * the host reads the control block once per chunk of levels.
+ * Level order (the reference's order, workingPad/main.cu:963-1008).  The
+ * queue is a pair of HBM buffers in structure-of-arrays form (known planes,
+ * up planes, allowed-adjacency rows, FP64 mass), each cut into one segment
+ * per workgroup of the expand grid.  One launch of k_expand_level consumes
+ * one tree level: every workgroup first scans the previous launch's
+ * per-segment counts into a prefix table in LDS; lane i of a wave then takes
+ * the level's item i, runs the BFS/path search of nr_device.h over the
+ * item's rows, and the wave reserves room for all of its children in its
+ * workgroup's own output segment with one LDS atomic; every lane then writes
+ * its children at its offset.  No global atomics and one launch per level,
+ * so consecutive levels run back to back with no host round trip; the host
+ * reads the control block once per chunk of levels.
  *
  * Masses are FP64 products taken left to right along the path; R, D and ESP
- * are per-lane Neumaier sums, merged per wave into double-double slots that
- * the host reduces at the end (replaces the single-address FP64 CAS of
- * main.cu:68-79).
+ * are per-lane Neumaier sums, merged per wave into double-double slots, and
+ * the counters go to per-wave slots; k_finalize reduces both in a fixed
+ * order (replaces the single-address FP64 CAS of main.cu:68-79 and the
+ * global counters of main.cu:58-63).
  */
 #include <hip/hip_runtime.h>
 
@@ -26,6 +29,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "nr_device.h"
@@ -33,20 +37,22 @@
 
 namespace nr {
 
+/* Per-wave counter slots. */
+enum { C_NODES = 0, C_PATHS, C_DEAD, C_TRUNC, C_OVER, C_FOLD, C_ERR, C_N = 8 };
+
 /* Control block of a level-order solve (device memory). */
 struct LevelCtl {
-  u64 count[2];   /* items in buffer 0/1; the write side doubles as the
-                     append counter and may exceed the capacity */
+  u64 last_items; /* items of the most recently launched level */
   u64 iterations; /* non-empty levels processed */
   u64 max_width;  /* widest level */
-  u64 nodes, paths, dead, truncated, overflowed, folded, errors;
-  u32 blocks_done;
-  u32 pad;
+  u64 counts[C_N];
+  double sum[3];  /* R, ESP, D after k_finalize */
 };
 
 struct QBuf {
-  u64* known; /* [w][cap] */
-  u64* up;    /* [w][cap] */
+  u64* known; /* [w][n_seg*seg_cap] */
+  u64* up;    /* [w][n_seg*seg_cap] */
+  void* rows; /* [n][n_seg*seg_cap] node sets (u32 or u64) */
   double* mass;
 };
 
@@ -54,40 +60,164 @@ struct LevelArgs {
   DevGraph g;
   Layout lay;
   QBuf buf[2];
+  u32* seg_count[2]; /* [n_seg] items written by each workgroup */
   LevelCtl* ctl;
   DD* partR;
   DD* partE;
   DD* partD;
-  u64 cap;
+  u64* cnt; /* [n_seg][C_N] per-workgroup counters */
+  u64 stride; /* n_seg * seg_cap: distance between planes */
+  u32 seg_cap;
+  int n_seg;
   double eps;
 };
 
 template <int W>
-__device__ inline void load_item(const QBuf& b, u64 cap, u64 i, u64 (&known)[W],
+__device__ inline void load_item(const QBuf& b, u64 stride, u64 i, u64 (&known)[W],
                                  u64 (&up)[W], double& mass) {
 #pragma unroll
   for (int w = 0; w < W; ++w) {
-    known[w] = b.known[w * cap + i];
-    up[w] = b.up[w * cap + i];
+    known[w] = b.known[w * stride + i];
+    up[w] = b.up[w * stride + i];
   }
   mass = b.mass[i];
 }
 
 template <int W>
-__device__ inline void store_item(const QBuf& b, u64 cap, u64 i, const u64 (&known)[W],
+__device__ inline void store_item(const QBuf& b, u64 stride, u64 i, const u64 (&known)[W],
                                   const u64 (&up)[W], double mass) {
 #pragma unroll
   for (int w = 0; w < W; ++w) {
-    b.known[w * cap + i] = known[w];
-    b.up[w * cap + i] = up[w];
+    b.known[w * stride + i] = known[w];
+    b.up[w * stride + i] = up[w];
   }
   b.mass[i] = mass;
 }
 
-/* One tree level: consume buf[parity], append children to buf[parity^1]. */
+/* Segment holding global item index `base`: the largest s with
+ * sp[s] <= base (sp[0] = 0 <= base < sp[n_seg]). */
+__device__ inline int segment_of(const u32* sp, int n_seg, u32 base) {
+  int lo = 0, hi = n_seg;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (sp[mid] <= base)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+/* Exclusive prefix of the per-segment counts into sp[0..n_seg] (LDS), by
+ * the whole workgroup; `part` is blockDim.x words of LDS scratch. */
+__device__ inline void block_prefix(const u32* counts, int n_seg, u32* sp, u32* part) {
+  const int t = threadIdx.x, T = blockDim.x;
+  const int per = (n_seg + T - 1) / T;
+  const int b0 = min(t * per, n_seg), b1 = min(b0 + per, n_seg);
+  u32 sum = 0;
+  for (int b = b0; b < b1; ++b) sum += counts[b];
+  part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < T; off <<= 1) {
+    const u32 v = t >= off ? part[t - off] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  u32 run = t ? part[t - 1] : 0u;
+  for (int b = b0; b < b1; ++b) {
+    sp[b] = run;
+    run += counts[b];
+  }
+  if (t == T - 1) sp[n_seg] = part[T - 1];
+  __syncthreads();
+}
+
+__device__ inline void flush_counts(u64* slot, const u32 (&c)[C_N]) {
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < C_N; ++j) any |= c[j] != 0;
+  if (!any) return;
+#pragma unroll
+  for (int j = 0; j < C_N; ++j) slot[j] += c[j];
+}
+
+/* Flush a lane's accumulators into its workgroup's slot: wave sums, then
+ * wave 0's lane 0 merges the workgroup's waves in wave order and updates the
+ * block's double-double and counter slots (owned by this workgroup for the
+ * launch; launches on one stream do not overlap).  `scratch` is LDS of at
+ * least 80 bytes per wave. */
+__device__ inline void block_flush(const LevelArgs& a, Acc2& accR, Acc2& accE, Acc2& accD,
+                                   const u32 (&c)[C_N], unsigned char* scratch) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int n_w = (blockDim.x + 63) >> 6;
+  wave_sum_dd(accR.s, accR.c);
+  wave_sum_dd(accE.s, accE.c);
+  wave_sum_dd(accD.s, accD.c);
+  double* sd = (double*)scratch;      /* [n_w][6] */
+  u32* sc = (u32*)(sd + 6 * n_w);     /* [n_w][C_N] */
+  u32 t[C_N];
+#pragma unroll
+  for (int j = 0; j < C_N; ++j) t[j] = (u32)wave_sum_u64(c[j]);
+  if (lane == 0) {
+    sd[wave * 6 + 0] = accR.s;
+    sd[wave * 6 + 1] = accR.c;
+    sd[wave * 6 + 2] = accE.s;
+    sd[wave * 6 + 3] = accE.c;
+    sd[wave * 6 + 4] = accD.s;
+    sd[wave * 6 + 5] = accD.c;
+#pragma unroll
+    for (int j = 0; j < C_N; ++j) sc[wave * C_N + j] = t[j];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    DD* dst[3] = {&a.partR[blockIdx.x], &a.partE[blockIdx.x], &a.partD[blockIdx.x]};
+    for (int k = 0; k < 3; ++k) {
+      double s = 0.0, cc = 0.0;
+      for (int w = 0; w < n_w; ++w) {
+        double x, e;
+        two_sum(s, sd[w * 6 + 2 * k], x, e);
+        s = x;
+        cc += sd[w * 6 + 2 * k + 1] + e;
+      }
+      if (s != 0.0 || cc != 0.0) dd_merge_slot(dst[k], s, cc);
+    }
+    u32 tot[C_N];
+    bool any = false;
+    for (int j = 0; j < C_N; ++j) {
+      tot[j] = 0;
+      for (int w = 0; w < n_w; ++w) tot[j] += sc[w * C_N + j];
+      any |= tot[j] != 0;
+    }
+    if (any) {
+      u64* slot = &a.cnt[(u64)blockIdx.x * C_N];
+      for (int j = 0; j < C_N; ++j) slot[j] += tot[j];
+    }
+  }
+}
+
+/* One tree level: consume buf[parity], append children to the workgroup's
+ * segment of buf[parity^1]. */
 template <int W, typename NS>
 __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity) {
   extern __shared__ __align__(16) unsigned char smem[];
+  u32* sp = (u32*)(smem + a.lay.off_extra);
+  u32* s_cnt = sp + a.n_seg + 1;
+  u32* part = s_cnt + 1;
+  block_prefix(a.seg_count[parity], a.n_seg, sp, part);
+  const u32 n_items = sp[a.n_seg];
+  if (blockIdx.x == 0 && threadIdx.x == 0) { /* count the level (main.cu:970-973) */
+    a.ctl->last_items = n_items;
+    if (n_items) {
+      a.ctl->iterations += 1;
+      if (n_items > a.ctl->max_width) a.ctl->max_width = n_items;
+    }
+  }
+  if (n_items == 0) {
+    if (threadIdx.x == 0) a.seg_count[parity ^ 1][blockIdx.x] = 0;
+    return;
+  }
+  if (threadIdx.x == 0) *s_cnt = 0;
   stage_tables(a.g, a.lay, smem);
   __syncthreads();
 
@@ -96,82 +226,84 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity) {
 
   const QBuf in = a.buf[parity];
   const QBuf out = a.buf[parity ^ 1];
-  const u64 n_items = __atomic_load_n(&a.ctl->count[parity], __ATOMIC_RELAXED);
   const int lane = threadIdx.x & 63;
   const u64 wave_id = (u64)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const u64 n_waves = (u64)(gridDim.x * blockDim.x) >> 6;
+  const u64 out_base = (u64)blockIdx.x * a.seg_cap;
 
   Acc2 accR, accE, accD;
   accR.init();
   accE.init();
   accD.init();
-  u32 c_nodes = 0, c_paths = 0, c_dead = 0, c_trunc = 0, c_over = 0, c_err = 0;
+  u32 c[C_N] = {0, 0, 0, 0, 0, 0, 0, 0};
 
-  for (u64 base = wave_id * 64; base < n_items; base += n_waves * 64) {
-    const u64 i = base + lane;
-    u64 known[W], up[W], Dn[W];
+  /* Small levels are spread over every wave (chunk < 64 lanes) so each
+   * workgroup receives, and appends to its segment, a fair share. */
+  const u64 per_wave = (n_items + n_waves - 1) / n_waves;
+  const u64 chunk = per_wave >= 64 ? 64 : per_wave;
+  for (u64 base = wave_id * chunk; base < n_items; base += n_waves * chunk) {
+    const u32 i = (u32)(base + lane);
+    u64 known[W], up[W];
     double mass = 0.0;
     int k = 0;
     bool live = false;
-    if (i < n_items) {
-      load_item<W>(in, a.cap, i, known, up, mass);
-      ++c_nodes;
+    if ((u64)lane < chunk && i < n_items) {
+      int s = segment_of(sp, a.n_seg, (u32)base);
+      while (sp[s + 1] <= i) ++s;
+      const u64 slot_in = (u64)s * a.seg_cap + (i - sp[s]);
+      load_item<W>(in, a.stride, slot_in, known, up, mass);
+      ++c[C_NODES];
       if (mass < a.eps) { /* main.cu:537-543 */
         accE.add(mass);
-        ++c_trunc;
+        ++c[C_TRUNC];
       } else {
-#pragma unroll
-        for (int w = 0; w < W; ++w) Dn[w] = known[w] & ~up[w];
-        k = ex.find(known, Dn);
+        const NS* rin = (const NS*)in.rows + slot_in;
+        for (int u = 0; u < ex.n; ++u) ex.rows[u * ex.stride] = rin[(u64)u * a.stride];
+        k = ex.find(known);
         if (k >= 0) {
           live = true;
         } else {
-          if (k == -2) ++c_err;
+          if (k == -2) ++c[C_ERR];
           accD.add(mass); /* main.cu:579-582 */
-          ++c_dead;
+          ++c[C_DEAD];
           k = 0;
         }
       }
     }
     const int incl = wave_incl_scan(k);
     const int total = __shfl(incl, 63, 64);
-    u64 wbase = 0;
+    u32 wbase = 0;
     if (total) {
-      unsigned lo = 0, hi = 0;
-      if (lane == 63) {
-        u64 b = atomicAdd((unsigned long long*)&a.ctl->count[parity ^ 1], (u64)total);
-        lo = (unsigned)b;
-        hi = (unsigned)(b >> 32);
-      }
-      lo = __shfl(lo, 63, 64);
-      hi = __shfl(hi, 63, 64);
-      wbase = ((u64)hi << 32) | lo;
+      if (lane == 63) wbase = atomicAdd(s_cnt, (u32)total);
+      wbase = __shfl(wbase, 63, 64);
     }
     if (live) {
       /* main.cu:214-266: child j has the earlier UNKNOWN hops WORKING and
        * hop j FAILED; masses multiply left to right along the path */
-      u64 slot = wbase + (u64)(incl - k);
+      u32 off = wbase + (u32)(incl - k);
       double run = mass;
       u64 pre[W];
 #pragma unroll
       for (int w = 0; w < W; ++w) pre[w] = 0;
-      for (int j = k - 1; j >= 0; --j, ++slot) {
+      for (int j = k - 1; j >= 0; --j, ++off) {
         const int e = ex.uh[j * ex.stride];
         const int wi = e >> 6;
         const u64 bit = 1ull << (e & 63);
         const double cm = run * ex.q[e];
-        if (slot < a.cap) {
+        if (off < a.seg_cap) {
           u64 ck[W], cu[W];
 #pragma unroll
           for (int w = 0; w < W; ++w) {
             ck[w] = known[w] | pre[w] | (w == wi ? bit : 0ull);
             cu[w] = up[w] | pre[w];
           }
-          store_item<W>(out, a.cap, slot, ck, cu, cm);
-        } else { /* queue full: main.cu:245-252 */
+          store_item<W>(out, a.stride, out_base + off, ck, cu, cm);
+          NS* rout = (NS*)out.rows + out_base + off;
+          for (int u = 0; u < ex.n; ++u) rout[(u64)u * a.stride] = ex.child_row(u, e);
+        } else { /* segment full: main.cu:245-252 */
           accE.add(cm);
-          ++c_paths;
-          ++c_over;
+          ++c[C_PATHS];
+          ++c[C_OVER];
         }
         run = run * ex.p[e];
 #pragma unroll
@@ -179,69 +311,49 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity) {
           if (w == wi) pre[w] |= bit;
       }
       accR.add(run); /* success term, main.cu:254-262 */
-      ++c_paths;
+      ++c[C_PATHS];
     }
   }
 
-  /* per-wave flush */
-  wave_sum_dd(accR.s, accR.c);
-  wave_sum_dd(accE.s, accE.c);
-  wave_sum_dd(accD.s, accD.c);
-  u64 t_nodes = wave_sum_u64(c_nodes), t_paths = wave_sum_u64(c_paths),
-      t_dead = wave_sum_u64(c_dead), t_trunc = wave_sum_u64(c_trunc),
-      t_over = wave_sum_u64(c_over), t_err = wave_sum_u64(c_err);
-  if (lane == 0) {
-    dd_merge_slot(&a.partR[wave_id], accR.s, accR.c);
-    dd_merge_slot(&a.partE[wave_id], accE.s, accE.c);
-    dd_merge_slot(&a.partD[wave_id], accD.s, accD.c);
-    if (t_nodes) atomicAdd((unsigned long long*)&a.ctl->nodes, t_nodes);
-    if (t_paths) atomicAdd((unsigned long long*)&a.ctl->paths, t_paths);
-    if (t_dead) atomicAdd((unsigned long long*)&a.ctl->dead, t_dead);
-    if (t_trunc) atomicAdd((unsigned long long*)&a.ctl->truncated, t_trunc);
-    if (t_over) atomicAdd((unsigned long long*)&a.ctl->overflowed, t_over);
-    if (t_err) atomicAdd((unsigned long long*)&a.ctl->errors, t_err);
-  }
-
-  /* last workgroup: close the level (main.cu:673-691 done on the device) */
-  __threadfence();
+  /* per-workgroup flush; then publish this workgroup's segment count */
   __syncthreads();
+  block_flush(a, accR, accE, accD, c, (unsigned char*)part);
   if (threadIdx.x == 0) {
-    u32 prev = atomicAdd(&a.ctl->blocks_done, 1u);
-    if (prev == gridDim.x - 1) {
-      __threadfence();
-      u64 produced = atomicAdd((unsigned long long*)&a.ctl->count[parity ^ 1], 0ull);
-      a.ctl->count[parity ^ 1] = produced < a.cap ? produced : a.cap;
-      a.ctl->count[parity] = 0;
-      if (n_items) {
-        a.ctl->iterations += 1;
-        if (n_items > a.ctl->max_width) a.ctl->max_width = n_items;
-      }
-      a.ctl->blocks_done = 0;
-      __threadfence();
-    }
+    const u32 n = *s_cnt;
+    a.seg_count[parity ^ 1][blockIdx.x] = n < a.seg_cap ? n : a.seg_cap;
   }
 }
 
 /* Fold every queued item of buf[parity] into ESP (time budget expiry;
  * original/Seq_16.cpp:44,227 PURGEQ restated on the device). */
 __global__ void __launch_bounds__(256) k_fold(LevelArgs a, int parity) {
-  const u64 n_items = __atomic_load_n(&a.ctl->count[parity], __ATOMIC_RELAXED);
+  extern __shared__ __align__(16) unsigned char smem[];
+  u32* sp = (u32*)smem;
+  u32* part = sp + a.n_seg + 2;
+  block_prefix(a.seg_count[parity], a.n_seg, sp, part);
+  const u32 n_items = sp[a.n_seg];
   const int lane = threadIdx.x & 63;
   const u64 wave_id = (u64)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const u64 n_waves = (u64)(gridDim.x * blockDim.x) >> 6;
   Acc2 e;
   e.init();
-  u32 c = 0;
-  for (u64 i = wave_id * 64 + lane; i < n_items; i += n_waves * 64) {
-    e.add(a.buf[parity].mass[i]);
-    ++c;
+  u32 c[C_N] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const u64 per_wave = (n_items + n_waves - 1) / n_waves;
+  const u64 chunk = per_wave >= 64 ? 64 : per_wave;
+  for (u64 base = wave_id * chunk; base < n_items; base += n_waves * chunk) {
+    const u32 i = (u32)(base + lane);
+    if ((u64)lane < chunk && i < n_items) {
+      int s = segment_of(sp, a.n_seg, (u32)base);
+      while (sp[s + 1] <= i) ++s;
+      e.add(a.buf[parity].mass[(u64)s * a.seg_cap + (i - sp[s])]);
+      ++c[C_FOLD];
+    }
   }
-  wave_sum_dd(e.s, e.c);
-  u64 t = wave_sum_u64(c);
-  if (lane == 0) {
-    dd_merge_slot(&a.partE[wave_id], e.s, e.c);
-    if (t) atomicAdd((unsigned long long*)&a.ctl->folded, t);
-  }
+  Acc2 zr, zd;
+  zr.init();
+  zd.init();
+  __syncthreads();
+  block_flush(a, zr, e, zd, c, (unsigned char*)part);
 }
 
 /* One expansion of each caller-supplied item into fixed output slots (the
@@ -277,7 +389,8 @@ __global__ void __launch_bounds__(256) k_expand_items(ItemsArgs a) {
     Dn[w] = known[w] & ~up[w];
   }
   const double mass = a.mass[i];
-  int k = ex.find(known, Dn);
+  ex.build_rows(Dn);
+  int k = ex.find(known);
   if (k < 0) {
     a.nchild[i] = k;
     a.succ[i] = 0.0;
@@ -310,9 +423,106 @@ __global__ void __launch_bounds__(256) k_expand_items(ItemsArgs a) {
   a.succ[i] = run;
 }
 
+/* Seed a solve: zero the control block, the partial-sum and counter slots,
+ * set the first prefix table to one item, and put the root item
+ * (main.cu:853-855) in slot 0 of buffer 0.  A kernel instead of memset /
+ * memcpy calls keeps the whole solve on one stream. */
+struct Root {
+  u64 known[8], up[8];
+  u64 rows[64];
+};
+
+__global__ void __launch_bounds__(256) k_init(LevelArgs a, Root root, int W, int n,
+                                              int ns_bytes, u64 n_slots) {
+  const u64 tid = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  const u64 stride = (u64)gridDim.x * blockDim.x;
+  for (u64 i = tid; i < n_slots; i += stride) {
+    a.partR[i].hi = a.partR[i].lo = 0.0;
+    a.partE[i].hi = a.partE[i].lo = 0.0;
+    a.partD[i].hi = a.partD[i].lo = 0.0;
+    for (int j = 0; j < C_N; ++j) a.cnt[i * C_N + j] = 0;
+  }
+  for (u64 s = tid; s < (u64)a.n_seg; s += stride) {
+    a.seg_count[0][s] = s == 0 ? 1u : 0u;
+    a.seg_count[1][s] = 0u;
+  }
+  if (tid == 0) {
+    LevelCtl z = {};
+    *a.ctl = z;
+    a.ctl->last_items = 1;
+    for (int w = 0; w < W; ++w) {
+      a.buf[0].known[(u64)w * a.stride] = root.known[w];
+      a.buf[0].up[(u64)w * a.stride] = root.up[w];
+    }
+    for (int u = 0; u < n; ++u) {
+      if (ns_bytes == 4)
+        ((u32*)a.buf[0].rows)[(u64)u * a.stride] = (u32)root.rows[u];
+      else
+        ((u64*)a.buf[0].rows)[(u64)u * a.stride] = root.rows[u];
+    }
+    a.buf[0].mass[0] = 1.0;
+  }
+}
+
+/* Reduce the per-wave slots in a fixed order (one block): thread t folds
+ * slots t, t+256, ...; thread 0 then folds the 256 partials in index order,
+ * so the result does not depend on timing. */
+__device__ inline void neu_add(double& s, double& c, double x) {
+  double u = s + x;
+  if (fabs(s) >= fabs(x))
+    c += (s - u) + x;
+  else
+    c += (x - u) + s;
+  s = u;
+}
+
+__global__ void __launch_bounds__(256) k_finalize(LevelArgs a, u64 n_slots) {
+  __shared__ double ps[3][256], pc[3][256];
+  __shared__ u64 pn[C_N][256];
+  const int t = threadIdx.x;
+  const DD* src[3] = {a.partR, a.partE, a.partD};
+  for (int k = 0; k < 3; ++k) {
+    double s = 0.0, c = 0.0;
+    for (u64 i = t; i < n_slots; i += blockDim.x) {
+      neu_add(s, c, src[k][i].hi);
+      neu_add(s, c, src[k][i].lo);
+    }
+    ps[k][t] = s;
+    pc[k][t] = c;
+  }
+  for (int j = 0; j < C_N; ++j) {
+    u64 x = 0;
+    for (u64 i = t; i < n_slots; i += blockDim.x) x += a.cnt[i * C_N + j];
+    pn[j][t] = x;
+  }
+  __syncthreads();
+  /* pairwise tree: slot t absorbs slot t + h with an error-free sum */
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if (t < h) {
+      for (int k = 0; k < 3; ++k) {
+        double s, e;
+        two_sum(ps[k][t], ps[k][t + h], s, e);
+        ps[k][t] = s;
+        pc[k][t] = pc[k][t] + pc[k][t + h] + e;
+      }
+      for (int j = 0; j < C_N; ++j) pn[j][t] += pn[j][t + h];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    for (int k = 0; k < 3; ++k) a.ctl->sum[k] = ps[k][0] + pc[k][0];
+    for (int j = 0; j < C_N; ++j) a.ctl->counts[j] = pn[j][0];
+  }
+}
+
 /* ------------------------------------------------------------------ */
 /* host runtime                                                        */
 /* ------------------------------------------------------------------ */
+
+hipError_t& last_hip_error() {
+  static thread_local hipError_t e = hipSuccess;
+  return e;
+}
 
 #define NR_HIP(call)                                   \
   do {                                                 \
@@ -323,100 +533,159 @@ __global__ void __launch_bounds__(256) k_expand_items(ItemsArgs a) {
     }                                                  \
   } while (0)
 
-hipError_t& last_hip_error() {
-  static thread_local hipError_t e = hipSuccess;
-  return e;
-}
+/* Per-device workspace kept across calls: queue buffers, graph tables,
+ * partial-sum slots, control block (device + pinned host copy), a stream
+ * and an event pool.  Buffers grow on demand and are reused, so repeated
+ * solves pay no allocation.  One mutex per device serialises calls. */
+struct Workspace {
+  bool ready = false;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipDeviceProp_t prop;
+  void* q = nullptr;
+  size_t q_bytes = 0;
+  void* tab = nullptr;
+  size_t tab_bytes = 0;
+  void* meta = nullptr; /* per-wave slots, segment counts, prefix tables */
+  size_t meta_bytes = 0;
+  LevelCtl* ctl = nullptr;
+  LevelCtl* hctl = nullptr; /* pinned */
+  std::vector<hipEvent_t> events;
+  std::mutex mu;
 
-/* RAII bag of device allocations. */
-struct DevMem {
-  std::vector<void*> ptrs;
-  ~DevMem() {
-    for (void* p : ptrs) (void)hipFree(p);
-  }
-  template <typename T>
-  int alloc(T** out, size_t count) {
-    void* p = nullptr;
-    if (count == 0) count = 1;
-    if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) return NR_ERR_NOMEM;
-    ptrs.push_back(p);
-    *out = (T*)p;
+  int grow(void** p, size_t* have, size_t need) {
+    if (need <= *have) return NR_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *have = 0;
+    if (hipMalloc(p, need) != hipSuccess) {
+      *p = nullptr;
+      return NR_ERR_NOMEM;
+    }
+    *have = need;
     return NR_OK;
   }
-};
-
-/* Host-built graph tables. */
-struct HostTables {
-  std::vector<u64> sadj, inc;
-  std::vector<u32> ep, deg;
-  std::vector<u16> eidm;
-};
-
-int build_tables(const nr_graph* g, int W, HostTables& t) {
-  const int n = g->n;
-  t.sadj.assign(n, 0);
-  t.inc.assign((size_t)n * W, 0);
-  t.ep.assign(std::max(g->e, 1), 0);
-  t.deg.assign(n, 0);
-  t.eidm.assign((size_t)n * n, 0xffff);
-  for (int e = 0; e < g->e; ++e) {
-    int a = g->ea[e], b = g->eb[e];
-    t.ep[e] = (u32)a | ((u32)b << 16);
-    t.inc[(size_t)a * W + (e >> 6)] |= 1ull << (e & 63);
-    t.inc[(size_t)b * W + (e >> 6)] |= 1ull << (e & 63);
-    if (a != b) {
-      t.sadj[a] |= 1ull << b;
-      t.sadj[b] |= 1ull << a;
+  int event(size_t i, hipEvent_t* e) {
+    while (events.size() <= i) {
+      hipEvent_t x;
+      NR_HIP(hipEventCreate(&x));
+      events.push_back(x);
     }
-    t.eidm[(size_t)a * n + b] = (u16)e;
-    t.eidm[(size_t)b * n + a] = (u16)e;
+    *e = events[i];
+    return NR_OK;
+  }
+  int open(int dev) {
+    if (ready) return NR_OK;
+    device = dev;
+    NR_HIP(hipSetDevice(dev));
+    NR_HIP(hipGetDeviceProperties(&prop, dev));
+    NR_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    NR_HIP(hipMalloc((void**)&ctl, sizeof(LevelCtl)));
+    NR_HIP(hipHostMalloc((void**)&hctl, sizeof(LevelCtl), hipHostMallocDefault));
+    ready = true;
+    return NR_OK;
+  }
+  void release() {
+    if (!ready) return;
+    (void)hipSetDevice(device);
+    (void)hipStreamSynchronize(stream);
+    for (auto e : events) (void)hipEventDestroy(e);
+    events.clear();
+    if (q) (void)hipFree(q);
+    if (tab) (void)hipFree(tab);
+    if (meta) (void)hipFree(meta);
+    if (ctl) (void)hipFree(ctl);
+    if (hctl) (void)hipHostFree(hctl);
+    (void)hipStreamDestroy(stream);
+    q = tab = meta = nullptr;
+    ctl = hctl = nullptr;
+    q_bytes = tab_bytes = meta_bytes = 0;
+    ready = false;
+  }
+};
+
+constexpr int kMaxDevices = 64;
+Workspace g_ws[kMaxDevices];
+
+/* Graph tables packed into one buffer: p, q (FP64), static neighbour sets,
+ * incident-edge sets, edge endpoints, incident counts, the (u,v) -> edge id
+ * matrix.  Offsets are 16-byte aligned. */
+struct TableBlob {
+  std::vector<unsigned char> bytes;
+  size_t off_p, off_q, off_sadj, off_inc, off_ep, off_deg, off_eidm;
+};
+
+void pack_tables(const nr_graph* g, int W, TableBlob& b) {
+  const int n = g->n, e = std::max(g->e, 1);
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    o = (o + 15) / 16 * 16;
+    size_t at = o;
+    o += bytes;
+    return at;
+  };
+  b.off_p = take(8 * (size_t)e);
+  b.off_q = take(8 * (size_t)e);
+  b.off_sadj = take(8 * (size_t)n);
+  b.off_inc = take(8 * (size_t)n * W);
+  b.off_ep = take(4 * (size_t)e);
+  b.off_deg = take(4 * (size_t)n);
+  b.off_eidm = take(2 * (size_t)n * n + 4);
+  b.bytes.assign((o + 15) / 16 * 16, 0);
+  double* p = (double*)&b.bytes[b.off_p];
+  double* q = (double*)&b.bytes[b.off_q];
+  u64* sadj = (u64*)&b.bytes[b.off_sadj];
+  u64* inc = (u64*)&b.bytes[b.off_inc];
+  u32* ep = (u32*)&b.bytes[b.off_ep];
+  u32* deg = (u32*)&b.bytes[b.off_deg];
+  u16* eidm = (u16*)&b.bytes[b.off_eidm];
+  for (size_t i = 0; i < (size_t)n * n; ++i) eidm[i] = 0xffff;
+  for (int id = 0; id < g->e; ++id) {
+    const int a = g->ea[id], c = g->eb[id];
+    p[id] = g->p[id];
+    q[id] = g->q[id];
+    ep[id] = (u32)a | ((u32)c << 16);
+    inc[(size_t)a * W + (id >> 6)] |= 1ull << (id & 63);
+    inc[(size_t)c * W + (id >> 6)] |= 1ull << (id & 63);
+    if (a != c) {
+      sadj[a] |= 1ull << c;
+      sadj[c] |= 1ull << a;
+    }
+    eidm[(size_t)a * n + c] = (u16)id;
+    eidm[(size_t)c * n + a] = (u16)id;
   }
   for (int u = 0; u < n; ++u) {
-    int c = 0;
-    for (int i = 0; i < W; ++i) c += __builtin_popcountll(t.inc[(size_t)u * W + i]);
-    t.deg[u] = (u32)c;
+    int cnt = 0;
+    for (int i = 0; i < W; ++i) cnt += __builtin_popcountll(inc[(size_t)u * W + i]);
+    deg[u] = (u32)cnt;
   }
-  return NR_OK;
 }
 
-struct DeviceGraph {
-  DevMem mem;
-  DevGraph g;
-};
-
-int upload_graph(const nr_graph* hg, int W, DeviceGraph& d) {
-  HostTables t;
-  build_tables(hg, W, t);
-  const int n = hg->n, e = std::max(hg->e, 1);
-  double *p, *q;
-  u64 *sadj, *inc;
-  u32 *ep, *deg;
-  u16* eidm;
-  int rc;
-  if ((rc = d.mem.alloc(&p, e)) || (rc = d.mem.alloc(&q, e)) ||
-      (rc = d.mem.alloc(&sadj, n)) || (rc = d.mem.alloc(&inc, (size_t)n * W)) ||
-      (rc = d.mem.alloc(&ep, e)) || (rc = d.mem.alloc(&deg, n)) ||
-      (rc = d.mem.alloc(&eidm, (size_t)n * n + 2)))
-    return rc;
-  std::vector<double> hp(e, 0.0), hq(e, 0.0);
-  for (int i = 0; i < hg->e; ++i) {
-    hp[i] = hg->p[i];
-    hq[i] = hg->q[i];
-  }
-  NR_HIP(hipMemcpy(p, hp.data(), sizeof(double) * e, hipMemcpyHostToDevice));
-  NR_HIP(hipMemcpy(q, hq.data(), sizeof(double) * e, hipMemcpyHostToDevice));
-  NR_HIP(hipMemcpy(sadj, t.sadj.data(), sizeof(u64) * n, hipMemcpyHostToDevice));
-  NR_HIP(hipMemcpy(inc, t.inc.data(), sizeof(u64) * n * W, hipMemcpyHostToDevice));
-  NR_HIP(hipMemcpy(ep, t.ep.data(), sizeof(u32) * e, hipMemcpyHostToDevice));
-  NR_HIP(hipMemcpy(deg, t.deg.data(), sizeof(u32) * n, hipMemcpyHostToDevice));
-  NR_HIP(hipMemcpy(eidm, t.eidm.data(), sizeof(u16) * n * n, hipMemcpyHostToDevice));
-  d.g = DevGraph{p, q, sadj, inc, ep, deg, eidm, hg->n, hg->e, W, hg->src, hg->dst};
+int upload_tables(Workspace& ws, const nr_graph* g, int W, DevGraph& d) {
+  TableBlob b;
+  pack_tables(g, W, b);
+  int rc = ws.grow(&ws.tab, &ws.tab_bytes, b.bytes.size());
+  if (rc) return rc;
+  NR_HIP(hipMemcpyAsync(ws.tab, b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice,
+                        ws.stream));
+  unsigned char* base = (unsigned char*)ws.tab;
+  d.p = (const double*)(base + b.off_p);
+  d.q = (const double*)(base + b.off_q);
+  d.sadj = (const u64*)(base + b.off_sadj);
+  d.inc = (const u64*)(base + b.off_inc);
+  d.ep = (const u32*)(base + b.off_ep);
+  d.deg = (const u32*)(base + b.off_deg);
+  d.eidm = (const u16*)(base + b.off_eidm);
+  d.n = g->n;
+  d.e = g->e;
+  d.w = W;
+  d.src = g->src;
+  d.dst = g->dst;
   return NR_OK;
 }
 
 /* Width classes: W 64-bit words per edge-state plane, node sets of 32 or 64
  * bits.  Each class is a separate instantiation of the kernels. */
-
 template <int W, typename NS>
 struct Kern {
   static void* level() { return (void*)&k_expand_level<W, NS>; }
@@ -443,11 +712,11 @@ Variant variant_for_w(int W) {
 
 /* Largest block (<= 256, multiple of 64) whose LDS footprint fits in 64 KiB;
  * a requested size is honoured when it fits. */
-int choose_block(const nr_graph* g, int W, int ns_bytes, int requested) {
+int choose_block(const nr_graph* g, int W, int ns_bytes, int requested, int extra = 0) {
   const int cands[4] = {requested, 256, 128, 64};
   for (int b : cands) {
     if (b <= 0 || b > 256 || b % 64) continue;
-    Layout L = make_layout(g->n, std::max(g->e, 1), W, b, ns_bytes);
+    Layout L = make_layout(g->n, std::max(g->e, 1), W, b, ns_bytes, extra);
     if (L.total_bytes <= 64 * 1024) return b;
   }
   return 0;
@@ -461,202 +730,213 @@ int pick_variant(const nr_graph* g, Variant& v) {
   return v.level ? NR_OK : NR_ERR_UNSUPPORTED;
 }
 
-void initial_planes(const nr_graph* g, int W, std::vector<u64>& known,
-                    std::vector<u64>& up) {
+Root root_item(const nr_graph* g, int W) {
   /* main.cu:717-748: p <= 0 FAILED, p >= 1 WORKING, otherwise UNKNOWN */
-  known.assign(W, 0);
-  up.assign(W, 0);
+  Root r;
+  std::memset(&r, 0, sizeof r);
   for (int e = 0; e < g->e; ++e) {
     u64 bit = 1ull << (e & 63);
     if (g->p[e] <= 0.0) {
-      known[e >> 6] |= bit;
+      r.known[e >> 6] |= bit;
     } else if (g->p[e] >= 1.0) {
-      known[e >> 6] |= bit;
-      up[e >> 6] |= bit;
+      r.known[e >> 6] |= bit;
+      r.up[e >> 6] |= bit;
+    }
+    /* rows: neighbours over edges that are not FAILED */
+    if (g->p[e] > 0.0 && g->ea[e] != g->eb[e]) {
+      r.rows[g->ea[e]] |= 1ull << g->eb[e];
+      r.rows[g->eb[e]] |= 1ull << g->ea[e];
     }
   }
+  (void)W;
+  return r;
 }
 
-double dd_reduce(const std::vector<DD>& v) {
-  double s = 0.0, c = 0.0;
-  auto add = [&](double x) {
-    double t = s + x;
-    if (std::fabs(s) >= std::fabs(x))
-      c += (s - t) + x;
-    else
-      c += (x - t) + s;
-    s = t;
-  };
-  for (const DD& d : v) {
-    add(d.hi);
-    add(d.lo);
-  }
-  return s + c;
-}
-
-int solve_level(const nr_graph* g, const nr_config* cfg, nr_result* out) {
+int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_result* out) {
   Variant var;
   int rc = pick_variant(g, var);
   if (rc) return rc;
   const int W = mask_words(g);
-  NR_HIP(hipSetDevice(cfg->device));
-  hipDeviceProp_t prop;
-  NR_HIP(hipGetDeviceProperties(&prop, cfg->device));
-
-  DeviceGraph dg;
-  if ((rc = upload_graph(g, W, dg))) return rc;
-  const int block = choose_block(g, W, var.ns_bytes, cfg->block_size);
-  if (block <= 0) return NR_ERR_UNSUPPORTED;
-  Layout lay = make_layout(g->n, std::max(g->e, 1), W, block, var.ns_bytes);
+  const int cus = ws.prop.multiProcessorCount;
+  /* The prefix table of the segment counts (one per workgroup) lives in
+   * LDS.  Size the grid from the occupancy without it, then add the table
+   * for that grid; occupancy can only drop, so the table stays large
+   * enough for the final grid. */
   const void* kfun = (const void*)var.level();
-  if (lay.total_bytes > 48 * 1024)
-    NR_HIP(hipFuncSetAttribute(kfun, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               lay.total_bytes));
+  int block = choose_block(g, W, var.ns_bytes, cfg->block_size, 4 * (1026 + 256));
+  if (block <= 0) return NR_ERR_UNSUPPORTED;
+  Layout lay0 = make_layout(g->n, std::max(g->e, 1), W, block, var.ns_bytes, 4 * block);
+  NR_HIP(hipFuncSetAttribute(kfun, hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
+  int per_cu0 = 0;
+  NR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu0, kfun, block, lay0.total_bytes));
+  if (per_cu0 < 1) return NR_ERR_UNSUPPORTED;
+  int grid = cfg->grid_blocks > 0 ? cfg->grid_blocks : per_cu0 * cus;
+  const int extra = 4 * (grid + 2 + block);
+  Layout lay = make_layout(g->n, std::max(g->e, 1), W, block, var.ns_bytes, extra);
+  if (lay.total_bytes > 64 * 1024) return NR_ERR_UNSUPPORTED;
   int per_cu = 0;
-  NR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfun, block,
-                                                      lay.total_bytes));
+  NR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfun, block, lay.total_bytes));
   if (per_cu < 1) return NR_ERR_UNSUPPORTED;
-  const int grid = cfg->grid_blocks > 0 ? cfg->grid_blocks : per_cu * prop.multiProcessorCount;
-  const u64 n_waves = (u64)grid * block / 64;
-
+  if (cfg->grid_blocks <= 0) grid = std::min(grid, per_cu * cus);
+  const u64 n_slots = (u64)grid; /* partial sums and counters per workgroup */
   const u64 cap = cfg->queue_capacity > 0 ? (u64)cfg->queue_capacity : (1ull << 20);
-  DevMem mem;
+  /* capacity is split evenly over the workgroup segments */
+  const u64 seg_cap64 = std::max<u64>(1, (cap + grid - 1) / grid);
+  if (seg_cap64 * (u64)grid >= (1ull << 32)) return NR_ERR_ARG;
+  const u32 seg_cap = (u32)seg_cap64;
+  const u64 slots = seg_cap64 * (u64)grid;
+
   LevelArgs a;
   std::memset(&a, 0, sizeof a);
-  a.g = dg.g;
+  if ((rc = upload_tables(ws, g, W, a.g))) return rc;
   a.lay = lay;
-  a.cap = cap;
+  a.seg_cap = seg_cap;
+  a.n_seg = grid;
+  a.stride = slots;
   a.eps = cfg->truncation_eps > 0 ? cfg->truncation_eps : 0.0;
+  /* SoA buffer: known[W], up[W], mass, rows[n] (each rows plane holds
+   * node sets of ns_bytes) */
+  const size_t rows_bytes = ((size_t)g->n * slots * var.ns_bytes + 255) / 256 * 256;
+  const size_t per_buf = (size_t)(2 * W + 1) * slots * 8 + rows_bytes;
+  if ((rc = ws.grow(&ws.q, &ws.q_bytes, 2 * per_buf))) return rc;
   for (int b = 0; b < 2; ++b) {
-    if ((rc = mem.alloc(&a.buf[b].known, cap * W)) || (rc = mem.alloc(&a.buf[b].up, cap * W)) ||
-        (rc = mem.alloc(&a.buf[b].mass, cap)))
-      return rc;
+    u64* base = (u64*)((unsigned char*)ws.q + b * per_buf);
+    a.buf[b].known = base;
+    a.buf[b].up = base + (size_t)W * slots;
+    a.buf[b].mass = (double*)(base + (size_t)2 * W * slots);
+    a.buf[b].rows = (void*)(base + (size_t)(2 * W + 1) * slots);
   }
-  if ((rc = mem.alloc(&a.ctl, 1)) || (rc = mem.alloc(&a.partR, n_waves)) ||
-      (rc = mem.alloc(&a.partE, n_waves)) || (rc = mem.alloc(&a.partD, n_waves)))
-    return rc;
-  NR_HIP(hipMemset(a.ctl, 0, sizeof(LevelCtl)));
-  NR_HIP(hipMemset(a.partR, 0, sizeof(DD) * n_waves));
-  NR_HIP(hipMemset(a.partE, 0, sizeof(DD) * n_waves));
-  NR_HIP(hipMemset(a.partD, 0, sizeof(DD) * n_waves));
-  /* seed the root item (main.cu:853-855) */
+  /* per-workgroup slots: 3 DD + C_N counters; segment counts */
+  const size_t meta_bytes = n_slots * (3 * sizeof(DD) + C_N * sizeof(u64)) +
+                            2 * sizeof(u32) * (2 * (size_t)grid + 4);
+  if ((rc = ws.grow(&ws.meta, &ws.meta_bytes, meta_bytes))) return rc;
   {
-    std::vector<u64> k0, u0;
-    initial_planes(g, W, k0, u0);
-    for (int w = 0; w < W; ++w) {
-      NR_HIP(hipMemcpy(a.buf[0].known + (size_t)w * cap, &k0[w], 8, hipMemcpyHostToDevice));
-      NR_HIP(hipMemcpy(a.buf[0].up + (size_t)w * cap, &u0[w], 8, hipMemcpyHostToDevice));
-    }
-    double one = 1.0;
-    NR_HIP(hipMemcpy(a.buf[0].mass, &one, 8, hipMemcpyHostToDevice));
-    u64 c1 = 1;
-    NR_HIP(hipMemcpy(&a.ctl->count[0], &c1, 8, hipMemcpyHostToDevice));
+    unsigned char* m = (unsigned char*)ws.meta;
+    a.partR = (DD*)m;
+    a.partE = a.partR + n_slots;
+    a.partD = a.partE + n_slots;
+    a.cnt = (u64*)(a.partD + n_slots);
+    u32* q = (u32*)(a.cnt + n_slots * C_N);
+    a.seg_count[0] = q;
+    a.seg_count[1] = q + grid;
   }
-  NR_HIP(hipDeviceSynchronize());
+  a.ctl = ws.ctl;
 
-  hipStream_t stream;
-  NR_HIP(hipStreamCreate(&stream));
-  struct StreamGuard {
-    hipStream_t s;
-    std::vector<hipEvent_t> ev;
-    ~StreamGuard() {
-      for (auto e : ev) (void)hipEventDestroy(e);
-      (void)hipStreamDestroy(s);
-    }
-  } sg{stream, {}};
-  auto new_event = [&](hipEvent_t* e) -> int {
-    NR_HIP(hipEventCreate(e));
-    sg.ev.push_back(*e);
-    return NR_OK;
-  };
+  hipStream_t st = ws.stream;
+  {
+    Root root = root_item(g, W);
+    int Wi = W, ni = g->n, nsb = var.ns_bytes;
+    u64 nw = n_slots;
+    void* args[] = {&a, &root, &Wi, &ni, &nsb, &nw};
+    NR_HIP(hipLaunchKernel((const void*)&k_init, dim3(64), dim3(256), args, 0, st));
+  }
   hipEvent_t ev_start, ev_stop;
-  if ((rc = new_event(&ev_start)) || (rc = new_event(&ev_stop))) return rc;
-  const size_t max_timed = 4096;
-  std::vector<hipEvent_t> k_beg, k_end;
+  if ((rc = ws.event(0, &ev_start)) || (rc = ws.event(1, &ev_stop))) return rc;
+  const size_t max_timed = (cfg->flags & NR_FLAG_TIME_LAUNCHES) ? 2048 : 0;
 
   auto t_begin = std::chrono::steady_clock::now();
   const bool has_budget = cfg->time_budget_s > 0;
-  NR_HIP(hipEventRecord(ev_start, stream));
-  int level = 0;
-  int chunk = 4;
+  NR_HIP(hipEventRecord(ev_start, st));
+  int level = 0, chunk = 8;
   bool finished = false, deadline = false;
-  LevelCtl hctl;
-  long long launches = 0;
+  long long timed = 0;
   while (!finished) {
     for (int c = 0; c < chunk; ++c, ++level) {
       const int parity = level & 1;
-      if (k_beg.size() < max_timed) {
-        hipEvent_t b, e;
-        if ((rc = new_event(&b)) || (rc = new_event(&e))) return rc;
-        k_beg.push_back(b);
-        k_end.push_back(e);
-        NR_HIP(hipEventRecord(b, stream));
+      hipEvent_t kb = nullptr, ke = nullptr;
+      if ((size_t)timed < max_timed) {
+        if ((rc = ws.event(2 + 2 * timed, &kb)) || (rc = ws.event(3 + 2 * timed, &ke)))
+          return rc;
+        NR_HIP(hipEventRecord(kb, st));
       }
       void* args[] = {&a, (void*)&parity};
-      NR_HIP(hipLaunchKernel(kfun, dim3(grid), dim3(block), args, lay.total_bytes, stream));
-      ++launches;
-      if (k_end.size() == (size_t)launches) NR_HIP(hipEventRecord(k_end.back(), stream));
+      NR_HIP(hipLaunchKernel(kfun, dim3(grid), dim3(block), args, lay.total_bytes, st));
+      if (kb) {
+        NR_HIP(hipEventRecord(ke, st));
+        ++timed;
+      }
     }
-    NR_HIP(hipMemcpyAsync(&hctl, a.ctl, sizeof hctl, hipMemcpyDeviceToHost, stream));
-    NR_HIP(hipStreamSynchronize(stream));
-    if (hctl.count[0] == 0 && hctl.count[1] == 0) finished = true;
+    NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
+    NR_HIP(hipStreamSynchronize(st));
+    if (ws.hctl->last_items == 0) finished = true;
     if (!finished && has_budget) {
       double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count();
       if (el >= cfg->time_budget_s) {
         const int parity = level & 1;
         void* args[] = {&a, (void*)&parity};
-        NR_HIP(hipLaunchKernel((const void*)&k_fold, dim3(grid), dim3(block), args, 0, stream));
+        NR_HIP(hipLaunchKernel((const void*)&k_fold, dim3(grid), dim3(block), args,
+                               4 * (grid + 2 + block), st));
         deadline = true;
         finished = true;
       }
     }
     chunk = std::min(chunk * 2, 16);
   }
-  NR_HIP(hipEventRecord(ev_stop, stream));
-  NR_HIP(hipEventSynchronize(ev_stop));
+  {
+    u64 nw = n_slots;
+    void* args[] = {&a, &nw};
+    NR_HIP(hipLaunchKernel((const void*)&k_finalize, dim3(1), dim3(256), args, 0, st));
+  }
+  NR_HIP(hipEventRecord(ev_stop, st));
+  NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
+  NR_HIP(hipStreamSynchronize(st));
   float ms = 0.f;
   NR_HIP(hipEventElapsedTime(&ms, ev_start, ev_stop));
+  /* summed time of every expand launch, including the few empty launches
+   * enqueued past the last level of the final chunk */
+  const LevelCtl& h = *ws.hctl;
   double kms = 0.0;
-  const long long useful = std::min<long long>((long long)k_end.size(), launches);
-  for (long long i = 0; i < useful; ++i) {
+  for (long long i = 0; i < timed; ++i) {
     float x = 0.f;
-    NR_HIP(hipEventElapsedTime(&x, k_beg[i], k_end[i]));
+    NR_HIP(hipEventElapsedTime(&x, ws.events[2 + 2 * i], ws.events[3 + 2 * i]));
     kms += x;
   }
-  NR_HIP(hipMemcpy(&hctl, a.ctl, sizeof hctl, hipMemcpyDeviceToHost));
-  std::vector<DD> pr(n_waves), pe(n_waves), pd(n_waves);
-  NR_HIP(hipMemcpy(pr.data(), a.partR, sizeof(DD) * n_waves, hipMemcpyDeviceToHost));
-  NR_HIP(hipMemcpy(pe.data(), a.partE, sizeof(DD) * n_waves, hipMemcpyDeviceToHost));
-  NR_HIP(hipMemcpy(pd.data(), a.partD, sizeof(DD) * n_waves, hipMemcpyDeviceToHost));
-  if (hctl.errors) return NR_ERR_DEVICE;
+  if (h.counts[C_ERR]) return NR_ERR_DEVICE;
 
   std::memset(out, 0, sizeof(*out));
-  out->R = dd_reduce(pr);
-  out->ESP = dd_reduce(pe);
-  out->D = dd_reduce(pd);
-  out->paths = hctl.paths;
-  out->nodes = hctl.nodes;
-  out->dead = hctl.dead;
-  out->truncated = hctl.truncated;
-  out->overflowed = hctl.overflowed;
-  out->folded = hctl.folded;
-  out->iterations = (long long)hctl.iterations;
-  out->max_queue_items = (long long)hctl.max_width;
+  out->R = h.sum[0];
+  out->ESP = h.sum[1];
+  out->D = h.sum[2];
+  out->paths = h.counts[C_PATHS];
+  out->nodes = h.counts[C_NODES];
+  out->dead = h.counts[C_DEAD];
+  out->truncated = h.counts[C_TRUNC];
+  out->overflowed = h.counts[C_OVER];
+  out->folded = h.counts[C_FOLD];
+  out->iterations = (long long)h.iterations;
+  out->max_queue_items = (long long)h.max_width;
   out->item_bytes = 16LL * W + 8;
   out->elapsed_ms = ms;
   out->kernel_ms = kms;
-  out->kernel_launches = launches;
+  out->kernel_launches = timed;
   out->deadline_hit = deadline ? 1 : 0;
-  out->overflowed_flag = hctl.overflowed ? 1 : 0;
+  out->overflowed_flag = h.counts[C_OVER] ? 1 : 0;
   out->grid_blocks = grid;
   out->block_size = block;
   out->lds_bytes = lay.total_bytes;
   out->blocks_per_cu = per_cu;
-  out->compute_units = prop.multiProcessorCount;
+  out->compute_units = cus;
   return NR_OK;
 }
 
-int expand_items(const nr_graph* g, int device, int count, const u64* known,
+/* RAII bag of device allocations (test entry point only). */
+struct DevMem {
+  std::vector<void*> ptrs;
+  ~DevMem() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <typename T>
+  int alloc(T** out, size_t count) {
+    void* p = nullptr;
+    if (count == 0) count = 1;
+    if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) return NR_ERR_NOMEM;
+    ptrs.push_back(p);
+    *out = (T*)p;
+    return NR_OK;
+  }
+};
+
+int expand_items(Workspace& ws, const nr_graph* g, int count, const u64* known,
                  const u64* up, const double* mass, int max_children, int* nchild,
                  double* succ, u64* ck, u64* cu, double* cm) {
   Variant var;
@@ -664,9 +944,6 @@ int expand_items(const nr_graph* g, int device, int count, const u64* known,
   if (rc) return rc;
   if (count <= 0) return NR_OK;
   const int W = mask_words(g);
-  NR_HIP(hipSetDevice(device));
-  DeviceGraph dg;
-  if ((rc = upload_graph(g, W, dg))) return rc;
   const int block = choose_block(g, W, var.ns_bytes, 0);
   if (block <= 0) return NR_ERR_UNSUPPORTED;
   Layout lay = make_layout(g->n, std::max(g->e, 1), W, block, var.ns_bytes);
@@ -677,7 +954,7 @@ int expand_items(const nr_graph* g, int device, int count, const u64* known,
   DevMem mem;
   ItemsArgs a;
   std::memset(&a, 0, sizeof a);
-  a.g = dg.g;
+  if ((rc = upload_tables(ws, g, W, a.g))) return rc;
   a.lay = lay;
   a.count = count;
   a.max_children = std::max(max_children, 1);
@@ -689,26 +966,44 @@ int expand_items(const nr_graph* g, int device, int count, const u64* known,
       (rc = mem.alloc(&a.succ, count)) || (rc = mem.alloc(&a.ck, nc * W)) ||
       (rc = mem.alloc(&a.cu, nc * W)) || (rc = mem.alloc(&a.cm, nc)))
     return rc;
-  NR_HIP(hipMemcpy(dk, known, sizeof(u64) * count * W, hipMemcpyHostToDevice));
-  NR_HIP(hipMemcpy(du, up, sizeof(u64) * count * W, hipMemcpyHostToDevice));
-  NR_HIP(hipMemcpy(dm, mass, sizeof(double) * count, hipMemcpyHostToDevice));
-  NR_HIP(hipMemset(a.ck, 0, sizeof(u64) * nc * W));
-  NR_HIP(hipMemset(a.cu, 0, sizeof(u64) * nc * W));
-  NR_HIP(hipMemset(a.cm, 0, sizeof(double) * nc));
+  hipStream_t st = ws.stream;
+  NR_HIP(hipMemcpyAsync(dk, known, sizeof(u64) * count * W, hipMemcpyHostToDevice, st));
+  NR_HIP(hipMemcpyAsync(du, up, sizeof(u64) * count * W, hipMemcpyHostToDevice, st));
+  NR_HIP(hipMemcpyAsync(dm, mass, sizeof(double) * count, hipMemcpyHostToDevice, st));
+  NR_HIP(hipMemsetAsync(a.ck, 0, sizeof(u64) * nc * W, st));
+  NR_HIP(hipMemsetAsync(a.cu, 0, sizeof(u64) * nc * W, st));
+  NR_HIP(hipMemsetAsync(a.cm, 0, sizeof(double) * nc, st));
   a.known = dk;
   a.up = du;
   a.mass = dm;
   const int grid = (count + block - 1) / block;
   void* args[] = {&a};
-  NR_HIP(hipLaunchKernel(kfun, dim3(grid), dim3(block), args, lay.total_bytes, 0));
-  NR_HIP(hipDeviceSynchronize());
-  NR_HIP(hipMemcpy(nchild, a.nchild, sizeof(int) * count, hipMemcpyDeviceToHost));
-  NR_HIP(hipMemcpy(succ, a.succ, sizeof(double) * count, hipMemcpyDeviceToHost));
-  if (ck) NR_HIP(hipMemcpy(ck, a.ck, sizeof(u64) * nc * W, hipMemcpyDeviceToHost));
-  if (cu) NR_HIP(hipMemcpy(cu, a.cu, sizeof(u64) * nc * W, hipMemcpyDeviceToHost));
-  if (cm) NR_HIP(hipMemcpy(cm, a.cm, sizeof(double) * nc, hipMemcpyDeviceToHost));
+  NR_HIP(hipLaunchKernel(kfun, dim3(grid), dim3(block), args, lay.total_bytes, st));
+  NR_HIP(hipMemcpyAsync(nchild, a.nchild, sizeof(int) * count, hipMemcpyDeviceToHost, st));
+  NR_HIP(hipMemcpyAsync(succ, a.succ, sizeof(double) * count, hipMemcpyDeviceToHost, st));
+  if (ck) NR_HIP(hipMemcpyAsync(ck, a.ck, sizeof(u64) * nc * W, hipMemcpyDeviceToHost, st));
+  if (cu) NR_HIP(hipMemcpyAsync(cu, a.cu, sizeof(u64) * nc * W, hipMemcpyDeviceToHost, st));
+  if (cm) NR_HIP(hipMemcpyAsync(cm, a.cm, sizeof(double) * nc, hipMemcpyDeviceToHost, st));
+  NR_HIP(hipStreamSynchronize(st));
   return NR_OK;
 }
+
+/* Locks and opens the workspace of a device. */
+struct WsLock {
+  Workspace* ws = nullptr;
+  std::unique_lock<std::mutex> lk;
+  int rc = NR_OK;
+  explicit WsLock(int device) {
+    if (device < 0 || device >= kMaxDevices) {
+      rc = NR_ERR_ARG;
+      return;
+    }
+    ws = &g_ws[device];
+    lk = std::unique_lock<std::mutex>(ws->mu);
+    rc = ws->open(device);
+    if (rc == NR_OK && hipSetDevice(device) != hipSuccess) rc = NR_ERR_DEVICE;
+  }
+};
 
 }  // namespace nr
 
@@ -723,10 +1018,13 @@ int nr_solve(const nr_graph* g, const nr_config* cfg, nr_result* out) {
   }
   if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0 || cfg->time_budget_s < 0)
     return NR_ERR_ARG;
-  switch (cfg->order) {
-    case NR_ORDER_LEVEL: return nr::solve_level(g, cfg, out);
-    default: return NR_ERR_UNSUPPORTED;
-  }
+  if (cfg->order != NR_ORDER_LEVEL) return NR_ERR_UNSUPPORTED;
+  nr::Variant var;
+  int rc = nr::pick_variant(g, var);
+  if (rc) return rc;
+  nr::WsLock L(cfg->device);
+  if (L.rc) return L.rc;
+  return nr::solve_level(*L.ws, g, cfg, out);
 }
 
 int nr_solve_batch(const nr_graph* const* graphs, int count, const nr_config* cfg,
@@ -746,8 +1044,21 @@ int nr_expand_items(const nr_graph* g, int device, int count,
                     double* child_mass) {
   if (!g || count < 0 || (count > 0 && (!known || !up || !mass || !nchild || !succ)))
     return NR_ERR_ARG;
-  return nr::expand_items(g, device, count, known, up, mass, max_children, nchild, succ,
+  nr::Variant var;
+  int rc = nr::pick_variant(g, var);
+  if (rc) return rc;
+  nr::WsLock L(device);
+  if (L.rc) return L.rc;
+  return nr::expand_items(*L.ws, g, count, known, up, mass, max_children, nchild, succ,
                           child_known, child_up, child_mass);
+}
+
+int nr_device_release(int device) {
+  if (device < 0 || device >= nr::kMaxDevices) return NR_ERR_ARG;
+  nr::Workspace& ws = nr::g_ws[device];
+  std::lock_guard<std::mutex> lk(ws.mu);
+  ws.release();
+  return NR_OK;
 }
 
 int nr_device_query(int device, char* name, int name_len, int* compute_units,
