@@ -28,6 +28,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -37,11 +38,25 @@
 
 namespace nr {
 
+/* Mass classes: two per binade, class 0 = heaviest (mass >= 0.75).  The
+ * class-round order expands the heaviest classes first. */
+constexpr int kClasses = 128;
+
+__host__ __device__ inline int mass_class(double m) {
+  unsigned long long bits;
+  memcpy(&bits, &m, sizeof bits);
+  const int e = (int)((bits >> 52) & 0x7ff) - 1023; /* m in [2^e, 2^(e+1)) */
+  const int heavy = (int)((bits >> 51) & 1);         /* m >= 1.5 * 2^e */
+  int c = 2 * (-e - 1) + (heavy ? 0 : 1);
+  return c < 0 ? 0 : (c >= kClasses ? kClasses - 1 : c);
+}
+
 /* Per-wave counter slots. */
 enum { C_NODES = 0, C_PATHS, C_DEAD, C_TRUNC, C_OVER, C_FOLD, C_ERR, C_N = 8 };
 
 /* Control block of a level-order solve (device memory). */
 struct LevelCtl {
+  u64 hist[kClasses]; /* class histogram of the pool (class rounds) */
   u64 last_items; /* items of the most recently launched level */
   u64 iterations; /* non-empty levels processed */
   u64 max_width;  /* widest level */
@@ -61,6 +76,7 @@ struct LevelArgs {
   Layout lay;
   QBuf buf[2];
   u32* seg_count[2]; /* [n_seg] items written by each workgroup */
+  u32* hist;         /* [n_seg][kClasses] class histogram of each segment */
   LevelCtl* ctl;
   DD* partR;
   DD* partE;
@@ -70,6 +86,12 @@ struct LevelArgs {
   u32 seg_cap;
   int n_seg;
   double eps;
+  /* class rounds (max-mass order): items of class <= thr_class are
+   * expanded, lighter ones are carried to the output unchanged; queued
+   * items of class >= drop_class are given up to ESP */
+  int mass_mode;
+  int thr_class;
+  int drop_class;
 };
 
 template <int W>
@@ -215,9 +237,15 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity) {
   }
   if (n_items == 0) {
     if (threadIdx.x == 0) a.seg_count[parity ^ 1][blockIdx.x] = 0;
+    if (a.mass_mode)
+      for (int i = threadIdx.x; i < kClasses; i += blockDim.x)
+        a.hist[(u64)blockIdx.x * kClasses + i] = 0;
     return;
   }
   if (threadIdx.x == 0) *s_cnt = 0;
+  u32* s_hist = part + blockDim.x; /* [kClasses], class rounds only */
+  if (a.mass_mode)
+    for (int i = threadIdx.x; i < kClasses; i += blockDim.x) s_hist[i] = 0;
   stage_tables(a.g, a.lay, smem);
   __syncthreads();
 
@@ -245,65 +273,111 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity) {
     const u32 i = (u32)(base + lane);
     u64 known[W], up[W];
     double mass = 0.0;
-    int k = 0;
-    bool live = false;
+    int k = 0, stored = 0;
+    bool live = false, carry = false;
+    u64 slot_in = 0;
     if ((u64)lane < chunk && i < n_items) {
       int s = segment_of(sp, a.n_seg, (u32)base);
       while (sp[s + 1] <= i) ++s;
-      const u64 slot_in = (u64)s * a.seg_cap + (i - sp[s]);
+      slot_in = (u64)s * a.seg_cap + (i - sp[s]);
       load_item<W>(in, a.stride, slot_in, known, up, mass);
-      ++c[C_NODES];
-      if (mass < a.eps) { /* main.cu:537-543 */
+      const int cl = a.mass_mode ? mass_class(mass) : 0;
+      if (a.mass_mode && cl >= a.drop_class) { /* given up: queue over capacity */
         accE.add(mass);
-        ++c[C_TRUNC];
+        ++c[C_FOLD];
+      } else if (a.mass_mode && cl > a.thr_class) { /* lighter: carried */
+        carry = true;
+        stored = 1;
       } else {
-        const NS* rin = (const NS*)in.rows + slot_in;
-        for (int u = 0; u < ex.n; ++u) ex.rows[u * ex.stride] = rin[(u64)u * a.stride];
-        k = ex.find(known);
-        if (k >= 0) {
-          live = true;
+        ++c[C_NODES];
+        if (mass < a.eps) { /* main.cu:537-543 */
+          accE.add(mass);
+          ++c[C_TRUNC];
         } else {
-          if (k == -2) ++c[C_ERR];
-          accD.add(mass); /* main.cu:579-582 */
-          ++c[C_DEAD];
-          k = 0;
+          const NS* rin = (const NS*)in.rows + slot_in;
+          for (int u = 0; u < ex.n; ++u) ex.rows[u * ex.stride] = rin[(u64)u * a.stride];
+          k = ex.find(known);
+          if (k >= 0) {
+            live = true;
+            stored = k;
+            if (a.mass_mode) { /* children below eps or over capacity are not queued */
+              stored = 0;
+              double run = mass;
+              for (int j = k - 1; j >= 0; --j) {
+                const int e = ex.uh[j * ex.stride];
+                const double cm = run * ex.q[e];
+                if (cm >= a.eps) ++stored;
+                run = run * ex.p[e];
+              }
+            }
+          } else {
+            if (k == -2) ++c[C_ERR];
+            accD.add(mass); /* main.cu:579-582 */
+            ++c[C_DEAD];
+            k = 0;
+          }
         }
       }
     }
-    const int incl = wave_incl_scan(k);
+    const int incl = wave_incl_scan(stored);
     const int total = __shfl(incl, 63, 64);
     u32 wbase = 0;
     if (total) {
       if (lane == 63) wbase = atomicAdd(s_cnt, (u32)total);
       wbase = __shfl(wbase, 63, 64);
     }
+    u32 off = wbase + (u32)(incl - stored);
+    if (carry) {
+      if (off < a.seg_cap) {
+        store_item<W>(out, a.stride, out_base + off, known, up, mass);
+        const NS* rin = (const NS*)in.rows + slot_in;
+        NS* rout = (NS*)out.rows + out_base + off;
+        for (int u = 0; u < ex.n; ++u) rout[(u64)u * a.stride] = rin[(u64)u * a.stride];
+        atomicAdd(&s_hist[mass_class(mass)], 1u);
+      } else {
+        accE.add(mass);
+        ++c[C_FOLD];
+      }
+    }
     if (live) {
       /* main.cu:214-266: child j has the earlier UNKNOWN hops WORKING and
        * hop j FAILED; masses multiply left to right along the path */
-      u32 off = wbase + (u32)(incl - k);
       double run = mass;
       u64 pre[W];
 #pragma unroll
       for (int w = 0; w < W; ++w) pre[w] = 0;
-      for (int j = k - 1; j >= 0; --j, ++off) {
+      for (int j = k - 1; j >= 0; --j) {
         const int e = ex.uh[j * ex.stride];
         const int wi = e >> 6;
         const u64 bit = 1ull << (e & 63);
         const double cm = run * ex.q[e];
-        if (off < a.seg_cap) {
-          u64 ck[W], cu[W];
-#pragma unroll
-          for (int w = 0; w < W; ++w) {
-            ck[w] = known[w] | pre[w] | (w == wi ? bit : 0ull);
-            cu[w] = up[w] | pre[w];
+        bool queue = true;
+        if (a.mass_mode) {
+          if (cm < a.eps) { /* settled at creation: counted as a truncated node */
+            accE.add(cm);
+            ++c[C_NODES];
+            ++c[C_TRUNC];
+            queue = false;
           }
-          store_item<W>(out, a.stride, out_base + off, ck, cu, cm);
-          NS* rout = (NS*)out.rows + out_base + off;
-          for (int u = 0; u < ex.n; ++u) rout[(u64)u * a.stride] = ex.child_row(u, e);
-        } else { /* segment full: main.cu:245-252 */
-          accE.add(cm);
-          ++c[C_PATHS];
-          ++c[C_OVER];
+        }
+        if (queue) {
+          if (off < a.seg_cap) {
+            u64 ck[W], cu[W];
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+              ck[w] = known[w] | pre[w] | (w == wi ? bit : 0ull);
+              cu[w] = up[w] | pre[w];
+            }
+            store_item<W>(out, a.stride, out_base + off, ck, cu, cm);
+            NS* rout = (NS*)out.rows + out_base + off;
+            for (int u = 0; u < ex.n; ++u) rout[(u64)u * a.stride] = ex.child_row(u, e);
+            if (a.mass_mode) atomicAdd(&s_hist[mass_class(cm)], 1u);
+          } else { /* segment full: main.cu:245-252 */
+            accE.add(cm);
+            ++c[C_PATHS];
+            ++c[C_OVER];
+          }
+          ++off;
         }
         run = run * ex.p[e];
 #pragma unroll
@@ -317,11 +391,23 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity) {
 
   /* per-workgroup flush; then publish this workgroup's segment count */
   __syncthreads();
+  if (a.mass_mode)
+    for (int i = threadIdx.x; i < kClasses; i += blockDim.x)
+      a.hist[(u64)blockIdx.x * kClasses + i] = s_hist[i];
   block_flush(a, accR, accE, accD, c, (unsigned char*)part);
   if (threadIdx.x == 0) {
     const u32 n = *s_cnt;
     a.seg_count[parity ^ 1][blockIdx.x] = n < a.seg_cap ? n : a.seg_cap;
   }
+}
+
+/* Sum the per-segment class histograms into the control block (one
+ * thread per class). */
+__global__ void __launch_bounds__(kClasses) k_hist_total(LevelArgs a) {
+  const int cl = threadIdx.x;
+  u64 sum = 0;
+  for (int b = 0; b < a.n_seg; ++b) sum += a.hist[(u64)b * kClasses + cl];
+  a.ctl->hist[cl] = sum;
 }
 
 /* Fold every queued item of buf[parity] into ESP (time budget expiry;
@@ -763,15 +849,16 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
    * for that grid; occupancy can only drop, so the table stays large
    * enough for the final grid. */
   const void* kfun = (const void*)var.level();
-  int block = choose_block(g, W, var.ns_bytes, cfg->block_size, 4 * (1026 + 256));
+  int block = choose_block(g, W, var.ns_bytes, cfg->block_size, 4 * (1026 + 256 + kClasses));
   if (block <= 0) return NR_ERR_UNSUPPORTED;
-  Layout lay0 = make_layout(g->n, std::max(g->e, 1), W, block, var.ns_bytes, 4 * block);
+  Layout lay0 = make_layout(g->n, std::max(g->e, 1), W, block, var.ns_bytes,
+                            4 * (block + kClasses));
   NR_HIP(hipFuncSetAttribute(kfun, hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
   int per_cu0 = 0;
   NR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu0, kfun, block, lay0.total_bytes));
   if (per_cu0 < 1) return NR_ERR_UNSUPPORTED;
   int grid = cfg->grid_blocks > 0 ? cfg->grid_blocks : per_cu0 * cus;
-  const int extra = 4 * (grid + 2 + block);
+  const int extra = 4 * (grid + 2 + block + kClasses);
   Layout lay = make_layout(g->n, std::max(g->e, 1), W, block, var.ns_bytes, extra);
   if (lay.total_bytes > 64 * 1024) return NR_ERR_UNSUPPORTED;
   int per_cu = 0;
@@ -780,8 +867,12 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   if (cfg->grid_blocks <= 0) grid = std::min(grid, per_cu * cus);
   const u64 n_slots = (u64)grid; /* partial sums and counters per workgroup */
   const u64 cap = cfg->queue_capacity > 0 ? (u64)cfg->queue_capacity : (1ull << 20);
+  const bool mass = cfg->order == NR_ORDER_MASS;
+  /* class rounds trim the pool to `cap` between rounds; the buffers get
+   * twice that so a round's children never overflow before the trim */
+  const u64 phys = mass ? 2 * cap : cap;
   /* capacity is split evenly over the workgroup segments */
-  const u64 seg_cap64 = std::max<u64>(1, (cap + grid - 1) / grid);
+  const u64 seg_cap64 = std::max<u64>(1, (phys + grid - 1) / grid);
   if (seg_cap64 * (u64)grid >= (1ull << 32)) return NR_ERR_ARG;
   const u32 seg_cap = (u32)seg_cap64;
   const u64 slots = seg_cap64 * (u64)grid;
@@ -808,7 +899,8 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   }
   /* per-workgroup slots: 3 DD + C_N counters; segment counts */
   const size_t meta_bytes = n_slots * (3 * sizeof(DD) + C_N * sizeof(u64)) +
-                            2 * sizeof(u32) * (2 * (size_t)grid + 4);
+                            2 * sizeof(u32) * (2 * (size_t)grid + 4) +
+                            sizeof(u32) * (size_t)grid * kClasses;
   if ((rc = ws.grow(&ws.meta, &ws.meta_bytes, meta_bytes))) return rc;
   {
     unsigned char* m = (unsigned char*)ws.meta;
@@ -819,8 +911,12 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
     u32* q = (u32*)(a.cnt + n_slots * C_N);
     a.seg_count[0] = q;
     a.seg_count[1] = q + grid;
+    a.hist = q + 2 * grid + 8;
   }
   a.ctl = ws.ctl;
+  a.mass_mode = mass ? 1 : 0;
+  a.thr_class = kClasses;
+  a.drop_class = kClasses;
 
   hipStream_t st = ws.stream;
   {
@@ -839,38 +935,114 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   NR_HIP(hipEventRecord(ev_start, st));
   int level = 0, chunk = 8;
   bool finished = false, deadline = false;
-  long long timed = 0;
-  while (!finished) {
-    for (int c = 0; c < chunk; ++c, ++level) {
-      const int parity = level & 1;
-      hipEvent_t kb = nullptr, ke = nullptr;
-      if ((size_t)timed < max_timed) {
-        if ((rc = ws.event(2 + 2 * timed, &kb)) || (rc = ws.event(3 + 2 * timed, &ke)))
-          return rc;
-        NR_HIP(hipEventRecord(kb, st));
-      }
-      void* args[] = {&a, (void*)&parity};
-      NR_HIP(hipLaunchKernel(kfun, dim3(grid), dim3(block), args, lay.total_bytes, st));
-      if (kb) {
-        NR_HIP(hipEventRecord(ke, st));
-        ++timed;
-      }
+  long long timed = 0, rounds = 0;
+  u64 max_pool = 1;
+  auto launch_level = [&](int parity) -> int {
+    hipEvent_t kb = nullptr, ke = nullptr;
+    if ((size_t)timed < max_timed) {
+      int r = ws.event(2 + 2 * timed, &kb);
+      if (!r) r = ws.event(3 + 2 * timed, &ke);
+      if (r) return r;
+      NR_HIP(hipEventRecord(kb, st));
     }
-    NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
-    NR_HIP(hipStreamSynchronize(st));
-    if (ws.hctl->last_items == 0) finished = true;
-    if (!finished && has_budget) {
-      double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count();
-      if (el >= cfg->time_budget_s) {
-        const int parity = level & 1;
-        void* args[] = {&a, (void*)&parity};
-        NR_HIP(hipLaunchKernel((const void*)&k_fold, dim3(grid), dim3(block), args,
-                               4 * (grid + 2 + block), st));
+    void* args[] = {&a, (void*)&parity};
+    NR_HIP(hipLaunchKernel(kfun, dim3(grid), dim3(block), args, lay.total_bytes, st));
+    if (kb) {
+      NR_HIP(hipEventRecord(ke, st));
+      ++timed;
+    }
+    return NR_OK;
+  };
+  auto expired = [&]() {
+    return has_budget &&
+           std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count() >=
+               cfg->time_budget_s;
+  };
+  auto fold = [&](int parity) -> int {
+    void* args[] = {&a, (void*)&parity};
+    NR_HIP(hipLaunchKernel((const void*)&k_fold, dim3(grid), dim3(block), args,
+                           4 * (grid + 2 + block), st));
+    return NR_OK;
+  };
+  if (!mass) {
+    /* level order: chunks of back-to-back level launches */
+    while (!finished) {
+      for (int c = 0; c < chunk; ++c, ++level)
+        if ((rc = launch_level(level & 1))) return rc;
+      NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
+      NR_HIP(hipStreamSynchronize(st));
+      if (ws.hctl->last_items == 0) finished = true;
+      if (!finished && expired()) {
+        if ((rc = fold(level & 1))) return rc;
         deadline = true;
         finished = true;
       }
+      chunk = std::min(chunk * 2, 16);
     }
-    chunk = std::min(chunk * 2, 16);
+  } else {
+    /* class rounds: each launch expands every queued item of the heaviest
+     * classes (at least `target` items, or all) and carries the rest; when
+     * the output would exceed the capacity the lightest classes are given
+     * up first. */
+    const bool trace = std::getenv("NR_TRACE_ROUNDS") != nullptr;
+    std::vector<u64> hist(kClasses, 0);
+    hist[mass_class(1.0)] = 1;
+    u64 pool = 1;
+    double ratio = 1.0; /* queued children per expanded item, running estimate */
+    const u64 target = std::max<u64>(1ull << 20, (u64)grid * block);
+    while (pool > 0) {
+      /* trim: when the pool exceeds the capacity, give up its lightest
+       * classes (at least the heaviest class is kept) */
+      int drop = kClasses;
+      u64 keep = 0;
+      for (int cl = 0; cl < kClasses; ++cl) {
+        if (keep + hist[cl] > cap && keep > 0) {
+          drop = cl;
+          break;
+        }
+        keep += hist[cl];
+      }
+      /* expand the heaviest kept classes: at least `target` items (or all),
+       * while the expected output fits the physical headroom */
+      u64 cum = 0;
+      int thr = -1;
+      for (int cl = 0; cl < drop; ++cl) {
+        if (!hist[cl]) continue;
+        const double out_if = ratio * (double)(cum + hist[cl]) + (double)(keep - cum - hist[cl]);
+        if (cum > 0 && (cum >= target || out_if > (double)phys)) break;
+        cum += hist[cl];
+        thr = cl;
+      }
+      const u64 carried = keep - cum;
+      a.thr_class = thr;
+      a.drop_class = drop;
+      const int parity = (int)(rounds & 1);
+      if ((rc = launch_level(parity))) return rc;
+      void* hargs[] = {&a};
+      NR_HIP(hipLaunchKernel((const void*)&k_hist_total, dim3(1), dim3(kClasses), hargs, 0, st));
+      NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
+      NR_HIP(hipStreamSynchronize(st));
+      ++rounds;
+      pool = 0;
+      for (int cl = 0; cl < kClasses; ++cl) {
+        hist[cl] = ws.hctl->hist[cl];
+        pool += hist[cl];
+      }
+      max_pool = std::max(max_pool, pool);
+      if (trace)
+        std::fprintf(stderr, "round %lld thr %d drop %d expand %llu carried %llu -> pool %llu\n",
+                     rounds, thr, drop, (unsigned long long)cum, (unsigned long long)carried,
+                     (unsigned long long)pool);
+      if (cum > 0) {
+        const double kids = pool > carried ? (double)(pool - carried) : 0.0;
+        ratio = std::max(1.0, 0.5 * ratio + 0.5 * kids / (double)cum);
+      }
+      if (pool > 0 && expired()) {
+        if ((rc = fold((int)(rounds & 1)))) return rc;
+        deadline = true;
+        break;
+      }
+    }
   }
   {
     u64 nw = n_slots;
@@ -903,8 +1075,8 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   out->truncated = h.counts[C_TRUNC];
   out->overflowed = h.counts[C_OVER];
   out->folded = h.counts[C_FOLD];
-  out->iterations = (long long)h.iterations;
-  out->max_queue_items = (long long)h.max_width;
+  out->iterations = mass ? rounds : (long long)h.iterations;
+  out->max_queue_items = mass ? (long long)max_pool : (long long)h.max_width;
   out->item_bytes = 16LL * W + 8;
   out->elapsed_ms = ms;
   out->kernel_ms = kms;
@@ -1018,7 +1190,7 @@ int nr_solve(const nr_graph* g, const nr_config* cfg, nr_result* out) {
   }
   if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0 || cfg->time_budget_s < 0)
     return NR_ERR_ARG;
-  if (cfg->order != NR_ORDER_LEVEL) return NR_ERR_UNSUPPORTED;
+  if (cfg->order != NR_ORDER_LEVEL && cfg->order != NR_ORDER_MASS) return NR_ERR_UNSUPPORTED;
   nr::Variant var;
   int rc = nr::pick_variant(g, var);
   if (rc) return rc;

@@ -168,3 +168,63 @@ def test_unsupported_shapes_fail_loudly():
     with pytest.raises(nr.NRError) as ei:
         nr.solve(g, truncation_eps=1e-3, queue_capacity=1024)
     assert ei.value.status == 5
+
+
+# ---------------------------------------------------------------- mass order
+@pytest.mark.parametrize("name,n", [("COMP8-09.txt", 8), ("COMP9.txt", 9), ("COMP10-05.txt", 10)])
+def test_mass_order_full_enumeration_counts(expected, name, n):
+    """Max-mass class rounds visit the same tree: counts bit-exact, R to 1e-12."""
+    print(f"case mass-order full {name}", flush=True)
+    g = nr.Graph.load(graph_path(name), 0, n - 1)
+    r = nr.solve(g, truncation_eps=0.0, queue_capacity=1 << 23, order="mass")
+    lv = nr.solve(g, truncation_eps=0.0, queue_capacity=1 << 23, order="level")
+    assert (r["paths"], r["nodes"], r["dead"]) == (lv["paths"], lv["nodes"], lv["dead"])
+    assert r["ESP"] == 0.0 and r["overflowed"] == 0 and r["folded"] == 0
+    assert abs(r["R"] - lv["R"]) <= REL * lv["R"]
+    assert abs(r["R"] + r["D"] - 1.0) <= 1e-13
+
+
+@pytest.mark.parametrize("eps", [1e-7, 1e-9])
+def test_mass_order_truncated_counts(eps):
+    print(f"case mass-order K15 eps={eps}", flush=True)
+    g, og = pair("COMP15-05.txt", 0, 14)
+    r = nr.solve(g, truncation_eps=eps, queue_capacity=1 << 24, order="mass")
+    o = og.solve(0, 14, order=ORDER_LEVEL, trunc_eps=eps)
+    check_same(r, o, levels=False)
+
+
+@pytest.mark.parametrize("cap", [64, 2000, 50000])
+def test_mass_order_capacity_bracket(expected, cap):
+    print(f"case mass-order K10 cap={cap}", flush=True)
+    g = nr.Graph.load(graph_path("COMP10-05.txt"), 0, 9)
+    r = nr.solve(g, truncation_eps=0.0, queue_capacity=cap, order="mass")
+    Rt = closed_form(expected, 10, "0.5")
+    assert r["overflowed"] + r["folded"] > 0
+    assert r["R"] <= Rt <= r["R"] + r["ESP"] + 1e-14
+    assert abs(r["R"] + r["D"] + r["ESP"] - 1.0) <= 1e-13
+
+
+@pytest.mark.parametrize("order", ["level", "mass"])
+def test_time_budget_bracket(expected, order):
+    """A 10 ms budget on K15 (delta 1e-15, a run of several hundred ms): the
+    queue is folded into ESP and the bracket R <= R_true <= R + ESP holds."""
+    print(f"case deadline K15 {order}", flush=True)
+    g = nr.Graph.load(graph_path("COMP15-05.txt"), 0, 14)
+    r = nr.solve(g, truncation_eps=1e-15, queue_capacity=1 << 24, order=order,
+                 time_budget_s=0.01)
+    Rt = closed_form(expected, 15, "0.5")
+    assert r["deadline_hit"] == 1
+    assert r["R"] <= Rt <= r["R"] + r["ESP"] + 1e-14
+    assert abs(r["R"] + r["D"] + r["ESP"] - 1.0) <= 1e-12
+
+
+def test_mass_order_beats_level_order_under_time_pressure(expected):
+    """With a budget far below the run time (K15, delta 1e-15), max-mass
+    order resolves heavier subproblems first and leaves a smaller gap."""
+    print("case mass vs level K15 20 ms", flush=True)
+    g = nr.Graph.load(graph_path("COMP15-05.txt"), 0, 14)
+    kw = dict(truncation_eps=1e-15, queue_capacity=1 << 24, time_budget_s=0.02)
+    lv = nr.solve(g, order="level", **kw)
+    ms = nr.solve(g, order="mass", **kw)
+    print(f"ESP level {lv['ESP']:.3e} mass {ms['ESP']:.3e}", flush=True)
+    assert ms["ESP"] < lv["ESP"]
