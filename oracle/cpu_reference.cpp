@@ -368,6 +368,151 @@ struct Engine {
   }
 };
 
+/* forkBest expansion, restating workingPad/main.cu:550-567 and 304-430:
+ * a confirmed (WORKING-only) BFS success ends the item; otherwise the
+ * optimistic path (lowest-index parent rule) is walked from dst towards src
+ * and the UNKNOWN hop of highest p is the pivot (ties: the first met from
+ * the dst side — main.cu keeps the path dst-first, main.cu:332-341, and its
+ * warp argmax keeps the lowest hop index on ties, 379-388).  Children: the
+ * pivot WORKING (mass * p) then FAILED (mass * q).  Returns -1 dead, 0
+ * success, 2 children. */
+int expand_forkbest_impl(const Engine& en, const u64* known, const u64* up, double mass,
+                         u64* ck, u64* cu, double* cm, double* succ, int* hops,
+                         std::vector<NodeSet>& levels) {
+  if (en.find_path(known, up, false, hops, levels) >= 0) {
+    *succ = mass;
+    return 0;
+  }
+  const int L = en.find_path(known, up, true, hops, levels);
+  if (L < 0) return -1;
+  int best = -1;
+  double bp = -1.0;
+  for (int h = L - 1; h >= 0; --h) {
+    const int e = hops[h];
+    const u64 bit = 1ull << (e & 63);
+    if (!(known[e >> 6] & bit) && en.g->p[e] > bp) {
+      bp = en.g->p[e];
+      best = e;
+    }
+  }
+  if (best < 0) { /* all-WORKING path: unreachable after the confirmed BFS */
+    *succ = mass;
+    return 0;
+  }
+  const int W = en.W;
+  const u64 bit = 1ull << (best & 63);
+  const int wi = best >> 6;
+  for (int i = 0; i < W; ++i) {
+    ck[i] = known[i];
+    cu[i] = up[i];
+    ck[W + i] = known[i];
+    cu[W + i] = up[i];
+  }
+  ck[wi] |= bit;
+  cu[wi] |= bit;
+  ck[W + wi] |= bit;
+  cm[0] = mass * en.g->p[best];
+  cm[1] = mass * en.g->q[best];
+  *succ = 0.0;
+  return 2;
+}
+
+/* Path decomposition on a path with the fewest UNKNOWN edges.  WORKING
+ * edges are contracted: components of WORKING edges are labelled by their
+ * lowest node; a level-synchronous BFS over components follows UNKNOWN
+ * edges; the parent of a component is the lowest-label component of the
+ * previous level joined to it by an UNKNOWN edge, through the lowest such
+ * edge id.  The path's UNKNOWN edges (src side first) get the children of
+ * main.cu:214-266; a path with none (src and dst already joined by WORKING
+ * edges) is a success. */
+int expand_minunk_impl(const Engine& en, const u64* known, const u64* up, double mass,
+                       u64* ck, u64* cu, double* cm, double* succ, int* hops) {
+  const orc_graph* g = en.g;
+  const int n = g->n, W = en.W;
+  std::vector<int> lab(n);
+  for (int v = 0; v < n; ++v) lab[v] = v;
+  /* label propagation to the minimum over WORKING edges */
+  for (bool changed = true; changed;) {
+    changed = false;
+    for (int e = 0; e < g->e; ++e) {
+      if (!en.working(known, up, e)) continue;
+      int a = -1, b = -1;
+      for (int u = 0; u < n && b < 0; ++u)
+        for (int j = g->row_ptr[u]; j < g->row_ptr[u + 1]; ++j)
+          if (g->edge_id[j] == e) {
+            if (a < 0) a = u; else if (u != a) { b = u; break; }
+          }
+      if (b < 0) continue;
+      int m = std::min(lab[a], lab[b]);
+      if (lab[a] != m || lab[b] != m) {
+        for (int v = 0; v < n; ++v)
+          if (lab[v] == lab[a] || lab[v] == lab[b]) lab[v] = m;
+        changed = true;
+      }
+    }
+  }
+  const int cs = lab[en.src], cd = lab[en.dst];
+  if (cs == cd) {
+    *succ = mass;
+    return 0;
+  }
+  /* BFS over components via UNKNOWN edges */
+  std::vector<int> lvl(n, -1), par(n, -1), pedge(n, -1);
+  std::vector<int> frontier{cs}, next;
+  lvl[cs] = 0;
+  int L = 0;
+  bool found = false;
+  while (!frontier.empty() && !found) {
+    next.clear();
+    ++L;
+    std::sort(frontier.begin(), frontier.end());
+    for (int c : frontier)
+      for (int u = 0; u < n; ++u) {
+        if (lab[u] != c) continue;
+        for (int j = g->row_ptr[u]; j < g->row_ptr[u + 1]; ++j) {
+          const int e = g->edge_id[j], v = g->col_idx[j];
+          if (!en.allowed(known, up, e) || en.working(known, up, e)) continue;
+          const int cv = lab[v];
+          if (lvl[cv] >= 0 && lvl[cv] < L) continue;
+          if (lvl[cv] < 0) {
+            lvl[cv] = L;
+            next.push_back(cv);
+            par[cv] = c;
+            pedge[cv] = e;
+          } else if (lvl[cv] == L && (c < par[cv] || (c == par[cv] && e < pedge[cv]))) {
+            par[cv] = c;
+            pedge[cv] = e;
+          }
+        }
+      }
+    if (lvl[cd] >= 0) found = true;
+    frontier.swap(next);
+  }
+  if (!found) return -1;
+  int k = 0;
+  for (int c = cd; c != cs; c = par[c]) hops[k++] = pedge[c];
+  std::reverse(hops, hops + k);
+  u64 pre[kMaxW] = {0};
+  double run = mass;
+  for (int h = 0; h < k; ++h) {
+    const int e = hops[h];
+    const u64 bit = 1ull << (e & 63);
+    const int wi = e >> 6;
+    u64* kk = ck + (size_t)h * W;
+    u64* uu = cu + (size_t)h * W;
+    for (int i = 0; i < W; ++i) {
+      kk[i] = known[i] | pre[i];
+      uu[i] = up[i] | pre[i];
+    }
+    kk[wi] |= bit;
+    cm[h] = run * g->q[e];
+    run = run * g->p[e];
+    pre[wi] |= bit;
+  }
+  *succ = run;
+  return k;
+}
+
 /* Flat item store: planes (known, up) of W words + FP64 mass. */
 struct Items {
   int W = 1;
@@ -438,6 +583,12 @@ struct Worker {
     if (en.mode == ORC_MODE_PIVOT)
       k = en.expand_pivot(known, up, mass, ck.data(), cu.data(), cm.data(), &succ,
                           hops.data(), levels);
+    else if (en.mode == ORC_MODE_FORKBEST)
+      k = expand_forkbest_impl(en, known, up, mass, ck.data(), cu.data(), cm.data(), &succ,
+                               hops.data(), levels);
+    else if (en.mode == ORC_MODE_MINUNK)
+      k = expand_minunk_impl(en, known, up, mass, ck.data(), cu.data(), cm.data(), &succ,
+                             hops.data());
     else
       k = en.expand_path(known, up, mass, ck.data(), cu.data(), cm.data(), &succ,
                          hops.data(), nullptr, levels);
@@ -446,7 +597,7 @@ struct Worker {
       acc.dead++;
       return 0;
     }
-    if (en.mode == ORC_MODE_PATH || k == 0) {
+    if (en.mode == ORC_MODE_PATH || en.mode == ORC_MODE_MINUNK || k == 0) {
       acc.R.add(succ);
       acc.paths++;
     }

@@ -2,7 +2,7 @@
  * cpu_reference_main.cpp - command line for the CPU oracle (test
  * infrastructure; see oracle.h).
  *
- *   cpu_reference <graph_file> <src> <dst> [--mode path|pivot]
+ *   cpu_reference <graph_file> <src> <dst> [--mode path|pivot|forkbest]
  *       [--order level|heap|dfs] [--threads T] [--truncation_eps f]
  *       [--queue_capacity n] [--time_budget s] [--json]
  *
@@ -18,7 +18,7 @@
 int main(int argc, char** argv) {
   if (argc < 4) {
     std::fprintf(stderr,
-                 "Usage: %s <graph_file> <src> <dst> [--mode path|pivot] "
+                 "Usage: %s <graph_file> <src> <dst> [--mode path|pivot|forkbest] "
                  "[--order level|heap|dfs] [--threads T] [--truncation_eps f] "
                  "[--queue_capacity n] [--time_budget s] [--json]\n",
                  argv[0]);
@@ -35,7 +35,10 @@ int main(int argc, char** argv) {
     bool has = i + 1 < argc;
     if (!std::strcmp(a, "--mode") && has) {
       ++i;
-      cfg.mode = !std::strcmp(argv[i], "pivot") ? ORC_MODE_PIVOT : ORC_MODE_PATH;
+      cfg.mode = !std::strcmp(argv[i], "pivot")      ? ORC_MODE_PIVOT
+                 : !std::strcmp(argv[i], "forkbest") ? ORC_MODE_FORKBEST
+                 : !std::strcmp(argv[i], "minunk")   ? ORC_MODE_MINUNK
+                                                     : ORC_MODE_PATH;
     } else if (!std::strcmp(a, "--order") && has) {
       ++i;
       cfg.order = !std::strcmp(argv[i], "heap")  ? ORC_ORDER_HEAP

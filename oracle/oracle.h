@@ -18,6 +18,9 @@
  *   - seq/seq.cpp:49-197,481-592       max-mass heap order, ESP on a full queue
  *   - original/Seq_16.cpp:44,227       time-budget fold of the queue into ESP
  *   - workingPad/cpu_reference.cpp:85-145 edge-pivot factoring ("pivot" mode)
+ *   - workingPad/main.cu:304-430,550-567 forkBest: confirmed BFS success, else
+ *                                       two-way branch on the path edge of
+ *                                       highest p ("forkbest" mode)
  * with FP64 masses and compensated (Neumaier) sums.
  */
 #ifndef NR_ORACLE_H
@@ -37,6 +40,10 @@ extern "C" {
 /* modes */
 #define ORC_MODE_PATH 0  /* path decomposition, k children per item */
 #define ORC_MODE_PIVOT 1 /* edge-pivot factoring (workingPad/cpu_reference.cpp) */
+#define ORC_MODE_FORKBEST 2 /* two-way branch on the best edge of the path
+                               (workingPad/main.cu:304-430, 550-567) */
+#define ORC_MODE_MINUNK 3   /* path decomposition on a path with the fewest
+                               UNKNOWN edges (BFS over WORKING components) */
 
 /* status codes */
 #define ORC_OK 0

@@ -221,3 +221,38 @@ def test_expand_children_partition_mass():
         for i, e in enumerate(unknown_hops):
             assert (int(ck[i][0]) >> e) & 1 and not (int(cu[i][0]) >> e) & 1
         assert W == 1
+
+
+MODE_FORKBEST, MODE_MINUNK = 2, 3
+
+
+@pytest.mark.parametrize("mode", [MODE_FORKBEST, MODE_MINUNK])
+def test_alternative_rules_exact(expected, mode):
+    """forkBest (main.cu:304-430) and the min-unknown path rule decompose
+    exactly: R equals brute force and the closed form."""
+    for bf in expected["brute_force"]:
+        r = load(bf["graph"]).solve(bf["src"], bf["dst"], mode=mode)
+        assert abs(r["R"] - bf["R"]) <= 1e-15, bf
+        assert abs(r["R"] + r["D"] - 1.0) <= 1e-15
+    for name, n, p in [("COMP8.txt", 8, "0.5"), ("COMP8-09.txt", 8, "0.9")]:
+        r = load(name).solve(0, n - 1, mode=mode)
+        assert abs(r["R"] - closed_form(expected, n, p)) <= 1e-14
+
+
+def test_forkbest_levels_match_logs(expected):
+    """forkBest restated from the source reproduces the logged level count
+    (workingPad/output_maxMem_forBest.txt:20) and binary-tree shape
+    (nodes = 2 * internal + 1); its logged paths/nodes (22560 / 78251) are
+    not reproduced by any tie rule and are not used as pins."""
+    r = load("COMP8.txt").solve(0, 7, mode=MODE_FORKBEST)
+    assert r["iterations"] == 29
+    leaves = r["paths"] + r["dead"]
+    assert r["nodes"] == 2 * leaves - 1
+
+
+def test_minunk_never_longer_in_unknowns():
+    """On the kite graph the min-unknown rule needs fewer subproblems."""
+    g = load("test_kite.txt")
+    a = g.solve(0, 4)
+    b = g.solve(0, 4, mode=MODE_MINUNK)
+    assert b["nodes"] <= a["nodes"] and abs(a["R"] - b["R"]) <= 1e-15
