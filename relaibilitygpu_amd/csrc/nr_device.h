@@ -60,7 +60,7 @@ struct Layout {
 };
 
 inline Layout make_layout(int n, int e, int w, int block, int ns_bytes,
-                          int extra_bytes = 0) {
+                          int extra_bytes = 0, bool tables = true) {
   Layout L;
   L.n = n;
   L.e = e;
@@ -74,14 +74,15 @@ inline Layout make_layout(int n, int e, int w, int block, int ns_bytes,
     o += bytes;
     return at;
   };
-  int ee = e > 0 ? e : 1;
-  L.off_p = take(8 * ee, 16);
-  L.off_q = take(8 * ee, 16);
-  L.off_sadj = take(8 * n, 16);
-  L.off_inc = take(8 * n * w, 16);
-  L.off_ep = take(4 * ee, 16);
-  L.off_deg = take(4 * n, 16);
-  L.off_eidm = take(2 * n * n, 16);
+  const int ee = e > 0 ? e : 1;
+  const int t = tables ? 1 : 0; /* batched instances read tables from HBM */
+  L.off_p = take(t * 8 * ee, 16);
+  L.off_q = take(t * 8 * ee, 16);
+  L.off_sadj = take(t * 8 * n, 16);
+  L.off_inc = take(t * 8 * n * w, 16);
+  L.off_ep = take(t * 4 * ee, 16);
+  L.off_deg = take(t * 4 * n, 16);
+  L.off_eidm = take(t * 2 * n * n, 16);
   L.table_bytes = (o + 15) / 16 * 16;
   L.off_extra = take(extra_bytes, 16);
   L.off_rows = take(ns_bytes * n * block, 16);

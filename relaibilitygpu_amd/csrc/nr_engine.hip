@@ -602,6 +602,240 @@ __global__ void __launch_bounds__(256) k_finalize(LevelArgs a, u64 n_slots) {
 }
 
 /* ------------------------------------------------------------------ */
+/* Batched instances: many graphs / terminal pairs in one level loop   */
+/* ------------------------------------------------------------------ */
+
+/* Per-instance exact accumulator: value * 2^96 as a 128-bit integer in two
+ * u64 words (lo, hi), added with two atomics and an explicit carry, so the
+ * total is exact (to 2^-96 per term) and independent of the order in which
+ * lanes add.  Masses are in [0, 1]. */
+__device__ inline void fx_add(u64* acc, double v) {
+  const double s = v * 4294967296.0; /* v * 2^32 */
+  const u64 hi = (u64)s;
+  const double frac = s - (double)hi;
+  const u64 lo = (u64)(frac * 18446744073709551616.0);
+  if (lo) {
+    const u64 old = atomicAdd((unsigned long long*)&acc[0], (unsigned long long)lo);
+    if (old + lo < old) atomicAdd((unsigned long long*)&acc[1], 1ull);
+  }
+  if (hi) atomicAdd((unsigned long long*)&acc[1], (unsigned long long)hi);
+}
+
+/* Per-instance description (device). */
+struct InstDesc {
+  int n, e, src, dst;
+  int off_e;    /* into p, q, ep */
+  int off_eidm; /* into eidm */
+};
+
+/* Per-instance results (device): fixed-point sums and counters. */
+enum { B_R = 0, B_E = 2, B_D = 4, B_NODES = 6, B_PATHS, B_DEAD, B_TRUNC, B_OVER, B_FOLD,
+       B_LEVELS, B_ERR, B_N = 16 };
+
+struct BatchArgs {
+  Layout lay;        /* per-lane LDS slices sized for the largest instance */
+  QBuf buf[2];
+  u32* inst[2];      /* instance id plane of each buffer */
+  u32* seg_count[2];
+  const InstDesc* desc;
+  const double* p;
+  const double* q;
+  const u32* ep;
+  const u16* eidm;
+  u64* res;          /* [count][B_N] */
+  LevelCtl* ctl;
+  u64 stride;
+  u32 seg_cap;
+  int n_seg;
+  double eps;
+};
+
+template <int W, typename NS>
+__global__ void __launch_bounds__(256) k_expand_batch(BatchArgs a, int parity, int level) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  u32* sp = (u32*)(smem + a.lay.off_extra);
+  u32* s_cnt = sp + a.n_seg + 1;
+  u32* part = s_cnt + 1;
+  block_prefix(a.seg_count[parity], a.n_seg, sp, part);
+  const u32 n_items = sp[a.n_seg];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.ctl->last_items = n_items;
+    if (n_items) {
+      a.ctl->iterations += 1;
+      if (n_items > a.ctl->max_width) a.ctl->max_width = n_items;
+    }
+  }
+  if (n_items == 0) {
+    if (threadIdx.x == 0) a.seg_count[parity ^ 1][blockIdx.x] = 0;
+    return;
+  }
+  if (threadIdx.x == 0) *s_cnt = 0;
+  __syncthreads();
+
+  Expander<W, NS> ex;
+  ex.rows = (NS*)(smem + a.lay.off_rows) + threadIdx.x;
+  ex.lvl = (NS*)(smem + a.lay.off_lvl) + threadIdx.x;
+  ex.uh = (u16*)(smem + a.lay.off_uh) + threadIdx.x;
+  ex.stride = a.lay.block;
+
+  const QBuf in = a.buf[parity];
+  const QBuf out = a.buf[parity ^ 1];
+  const u32* iin = a.inst[parity];
+  u32* iout = a.inst[parity ^ 1];
+  const int lane = threadIdx.x & 63;
+  const u64 wave_id = (u64)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const u64 n_waves = (u64)(gridDim.x * blockDim.x) >> 6;
+  const u64 out_base = (u64)blockIdx.x * a.seg_cap;
+  const u64 per_wave = (n_items + n_waves - 1) / n_waves;
+  const u64 chunk = per_wave >= 64 ? 64 : per_wave;
+
+  for (u64 base = wave_id * chunk; base < n_items; base += n_waves * chunk) {
+    const u32 i = (u32)(base + lane);
+    u64 known[W], up[W];
+    double mass = 0.0;
+    int k = 0;
+    bool live = false;
+    u32 id = 0;
+    u64* res = nullptr;
+    if ((u64)lane < chunk && i < n_items) {
+      int s = segment_of(sp, a.n_seg, (u32)base);
+      while (sp[s + 1] <= i) ++s;
+      const u64 slot_in = (u64)s * a.seg_cap + (i - sp[s]);
+      load_item<W>(in, a.stride, slot_in, known, up, mass);
+      id = iin[slot_in];
+      const InstDesc d = a.desc[id];
+      res = a.res + (u64)id * B_N;
+      ex.p = a.p + d.off_e;
+      ex.q = a.q + d.off_e;
+      ex.ep = a.ep + d.off_e;
+      ex.eidm = a.eidm + d.off_eidm;
+      ex.n = d.n;
+      ex.src = d.src;
+      ex.dst = d.dst;
+      atomicAdd((unsigned long long*)&res[B_NODES], 1ull);
+      atomicMax((unsigned long long*)&res[B_LEVELS], (unsigned long long)(level + 1));
+      if (mass < a.eps) { /* main.cu:537-543 */
+        fx_add(&res[B_E], mass);
+        atomicAdd((unsigned long long*)&res[B_TRUNC], 1ull);
+      } else {
+        const NS* rin = (const NS*)in.rows + slot_in;
+        for (int u = 0; u < ex.n; ++u) ex.rows[u * ex.stride] = rin[(u64)u * a.stride];
+        k = ex.find(known);
+        if (k >= 0) {
+          live = true;
+        } else {
+          if (k == -2) atomicAdd((unsigned long long*)&res[B_ERR], 1ull);
+          fx_add(&res[B_D], mass); /* main.cu:579-582 */
+          atomicAdd((unsigned long long*)&res[B_DEAD], 1ull);
+          k = 0;
+        }
+      }
+    }
+    const int incl = wave_incl_scan(k);
+    const int total = __shfl(incl, 63, 64);
+    u32 wbase = 0;
+    if (total) {
+      if (lane == 63) wbase = atomicAdd(s_cnt, (u32)total);
+      wbase = __shfl(wbase, 63, 64);
+    }
+    if (live) {
+      u32 off = wbase + (u32)(incl - k);
+      double run = mass;
+      u64 pre[W];
+#pragma unroll
+      for (int w = 0; w < W; ++w) pre[w] = 0;
+      for (int j = k - 1; j >= 0; --j, ++off) {
+        const int e = ex.uh[j * ex.stride];
+        const int wi = e >> 6;
+        const u64 bit = 1ull << (e & 63);
+        const double cm = run * ex.q[e];
+        if (off < a.seg_cap) {
+          u64 ck[W], cu[W];
+#pragma unroll
+          for (int w = 0; w < W; ++w) {
+            ck[w] = known[w] | pre[w] | (w == wi ? bit : 0ull);
+            cu[w] = up[w] | pre[w];
+          }
+          store_item<W>(out, a.stride, out_base + off, ck, cu, cm);
+          iout[out_base + off] = id;
+          NS* rout = (NS*)out.rows + out_base + off;
+          for (int u = 0; u < ex.n; ++u) rout[(u64)u * a.stride] = ex.child_row(u, e);
+        } else { /* segment full: main.cu:245-252 */
+          fx_add(&res[B_E], cm);
+          atomicAdd((unsigned long long*)&res[B_PATHS], 1ull);
+          atomicAdd((unsigned long long*)&res[B_OVER], 1ull);
+        }
+        run = run * ex.p[e];
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+          if (w == wi) pre[w] |= bit;
+      }
+      fx_add(&res[B_R], run); /* success term, main.cu:254-262 */
+      atomicAdd((unsigned long long*)&res[B_PATHS], 1ull);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const u32 n = *s_cnt;
+    a.seg_count[parity ^ 1][blockIdx.x] = n < a.seg_cap ? n : a.seg_cap;
+  }
+}
+
+/* Time-budget fold of a batch's pending items into each instance's ESP. */
+__global__ void __launch_bounds__(256) k_fold_batch(BatchArgs a, int parity) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  u32* sp = (u32*)smem;
+  u32* part = sp + a.n_seg + 2;
+  block_prefix(a.seg_count[parity], a.n_seg, sp, part);
+  const u32 n_items = sp[a.n_seg];
+  const u64 tid = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  const u64 nthreads = (u64)gridDim.x * blockDim.x;
+  for (u64 i = tid; i < n_items; i += nthreads) {
+    int s = segment_of(sp, a.n_seg, (u32)i);
+    const u64 slot = (u64)s * a.seg_cap + (i - sp[s]);
+    u64* res = a.res + (u64)a.inst[parity][slot] * B_N;
+    fx_add(&res[B_E], a.buf[parity].mass[slot]);
+    atomicAdd((unsigned long long*)&res[B_FOLD], 1ull);
+  }
+}
+
+/* Seed a batch: zero results and counters, scatter the roots (one per
+ * instance) round-robin over the segments. */
+__global__ void __launch_bounds__(256) k_init_batch(BatchArgs a, const u64* roots, int count,
+                                                    int W, int nmax, int ns_bytes) {
+  const u64 tid = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  const u64 stride = (u64)gridDim.x * blockDim.x;
+  for (u64 i = tid; i < (u64)count * B_N; i += stride) a.res[i] = 0;
+  for (u64 s = tid; s < (u64)a.n_seg; s += stride) {
+    const u64 c = (u64)count;
+    a.seg_count[0][s] = (u32)(c / a.n_seg + (s < c % a.n_seg ? 1 : 0));
+    a.seg_count[1][s] = 0u;
+  }
+  if (tid == 0) {
+    LevelCtl z = {};
+    *a.ctl = z;
+    a.ctl->last_items = (u64)count;
+  }
+  const int rec = 2 * W + nmax;
+  for (u64 i = tid; i < (u64)count; i += stride) {
+    const u64 slot = (i % a.n_seg) * a.seg_cap + i / a.n_seg;
+    const u64* r = roots + i * rec;
+    for (int w = 0; w < W; ++w) {
+      a.buf[0].known[(u64)w * a.stride + slot] = r[w];
+      a.buf[0].up[(u64)w * a.stride + slot] = r[W + w];
+    }
+    for (int u = 0; u < nmax; ++u) {
+      if (ns_bytes == 4)
+        ((u32*)a.buf[0].rows)[(u64)u * a.stride + slot] = (u32)r[2 * W + u];
+      else
+        ((u64*)a.buf[0].rows)[(u64)u * a.stride + slot] = r[2 * W + u];
+    }
+    a.buf[0].mass[slot] = 1.0;
+    a.inst[0][slot] = (u32)i;
+  }
+}
+
+/* ------------------------------------------------------------------ */
 /* host runtime                                                        */
 /* ------------------------------------------------------------------ */
 
@@ -776,12 +1010,13 @@ template <int W, typename NS>
 struct Kern {
   static void* level() { return (void*)&k_expand_level<W, NS>; }
   static void* items() { return (void*)&k_expand_items<W, NS>; }
+  static void* batch() { return (void*)&k_expand_batch<W, NS>; }
 };
 
 typedef void* (*KernelGetter)();
 
 struct Variant {
-  KernelGetter level, items;
+  KernelGetter level, items, batch;
   int ns_bytes;
 };
 
@@ -789,10 +1024,11 @@ template <typename NS>
 Variant variant_for_w(int W) {
   switch (W) {
 #define NR_W(X) \
-  case X: return Variant{&Kern<X, NS>::level, &Kern<X, NS>::items, (int)sizeof(NS)};
+  case X:      \
+    return Variant{&Kern<X, NS>::level, &Kern<X, NS>::items, &Kern<X, NS>::batch, (int)sizeof(NS)};
     NR_W(1) NR_W(2) NR_W(3) NR_W(4) NR_W(5) NR_W(6) NR_W(7) NR_W(8)
 #undef NR_W
-    default: return Variant{nullptr, nullptr, 0};
+    default: return Variant{nullptr, nullptr, nullptr, 0};
   }
 }
 
@@ -1091,6 +1327,214 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   return NR_OK;
 }
 
+/* Batched solve: every instance's root in one level-order queue; each item
+ * carries its instance id, instance tables are read from HBM, per-instance
+ * sums are exact fixed-point atomics (fx_add). */
+int solve_batch(Workspace& ws, const nr_graph* const* gs, int count, const nr_config* cfg,
+                nr_result* outs) {
+  int Wmax = 1, Nmax = 1;
+  for (int i = 0; i < count; ++i) {
+    const nr_graph* g = gs[i];
+    if (!g) return NR_ERR_ARG;
+    if (!g->symmetric || g->n > 64 || mask_words(g) > 8) return NR_ERR_UNSUPPORTED;
+    Wmax = std::max(Wmax, mask_words(g));
+    Nmax = std::max(Nmax, g->n);
+  }
+  Variant var = Nmax <= 32 ? variant_for_w<u32>(Wmax) : variant_for_w<u64>(Wmax);
+  if (!var.batch) return NR_ERR_UNSUPPORTED;
+  const int W = Wmax;
+  const void* kfun = (const void*)var.batch();
+  const int cus = ws.prop.multiProcessorCount;
+  int block = 256;
+  while (block >= 64 && make_layout(Nmax, 1, W, block, var.ns_bytes, 4 * (1026 + block), false)
+                                .total_bytes > 64 * 1024)
+    block /= 2;
+  if (block < 64) return NR_ERR_UNSUPPORTED;
+  NR_HIP(hipFuncSetAttribute(kfun, hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
+  Layout lay0 = make_layout(Nmax, 1, W, block, var.ns_bytes, 4 * (block + 2), false);
+  int per_cu0 = 0;
+  NR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu0, kfun, block, lay0.total_bytes));
+  if (per_cu0 < 1) return NR_ERR_UNSUPPORTED;
+  int grid = cfg->grid_blocks > 0 ? cfg->grid_blocks : per_cu0 * cus;
+  const int extra = 4 * (grid + 2 + block);
+  Layout lay = make_layout(Nmax, 1, W, block, var.ns_bytes, extra, false);
+  if (lay.total_bytes > 64 * 1024) return NR_ERR_UNSUPPORTED;
+  int per_cu = 0;
+  NR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfun, block, lay.total_bytes));
+  if (per_cu < 1) return NR_ERR_UNSUPPORTED;
+  if (cfg->grid_blocks <= 0) grid = std::min(grid, per_cu * cus);
+  const u64 cap0 = cfg->queue_capacity > 0 ? (u64)cfg->queue_capacity : (1ull << 20);
+  const u64 cap = std::max<u64>(cap0, (u64)count);
+  const u64 seg_cap64 = std::max<u64>(1, (cap + grid - 1) / grid + 1);
+  if (seg_cap64 * (u64)grid >= (1ull << 32)) return NR_ERR_ARG;
+  const u32 seg_cap = (u32)seg_cap64;
+  const u64 slots = seg_cap64 * (u64)grid;
+
+  /* instance tables */
+  std::vector<InstDesc> desc(count);
+  std::vector<double> hp, hq;
+  std::vector<u32> hep;
+  std::vector<u16> heidm;
+  const int rec = 2 * W + Nmax;
+  std::vector<u64> roots((size_t)count * rec, 0);
+  for (int i = 0; i < count; ++i) {
+    const nr_graph* g = gs[i];
+    InstDesc d;
+    d.n = g->n;
+    d.e = g->e;
+    d.src = g->src;
+    d.dst = g->dst;
+    d.off_e = (int)hp.size();
+    d.off_eidm = (int)heidm.size();
+    desc[i] = d;
+    for (int e = 0; e < g->e; ++e) {
+      hp.push_back(g->p[e]);
+      hq.push_back(g->q[e]);
+      hep.push_back((u32)g->ea[e] | ((u32)g->eb[e] << 16));
+    }
+    const size_t b = heidm.size();
+    heidm.resize(b + (size_t)g->n * g->n, 0xffff);
+    for (int e = 0; e < g->e; ++e) {
+      heidm[b + (size_t)g->ea[e] * g->n + g->eb[e]] = (u16)e;
+      heidm[b + (size_t)g->eb[e] * g->n + g->ea[e]] = (u16)e;
+    }
+    Root r = root_item(g, mask_words(g));
+    u64* dst = &roots[(size_t)i * rec];
+    for (int w = 0; w < W; ++w) {
+      dst[w] = w < mask_words(g) ? r.known[w] : 0;
+      dst[W + w] = w < mask_words(g) ? r.up[w] : 0;
+    }
+    for (int u = 0; u < g->n; ++u) dst[2 * W + u] = r.rows[u];
+  }
+  hp.push_back(0.0);
+  hq.push_back(0.0);
+  hep.push_back(0);
+  heidm.push_back(0);
+  /* one table buffer: desc | p | q | ep | eidm | roots */
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    o = (o + 255) / 256 * 256;
+    size_t at = o;
+    o += bytes;
+    return at;
+  };
+  const size_t o_desc = take(sizeof(InstDesc) * count), o_p = take(8 * hp.size()),
+               o_q = take(8 * hq.size()), o_ep = take(4 * hep.size()),
+               o_eidm = take(2 * heidm.size()), o_roots = take(8 * roots.size());
+  std::vector<unsigned char> blob(o, 0);
+  std::memcpy(&blob[o_desc], desc.data(), sizeof(InstDesc) * count);
+  std::memcpy(&blob[o_p], hp.data(), 8 * hp.size());
+  std::memcpy(&blob[o_q], hq.data(), 8 * hq.size());
+  std::memcpy(&blob[o_ep], hep.data(), 4 * hep.size());
+  std::memcpy(&blob[o_eidm], heidm.data(), 2 * heidm.size());
+  std::memcpy(&blob[o_roots], roots.data(), 8 * roots.size());
+  int rc;
+  if ((rc = ws.grow(&ws.tab, &ws.tab_bytes, blob.size()))) return rc;
+  hipStream_t st = ws.stream;
+  NR_HIP(hipMemcpyAsync(ws.tab, blob.data(), blob.size(), hipMemcpyHostToDevice, st));
+  unsigned char* tb = (unsigned char*)ws.tab;
+
+  BatchArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.lay = lay;
+  a.desc = (const InstDesc*)(tb + o_desc);
+  a.p = (const double*)(tb + o_p);
+  a.q = (const double*)(tb + o_q);
+  a.ep = (const u32*)(tb + o_ep);
+  a.eidm = (const u16*)(tb + o_eidm);
+  a.seg_cap = seg_cap;
+  a.n_seg = grid;
+  a.stride = slots;
+  a.eps = cfg->truncation_eps > 0 ? cfg->truncation_eps : 0.0;
+  const size_t rows_bytes = ((size_t)Nmax * slots * var.ns_bytes + 255) / 256 * 256;
+  const size_t inst_bytes = (slots * 4 + 255) / 256 * 256;
+  const size_t per_buf = (size_t)(2 * W + 1) * slots * 8 + rows_bytes + inst_bytes;
+  if ((rc = ws.grow(&ws.q, &ws.q_bytes, 2 * per_buf))) return rc;
+  for (int b = 0; b < 2; ++b) {
+    unsigned char* base = (unsigned char*)ws.q + b * per_buf;
+    a.buf[b].known = (u64*)base;
+    a.buf[b].up = (u64*)base + (size_t)W * slots;
+    a.buf[b].mass = (double*)((u64*)base + (size_t)2 * W * slots);
+    a.buf[b].rows = (void*)((u64*)base + (size_t)(2 * W + 1) * slots);
+    a.inst[b] = (u32*)((unsigned char*)a.buf[b].rows + rows_bytes);
+  }
+  const size_t meta_bytes = sizeof(u64) * (size_t)count * B_N + sizeof(u32) * (2 * (size_t)grid + 8);
+  if ((rc = ws.grow(&ws.meta, &ws.meta_bytes, meta_bytes))) return rc;
+  a.res = (u64*)ws.meta;
+  a.seg_count[0] = (u32*)(a.res + (size_t)count * B_N);
+  a.seg_count[1] = a.seg_count[0] + grid;
+  a.ctl = ws.ctl;
+  {
+    const u64* droots = (const u64*)(tb + o_roots);
+    int cnt = count, Wi = W, nm = Nmax, nsb = var.ns_bytes;
+    void* args[] = {&a, (void*)&droots, &cnt, &Wi, &nm, &nsb};
+    NR_HIP(hipLaunchKernel((const void*)&k_init_batch, dim3(256), dim3(256), args, 0, st));
+  }
+  hipEvent_t ev_start, ev_stop;
+  if ((rc = ws.event(0, &ev_start)) || (rc = ws.event(1, &ev_stop))) return rc;
+  auto t_begin = std::chrono::steady_clock::now();
+  NR_HIP(hipEventRecord(ev_start, st));
+  int level = 0, chunk = 8;
+  bool finished = false, deadline = false;
+  while (!finished) {
+    for (int c = 0; c < chunk; ++c, ++level) {
+      int parity = level & 1, lv = level;
+      void* args[] = {&a, &parity, &lv};
+      NR_HIP(hipLaunchKernel(kfun, dim3(grid), dim3(block), args, lay.total_bytes, st));
+    }
+    NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
+    NR_HIP(hipStreamSynchronize(st));
+    if (ws.hctl->last_items == 0) finished = true;
+    if (!finished && cfg->time_budget_s > 0 &&
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count() >=
+            cfg->time_budget_s) {
+      int parity = level & 1;
+      void* args[] = {&a, &parity};
+      NR_HIP(hipLaunchKernel((const void*)&k_fold_batch, dim3(grid), dim3(block), args,
+                             4 * (grid + 2 + block), st));
+      deadline = true;
+      finished = true;
+    }
+    chunk = std::min(chunk * 2, 16);
+  }
+  NR_HIP(hipEventRecord(ev_stop, st));
+  std::vector<u64> res((size_t)count * B_N);
+  NR_HIP(hipMemcpyAsync(res.data(), a.res, sizeof(u64) * res.size(), hipMemcpyDeviceToHost, st));
+  NR_HIP(hipStreamSynchronize(st));
+  float ms = 0.f;
+  NR_HIP(hipEventElapsedTime(&ms, ev_start, ev_stop));
+  auto fx = [](const u64* w) {
+    return (double)w[1] * 2.3283064365386963e-10 + (double)w[0] * 1.2621774483536189e-29;
+  };
+  for (int i = 0; i < count; ++i) {
+    const u64* r = &res[(size_t)i * B_N];
+    if (r[B_ERR]) return NR_ERR_DEVICE;
+    nr_result& o2 = outs[i];
+    std::memset(&o2, 0, sizeof o2);
+    o2.R = fx(&r[B_R]);
+    o2.ESP = fx(&r[B_E]);
+    o2.D = fx(&r[B_D]);
+    o2.nodes = r[B_NODES];
+    o2.paths = r[B_PATHS];
+    o2.dead = r[B_DEAD];
+    o2.truncated = r[B_TRUNC];
+    o2.overflowed = r[B_OVER];
+    o2.folded = r[B_FOLD];
+    o2.iterations = (long long)r[B_LEVELS];
+    o2.max_queue_items = (long long)ws.hctl->max_width;
+    o2.item_bytes = 16LL * W + 8;
+    o2.elapsed_ms = ms;
+    o2.deadline_hit = deadline ? 1 : 0;
+    o2.overflowed_flag = r[B_OVER] ? 1 : 0;
+    o2.grid_blocks = grid;
+    o2.block_size = block;
+    o2.lds_bytes = lay.total_bytes;
+    o2.blocks_per_cu = per_cu;
+    o2.compute_units = cus;
+  }
+  return NR_OK;
+}
+
 /* RAII bag of device allocations (test entry point only). */
 struct DevMem {
   std::vector<void*> ptrs;
@@ -1202,11 +1646,18 @@ int nr_solve(const nr_graph* g, const nr_config* cfg, nr_result* out) {
 int nr_solve_batch(const nr_graph* const* graphs, int count, const nr_config* cfg,
                    nr_result* results) {
   if (!graphs || count < 0 || !results) return NR_ERR_ARG;
-  for (int i = 0; i < count; ++i) {
-    int rc = nr_solve(graphs[i], cfg, &results[i]);
-    if (rc) return rc;
+  if (count == 0) return NR_OK;
+  nr_config def;
+  if (!cfg) {
+    nr_config_init(&def);
+    cfg = &def;
   }
-  return NR_OK;
+  if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0 || cfg->time_budget_s < 0)
+    return NR_ERR_ARG;
+  if (cfg->order != NR_ORDER_LEVEL) return NR_ERR_UNSUPPORTED;
+  nr::WsLock L(cfg->device);
+  if (L.rc) return L.rc;
+  return nr::solve_batch(*L.ws, graphs, count, cfg, results);
 }
 
 int nr_expand_items(const nr_graph* g, int device, int count,
