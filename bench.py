@@ -30,6 +30,10 @@ WORKLOADS = {
             "desc": "K10 (N=10, E=45, p=0.5) src 0 -> dst 9, full enumeration, level order"},
     "k8p09": {"graph": "COMP8-09.txt", "src": 0, "dst": 7, "eps": 0.0, "cap": 1 << 16,
               "desc": "K8 (N=8, E=28, p=0.9) src 0 -> dst 7, full enumeration, level order"},
+    "er4096": {"batch": 4096, "eps": 1e-12, "cap": 1 << 29,
+               "desc": ("4096 seeded Erdos-Renyi instances (n 12..40, mean degree 4..8, "
+                        "p 0.80..0.99, terminals at distance >= 3), truncation 1e-12, "
+                        "one batched solve")},
 }
 METRIC = "subproblems/sec; certified gap (ESP) at 60 s, learned vs plain order"
 
@@ -78,6 +82,113 @@ def cpu_baseline(wl: dict) -> dict:
     }
 
 
+def er_graphs(count: int):
+    from relaibilitygpu_amd import workloads
+
+    return [workloads.er_instance(i) for i in range(count)]
+
+
+def cpu_baseline_batch(insts, eps: float, sample: int) -> dict:
+    """The oracle on the first `sample` instances, one instance per host
+    thread (single-threaded depth-first solves in a thread pool)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle_ffi import ORDER_DFS, OracleGraph
+    from relaibilitygpu_amd import workloads
+
+    threads = min(os.cpu_count() or 1, 16)
+    graphs = []
+    for n, edges, probs, s, t in insts[:sample]:
+        rp, ci, ei = workloads.to_csr(n, edges)
+        graphs.append((OracleGraph.from_csr(rp, ci, ei, probs), s, t))
+
+    def one(item):
+        g, s, t = item
+        return g.solve(s, t, order=ORDER_DFS, threads=1, trunc_eps=eps)["nodes"]
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        nodes = sum(ex.map(one, graphs))
+    dt = time.perf_counter() - t0
+    return {
+        "value": nodes / dt,
+        "unit": "subproblems/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"first {sample} instances ({nodes} subproblems) by the oracle "
+                   f"(oracle/cpu_reference.cpp, depth-first, one instance per thread, "
+                   f"{threads} threads) in {dt:.2f} s"),
+    }
+
+
+def run_batch(args, wl: dict) -> dict:
+    import relaibilitygpu_amd as nr
+    from relaibilitygpu_amd import workloads
+
+    insts = er_graphs(wl["batch"])
+    graphs = []
+    wmax = 1
+    for n, edges, probs, s, t in insts:
+        rp, ci, ei = workloads.to_csr(n, edges)
+        g = nr.Graph.from_csr(rp, ci, ei, probs, s, t)
+        wmax = max(wmax, g.info().mask_words)
+        graphs.append(g)
+    cfg = nr.make_config(queue_capacity=wl["cap"], truncation_eps=wl["eps"], order="level")
+    for w in range(args.warmup):
+        r = nr.solve_batch(graphs, cfg)
+        print(f"warmup {w}: {r[0]['elapsed_ms']:.1f} ms, reruns {sum(x['rerun'] for x in r)}",
+              file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    runs = []
+    for k in range(args.steps):
+        runs.append(nr.solve_batch(graphs, cfg))
+        print(f"step {k}: {runs[-1][0]['elapsed_ms']:.1f} ms", file=sys.stderr, flush=True)
+    t1 = time.perf_counter()
+    nodes = sum(r["nodes"] for r in runs[0])
+    for res in runs[1:]:
+        assert [r["nodes"] for r in res] == [r["nodes"] for r in runs[0]]
+    wall_s = t1 - t0
+    tcfg = nr.make_config(queue_capacity=wl["cap"], truncation_eps=wl["eps"], order="level",
+                          time_launches=True)
+    timed = nr.solve_batch(graphs, tcfg)[0]
+    bytes_per_sub = 2 * (16 * wmax + 8)
+    achieved = nodes * bytes_per_sub / (timed["kernel_ms"] / 1e3) / 1e9
+    r0 = runs[0]
+    return {
+        "value": nodes * args.steps / wall_s,
+        "ms_per_step": wall_s * 1e3 / args.steps,
+        "config": {
+            "workload": wl["desc"],
+            "instances": len(graphs),
+            "subproblems_per_step": nodes,
+            "paths_per_step": sum(r["paths"] for r in r0),
+            "rerun_instances": sum(r["rerun"] for r in r0),
+            "overflowed_instances": sum(1 for r in r0 if r["overflowed"]),
+            "truncation_eps": wl["eps"],
+            "queue_capacity": wl["cap"],
+            "order": "level",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic_from_profiles("k_expand_batch", args.workload),
+            "kernel": "k_expand_batch",
+            "bytes_per_subproblem": bytes_per_sub,
+            "avg_launch_ms": timed["kernel_ms"] / max(1, timed["kernel_launches"]),
+            "launches_per_step": timed["kernel_launches"],
+        },
+        "gpu_elapsed_ms_per_step": sum(r[0]["elapsed_ms"] for r in runs) / len(runs),
+        "kernel_ms_per_step": timed["kernel_ms"],
+        "sum_R": sum(r["R"] for r in r0),
+        "max_ESP": max(r["ESP"] for r in r0),
+        "_insts": insts,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +196,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="k10", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-sample", type=int, default=512,
+                    help="batched workloads: instances timed on the CPU")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -108,6 +221,19 @@ def main() -> int:
     import relaibilitygpu_amd as nr
 
     wl = WORKLOADS[args.workload]
+    if "batch" in wl:
+        b = run_batch(args, wl)
+        insts = b.pop("_insts")
+        line = {"metric": METRIC, "value": b.pop("value"), "unit": "subproblems/s",
+                "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                "ms_per_step": b.pop("ms_per_step"), "higher_is_better": True,
+                "scaling": "strong", "vs_baseline": None, "dtype": "u64+f64",
+                "data": "synthetic"}
+        line.update(b)
+        if not args.no_cpu:
+            line["cpu_baseline"] = cpu_baseline_batch(insts, wl["eps"], args.cpu_sample)
+        print(json.dumps(line), flush=True)
+        return 0
     path = os.path.join(ROOT, "tests", "golden", "graphs", wl["graph"])
     g = nr.Graph.load(path, wl["src"], wl["dst"])
     info = g.info()

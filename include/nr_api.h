@@ -98,7 +98,8 @@ typedef struct nr_result {
   int lds_bytes;                /* dynamic LDS per workgroup */
   int blocks_per_cu;            /* occupancy reported by the runtime */
   int compute_units;
-  int reserved;
+  int rerun;                    /* nr_solve_batch: 1 if the instance overflowed
+                                   the shared queue and was solved again alone */
 } nr_result;
 
 typedef struct nr_graph_info {

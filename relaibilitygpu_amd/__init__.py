@@ -89,7 +89,7 @@ class Result(ctypes.Structure):
         ("lds_bytes", ctypes.c_int),
         ("blocks_per_cu", ctypes.c_int),
         ("compute_units", ctypes.c_int),
-        ("reserved", ctypes.c_int),
+        ("rerun", ctypes.c_int),
     ]
 
     def as_dict(self) -> dict:

@@ -267,6 +267,59 @@ struct Expander {
     return k;
   }
 
+  /* Row u derived on first use from the item's FAILED plane Dn and cached in
+   * this lane's LDS slice (bit u of `have`): the form used when items carry
+   * no rows. */
+  __device__ NS lazy_row(int u, const u64 (&Dn)[W], NS& have) const {
+    if ((have >> u) & (NS)1) return rows[u * stride];
+    const NS a = adj_of(u, Dn);
+    rows[u * stride] = a;
+    have |= (NS)1 << u;
+    return a;
+  }
+
+  /* find() for an item given by its planes alone; same BFS, same parent
+   * rule, same unknown-hop stack. */
+  __device__ int find_planes(const u64 (&known)[W], const u64 (&up)[W]) const {
+    if (src == dst) return 0;
+    u64 Dn[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i) Dn[i] = known[i] & ~up[i];
+    NS have = 0;
+    const NS dbit = (NS)1 << dst;
+    NS vis = (NS)1 << src;
+    NS F = vis;
+    lvl[0] = F;
+    int L = 0;
+    for (;;) {
+      NS nx = 0;
+      NS f = F;
+      while (f) {
+        const int u = ctzs<NS>(f);
+        f &= f - 1;
+        nx |= lazy_row(u, Dn, have);
+        if (nx & dbit) break;
+      }
+      nx &= ~vis;
+      if (!nx) return -1;
+      ++L;
+      if (nx & dbit) break;
+      lvl[L * stride] = nx;
+      vis |= nx;
+      F = nx;
+    }
+    int v = dst, k = 0;
+    for (int l = L; l >= 1; --l) {
+      const NS cand = lazy_row(v, Dn, have) & lvl[(l - 1) * stride];
+      if (!cand) return -2;
+      const int u = ctzs<NS>(cand);
+      const int e = eidm[u * n + v];
+      if (!((pick<W>(known, e >> 6) >> (e & 63)) & 1ull)) uh[(k++) * stride] = (u16)e;
+      v = u;
+    }
+    return k;
+  }
+
   /* Row u of a child whose FAILED hop is edge e: the parent's row with the
    * ends of e removed from each other's rows. */
   __device__ NS child_row(int u, int e) const {

@@ -621,50 +621,165 @@ __device__ inline void fx_add(u64* acc, double v) {
   if (hi) atomicAdd((unsigned long long*)&acc[1], (unsigned long long)hi);
 }
 
+/* The same fixed-point value split into three 32-bit limbs held in u64
+ * lanes (value * 2^96 = l2 * 2^64 + l1 * 2^32 + l0), so a wave can add up to
+ * 2^25 terms with plain integer sums before one flush. */
+__device__ inline void fx_limbs_add(u64& l2, u64& l1, u64& l0, double v) {
+  const double s = v * 4294967296.0;
+  const u64 hi = (u64)s;
+  const u64 lo = (u64)((s - (double)hi) * 18446744073709551616.0);
+  l2 += hi;
+  l1 += lo >> 32;
+  l0 += lo & 0xffffffffull;
+}
+
+__device__ inline void fx_flush(u64* acc, u64 l2, u64 l1, u64 l0) {
+  const u64 lo = l0 + ((l1 & 0xffffffffull) << 32);
+  u64 hi = l2 + (l1 >> 32) + (lo < l0 ? 1ull : 0ull);
+  if (lo) {
+    const u64 old = atomicAdd((unsigned long long*)&acc[0], (unsigned long long)lo);
+    if (old + lo < old) hi += 1;
+  }
+  if (hi) atomicAdd((unsigned long long*)&acc[1], (unsigned long long)hi);
+}
+
+__device__ inline u64 wave_sum_if(u64 x, u64 any) { return any ? wave_sum_u64(x) : 0ull; }
+
 /* Per-instance description (device). */
 struct InstDesc {
   int n, e, src, dst;
-  int off_e;    /* into p, q, ep */
-  int off_eidm; /* into eidm */
+  /* byte offsets of this instance's tables from BatchArgs::tab */
+  u32 off_p, off_q, off_sadj, off_inc, off_ep, off_deg, off_eidm, pad;
 };
 
 /* Per-instance results (device): fixed-point sums and counters. */
 enum { B_R = 0, B_E = 2, B_D = 4, B_NODES = 6, B_PATHS, B_DEAD, B_TRUNC, B_OVER, B_FOLD,
        B_LEVELS, B_ERR, B_N = 16 };
 
+/* Item tag in a batch: instance id (low 22 bits) and tree depth. */
+constexpr u32 kTagIdBits = 22;
+constexpr u32 kTagIdMask = (1u << kTagIdBits) - 1u;
+constexpr u32 kMaxBatch = 1u << kTagIdBits;
+constexpr u32 kMaxDepth = (1u << (32 - kTagIdBits)) - 1u;
+
+/* Stream state, double-buffered by the parity of the buffer a refill writes:
+ * refill(out = b) reads top[b] / next[b] and writes top[b ^ 1] / next[b ^ 1];
+ * the expand launch reading buffer b pushes its spill on top[b ^ 1]. */
+struct StreamCtl {
+  u64 top[2];  /* spill stack height */
+  u32 next[2]; /* instances admitted so far (index into perm) */
+};
+
 struct BatchArgs {
   Layout lay;        /* per-lane LDS slices sized for the largest instance */
   QBuf buf[2];
-  u32* inst[2];      /* instance id plane of each buffer */
-  u32* seg_count[2];
+  u32* tag[2];       /* instance id | depth << kTagIdBits, per slot */
+  u32* seg_count[2]; /* [n_seg]: segments 0..grid-1 (seg_cap slots each)
+                        belong to workgroups; segment grid (fill_step slots,
+                        the last) is the refill segment */
+  QBuf spill;        /* LIFO of children that did not fit their segment */
+  u32* spill_tag;
+  u64 spill_cap;
   const InstDesc* desc;
-  const double* p;
-  const double* q;
-  const u32* ep;
-  const u16* eidm;
+  const unsigned char* tab; /* every instance's tables (graph-table layout) */
+  const u64* roots;  /* [count][2W]: known[W] | up[W] */
+  const u32* perm;   /* admission order */
+  StreamCtl* sc;
   u64* res;          /* [count][B_N] */
   LevelCtl* ctl;
   u64 stride;
   u32 seg_cap;
-  int n_seg;
+  int n_seg;         /* grid + 1 */
+  u32 count;         /* instances in perm */
+  u32 fill_step;     /* at most this many items refilled per level */
+  u32 fill_to;       /* refill while the next level holds fewer items */
+  u32 root_step;     /* at most this many new roots per refill */
+  int roots_when_empty; /* admit roots only into an empty queue and stack */
   double eps;
 };
 
+template <int W>
+__device__ inline void copy_item(const QBuf& from, u64 fstride, u64 fi, const QBuf& to,
+                                 u64 tstride, u64 ti) {
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    to.known[(u64)w * tstride + ti] = from.known[(u64)w * fstride + fi];
+    to.up[(u64)w * tstride + ti] = from.up[(u64)w * fstride + fi];
+  }
+  to.mass[ti] = from.mass[fi];
+}
+
+/* Between two expand launches: top up the level about to run (buffer
+ * `out`) from the spill stack, deepest items first, then with new instance
+ * roots, through the refill segment.  Every workgroup derives the same
+ * counts from the same inputs; workgroup 0 publishes the new state. */
+template <int W>
+__global__ void __launch_bounds__(256) k_refill_batch(BatchArgs a, int out) {
+  __shared__ u32 s_part[256];
+  const int grid = a.n_seg - 1;
+  u32 sum = 0;
+  for (int b = threadIdx.x; b < grid; b += blockDim.x) sum += a.seg_count[out][b];
+  s_part[threadIdx.x] = sum;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) s_part[threadIdx.x] += s_part[threadIdx.x + off];
+    __syncthreads();
+  }
+  const u64 n_next = s_part[0];
+  /* pushes past spill_cap were settled as overflow and never stored */
+  const u64 top = a.sc->top[out] < a.spill_cap ? a.sc->top[out] : a.spill_cap;
+  const u32 next = a.sc->next[out];
+  u64 room = n_next < a.fill_to ? a.fill_to - n_next : 0;
+  if (room > a.fill_step) room = a.fill_step;
+  const u64 n_pop = top < room ? top : room;
+  u64 n_new = a.count - next;
+  if (n_new > room - n_pop) n_new = room - n_pop;
+  if (n_new > a.root_step) n_new = a.root_step;
+  if (a.roots_when_empty && (n_next || top)) n_new = 0;
+  const u64 seg0 = (u64)grid * a.seg_cap;
+  const QBuf dst = a.buf[out];
+  for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < n_pop + n_new;
+       j += (u64)gridDim.x * blockDim.x) {
+    const u64 slot = seg0 + j;
+    if (j < n_pop) {
+      const u64 si = top - n_pop + j;
+      copy_item<W>(a.spill, a.spill_cap, si, dst, a.stride, slot);
+      a.tag[out][slot] = a.spill_tag[si];
+    } else {
+      const u32 id = a.perm[next + (u32)(j - n_pop)];
+      const u64* r = a.roots + (u64)id * (2 * W);
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        dst.known[(u64)w * a.stride + slot] = r[w];
+        dst.up[(u64)w * a.stride + slot] = r[W + w];
+      }
+      dst.mass[slot] = 1.0;
+      a.tag[out][slot] = id; /* depth 0 */
+      u64* res = a.res + (u64)id * B_N;
+      for (int k = 0; k < B_N; ++k) res[k] = 0;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.sc->top[out ^ 1] = top - n_pop;
+    a.sc->next[out ^ 1] = next + (u32)n_new;
+    a.seg_count[out][grid] = (u32)(n_pop + n_new);
+    const u64 total = n_next + n_pop + n_new;
+    a.ctl->last_items = total;
+    if (total) {
+      a.ctl->iterations += 1;
+      if (total > a.ctl->max_width) a.ctl->max_width = total;
+    }
+  }
+}
+
 template <int W, typename NS>
-__global__ void __launch_bounds__(256) k_expand_batch(BatchArgs a, int parity, int level) {
+__global__ void __launch_bounds__(256) k_expand_batch(BatchArgs a, int parity) {
   extern __shared__ __align__(16) unsigned char smem[];
   u32* sp = (u32*)(smem + a.lay.off_extra);
   u32* s_cnt = sp + a.n_seg + 1;
   u32* part = s_cnt + 1;
   block_prefix(a.seg_count[parity], a.n_seg, sp, part);
   const u32 n_items = sp[a.n_seg];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    a.ctl->last_items = n_items;
-    if (n_items) {
-      a.ctl->iterations += 1;
-      if (n_items > a.ctl->max_width) a.ctl->max_width = n_items;
-    }
-  }
   if (n_items == 0) {
     if (threadIdx.x == 0) a.seg_count[parity ^ 1][blockIdx.x] = 0;
     return;
@@ -680,8 +795,9 @@ __global__ void __launch_bounds__(256) k_expand_batch(BatchArgs a, int parity, i
 
   const QBuf in = a.buf[parity];
   const QBuf out = a.buf[parity ^ 1];
-  const u32* iin = a.inst[parity];
-  u32* iout = a.inst[parity ^ 1];
+  const u32* tin = a.tag[parity];
+  u32* tout = a.tag[parity ^ 1];
+  u64* spill_top = &a.sc->top[parity ^ 1];
   const int lane = threadIdx.x & 63;
   const u64 wave_id = (u64)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const u64 n_waves = (u64)(gridDim.x * blockDim.x) >> 6;
@@ -694,39 +810,44 @@ __global__ void __launch_bounds__(256) k_expand_batch(BatchArgs a, int parity, i
     u64 known[W], up[W];
     double mass = 0.0;
     int k = 0;
-    bool live = false;
-    u32 id = 0;
-    u64* res = nullptr;
+    bool live = false, active = false;
+    u32 id = 0, depth = 0;
+    /* this lane's contributions, flushed per instance once per wave pass:
+     * counts packed nodes | truncated << 16 | dead << 32 | overflowed << 48 */
+    u64 cnt = 0, paths = 0;
+    u64 r2 = 0, r1 = 0, r0 = 0, e2 = 0, e1 = 0, e0 = 0, d2 = 0, d1 = 0, d0 = 0;
     if ((u64)lane < chunk && i < n_items) {
       int s = segment_of(sp, a.n_seg, (u32)base);
       while (sp[s + 1] <= i) ++s;
       const u64 slot_in = (u64)s * a.seg_cap + (i - sp[s]);
       load_item<W>(in, a.stride, slot_in, known, up, mass);
-      id = iin[slot_in];
+      const u32 tg = tin[slot_in];
+      id = tg & kTagIdMask;
+      depth = tg >> kTagIdBits;
+      active = true;
       const InstDesc d = a.desc[id];
-      res = a.res + (u64)id * B_N;
-      ex.p = a.p + d.off_e;
-      ex.q = a.q + d.off_e;
-      ex.ep = a.ep + d.off_e;
-      ex.eidm = a.eidm + d.off_eidm;
+      ex.p = (const double*)(a.tab + d.off_p);
+      ex.q = (const double*)(a.tab + d.off_q);
+      ex.sadj = (const u64*)(a.tab + d.off_sadj);
+      ex.inc = (const u64*)(a.tab + d.off_inc);
+      ex.ep = (const u32*)(a.tab + d.off_ep);
+      ex.deg = (const u32*)(a.tab + d.off_deg);
+      ex.eidm = (const u16*)(a.tab + d.off_eidm);
       ex.n = d.n;
       ex.src = d.src;
       ex.dst = d.dst;
-      atomicAdd((unsigned long long*)&res[B_NODES], 1ull);
-      atomicMax((unsigned long long*)&res[B_LEVELS], (unsigned long long)(level + 1));
+      cnt = 1;
       if (mass < a.eps) { /* main.cu:537-543 */
-        fx_add(&res[B_E], mass);
-        atomicAdd((unsigned long long*)&res[B_TRUNC], 1ull);
+        fx_limbs_add(e2, e1, e0, mass);
+        cnt |= 1ull << 16;
       } else {
-        const NS* rin = (const NS*)in.rows + slot_in;
-        for (int u = 0; u < ex.n; ++u) ex.rows[u * ex.stride] = rin[(u64)u * a.stride];
-        k = ex.find(known);
+        k = ex.find_planes(known, up);
         if (k >= 0) {
           live = true;
         } else {
-          if (k == -2) atomicAdd((unsigned long long*)&res[B_ERR], 1ull);
-          fx_add(&res[B_D], mass); /* main.cu:579-582 */
-          atomicAdd((unsigned long long*)&res[B_DEAD], 1ull);
+          if (k == -2) atomicAdd((unsigned long long*)&a.res[(u64)id * B_N + B_ERR], 1ull);
+          fx_limbs_add(d2, d1, d0, mass); /* main.cu:579-582 */
+          cnt |= 1ull << 32;
           k = 0;
         }
       }
@@ -734,12 +855,22 @@ __global__ void __launch_bounds__(256) k_expand_batch(BatchArgs a, int parity, i
     const int incl = wave_incl_scan(k);
     const int total = __shfl(incl, 63, 64);
     u32 wbase = 0;
+    u64 sbase = 0;
     if (total) {
-      if (lane == 63) wbase = atomicAdd(s_cnt, (u32)total);
+      if (lane == 63) {
+        wbase = atomicAdd(s_cnt, (u32)total);
+        /* children past the end of the segment go to the spill stack */
+        const u32 fit = wbase < a.seg_cap ? a.seg_cap - wbase : 0u;
+        if ((u32)total > fit)
+          sbase = atomicAdd((unsigned long long*)spill_top, (unsigned long long)(total - fit));
+      }
       wbase = __shfl(wbase, 63, 64);
+      sbase = __shfl(sbase, 63, 64);
     }
     if (live) {
       u32 off = wbase + (u32)(incl - k);
+      const u32 seg_end = wbase > a.seg_cap ? wbase : a.seg_cap;
+      const u32 child_tag = id | ((depth + 1u) << kTagIdBits);
       double run = mass;
       u64 pre[W];
 #pragma unroll
@@ -749,29 +880,70 @@ __global__ void __launch_bounds__(256) k_expand_batch(BatchArgs a, int parity, i
         const int wi = e >> 6;
         const u64 bit = 1ull << (e & 63);
         const double cm = run * ex.q[e];
-        if (off < a.seg_cap) {
-          u64 ck[W], cu[W];
+        u64 ck[W], cu[W];
 #pragma unroll
-          for (int w = 0; w < W; ++w) {
-            ck[w] = known[w] | pre[w] | (w == wi ? bit : 0ull);
-            cu[w] = up[w] | pre[w];
-          }
+        for (int w = 0; w < W; ++w) {
+          ck[w] = known[w] | pre[w] | (w == wi ? bit : 0ull);
+          cu[w] = up[w] | pre[w];
+        }
+        if (off < a.seg_cap) {
           store_item<W>(out, a.stride, out_base + off, ck, cu, cm);
-          iout[out_base + off] = id;
-          NS* rout = (NS*)out.rows + out_base + off;
-          for (int u = 0; u < ex.n; ++u) rout[(u64)u * a.stride] = ex.child_row(u, e);
-        } else { /* segment full: main.cu:245-252 */
-          fx_add(&res[B_E], cm);
-          atomicAdd((unsigned long long*)&res[B_PATHS], 1ull);
-          atomicAdd((unsigned long long*)&res[B_OVER], 1ull);
+          tout[out_base + off] = child_tag;
+        } else {
+          const u64 si = sbase + (off - seg_end);
+          if (si < a.spill_cap) {
+            store_item<W>(a.spill, a.spill_cap, si, ck, cu, cm);
+            a.spill_tag[si] = child_tag;
+          } else { /* spill full: main.cu:245-252 */
+            fx_limbs_add(e2, e1, e0, cm);
+            paths += 1;
+            cnt += 1ull << 48;
+          }
         }
         run = run * ex.p[e];
 #pragma unroll
         for (int w = 0; w < W; ++w)
           if (w == wi) pre[w] |= bit;
       }
-      fx_add(&res[B_R], run); /* success term, main.cu:254-262 */
-      atomicAdd((unsigned long long*)&res[B_PATHS], 1ull);
+      fx_limbs_add(r2, r1, r0, run); /* success term, main.cu:254-262 */
+      paths += 1;
+    }
+    /* flush: one group of lanes per distinct instance in the wave; the
+     * group's lowest lane adds the group's sums to that instance */
+    u64 pending = __ballot(active);
+    const u64 any_e = __ballot((e2 | e1 | e0) != 0), any_d = __ballot((d2 | d1 | d0) != 0);
+    while (pending) {
+      const int leader = __builtin_ctzll(pending);
+      const u32 lid = __shfl(id, leader, 64);
+      const bool mine = active && id == lid;
+      pending &= ~__ballot(mine);
+      const u64 gc = wave_sum_u64(mine ? cnt : 0ull);
+      const u64 gp = wave_sum_u64(mine ? paths : 0ull);
+      const u64 g_r2 = wave_sum_u64(mine ? r2 : 0ull), g_r1 = wave_sum_u64(mine ? r1 : 0ull),
+                g_r0 = wave_sum_u64(mine ? r0 : 0ull);
+      const u64 g_e2 = wave_sum_if(mine ? e2 : 0ull, any_e),
+                g_e1 = wave_sum_if(mine ? e1 : 0ull, any_e),
+                g_e0 = wave_sum_if(mine ? e0 : 0ull, any_e);
+      const u64 g_d2 = wave_sum_if(mine ? d2 : 0ull, any_d),
+                g_d1 = wave_sum_if(mine ? d1 : 0ull, any_d),
+                g_d0 = wave_sum_if(mine ? d0 : 0ull, any_d);
+      u32 gdepth = mine ? depth + 1u : 0u;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) gdepth = max(gdepth, (u32)__shfl_xor(gdepth, off, 64));
+      if (lane == leader) {
+        u64* res = a.res + (u64)lid * B_N;
+        const u64 c_nodes = gc & 0xffff, c_tr = (gc >> 16) & 0xffff, c_dead = (gc >> 32) & 0xffff,
+                  c_over = gc >> 48;
+        atomicAdd((unsigned long long*)&res[B_NODES], (unsigned long long)c_nodes);
+        atomicMax((unsigned long long*)&res[B_LEVELS], (unsigned long long)gdepth);
+        if (c_tr) atomicAdd((unsigned long long*)&res[B_TRUNC], (unsigned long long)c_tr);
+        if (c_dead) atomicAdd((unsigned long long*)&res[B_DEAD], (unsigned long long)c_dead);
+        if (c_over) atomicAdd((unsigned long long*)&res[B_OVER], (unsigned long long)c_over);
+        if (gp) atomicAdd((unsigned long long*)&res[B_PATHS], (unsigned long long)gp);
+        fx_flush(&res[B_R], g_r2, g_r1, g_r0);
+        fx_flush(&res[B_E], g_e2, g_e1, g_e0);
+        fx_flush(&res[B_D], g_d2, g_d1, g_d0);
+      }
     }
   }
   __syncthreads();
@@ -781,57 +953,48 @@ __global__ void __launch_bounds__(256) k_expand_batch(BatchArgs a, int parity, i
   }
 }
 
-/* Time-budget fold of a batch's pending items into each instance's ESP. */
+/* Time-budget fold of a batch's pending items (the level about to run,
+ * buffer `parity`, and the spill stack) into each instance's ESP. */
 __global__ void __launch_bounds__(256) k_fold_batch(BatchArgs a, int parity) {
   extern __shared__ __align__(16) unsigned char smem[];
   u32* sp = (u32*)smem;
   u32* part = sp + a.n_seg + 2;
   block_prefix(a.seg_count[parity], a.n_seg, sp, part);
   const u32 n_items = sp[a.n_seg];
+  const u64 top = a.sc->top[parity ^ 1] < a.spill_cap ? a.sc->top[parity ^ 1] : a.spill_cap;
   const u64 tid = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   const u64 nthreads = (u64)gridDim.x * blockDim.x;
-  for (u64 i = tid; i < n_items; i += nthreads) {
-    int s = segment_of(sp, a.n_seg, (u32)i);
-    const u64 slot = (u64)s * a.seg_cap + (i - sp[s]);
-    u64* res = a.res + (u64)a.inst[parity][slot] * B_N;
-    fx_add(&res[B_E], a.buf[parity].mass[slot]);
+  for (u64 i = tid; i < (u64)n_items + top; i += nthreads) {
+    u32 tg;
+    double m;
+    if (i < n_items) {
+      int s = segment_of(sp, a.n_seg, (u32)i);
+      const u64 slot = (u64)s * a.seg_cap + (i - sp[s]);
+      tg = a.tag[parity][slot];
+      m = a.buf[parity].mass[slot];
+    } else {
+      tg = a.spill_tag[i - n_items];
+      m = a.spill.mass[i - n_items];
+    }
+    u64* res = a.res + (u64)(tg & kTagIdMask) * B_N;
+    fx_add(&res[B_E], m);
     atomicAdd((unsigned long long*)&res[B_FOLD], 1ull);
   }
 }
 
-/* Seed a batch: zero results and counters, scatter the roots (one per
- * instance) round-robin over the segments. */
-__global__ void __launch_bounds__(256) k_init_batch(BatchArgs a, const u64* roots, int count,
-                                                    int W, int nmax, int ns_bytes) {
+/* Reset a batch pass: empty segments and spill, admission from perm[0]. */
+__global__ void __launch_bounds__(256) k_init_batch(BatchArgs a) {
   const u64 tid = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   const u64 stride = (u64)gridDim.x * blockDim.x;
-  for (u64 i = tid; i < (u64)count * B_N; i += stride) a.res[i] = 0;
   for (u64 s = tid; s < (u64)a.n_seg; s += stride) {
-    const u64 c = (u64)count;
-    a.seg_count[0][s] = (u32)(c / a.n_seg + (s < c % a.n_seg ? 1 : 0));
+    a.seg_count[0][s] = 0u;
     a.seg_count[1][s] = 0u;
   }
   if (tid == 0) {
     LevelCtl z = {};
     *a.ctl = z;
-    a.ctl->last_items = (u64)count;
-  }
-  const int rec = 2 * W + nmax;
-  for (u64 i = tid; i < (u64)count; i += stride) {
-    const u64 slot = (i % a.n_seg) * a.seg_cap + i / a.n_seg;
-    const u64* r = roots + i * rec;
-    for (int w = 0; w < W; ++w) {
-      a.buf[0].known[(u64)w * a.stride + slot] = r[w];
-      a.buf[0].up[(u64)w * a.stride + slot] = r[W + w];
-    }
-    for (int u = 0; u < nmax; ++u) {
-      if (ns_bytes == 4)
-        ((u32*)a.buf[0].rows)[(u64)u * a.stride + slot] = (u32)r[2 * W + u];
-      else
-        ((u64*)a.buf[0].rows)[(u64)u * a.stride + slot] = r[2 * W + u];
-    }
-    a.buf[0].mass[slot] = 1.0;
-    a.inst[0][slot] = (u32)i;
+    StreamCtl c = {};
+    *a.sc = c;
   }
 }
 
@@ -1011,12 +1174,13 @@ struct Kern {
   static void* level() { return (void*)&k_expand_level<W, NS>; }
   static void* items() { return (void*)&k_expand_items<W, NS>; }
   static void* batch() { return (void*)&k_expand_batch<W, NS>; }
+  static void* refill() { return (void*)&k_refill_batch<W>; }
 };
 
 typedef void* (*KernelGetter)();
 
 struct Variant {
-  KernelGetter level, items, batch;
+  KernelGetter level, items, batch, refill;
   int ns_bytes;
 };
 
@@ -1025,10 +1189,11 @@ Variant variant_for_w(int W) {
   switch (W) {
 #define NR_W(X) \
   case X:      \
-    return Variant{&Kern<X, NS>::level, &Kern<X, NS>::items, &Kern<X, NS>::batch, (int)sizeof(NS)};
+    return Variant{&Kern<X, NS>::level, &Kern<X, NS>::items, &Kern<X, NS>::batch,               \
+                   &Kern<X, NS>::refill, (int)sizeof(NS)};
     NR_W(1) NR_W(2) NR_W(3) NR_W(4) NR_W(5) NR_W(6) NR_W(7) NR_W(8)
 #undef NR_W
-    default: return Variant{nullptr, nullptr, nullptr, 0};
+    default: return Variant{nullptr, nullptr, nullptr, nullptr, 0};
   }
 }
 
@@ -1327,9 +1492,17 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   return NR_OK;
 }
 
-/* Batched solve: every instance's root in one level-order queue; each item
- * carries its instance id, instance tables are read from HBM, per-instance
- * sums are exact fixed-point atomics (fx_add). */
+/* Batched solve.  All instances share one level-synchronous queue; an item
+ * is its planes, its mass and a tag (instance id, tree depth); adjacency is
+ * derived from the planes and the instance's tables in HBM, and
+ * per-instance sums are exact fixed-point integers, added once per wave and
+ * instance.  Children that do not fit their workgroup's segment go to a
+ * spill stack; between two levels a refill launch tops the next level up to
+ * a quarter of the capacity, first from the top of the stack (the deepest
+ * items, so the stack stays shallow), then with new instance roots.  Nothing
+ * is dropped unless the stack itself is full; an instance that overflowed
+ * then is solved again alone, so its counts are those of an unbounded queue
+ * unless it overflows on its own. */
 int solve_batch(Workspace& ws, const nr_graph* const* gs, int count, const nr_config* cfg,
                 nr_result* outs) {
   int Wmax = 1, Nmax = 1;
@@ -1337,6 +1510,7 @@ int solve_batch(Workspace& ws, const nr_graph* const* gs, int count, const nr_co
     const nr_graph* g = gs[i];
     if (!g) return NR_ERR_ARG;
     if (!g->symmetric || g->n > 64 || mask_words(g) > 8) return NR_ERR_UNSUPPORTED;
+    if ((u32)g->e >= kMaxDepth) return NR_ERR_UNSUPPORTED; /* depth <= E fits the tag */
     Wmax = std::max(Wmax, mask_words(g));
     Nmax = std::max(Nmax, g->n);
   }
@@ -1345,72 +1519,74 @@ int solve_batch(Workspace& ws, const nr_graph* const* gs, int count, const nr_co
   const int W = Wmax;
   const void* kfun = (const void*)var.batch();
   const int cus = ws.prop.multiProcessorCount;
-  int block = 256;
-  while (block >= 64 && make_layout(Nmax, 1, W, block, var.ns_bytes, 4 * (1026 + block), false)
-                                .total_bytes > 64 * 1024)
-    block /= 2;
-  if (block < 64) return NR_ERR_UNSUPPORTED;
-  NR_HIP(hipFuncSetAttribute(kfun, hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
-  Layout lay0 = make_layout(Nmax, 1, W, block, var.ns_bytes, 4 * (block + 2), false);
-  int per_cu0 = 0;
-  NR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu0, kfun, block, lay0.total_bytes));
-  if (per_cu0 < 1) return NR_ERR_UNSUPPORTED;
-  int grid = cfg->grid_blocks > 0 ? cfg->grid_blocks : per_cu0 * cus;
-  const int extra = 4 * (grid + 2 + block);
-  Layout lay = make_layout(Nmax, 1, W, block, var.ns_bytes, extra, false);
-  if (lay.total_bytes > 64 * 1024) return NR_ERR_UNSUPPORTED;
+  const int lds_max = 64 * 1024;
+  NR_HIP(hipFuncSetAttribute(kfun, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+  /* block: requested, else the largest of 256/128/64 whose per-lane slices
+   * leave room for 2 workgroups per CU */
+  int block = cfg->block_size > 0 ? cfg->block_size : 256;
+  if (block % 64 || block > 256) return NR_ERR_ARG;
+  auto lay_for = [&](int blk, int grid) {
+    return make_layout(Nmax, 1, W, blk, var.ns_bytes, 4 * (grid + 3 + blk), false);
+  };
+  if (cfg->block_size <= 0)
+    while (block > 64 && lay_for(block, 4 * cus).total_bytes > lds_max) block /= 2;
   int per_cu = 0;
-  NR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfun, block, lay.total_bytes));
+  NR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfun, block,
+                                                      lay_for(block, 4 * cus).total_bytes));
   if (per_cu < 1) return NR_ERR_UNSUPPORTED;
-  if (cfg->grid_blocks <= 0) grid = std::min(grid, per_cu * cus);
+  const int grid = cfg->grid_blocks > 0 ? cfg->grid_blocks : per_cu * cus;
+  const Layout lay = lay_for(block, grid);
+  if (lay.total_bytes > lds_max) return NR_ERR_UNSUPPORTED;
+  const int n_seg = grid + 1;
   const u64 cap0 = cfg->queue_capacity > 0 ? (u64)cfg->queue_capacity : (1ull << 20);
-  const u64 cap = std::max<u64>(cap0, (u64)count);
-  const u64 seg_cap64 = std::max<u64>(1, (cap + grid - 1) / grid + 1);
-  if (seg_cap64 * (u64)grid >= (1ull << 32)) return NR_ERR_ARG;
+  const u64 seg_cap64 = std::max<u64>(64, (cap0 + grid - 1) / grid);
+  const u64 cap = seg_cap64 * (u64)grid;
+  const u64 fill_cap = std::max<u64>(seg_cap64, cap / 4); /* refill segment */
+  const u64 slots = cap + fill_cap;
+  if (slots >= (1ull << 32)) return NR_ERR_ARG;
   const u32 seg_cap = (u32)seg_cap64;
-  const u64 slots = seg_cap64 * (u64)grid;
+  if ((u64)count > kMaxBatch) return NR_ERR_ARG;
 
-  /* instance tables */
+  /* one buffer: desc | instance tables | roots | perm | perm2 */
   std::vector<InstDesc> desc(count);
-  std::vector<double> hp, hq;
-  std::vector<u32> hep;
-  std::vector<u16> heidm;
-  const int rec = 2 * W + Nmax;
-  std::vector<u64> roots((size_t)count * rec, 0);
+  std::vector<unsigned char> tabs;
+  std::vector<u64> roots((size_t)count * 2 * W, 0);
   for (int i = 0; i < count; ++i) {
     const nr_graph* g = gs[i];
+    TableBlob tb;
+    pack_tables(g, W, tb);
+    const size_t base = (tabs.size() + 255) / 256 * 256;
+    if (base + tb.bytes.size() >= (1ull << 32)) return NR_ERR_ARG;
+    tabs.resize(base + tb.bytes.size(), 0);
+    std::memcpy(&tabs[base], tb.bytes.data(), tb.bytes.size());
     InstDesc d;
+    std::memset(&d, 0, sizeof d);
     d.n = g->n;
     d.e = g->e;
     d.src = g->src;
     d.dst = g->dst;
-    d.off_e = (int)hp.size();
-    d.off_eidm = (int)heidm.size();
+    d.off_p = (u32)(base + tb.off_p);
+    d.off_q = (u32)(base + tb.off_q);
+    d.off_sadj = (u32)(base + tb.off_sadj);
+    d.off_inc = (u32)(base + tb.off_inc);
+    d.off_ep = (u32)(base + tb.off_ep);
+    d.off_deg = (u32)(base + tb.off_deg);
+    d.off_eidm = (u32)(base + tb.off_eidm);
     desc[i] = d;
-    for (int e = 0; e < g->e; ++e) {
-      hp.push_back(g->p[e]);
-      hq.push_back(g->q[e]);
-      hep.push_back((u32)g->ea[e] | ((u32)g->eb[e] << 16));
-    }
-    const size_t b = heidm.size();
-    heidm.resize(b + (size_t)g->n * g->n, 0xffff);
-    for (int e = 0; e < g->e; ++e) {
-      heidm[b + (size_t)g->ea[e] * g->n + g->eb[e]] = (u16)e;
-      heidm[b + (size_t)g->eb[e] * g->n + g->ea[e]] = (u16)e;
-    }
     Root r = root_item(g, mask_words(g));
-    u64* dst = &roots[(size_t)i * rec];
+    u64* dst = &roots[(size_t)i * 2 * W];
     for (int w = 0; w < W; ++w) {
       dst[w] = w < mask_words(g) ? r.known[w] : 0;
       dst[W + w] = w < mask_words(g) ? r.up[w] : 0;
     }
-    for (int u = 0; u < g->n; ++u) dst[2 * W + u] = r.rows[u];
   }
-  hp.push_back(0.0);
-  hq.push_back(0.0);
-  hep.push_back(0);
-  heidm.push_back(0);
-  /* one table buffer: desc | p | q | ep | eidm | roots */
+  /* admission order: larger instances first, so the longest subtrees start
+   * early and the tail of the stream holds the small ones */
+  std::vector<u32> perm(count);
+  for (int i = 0; i < count; ++i) perm[i] = (u32)i;
+  std::stable_sort(perm.begin(), perm.end(), [&](u32 x, u32 y) {
+    return (long long)gs[x]->e * gs[x]->n > (long long)gs[y]->e * gs[y]->n;
+  });
   size_t o = 0;
   auto take = [&](size_t bytes) {
     o = (o + 255) / 256 * 256;
@@ -1418,16 +1594,14 @@ int solve_batch(Workspace& ws, const nr_graph* const* gs, int count, const nr_co
     o += bytes;
     return at;
   };
-  const size_t o_desc = take(sizeof(InstDesc) * count), o_p = take(8 * hp.size()),
-               o_q = take(8 * hq.size()), o_ep = take(4 * hep.size()),
-               o_eidm = take(2 * heidm.size()), o_roots = take(8 * roots.size());
+  const size_t o_desc = take(sizeof(InstDesc) * count), o_tabs = take(tabs.size()),
+               o_roots = take(8 * roots.size()), o_perm = take(4 * (size_t)count),
+               o_perm2 = take(4 * (size_t)count);
   std::vector<unsigned char> blob(o, 0);
   std::memcpy(&blob[o_desc], desc.data(), sizeof(InstDesc) * count);
-  std::memcpy(&blob[o_p], hp.data(), 8 * hp.size());
-  std::memcpy(&blob[o_q], hq.data(), 8 * hq.size());
-  std::memcpy(&blob[o_ep], hep.data(), 4 * hep.size());
-  std::memcpy(&blob[o_eidm], heidm.data(), 2 * heidm.size());
+  std::memcpy(&blob[o_tabs], tabs.data(), tabs.size());
   std::memcpy(&blob[o_roots], roots.data(), 8 * roots.size());
+  std::memcpy(&blob[o_perm], perm.data(), 4 * perm.size());
   int rc;
   if ((rc = ws.grow(&ws.tab, &ws.tab_bytes, blob.size()))) return rc;
   hipStream_t st = ws.stream;
@@ -1438,94 +1612,164 @@ int solve_batch(Workspace& ws, const nr_graph* const* gs, int count, const nr_co
   std::memset(&a, 0, sizeof a);
   a.lay = lay;
   a.desc = (const InstDesc*)(tb + o_desc);
-  a.p = (const double*)(tb + o_p);
-  a.q = (const double*)(tb + o_q);
-  a.ep = (const u32*)(tb + o_ep);
-  a.eidm = (const u16*)(tb + o_eidm);
+  a.tab = tb + o_tabs;
+  a.roots = (const u64*)(tb + o_roots);
   a.seg_cap = seg_cap;
-  a.n_seg = grid;
+  a.n_seg = n_seg;
   a.stride = slots;
   a.eps = cfg->truncation_eps > 0 ? cfg->truncation_eps : 0.0;
-  const size_t rows_bytes = ((size_t)Nmax * slots * var.ns_bytes + 255) / 256 * 256;
-  const size_t inst_bytes = (slots * 4 + 255) / 256 * 256;
-  const size_t per_buf = (size_t)(2 * W + 1) * slots * 8 + rows_bytes + inst_bytes;
-  if ((rc = ws.grow(&ws.q, &ws.q_bytes, 2 * per_buf))) return rc;
+  a.spill_cap = 4 * cap; /* the stack holds what the levels cannot */
+  const size_t tag_bytes = (slots * 4 + 255) / 256 * 256;
+  const size_t per_buf = (size_t)(2 * W + 1) * slots * 8 + tag_bytes;
+  const size_t spill_bytes =
+      (size_t)(2 * W + 1) * a.spill_cap * 8 + (a.spill_cap * 4 + 255) / 256 * 256;
+  if ((rc = ws.grow(&ws.q, &ws.q_bytes, 2 * per_buf + spill_bytes))) return rc;
   for (int b = 0; b < 2; ++b) {
     unsigned char* base = (unsigned char*)ws.q + b * per_buf;
     a.buf[b].known = (u64*)base;
     a.buf[b].up = (u64*)base + (size_t)W * slots;
     a.buf[b].mass = (double*)((u64*)base + (size_t)2 * W * slots);
-    a.buf[b].rows = (void*)((u64*)base + (size_t)(2 * W + 1) * slots);
-    a.inst[b] = (u32*)((unsigned char*)a.buf[b].rows + rows_bytes);
+    a.buf[b].rows = nullptr;
+    a.tag[b] = (u32*)((u64*)base + (size_t)(2 * W + 1) * slots);
   }
-  const size_t meta_bytes = sizeof(u64) * (size_t)count * B_N + sizeof(u32) * (2 * (size_t)grid + 8);
+  {
+    unsigned char* base = (unsigned char*)ws.q + 2 * per_buf;
+    a.spill.known = (u64*)base;
+    a.spill.up = (u64*)base + (size_t)W * a.spill_cap;
+    a.spill.mass = (double*)((u64*)base + (size_t)2 * W * a.spill_cap);
+    a.spill.rows = nullptr;
+    a.spill_tag = (u32*)((u64*)base + (size_t)(2 * W + 1) * a.spill_cap);
+  }
+  const size_t meta_bytes = sizeof(u64) * (size_t)count * B_N + sizeof(StreamCtl) +
+                            sizeof(u32) * (2 * (size_t)n_seg + 8);
   if ((rc = ws.grow(&ws.meta, &ws.meta_bytes, meta_bytes))) return rc;
   a.res = (u64*)ws.meta;
-  a.seg_count[0] = (u32*)(a.res + (size_t)count * B_N);
-  a.seg_count[1] = a.seg_count[0] + grid;
+  a.sc = (StreamCtl*)(a.res + (size_t)count * B_N);
+  a.seg_count[0] = (u32*)(a.sc + 1);
+  a.seg_count[1] = a.seg_count[0] + n_seg;
   a.ctl = ws.ctl;
-  {
-    const u64* droots = (const u64*)(tb + o_roots);
-    int cnt = count, Wi = W, nm = Nmax, nsb = var.ns_bytes;
-    void* args[] = {&a, (void*)&droots, &cnt, &Wi, &nm, &nsb};
-    NR_HIP(hipLaunchKernel((const void*)&k_init_batch, dim3(256), dim3(256), args, 0, st));
-  }
+  const void* kref = (const void*)var.refill();
+  const int refill_grid = 2 * cus;
+
   hipEvent_t ev_start, ev_stop;
   if ((rc = ws.event(0, &ev_start)) || (rc = ws.event(1, &ev_stop))) return rc;
+  const bool timing = (cfg->flags & NR_FLAG_TIME_LAUNCHES) != 0;
+  double kernel_ms = 0.0;
+  for (int c = 0; timing && c < 16; ++c) { /* events 2.. bracket one chunk's launches */
+    hipEvent_t e0, e1;
+    if ((rc = ws.event(2 + 2 * c, &e0)) || (rc = ws.event(3 + 2 * c, &e1))) return rc;
+  }
   auto t_begin = std::chrono::steady_clock::now();
   NR_HIP(hipEventRecord(ev_start, st));
-  int level = 0, chunk = 8;
-  bool finished = false, deadline = false;
-  while (!finished) {
-    for (int c = 0; c < chunk; ++c, ++level) {
-      int parity = level & 1, lv = level;
-      void* args[] = {&a, &parity, &lv};
-      NR_HIP(hipLaunchKernel(kfun, dim3(grid), dim3(block), args, lay.total_bytes, st));
+  std::vector<u64> res((size_t)count * B_N, 0);
+  std::vector<char> admitted(count, 0), rerun(count, 0);
+  bool deadline = false;
+  long long launches = 0, max_width = 0;
+  std::vector<u32> todo(perm);
+  /* pass 0 streams every instance; an instance that still overflowed (spill
+   * stack full) is solved again in a pass of its own */
+  for (int pass = 0; !todo.empty() && !deadline; ++pass) {
+    const bool alone = pass >= 1;
+    a.perm = (const u32*)(tb + (pass ? o_perm2 : o_perm));
+    if (pass)
+      NR_HIP(hipMemcpyAsync((void*)a.perm, todo.data(), 4 * todo.size(), hipMemcpyHostToDevice,
+                            st));
+    a.count = (u32)todo.size();
+    a.fill_step = (u32)fill_cap;
+    a.fill_to = (u32)std::max<u64>(cap / 4, 1);
+    a.root_step = alone ? 1u : (u32)fill_cap;
+    a.roots_when_empty = alone ? 1 : 0;
+    {
+      void* args[] = {&a};
+      NR_HIP(hipLaunchKernel((const void*)&k_init_batch, dim3(64), dim3(256), args, 0, st));
+      int out0 = 0;
+      void* rargs[] = {&a, &out0};
+      NR_HIP(hipLaunchKernel(kref, dim3(refill_grid), dim3(256), rargs, 0, st));
     }
-    NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
+    int level = 0, chunk = 8;
+    bool finished = false;
+    StreamCtl hsc;
+    while (!finished) {
+      for (int c = 0; c < chunk; ++c, ++level) {
+        int parity = level & 1, nxt = parity ^ 1;
+        void* args[] = {&a, &parity};
+        if (timing) NR_HIP(hipEventRecord(ws.events[2 + 2 * c], st));
+        NR_HIP(hipLaunchKernel(kfun, dim3(grid), dim3(block), args, lay.total_bytes, st));
+        if (timing) NR_HIP(hipEventRecord(ws.events[3 + 2 * c], st));
+        void* rargs[] = {&a, &nxt};
+        NR_HIP(hipLaunchKernel(kref, dim3(refill_grid), dim3(256), rargs, 0, st));
+      }
+      launches += chunk;
+      NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
+      NR_HIP(hipMemcpyAsync(&hsc, a.sc, sizeof hsc, hipMemcpyDeviceToHost, st));
+      NR_HIP(hipStreamSynchronize(st));
+      for (int c = 0; timing && c < chunk; ++c) {
+        float x = 0.f;
+        NR_HIP(hipEventElapsedTime(&x, ws.events[2 + 2 * c], ws.events[3 + 2 * c]));
+        kernel_ms += x;
+      }
+      max_width = std::max<long long>(max_width, (long long)ws.hctl->max_width);
+      if (ws.hctl->last_items == 0) finished = true;
+      if (!finished && cfg->time_budget_s > 0 &&
+          std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count() >=
+              cfg->time_budget_s) {
+        int parity = level & 1;
+        void* args[] = {&a, &parity};
+        NR_HIP(hipLaunchKernel((const void*)&k_fold_batch, dim3(grid), dim3(block), args,
+                               4 * (n_seg + 2 + block), st));
+        deadline = true;
+        finished = true;
+      }
+      chunk = std::min(chunk * 2, 16);
+    }
+    const u32 next_seen = hsc.next[(level & 1) ^ 1];
+    std::vector<u64> r2((size_t)count * B_N);
+    NR_HIP(hipMemcpyAsync(r2.data(), a.res, sizeof(u64) * r2.size(), hipMemcpyDeviceToHost, st));
     NR_HIP(hipStreamSynchronize(st));
-    if (ws.hctl->last_items == 0) finished = true;
-    if (!finished && cfg->time_budget_s > 0 &&
-        std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count() >=
-            cfg->time_budget_s) {
-      int parity = level & 1;
-      void* args[] = {&a, &parity};
-      NR_HIP(hipLaunchKernel((const void*)&k_fold_batch, dim3(grid), dim3(block), args,
-                             4 * (grid + 2 + block), st));
-      deadline = true;
-      finished = true;
+    std::vector<u32> again;
+    for (u32 j = 0; j < next_seen && j < todo.size(); ++j) {
+      const u32 id = todo[j];
+      admitted[id] = 1;
+      std::memcpy(&res[(size_t)id * B_N], &r2[(size_t)id * B_N], sizeof(u64) * B_N);
+      if (pass) rerun[id] = 1;
+      if (!alone && !deadline && r2[(size_t)id * B_N + B_OVER]) again.push_back(id);
     }
-    chunk = std::min(chunk * 2, 16);
+    todo.swap(again);
   }
   NR_HIP(hipEventRecord(ev_stop, st));
-  std::vector<u64> res((size_t)count * B_N);
-  NR_HIP(hipMemcpyAsync(res.data(), a.res, sizeof(u64) * res.size(), hipMemcpyDeviceToHost, st));
-  NR_HIP(hipStreamSynchronize(st));
+  NR_HIP(hipEventSynchronize(ev_stop));
   float ms = 0.f;
   NR_HIP(hipEventElapsedTime(&ms, ev_start, ev_stop));
   auto fx = [](const u64* w) {
     return (double)w[1] * 2.3283064365386963e-10 + (double)w[0] * 1.2621774483536189e-29;
   };
   for (int i = 0; i < count; ++i) {
-    const u64* r = &res[(size_t)i * B_N];
-    if (r[B_ERR]) return NR_ERR_DEVICE;
     nr_result& o2 = outs[i];
     std::memset(&o2, 0, sizeof o2);
-    o2.R = fx(&r[B_R]);
-    o2.ESP = fx(&r[B_E]);
-    o2.D = fx(&r[B_D]);
-    o2.nodes = r[B_NODES];
-    o2.paths = r[B_PATHS];
-    o2.dead = r[B_DEAD];
-    o2.truncated = r[B_TRUNC];
-    o2.overflowed = r[B_OVER];
-    o2.folded = r[B_FOLD];
-    o2.iterations = (long long)r[B_LEVELS];
-    o2.max_queue_items = (long long)ws.hctl->max_width;
-    o2.item_bytes = 16LL * W + 8;
+    const u64* r = &res[(size_t)i * B_N];
+    if (admitted[i] && r[B_ERR]) return NR_ERR_DEVICE;
+    if (admitted[i]) {
+      o2.R = fx(&r[B_R]);
+      o2.ESP = fx(&r[B_E]);
+      o2.D = fx(&r[B_D]);
+      o2.nodes = r[B_NODES];
+      o2.paths = r[B_PATHS];
+      o2.dead = r[B_DEAD];
+      o2.truncated = r[B_TRUNC];
+      o2.overflowed = r[B_OVER];
+      o2.folded = r[B_FOLD];
+      o2.iterations = (long long)r[B_LEVELS];
+    } else { /* never started before the deadline: the whole mass is the gap */
+      o2.ESP = 1.0;
+    }
+    o2.max_queue_items = max_width;
+    o2.item_bytes = 16LL * W + 12;
     o2.elapsed_ms = ms;
+    o2.kernel_launches = launches;
+    o2.kernel_ms = timing ? kernel_ms : 0.0;
     o2.deadline_hit = deadline ? 1 : 0;
-    o2.overflowed_flag = r[B_OVER] ? 1 : 0;
+    o2.overflowed_flag = o2.overflowed ? 1 : 0;
+    o2.rerun = rerun[i];
     o2.grid_blocks = grid;
     o2.block_size = block;
     o2.lds_bytes = lay.total_bytes;

@@ -56,6 +56,26 @@ def test_er_batch_vs_oracle(count, eps):
         assert abs(r["R"] + r["ESP"] + r["D"] - 1.0) <= 1e-12
 
 
+def test_er_stream_spill_vs_oracle():
+    """A queue far smaller than the batch (64 workgroups x 256 slots; the
+    batch's widest level is ~40x that): children that do not fit go to the
+    spill stack and come back through the refill segment, so levels are
+    interleaved across instances and depths.  Every instance still matches
+    the oracle exactly, tree depth included."""
+    print("case ER stream 512 eps=1e-7 cap=16384 grid=64", flush=True)
+    count, eps = 512, 1e-7
+    pairs = [er_pair(i) for i in range(count)]
+    cfg = nr.make_config(truncation_eps=eps, queue_capacity=1 << 14)
+    cfg.grid_blocks = 64
+    res = nr.solve_batch([p[0] for p in pairs], cfg)
+    for (g, og, s, t), r in zip(pairs, res):
+        o = og.solve(s, t, order=ORDER_LEVEL, trunc_eps=eps)
+        assert r["overflowed"] == 0
+        check_inst(r, o)
+    assert res[0]["max_queue_items"] <= (1 << 14) + (1 << 12)
+    assert res[0]["kernel_launches"] > 2 * max(r["iterations"] for r in res)
+
+
 def test_batch_matches_single_solves():
     """Mixed sizes and mask widths in one batch: files (N <= 32, one mask
     word) next to ER graphs up to N = 40 (two-word rows); each result equals
@@ -93,7 +113,10 @@ def test_batch_capacity_bracket():
     every instance still brackets its exact R."""
     print("case batch capacity bracket", flush=True)
     pairs = [er_pair(i) for i in range(8)]
-    res = nr.solve_batch([p[0] for p in pairs], truncation_eps=1e-9, queue_capacity=4096)
+    # 4 workgroups x 64-item segments: each instance overflows even alone
+    cfg = nr.make_config(truncation_eps=1e-9, queue_capacity=256)
+    cfg.grid_blocks = 4
+    res = nr.solve_batch([p[0] for p in pairs], cfg)
     over = 0
     for (g, og, s, t), r in zip(pairs, res):
         # the oracle's own bracket at a finer eps; both contain the exact R
