@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstring>
+
 namespace nr {
 
 typedef unsigned long long u64;
@@ -96,12 +98,25 @@ inline Layout make_layout(int n, int e, int w, int block, int ns_bytes,
 struct DevGraph {
   const double* p;
   const double* q;
-  const u64* sadj;  /* [n] static neighbour set */
-  const u64* inc;   /* [n*w] incident edge set */
+  const u64* sadj;  /* [n] static neighbour set (n <= 64) */
+  const u64* inc;   /* [n*w] incident edge set (n <= 64) */
   const u32* ep;    /* [e] endpoints a | b << 16 */
-  const u32* deg;   /* [n] incident edge count */
-  const u16* eidm;  /* [n*n] edge id of (u,v), 0xffff if none */
+  const u32* deg;   /* [n] incident edge count (n <= 64) */
+  const u16* eidm;  /* [n*n] edge id of (u,v), 0xffff if none (n <= 64) */
+  const int* rp;    /* [n+1] out-arc rows */
+  const u32* arc;   /* out-arcs v | e << 16, each row sorted by (v, e) */
+  const int* irp;   /* [n+1] in-arc rows */
+  const u32* iarc;  /* in-arcs u | e << 16, each row sorted by (u, e) */
   int n, e, w, src, dst;
+};
+
+/* One queue buffer in structure-of-arrays form: plane w of item i at
+ * known[w * stride + i]; rows (when items carry them) likewise per node. */
+struct QBuf {
+  u64* known; /* [w][stride] */
+  u64* up;    /* [w][stride] */
+  void* rows; /* [n][stride] node sets (u32 or u64), or null */
+  double* mass;
 };
 
 /* Cooperative copy of the tables into LDS (all threads of the block). */
@@ -175,6 +190,36 @@ struct Expander {
     n = L.n;
     src = src_;
     dst = dst_;
+  }
+
+  /* ---- expander interface used by the level and items kernels ---- */
+  static constexpr bool kStage = true; /* graph tables staged in LDS */
+  static constexpr int kRowBytes = (int)sizeof(NS);
+  __device__ void bind_graph(unsigned char* smem, const Layout& L, const DevGraph& g, int slot) {
+    bind(smem, L, slot, g.src, g.dst);
+  }
+  /* the item's rows into this lane's slice */
+  __device__ void load(const QBuf& in, u64 qstride, u64 slot) const {
+    const NS* rin = (const NS*)in.rows + slot;
+    for (int u = 0; u < n; ++u) rows[u * stride] = rin[(u64)u * qstride];
+  }
+  /* rows of an item given by its planes alone */
+  __device__ void derive(const u64 (&known)[W], const u64 (&up)[W]) const {
+    u64 Dn[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i) Dn[i] = known[i] & ~up[i];
+    build_rows(Dn);
+  }
+  __device__ int find_item(const u64 (&known)[W], const u64 (&)[W]) const { return find(known); }
+  __device__ void store_child(const QBuf& out, u64 qstride, u64 slot, int e) const {
+    NS* rout = (NS*)out.rows + slot;
+    for (int u = 0; u < n; ++u) rout[(u64)u * qstride] = child_row(u, e);
+  }
+  __device__ void carry(const QBuf& in, u64 slot_in, const QBuf& out, u64 slot_out,
+                        u64 qstride) const {
+    const NS* rin = (const NS*)in.rows + slot_in;
+    NS* rout = (NS*)out.rows + slot_out;
+    for (int u = 0; u < n; ++u) rout[(u64)u * qstride] = rin[(u64)u * qstride];
   }
 
   /* Allowed neighbours of u (edges not FAILED): the static neighbour set
@@ -329,6 +374,170 @@ struct Expander {
     if (u == a) r &= ~((NS)1 << b);
     if (u == b) r &= ~((NS)1 << a);
     return r;
+  }
+};
+
+/* NW-word node sets held in VGPRs; every index is resolved by unrolled
+ * compares so the arrays never spill to scratch. */
+template <int NW>
+__device__ inline bool bs_test(const u64 (&s)[NW], int v) {
+  u64 w = 0;
+#pragma unroll
+  for (int k = 0; k < NW; ++k)
+    if (k == (v >> 6)) w = s[k];
+  return (w >> (v & 63)) & 1ull;
+}
+template <int NW>
+__device__ inline void bs_set(u64 (&s)[NW], int v) {
+#pragma unroll
+  for (int k = 0; k < NW; ++k)
+    if (k == (v >> 6)) s[k] |= 1ull << (v & 63);
+}
+
+/* LDS layout of the CSR expander: per-lane node levels (one byte per node,
+ * four nodes per 32-bit word, laid out [word][lane]) and the unknown-hop
+ * stack ([slot][lane] u16). */
+inline Layout make_layout_csr(int n, int block, int extra_bytes = 0) {
+  Layout L;
+  std::memset(&L, 0, sizeof L);
+  L.n = n;
+  L.block = block;
+  L.ns_bytes = 0;
+  int o = 0;
+  auto take = [&](int bytes, int align) {
+    o = (o + align - 1) / align * align;
+    int at = o;
+    o += bytes;
+    return at;
+  };
+  L.table_bytes = 0;
+  L.off_extra = take(extra_bytes, 16);
+  L.off_lvl = take(4 * ((n + 3) / 4) * block, 16);
+  L.off_rows = L.off_lvl;
+  L.off_uh = take(2 * n * block, 16);
+  L.total_bytes = (o + 15) / 16 * 16;
+  return L;
+}
+
+/* Per-lane expander over the CSR arcs, for graphs wider than a 64-bit node
+ * set or with one-way arcs (workingPad 'D' files).  Same search as Expander:
+ * BFS from src over arcs whose edge is not FAILED, a node marked visited
+ * when first discovered; the parent of v is the lowest-index node of the
+ * previous level with an allowed arc into v, lowest edge id on ties (the
+ * in-arc rows are sorted by (u, e), so the first match is the rule's). */
+template <int W, int NW>
+struct CsrExpander {
+  static constexpr bool kStage = false;
+  static constexpr int kRowBytes = 0;
+  const double* p;
+  const double* q;
+  const int* rp;
+  const u32* arc;
+  const int* irp;
+  const u32* iarc;
+  unsigned char* lvb; /* this lane's level bytes: node u at (u>>2)*4*stride + (u&3) */
+  u16* uh;            /* this lane's unknown-hop stack, stride `stride` */
+  int stride, n, src, dst;
+
+  __device__ void bind_graph(unsigned char* smem, const Layout& L, const DevGraph& g, int slot) {
+    p = g.p;
+    q = g.q;
+    rp = g.rp;
+    arc = g.arc;
+    irp = g.irp;
+    iarc = g.iarc;
+    lvb = smem + L.off_lvl + 4 * slot;
+    uh = (u16*)(smem + L.off_uh) + slot;
+    stride = L.block;
+    n = g.n;
+    src = g.src;
+    dst = g.dst;
+  }
+  __device__ void load(const QBuf&, u64, u64) const {}
+  __device__ void derive(const u64 (&)[W], const u64 (&)[W]) const {}
+  __device__ void store_child(const QBuf&, u64, u64, int) const {}
+  __device__ void carry(const QBuf&, u64, const QBuf&, u64, u64) const {}
+
+  __device__ int level_of(int u) const { return lvb[(u >> 2) * 4 * stride + (u & 3)]; }
+  __device__ void set_level(int u, int l) const {
+    lvb[(u >> 2) * 4 * stride + (u & 3)] = (unsigned char)l;
+  }
+  __device__ static bool allowed(const u64 (&known)[W], const u64 (&up)[W], int e) {
+    const int wi = e >> 6;
+    const u64 b = 1ull << (e & 63);
+    return !((pick<W>(known, wi) & b) && !(pick<W>(up, wi) & b));
+  }
+
+  /* Pushes the path's UNKNOWN hops on uh[] in dst->src order; returns their
+   * number, -1 if dst is unreachable, -2 on an internal inconsistency. */
+  __device__ int find_item(const u64 (&known)[W], const u64 (&up)[W]) const {
+    if (src == dst) return 0;
+    for (int i = 0; i < (n + 3) / 4; ++i) *(u32*)(lvb + i * 4 * stride) = 0xffffffffu;
+    u64 vis[NW], F[NW];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) vis[k] = F[k] = 0;
+    bs_set<NW>(vis, src);
+    bs_set<NW>(F, src);
+    set_level(src, 0);
+    int L = 0;
+    for (;;) {
+      u64 nx[NW];
+#pragma unroll
+      for (int k = 0; k < NW; ++k) nx[k] = 0;
+      bool any = false, hit = false;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) {
+        u64 fw = F[k];
+        while (fw && !hit) {
+          const int u = k * 64 + __builtin_ctzll(fw);
+          fw &= fw - 1;
+          const int j1 = rp[u + 1];
+          for (int j = rp[u]; j < j1; ++j) {
+            const u32 x = arc[j];
+            const int v = (int)(x & 0xffffu), e = (int)(x >> 16);
+            if (bs_test<NW>(vis, v) || !allowed(known, up, e)) continue;
+            bs_set<NW>(vis, v);
+            bs_set<NW>(nx, v);
+            any = true;
+            if (v == dst) {
+              hit = true;
+              break;
+            }
+          }
+        }
+      }
+      if (!any) return -1;
+      ++L;
+      if (hit) break;
+      if (L >= 255) return -2;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) {
+        u64 m = nx[k];
+        while (m) {
+          set_level(k * 64 + __builtin_ctzll(m), L);
+          m &= m - 1;
+        }
+        F[k] = nx[k];
+      }
+    }
+    int v = dst, k = 0;
+    for (int l = L; l >= 1; --l) {
+      int pu = -1, pe = -1;
+      const int j1 = irp[v + 1];
+      for (int j = irp[v]; j < j1; ++j) {
+        const u32 x = iarc[j];
+        const int u = (int)(x & 0xffffu), e = (int)(x >> 16);
+        if (level_of(u) == l - 1 && allowed(known, up, e)) {
+          pu = u;
+          pe = e;
+          break;
+        }
+      }
+      if (pu < 0) return -2;
+      if (!((pick<W>(known, pe >> 6) >> (pe & 63)) & 1ull)) uh[(k++) * stride] = (u16)pe;
+      v = pu;
+    }
+    return k;
   }
 };
 

@@ -35,371 +35,9 @@
 
 #include "nr_device.h"
 #include "nr_internal.h"
+#include "nr_level.h"
 
 namespace nr {
-
-/* Mass classes: two per binade, class 0 = heaviest (mass >= 0.75).  The
- * class-round order expands the heaviest classes first. */
-constexpr int kClasses = 128;
-
-__host__ __device__ inline int mass_class(double m) {
-  unsigned long long bits;
-  memcpy(&bits, &m, sizeof bits);
-  const int e = (int)((bits >> 52) & 0x7ff) - 1023; /* m in [2^e, 2^(e+1)) */
-  const int heavy = (int)((bits >> 51) & 1);         /* m >= 1.5 * 2^e */
-  int c = 2 * (-e - 1) + (heavy ? 0 : 1);
-  return c < 0 ? 0 : (c >= kClasses ? kClasses - 1 : c);
-}
-
-/* Per-wave counter slots. */
-enum { C_NODES = 0, C_PATHS, C_DEAD, C_TRUNC, C_OVER, C_FOLD, C_ERR, C_N = 8 };
-
-/* Control block of a level-order solve (device memory). */
-struct LevelCtl {
-  u64 hist[kClasses]; /* class histogram of the pool (class rounds) */
-  u64 last_items; /* items of the most recently launched level */
-  u64 iterations; /* non-empty levels processed */
-  u64 max_width;  /* widest level */
-  u64 counts[C_N];
-  double sum[3];  /* R, ESP, D after k_finalize */
-};
-
-struct QBuf {
-  u64* known; /* [w][n_seg*seg_cap] */
-  u64* up;    /* [w][n_seg*seg_cap] */
-  void* rows; /* [n][n_seg*seg_cap] node sets (u32 or u64) */
-  double* mass;
-};
-
-struct LevelArgs {
-  DevGraph g;
-  Layout lay;
-  QBuf buf[2];
-  u32* seg_count[2]; /* [n_seg] items written by each workgroup */
-  u32* hist;         /* [n_seg][kClasses] class histogram of each segment */
-  LevelCtl* ctl;
-  DD* partR;
-  DD* partE;
-  DD* partD;
-  u64* cnt; /* [n_seg][C_N] per-workgroup counters */
-  u64 stride; /* n_seg * seg_cap: distance between planes */
-  u32 seg_cap;
-  int n_seg;
-  double eps;
-  /* class rounds (max-mass order): items of class <= thr_class are
-   * expanded, lighter ones are carried to the output unchanged; queued
-   * items of class >= drop_class are given up to ESP */
-  int mass_mode;
-  int thr_class;
-  int drop_class;
-};
-
-template <int W>
-__device__ inline void load_item(const QBuf& b, u64 stride, u64 i, u64 (&known)[W],
-                                 u64 (&up)[W], double& mass) {
-#pragma unroll
-  for (int w = 0; w < W; ++w) {
-    known[w] = b.known[w * stride + i];
-    up[w] = b.up[w * stride + i];
-  }
-  mass = b.mass[i];
-}
-
-template <int W>
-__device__ inline void store_item(const QBuf& b, u64 stride, u64 i, const u64 (&known)[W],
-                                  const u64 (&up)[W], double mass) {
-#pragma unroll
-  for (int w = 0; w < W; ++w) {
-    b.known[w * stride + i] = known[w];
-    b.up[w * stride + i] = up[w];
-  }
-  b.mass[i] = mass;
-}
-
-/* Segment holding global item index `base`: the largest s with
- * sp[s] <= base (sp[0] = 0 <= base < sp[n_seg]). */
-__device__ inline int segment_of(const u32* sp, int n_seg, u32 base) {
-  int lo = 0, hi = n_seg;
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (sp[mid] <= base)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
-/* Exclusive prefix of the per-segment counts into sp[0..n_seg] (LDS), by
- * the whole workgroup; `part` is blockDim.x words of LDS scratch. */
-__device__ inline void block_prefix(const u32* counts, int n_seg, u32* sp, u32* part) {
-  const int t = threadIdx.x, T = blockDim.x;
-  const int per = (n_seg + T - 1) / T;
-  const int b0 = min(t * per, n_seg), b1 = min(b0 + per, n_seg);
-  u32 sum = 0;
-  for (int b = b0; b < b1; ++b) sum += counts[b];
-  part[t] = sum;
-  __syncthreads();
-  for (int off = 1; off < T; off <<= 1) {
-    const u32 v = t >= off ? part[t - off] : 0u;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  u32 run = t ? part[t - 1] : 0u;
-  for (int b = b0; b < b1; ++b) {
-    sp[b] = run;
-    run += counts[b];
-  }
-  if (t == T - 1) sp[n_seg] = part[T - 1];
-  __syncthreads();
-}
-
-__device__ inline void flush_counts(u64* slot, const u32 (&c)[C_N]) {
-  bool any = false;
-#pragma unroll
-  for (int j = 0; j < C_N; ++j) any |= c[j] != 0;
-  if (!any) return;
-#pragma unroll
-  for (int j = 0; j < C_N; ++j) slot[j] += c[j];
-}
-
-/* Flush a lane's accumulators into its workgroup's slot: wave sums, then
- * wave 0's lane 0 merges the workgroup's waves in wave order and updates the
- * block's double-double and counter slots (owned by this workgroup for the
- * launch; launches on one stream do not overlap).  `scratch` is LDS of at
- * least 80 bytes per wave. */
-__device__ inline void block_flush(const LevelArgs& a, Acc2& accR, Acc2& accE, Acc2& accD,
-                                   const u32 (&c)[C_N], unsigned char* scratch) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int n_w = (blockDim.x + 63) >> 6;
-  wave_sum_dd(accR.s, accR.c);
-  wave_sum_dd(accE.s, accE.c);
-  wave_sum_dd(accD.s, accD.c);
-  double* sd = (double*)scratch;      /* [n_w][6] */
-  u32* sc = (u32*)(sd + 6 * n_w);     /* [n_w][C_N] */
-  u32 t[C_N];
-#pragma unroll
-  for (int j = 0; j < C_N; ++j) t[j] = (u32)wave_sum_u64(c[j]);
-  if (lane == 0) {
-    sd[wave * 6 + 0] = accR.s;
-    sd[wave * 6 + 1] = accR.c;
-    sd[wave * 6 + 2] = accE.s;
-    sd[wave * 6 + 3] = accE.c;
-    sd[wave * 6 + 4] = accD.s;
-    sd[wave * 6 + 5] = accD.c;
-#pragma unroll
-    for (int j = 0; j < C_N; ++j) sc[wave * C_N + j] = t[j];
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    DD* dst[3] = {&a.partR[blockIdx.x], &a.partE[blockIdx.x], &a.partD[blockIdx.x]};
-    for (int k = 0; k < 3; ++k) {
-      double s = 0.0, cc = 0.0;
-      for (int w = 0; w < n_w; ++w) {
-        double x, e;
-        two_sum(s, sd[w * 6 + 2 * k], x, e);
-        s = x;
-        cc += sd[w * 6 + 2 * k + 1] + e;
-      }
-      if (s != 0.0 || cc != 0.0) dd_merge_slot(dst[k], s, cc);
-    }
-    u32 tot[C_N];
-    bool any = false;
-    for (int j = 0; j < C_N; ++j) {
-      tot[j] = 0;
-      for (int w = 0; w < n_w; ++w) tot[j] += sc[w * C_N + j];
-      any |= tot[j] != 0;
-    }
-    if (any) {
-      u64* slot = &a.cnt[(u64)blockIdx.x * C_N];
-      for (int j = 0; j < C_N; ++j) slot[j] += tot[j];
-    }
-  }
-}
-
-/* One tree level: consume buf[parity], append children to the workgroup's
- * segment of buf[parity^1]. */
-template <int W, typename NS>
-__global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  u32* sp = (u32*)(smem + a.lay.off_extra);
-  u32* s_cnt = sp + a.n_seg + 1;
-  u32* part = s_cnt + 1;
-  block_prefix(a.seg_count[parity], a.n_seg, sp, part);
-  const u32 n_items = sp[a.n_seg];
-  if (blockIdx.x == 0 && threadIdx.x == 0) { /* count the level (main.cu:970-973) */
-    a.ctl->last_items = n_items;
-    if (n_items) {
-      a.ctl->iterations += 1;
-      if (n_items > a.ctl->max_width) a.ctl->max_width = n_items;
-    }
-  }
-  if (n_items == 0) {
-    if (threadIdx.x == 0) a.seg_count[parity ^ 1][blockIdx.x] = 0;
-    if (a.mass_mode)
-      for (int i = threadIdx.x; i < kClasses; i += blockDim.x)
-        a.hist[(u64)blockIdx.x * kClasses + i] = 0;
-    return;
-  }
-  if (threadIdx.x == 0) *s_cnt = 0;
-  u32* s_hist = part + blockDim.x; /* [kClasses], class rounds only */
-  if (a.mass_mode)
-    for (int i = threadIdx.x; i < kClasses; i += blockDim.x) s_hist[i] = 0;
-  stage_tables(a.g, a.lay, smem);
-  __syncthreads();
-
-  Expander<W, NS> ex;
-  ex.bind(smem, a.lay, threadIdx.x, a.g.src, a.g.dst);
-
-  const QBuf in = a.buf[parity];
-  const QBuf out = a.buf[parity ^ 1];
-  const int lane = threadIdx.x & 63;
-  const u64 wave_id = (u64)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const u64 n_waves = (u64)(gridDim.x * blockDim.x) >> 6;
-  const u64 out_base = (u64)blockIdx.x * a.seg_cap;
-
-  Acc2 accR, accE, accD;
-  accR.init();
-  accE.init();
-  accD.init();
-  u32 c[C_N] = {0, 0, 0, 0, 0, 0, 0, 0};
-
-  /* Small levels are spread over every wave (chunk < 64 lanes) so each
-   * workgroup receives, and appends to its segment, a fair share. */
-  const u64 per_wave = (n_items + n_waves - 1) / n_waves;
-  const u64 chunk = per_wave >= 64 ? 64 : per_wave;
-  for (u64 base = wave_id * chunk; base < n_items; base += n_waves * chunk) {
-    const u32 i = (u32)(base + lane);
-    u64 known[W], up[W];
-    double mass = 0.0;
-    int k = 0, stored = 0;
-    bool live = false, carry = false;
-    u64 slot_in = 0;
-    if ((u64)lane < chunk && i < n_items) {
-      int s = segment_of(sp, a.n_seg, (u32)base);
-      while (sp[s + 1] <= i) ++s;
-      slot_in = (u64)s * a.seg_cap + (i - sp[s]);
-      load_item<W>(in, a.stride, slot_in, known, up, mass);
-      const int cl = a.mass_mode ? mass_class(mass) : 0;
-      if (a.mass_mode && cl >= a.drop_class) { /* given up: queue over capacity */
-        accE.add(mass);
-        ++c[C_FOLD];
-      } else if (a.mass_mode && cl > a.thr_class) { /* lighter: carried */
-        carry = true;
-        stored = 1;
-      } else {
-        ++c[C_NODES];
-        if (mass < a.eps) { /* main.cu:537-543 */
-          accE.add(mass);
-          ++c[C_TRUNC];
-        } else {
-          const NS* rin = (const NS*)in.rows + slot_in;
-          for (int u = 0; u < ex.n; ++u) ex.rows[u * ex.stride] = rin[(u64)u * a.stride];
-          k = ex.find(known);
-          if (k >= 0) {
-            live = true;
-            stored = k;
-            if (a.mass_mode) { /* children below eps or over capacity are not queued */
-              stored = 0;
-              double run = mass;
-              for (int j = k - 1; j >= 0; --j) {
-                const int e = ex.uh[j * ex.stride];
-                const double cm = run * ex.q[e];
-                if (cm >= a.eps) ++stored;
-                run = run * ex.p[e];
-              }
-            }
-          } else {
-            if (k == -2) ++c[C_ERR];
-            accD.add(mass); /* main.cu:579-582 */
-            ++c[C_DEAD];
-            k = 0;
-          }
-        }
-      }
-    }
-    const int incl = wave_incl_scan(stored);
-    const int total = __shfl(incl, 63, 64);
-    u32 wbase = 0;
-    if (total) {
-      if (lane == 63) wbase = atomicAdd(s_cnt, (u32)total);
-      wbase = __shfl(wbase, 63, 64);
-    }
-    u32 off = wbase + (u32)(incl - stored);
-    if (carry) {
-      if (off < a.seg_cap) {
-        store_item<W>(out, a.stride, out_base + off, known, up, mass);
-        const NS* rin = (const NS*)in.rows + slot_in;
-        NS* rout = (NS*)out.rows + out_base + off;
-        for (int u = 0; u < ex.n; ++u) rout[(u64)u * a.stride] = rin[(u64)u * a.stride];
-        atomicAdd(&s_hist[mass_class(mass)], 1u);
-      } else {
-        accE.add(mass);
-        ++c[C_FOLD];
-      }
-    }
-    if (live) {
-      /* main.cu:214-266: child j has the earlier UNKNOWN hops WORKING and
-       * hop j FAILED; masses multiply left to right along the path */
-      double run = mass;
-      u64 pre[W];
-#pragma unroll
-      for (int w = 0; w < W; ++w) pre[w] = 0;
-      for (int j = k - 1; j >= 0; --j) {
-        const int e = ex.uh[j * ex.stride];
-        const int wi = e >> 6;
-        const u64 bit = 1ull << (e & 63);
-        const double cm = run * ex.q[e];
-        bool queue = true;
-        if (a.mass_mode) {
-          if (cm < a.eps) { /* settled at creation: counted as a truncated node */
-            accE.add(cm);
-            ++c[C_NODES];
-            ++c[C_TRUNC];
-            queue = false;
-          }
-        }
-        if (queue) {
-          if (off < a.seg_cap) {
-            u64 ck[W], cu[W];
-#pragma unroll
-            for (int w = 0; w < W; ++w) {
-              ck[w] = known[w] | pre[w] | (w == wi ? bit : 0ull);
-              cu[w] = up[w] | pre[w];
-            }
-            store_item<W>(out, a.stride, out_base + off, ck, cu, cm);
-            NS* rout = (NS*)out.rows + out_base + off;
-            for (int u = 0; u < ex.n; ++u) rout[(u64)u * a.stride] = ex.child_row(u, e);
-            if (a.mass_mode) atomicAdd(&s_hist[mass_class(cm)], 1u);
-          } else { /* segment full: main.cu:245-252 */
-            accE.add(cm);
-            ++c[C_PATHS];
-            ++c[C_OVER];
-          }
-          ++off;
-        }
-        run = run * ex.p[e];
-#pragma unroll
-        for (int w = 0; w < W; ++w)
-          if (w == wi) pre[w] |= bit;
-      }
-      accR.add(run); /* success term, main.cu:254-262 */
-      ++c[C_PATHS];
-    }
-  }
-
-  /* per-workgroup flush; then publish this workgroup's segment count */
-  __syncthreads();
-  if (a.mass_mode)
-    for (int i = threadIdx.x; i < kClasses; i += blockDim.x)
-      a.hist[(u64)blockIdx.x * kClasses + i] = s_hist[i];
-  block_flush(a, accR, accE, accD, c, (unsigned char*)part);
-  if (threadIdx.x == 0) {
-    const u32 n = *s_cnt;
-    a.seg_count[parity ^ 1][blockIdx.x] = n < a.seg_cap ? n : a.seg_cap;
-  }
-}
 
 /* Sum the per-segment class histograms into the control block (one
  * thread per class). */
@@ -442,73 +80,6 @@ __global__ void __launch_bounds__(256) k_fold(LevelArgs a, int parity) {
   block_flush(a, zr, e, zd, c, (unsigned char*)part);
 }
 
-/* One expansion of each caller-supplied item into fixed output slots (the
- * parity entry point nr_expand_items). */
-struct ItemsArgs {
-  DevGraph g;
-  Layout lay;
-  const u64* known; /* [count][w] */
-  const u64* up;
-  const double* mass;
-  int count, max_children;
-  int* nchild;
-  double* succ;
-  u64* ck; /* [count][max_children][w] */
-  u64* cu;
-  double* cm;
-};
-
-template <int W, typename NS>
-__global__ void __launch_bounds__(256) k_expand_items(ItemsArgs a) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  stage_tables(a.g, a.lay, smem);
-  __syncthreads();
-  Expander<W, NS> ex;
-  ex.bind(smem, a.lay, threadIdx.x, a.g.src, a.g.dst);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.count) return;
-  u64 known[W], up[W], Dn[W];
-#pragma unroll
-  for (int w = 0; w < W; ++w) {
-    known[w] = a.known[(u64)i * W + w];
-    up[w] = a.up[(u64)i * W + w];
-    Dn[w] = known[w] & ~up[w];
-  }
-  const double mass = a.mass[i];
-  ex.build_rows(Dn);
-  int k = ex.find(known);
-  if (k < 0) {
-    a.nchild[i] = k;
-    a.succ[i] = 0.0;
-    return;
-  }
-  a.nchild[i] = k;
-  double run = mass;
-  u64 pre[W];
-#pragma unroll
-  for (int w = 0; w < W; ++w) pre[w] = 0;
-  int c = 0;
-  for (int j = k - 1; j >= 0; --j, ++c) {
-    const int e = ex.uh[j * ex.stride];
-    const int wi = e >> 6;
-    const u64 bit = 1ull << (e & 63);
-    if (c < a.max_children) {
-      u64 base = ((u64)i * a.max_children + c) * W;
-#pragma unroll
-      for (int w = 0; w < W; ++w) {
-        a.ck[base + w] = known[w] | pre[w] | (w == wi ? bit : 0ull);
-        a.cu[base + w] = up[w] | pre[w];
-      }
-      a.cm[(u64)i * a.max_children + c] = run * ex.q[e];
-    }
-    run = run * ex.p[e];
-#pragma unroll
-    for (int w = 0; w < W; ++w)
-      if (w == wi) pre[w] |= bit;
-  }
-  a.succ[i] = run;
-}
-
 /* Seed a solve: zero the control block, the partial-sum and counter slots,
  * set the first prefix table to one item, and put the root item
  * (main.cu:853-855) in slot 0 of buffer 0.  A kernel instead of memset /
@@ -540,7 +111,7 @@ __global__ void __launch_bounds__(256) k_init(LevelArgs a, Root root, int W, int
       a.buf[0].known[(u64)w * a.stride] = root.known[w];
       a.buf[0].up[(u64)w * a.stride] = root.up[w];
     }
-    for (int u = 0; u < n; ++u) {
+    for (int u = 0; u < n && ns_bytes; ++u) {
       if (ns_bytes == 4)
         ((u32*)a.buf[0].rows)[(u64)u * a.stride] = (u32)root.rows[u];
       else
@@ -1096,10 +667,13 @@ Workspace g_ws[kMaxDevices];
 struct TableBlob {
   std::vector<unsigned char> bytes;
   size_t off_p, off_q, off_sadj, off_inc, off_ep, off_deg, off_eidm;
+  size_t off_rp, off_arc, off_irp, off_iarc;
 };
 
 void pack_tables(const nr_graph* g, int W, TableBlob& b) {
   const int n = g->n, e = std::max(g->e, 1);
+  /* node-set tables exist for the row form only (n <= 64) */
+  const int ns = n <= 64 ? n : 0;
   size_t o = 0;
   auto take = [&](size_t bytes) {
     o = (o + 15) / 16 * 16;
@@ -1109,11 +683,15 @@ void pack_tables(const nr_graph* g, int W, TableBlob& b) {
   };
   b.off_p = take(8 * (size_t)e);
   b.off_q = take(8 * (size_t)e);
-  b.off_sadj = take(8 * (size_t)n);
-  b.off_inc = take(8 * (size_t)n * W);
+  b.off_sadj = take(8 * (size_t)ns + 8);
+  b.off_inc = take(8 * (size_t)ns * W + 8);
   b.off_ep = take(4 * (size_t)e);
-  b.off_deg = take(4 * (size_t)n);
-  b.off_eidm = take(2 * (size_t)n * n + 4);
+  b.off_deg = take(4 * (size_t)ns + 4);
+  b.off_eidm = take(2 * (size_t)ns * ns + 4);
+  b.off_rp = take(4 * (size_t)(n + 1));
+  b.off_arc = take(4 * (size_t)std::max(g->nnz, 1));
+  b.off_irp = take(4 * (size_t)(n + 1));
+  b.off_iarc = take(4 * (size_t)std::max(g->nnz, 1));
   b.bytes.assign((o + 15) / 16 * 16, 0);
   double* p = (double*)&b.bytes[b.off_p];
   double* q = (double*)&b.bytes[b.off_q];
@@ -1122,12 +700,13 @@ void pack_tables(const nr_graph* g, int W, TableBlob& b) {
   u32* ep = (u32*)&b.bytes[b.off_ep];
   u32* deg = (u32*)&b.bytes[b.off_deg];
   u16* eidm = (u16*)&b.bytes[b.off_eidm];
-  for (size_t i = 0; i < (size_t)n * n; ++i) eidm[i] = 0xffff;
+  for (size_t i = 0; i < (size_t)ns * ns; ++i) eidm[i] = 0xffff;
   for (int id = 0; id < g->e; ++id) {
     const int a = g->ea[id], c = g->eb[id];
     p[id] = g->p[id];
     q[id] = g->q[id];
     ep[id] = (u32)a | ((u32)c << 16);
+    if (!ns) continue;
     inc[(size_t)a * W + (id >> 6)] |= 1ull << (id & 63);
     inc[(size_t)c * W + (id >> 6)] |= 1ull << (id & 63);
     if (a != c) {
@@ -1137,11 +716,38 @@ void pack_tables(const nr_graph* g, int W, TableBlob& b) {
     eidm[(size_t)a * n + c] = (u16)id;
     eidm[(size_t)c * n + a] = (u16)id;
   }
-  for (int u = 0; u < n; ++u) {
+  for (int u = 0; u < ns; ++u) {
     int cnt = 0;
     for (int i = 0; i < W; ++i) cnt += __builtin_popcountll(inc[(size_t)u * W + i]);
     deg[u] = (u32)cnt;
   }
+  /* CSR arcs for the CSR expander: out-rows sorted by (v, e), in-rows by
+   * (u, e); an arc is packed node | edge << 16 */
+  std::vector<std::vector<u32>> out(n), in(n);
+  for (int u = 0; u < n; ++u)
+    for (int j = g->row_ptr[u]; j < g->row_ptr[u + 1]; ++j) {
+      const int v = g->col_idx[j], id = g->edge_id[j];
+      out[u].push_back((u32)v | ((u32)id << 16));
+      in[v].push_back((u32)u | ((u32)id << 16));
+    }
+  auto by_node_then_edge = [](u32 x, u32 y) {
+    return (x & 0xffffu) != (y & 0xffffu) ? (x & 0xffffu) < (y & 0xffffu) : (x >> 16) < (y >> 16);
+  };
+  int* rp = (int*)&b.bytes[b.off_rp];
+  int* irp = (int*)&b.bytes[b.off_irp];
+  u32* arc = (u32*)&b.bytes[b.off_arc];
+  u32* iarc = (u32*)&b.bytes[b.off_iarc];
+  int k = 0, ik = 0;
+  for (int u = 0; u < n; ++u) {
+    std::sort(out[u].begin(), out[u].end(), by_node_then_edge);
+    std::sort(in[u].begin(), in[u].end(), by_node_then_edge);
+    rp[u] = k;
+    irp[u] = ik;
+    for (u32 x : out[u]) arc[k++] = x;
+    for (u32 x : in[u]) iarc[ik++] = x;
+  }
+  rp[n] = k;
+  irp[n] = ik;
 }
 
 int upload_tables(Workspace& ws, const nr_graph* g, int W, DevGraph& d) {
@@ -1159,6 +765,10 @@ int upload_tables(Workspace& ws, const nr_graph* g, int W, DevGraph& d) {
   d.ep = (const u32*)(base + b.off_ep);
   d.deg = (const u32*)(base + b.off_deg);
   d.eidm = (const u16*)(base + b.off_eidm);
+  d.rp = (const int*)(base + b.off_rp);
+  d.arc = (const u32*)(base + b.off_arc);
+  d.irp = (const int*)(base + b.off_irp);
+  d.iarc = (const u32*)(base + b.off_iarc);
   d.n = g->n;
   d.e = g->e;
   d.w = W;
@@ -1171,17 +781,10 @@ int upload_tables(Workspace& ws, const nr_graph* g, int W, DevGraph& d) {
  * bits.  Each class is a separate instantiation of the kernels. */
 template <int W, typename NS>
 struct Kern {
-  static void* level() { return (void*)&k_expand_level<W, NS>; }
-  static void* items() { return (void*)&k_expand_items<W, NS>; }
+  static void* level() { return (void*)&k_expand_level<W, Expander<W, NS>>; }
+  static void* items() { return (void*)&k_expand_items<W, Expander<W, NS>>; }
   static void* batch() { return (void*)&k_expand_batch<W, NS>; }
   static void* refill() { return (void*)&k_refill_batch<W>; }
-};
-
-typedef void* (*KernelGetter)();
-
-struct Variant {
-  KernelGetter level, items, batch, refill;
-  int ns_bytes;
 };
 
 template <typename NS>
@@ -1190,30 +793,39 @@ Variant variant_for_w(int W) {
 #define NR_W(X) \
   case X:      \
     return Variant{&Kern<X, NS>::level, &Kern<X, NS>::items, &Kern<X, NS>::batch,               \
-                   &Kern<X, NS>::refill, (int)sizeof(NS)};
+                   &Kern<X, NS>::refill, (int)sizeof(NS), 0};
     NR_W(1) NR_W(2) NR_W(3) NR_W(4) NR_W(5) NR_W(6) NR_W(7) NR_W(8)
 #undef NR_W
-    default: return Variant{nullptr, nullptr, nullptr, nullptr, 0};
+    default: return Variant{nullptr, nullptr, nullptr, nullptr, 0, 0};
   }
+}
+
+/* LDS layout of a variant's expand kernels for graph g. */
+Layout layout_for(const Variant& v, const nr_graph* g, int W, int block, int extra = 0) {
+  if (v.nw) return make_layout_csr(g->n, block, extra);
+  return make_layout(g->n, std::max(g->e, 1), W, block, v.ns_bytes, extra);
 }
 
 /* Largest block (<= 256, multiple of 64) whose LDS footprint fits in 64 KiB;
  * a requested size is honoured when it fits. */
-int choose_block(const nr_graph* g, int W, int ns_bytes, int requested, int extra = 0) {
+int choose_block(const Variant& v, const nr_graph* g, int W, int requested, int extra = 0) {
   const int cands[4] = {requested, 256, 128, 64};
   for (int b : cands) {
     if (b <= 0 || b > 256 || b % 64) continue;
-    Layout L = make_layout(g->n, std::max(g->e, 1), W, b, ns_bytes, extra);
-    if (L.total_bytes <= 64 * 1024) return b;
+    if (layout_for(v, g, W, b, extra).total_bytes <= 64 * 1024) return b;
   }
   return 0;
 }
 
+/* Row form (items carry adjacency rows) for undirected graphs of up to 64
+ * nodes; CSR form for up to 256 nodes or one-way arcs.  Up to 512 edges. */
 int pick_variant(const nr_graph* g, Variant& v) {
   const int W = mask_words(g);
-  if (g->n > 64 || W > 8) return NR_ERR_UNSUPPORTED;
-  if (!g->symmetric) return NR_ERR_UNSUPPORTED;
-  v = g->n <= 32 ? variant_for_w<u32>(W) : variant_for_w<u64>(W);
+  if (W > 8 || g->n > 256 || g->n < 1) return NR_ERR_UNSUPPORTED;
+  if (g->symmetric && g->n <= 64)
+    v = g->n <= 32 ? variant_for_w<u32>(W) : variant_for_w<u64>(W);
+  else
+    v = variant_csr(W, (g->n + 63) / 64);
   return v.level ? NR_OK : NR_ERR_UNSUPPORTED;
 }
 
@@ -1229,8 +841,8 @@ Root root_item(const nr_graph* g, int W) {
       r.known[e >> 6] |= bit;
       r.up[e >> 6] |= bit;
     }
-    /* rows: neighbours over edges that are not FAILED */
-    if (g->p[e] > 0.0 && g->ea[e] != g->eb[e]) {
+    /* rows (row form, n <= 64): neighbours over edges that are not FAILED */
+    if (g->n <= 64 && g->p[e] > 0.0 && g->ea[e] != g->eb[e]) {
       r.rows[g->ea[e]] |= 1ull << g->eb[e];
       r.rows[g->eb[e]] |= 1ull << g->ea[e];
     }
@@ -1250,17 +862,16 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
    * for that grid; occupancy can only drop, so the table stays large
    * enough for the final grid. */
   const void* kfun = (const void*)var.level();
-  int block = choose_block(g, W, var.ns_bytes, cfg->block_size, 4 * (1026 + 256 + kClasses));
+  int block = choose_block(var, g, W, cfg->block_size, 4 * (1026 + 256 + kClasses));
   if (block <= 0) return NR_ERR_UNSUPPORTED;
-  Layout lay0 = make_layout(g->n, std::max(g->e, 1), W, block, var.ns_bytes,
-                            4 * (block + kClasses));
+  Layout lay0 = layout_for(var, g, W, block, 4 * (block + kClasses));
   NR_HIP(hipFuncSetAttribute(kfun, hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
   int per_cu0 = 0;
   NR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu0, kfun, block, lay0.total_bytes));
   if (per_cu0 < 1) return NR_ERR_UNSUPPORTED;
   int grid = cfg->grid_blocks > 0 ? cfg->grid_blocks : per_cu0 * cus;
   const int extra = 4 * (grid + 2 + block + kClasses);
-  Layout lay = make_layout(g->n, std::max(g->e, 1), W, block, var.ns_bytes, extra);
+  Layout lay = layout_for(var, g, W, block, extra);
   if (lay.total_bytes > 64 * 1024) return NR_ERR_UNSUPPORTED;
   int per_cu = 0;
   NR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfun, block, lay.total_bytes));
@@ -1804,9 +1415,9 @@ int expand_items(Workspace& ws, const nr_graph* g, int count, const u64* known,
   if (rc) return rc;
   if (count <= 0) return NR_OK;
   const int W = mask_words(g);
-  const int block = choose_block(g, W, var.ns_bytes, 0);
+  const int block = choose_block(var, g, W, 0);
   if (block <= 0) return NR_ERR_UNSUPPORTED;
-  Layout lay = make_layout(g->n, std::max(g->e, 1), W, block, var.ns_bytes);
+  Layout lay = layout_for(var, g, W, block);
   const void* kfun = (const void*)var.items();
   if (lay.total_bytes > 48 * 1024)
     NR_HIP(hipFuncSetAttribute(kfun, hipFuncAttributeMaxDynamicSharedMemorySize,

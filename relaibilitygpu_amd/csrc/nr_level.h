@@ -1,0 +1,453 @@
+/*
+ * nr_level.h - the level-order expand kernel and the single-expansion
+ * kernel, templated on the per-lane expander (nr_device.h): Expander for
+ * undirected graphs of up to 64 nodes (items carry adjacency rows),
+ * CsrExpander for larger or directed graphs (items are planes only).  Shared
+ * by the translation units that instantiate them.
+ */
+#ifndef NR_LEVEL_H
+#define NR_LEVEL_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "nr_device.h"
+
+namespace nr {
+
+/* Mass classes: two per binade, class 0 = heaviest (mass >= 0.75).  The
+ * class-round order expands the heaviest classes first. */
+constexpr int kClasses = 128;
+
+__host__ __device__ inline int mass_class(double m) {
+  unsigned long long bits;
+  memcpy(&bits, &m, sizeof bits);
+  const int e = (int)((bits >> 52) & 0x7ff) - 1023; /* m in [2^e, 2^(e+1)) */
+  const int heavy = (int)((bits >> 51) & 1);         /* m >= 1.5 * 2^e */
+  int c = 2 * (-e - 1) + (heavy ? 0 : 1);
+  return c < 0 ? 0 : (c >= kClasses ? kClasses - 1 : c);
+}
+
+/* Per-wave counter slots. */
+enum { C_NODES = 0, C_PATHS, C_DEAD, C_TRUNC, C_OVER, C_FOLD, C_ERR, C_N = 8 };
+
+/* Control block of a level-order solve (device memory). */
+struct LevelCtl {
+  u64 hist[kClasses]; /* class histogram of the pool (class rounds) */
+  u64 last_items; /* items of the most recently launched level */
+  u64 iterations; /* non-empty levels processed */
+  u64 max_width;  /* widest level */
+  u64 counts[C_N];
+  double sum[3];  /* R, ESP, D after k_finalize */
+};
+
+
+struct LevelArgs {
+  DevGraph g;
+  Layout lay;
+  QBuf buf[2];
+  u32* seg_count[2]; /* [n_seg] items written by each workgroup */
+  u32* hist;         /* [n_seg][kClasses] class histogram of each segment */
+  LevelCtl* ctl;
+  DD* partR;
+  DD* partE;
+  DD* partD;
+  u64* cnt; /* [n_seg][C_N] per-workgroup counters */
+  u64 stride; /* n_seg * seg_cap: distance between planes */
+  u32 seg_cap;
+  int n_seg;
+  double eps;
+  /* class rounds (max-mass order): items of class <= thr_class are
+   * expanded, lighter ones are carried to the output unchanged; queued
+   * items of class >= drop_class are given up to ESP */
+  int mass_mode;
+  int thr_class;
+  int drop_class;
+};
+
+template <int W>
+__device__ inline void load_item(const QBuf& b, u64 stride, u64 i, u64 (&known)[W],
+                                 u64 (&up)[W], double& mass) {
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    known[w] = b.known[w * stride + i];
+    up[w] = b.up[w * stride + i];
+  }
+  mass = b.mass[i];
+}
+
+template <int W>
+__device__ inline void store_item(const QBuf& b, u64 stride, u64 i, const u64 (&known)[W],
+                                  const u64 (&up)[W], double mass) {
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    b.known[w * stride + i] = known[w];
+    b.up[w * stride + i] = up[w];
+  }
+  b.mass[i] = mass;
+}
+
+/* Segment holding global item index `base`: the largest s with
+ * sp[s] <= base (sp[0] = 0 <= base < sp[n_seg]). */
+__device__ inline int segment_of(const u32* sp, int n_seg, u32 base) {
+  int lo = 0, hi = n_seg;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (sp[mid] <= base)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+/* Exclusive prefix of the per-segment counts into sp[0..n_seg] (LDS), by
+ * the whole workgroup; `part` is blockDim.x words of LDS scratch. */
+__device__ inline void block_prefix(const u32* counts, int n_seg, u32* sp, u32* part) {
+  const int t = threadIdx.x, T = blockDim.x;
+  const int per = (n_seg + T - 1) / T;
+  const int b0 = min(t * per, n_seg), b1 = min(b0 + per, n_seg);
+  u32 sum = 0;
+  for (int b = b0; b < b1; ++b) sum += counts[b];
+  part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < T; off <<= 1) {
+    const u32 v = t >= off ? part[t - off] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  u32 run = t ? part[t - 1] : 0u;
+  for (int b = b0; b < b1; ++b) {
+    sp[b] = run;
+    run += counts[b];
+  }
+  if (t == T - 1) sp[n_seg] = part[T - 1];
+  __syncthreads();
+}
+
+__device__ inline void flush_counts(u64* slot, const u32 (&c)[C_N]) {
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < C_N; ++j) any |= c[j] != 0;
+  if (!any) return;
+#pragma unroll
+  for (int j = 0; j < C_N; ++j) slot[j] += c[j];
+}
+
+/* Flush a lane's accumulators into its workgroup's slot: wave sums, then
+ * wave 0's lane 0 merges the workgroup's waves in wave order and updates the
+ * block's double-double and counter slots (owned by this workgroup for the
+ * launch; launches on one stream do not overlap).  `scratch` is LDS of at
+ * least 80 bytes per wave. */
+__device__ inline void block_flush(const LevelArgs& a, Acc2& accR, Acc2& accE, Acc2& accD,
+                                   const u32 (&c)[C_N], unsigned char* scratch) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int n_w = (blockDim.x + 63) >> 6;
+  wave_sum_dd(accR.s, accR.c);
+  wave_sum_dd(accE.s, accE.c);
+  wave_sum_dd(accD.s, accD.c);
+  double* sd = (double*)scratch;      /* [n_w][6] */
+  u32* sc = (u32*)(sd + 6 * n_w);     /* [n_w][C_N] */
+  u32 t[C_N];
+#pragma unroll
+  for (int j = 0; j < C_N; ++j) t[j] = (u32)wave_sum_u64(c[j]);
+  if (lane == 0) {
+    sd[wave * 6 + 0] = accR.s;
+    sd[wave * 6 + 1] = accR.c;
+    sd[wave * 6 + 2] = accE.s;
+    sd[wave * 6 + 3] = accE.c;
+    sd[wave * 6 + 4] = accD.s;
+    sd[wave * 6 + 5] = accD.c;
+#pragma unroll
+    for (int j = 0; j < C_N; ++j) sc[wave * C_N + j] = t[j];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    DD* dst[3] = {&a.partR[blockIdx.x], &a.partE[blockIdx.x], &a.partD[blockIdx.x]};
+    for (int k = 0; k < 3; ++k) {
+      double s = 0.0, cc = 0.0;
+      for (int w = 0; w < n_w; ++w) {
+        double x, e;
+        two_sum(s, sd[w * 6 + 2 * k], x, e);
+        s = x;
+        cc += sd[w * 6 + 2 * k + 1] + e;
+      }
+      if (s != 0.0 || cc != 0.0) dd_merge_slot(dst[k], s, cc);
+    }
+    u32 tot[C_N];
+    bool any = false;
+    for (int j = 0; j < C_N; ++j) {
+      tot[j] = 0;
+      for (int w = 0; w < n_w; ++w) tot[j] += sc[w * C_N + j];
+      any |= tot[j] != 0;
+    }
+    if (any) {
+      u64* slot = &a.cnt[(u64)blockIdx.x * C_N];
+      for (int j = 0; j < C_N; ++j) slot[j] += tot[j];
+    }
+  }
+}
+
+/* One tree level: consume buf[parity], append children to the workgroup's
+ * segment of buf[parity^1]. */
+template <int W, class EX>
+__global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  u32* sp = (u32*)(smem + a.lay.off_extra);
+  u32* s_cnt = sp + a.n_seg + 1;
+  u32* part = s_cnt + 1;
+  block_prefix(a.seg_count[parity], a.n_seg, sp, part);
+  const u32 n_items = sp[a.n_seg];
+  if (blockIdx.x == 0 && threadIdx.x == 0) { /* count the level (main.cu:970-973) */
+    a.ctl->last_items = n_items;
+    if (n_items) {
+      a.ctl->iterations += 1;
+      if (n_items > a.ctl->max_width) a.ctl->max_width = n_items;
+    }
+  }
+  if (n_items == 0) {
+    if (threadIdx.x == 0) a.seg_count[parity ^ 1][blockIdx.x] = 0;
+    if (a.mass_mode)
+      for (int i = threadIdx.x; i < kClasses; i += blockDim.x)
+        a.hist[(u64)blockIdx.x * kClasses + i] = 0;
+    return;
+  }
+  if (threadIdx.x == 0) *s_cnt = 0;
+  u32* s_hist = part + blockDim.x; /* [kClasses], class rounds only */
+  if (a.mass_mode)
+    for (int i = threadIdx.x; i < kClasses; i += blockDim.x) s_hist[i] = 0;
+  if (EX::kStage) stage_tables(a.g, a.lay, smem);
+  __syncthreads();
+
+  EX ex;
+  ex.bind_graph(smem, a.lay, a.g, threadIdx.x);
+
+  const QBuf in = a.buf[parity];
+  const QBuf out = a.buf[parity ^ 1];
+  const int lane = threadIdx.x & 63;
+  const u64 wave_id = (u64)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const u64 n_waves = (u64)(gridDim.x * blockDim.x) >> 6;
+  const u64 out_base = (u64)blockIdx.x * a.seg_cap;
+
+  Acc2 accR, accE, accD;
+  accR.init();
+  accE.init();
+  accD.init();
+  u32 c[C_N] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+  /* Small levels are spread over every wave (chunk < 64 lanes) so each
+   * workgroup receives, and appends to its segment, a fair share. */
+  const u64 per_wave = (n_items + n_waves - 1) / n_waves;
+  const u64 chunk = per_wave >= 64 ? 64 : per_wave;
+  for (u64 base = wave_id * chunk; base < n_items; base += n_waves * chunk) {
+    const u32 i = (u32)(base + lane);
+    u64 known[W], up[W];
+    double mass = 0.0;
+    int k = 0, stored = 0;
+    bool live = false, carry = false;
+    u64 slot_in = 0;
+    if ((u64)lane < chunk && i < n_items) {
+      int s = segment_of(sp, a.n_seg, (u32)base);
+      while (sp[s + 1] <= i) ++s;
+      slot_in = (u64)s * a.seg_cap + (i - sp[s]);
+      load_item<W>(in, a.stride, slot_in, known, up, mass);
+      const int cl = a.mass_mode ? mass_class(mass) : 0;
+      if (a.mass_mode && cl >= a.drop_class) { /* given up: queue over capacity */
+        accE.add(mass);
+        ++c[C_FOLD];
+      } else if (a.mass_mode && cl > a.thr_class) { /* lighter: carried */
+        carry = true;
+        stored = 1;
+      } else {
+        ++c[C_NODES];
+        if (mass < a.eps) { /* main.cu:537-543 */
+          accE.add(mass);
+          ++c[C_TRUNC];
+        } else {
+          ex.load(in, a.stride, slot_in);
+          k = ex.find_item(known, up);
+          if (k >= 0) {
+            live = true;
+            stored = k;
+            if (a.mass_mode) { /* children below eps or over capacity are not queued */
+              stored = 0;
+              double run = mass;
+              for (int j = k - 1; j >= 0; --j) {
+                const int e = ex.uh[j * ex.stride];
+                const double cm = run * ex.q[e];
+                if (cm >= a.eps) ++stored;
+                run = run * ex.p[e];
+              }
+            }
+          } else {
+            if (k == -2) ++c[C_ERR];
+            accD.add(mass); /* main.cu:579-582 */
+            ++c[C_DEAD];
+            k = 0;
+          }
+        }
+      }
+    }
+    const int incl = wave_incl_scan(stored);
+    const int total = __shfl(incl, 63, 64);
+    u32 wbase = 0;
+    if (total) {
+      if (lane == 63) wbase = atomicAdd(s_cnt, (u32)total);
+      wbase = __shfl(wbase, 63, 64);
+    }
+    u32 off = wbase + (u32)(incl - stored);
+    if (carry) {
+      if (off < a.seg_cap) {
+        store_item<W>(out, a.stride, out_base + off, known, up, mass);
+        ex.carry(in, slot_in, out, out_base + off, a.stride);
+        atomicAdd(&s_hist[mass_class(mass)], 1u);
+      } else {
+        accE.add(mass);
+        ++c[C_FOLD];
+      }
+    }
+    if (live) {
+      /* main.cu:214-266: child j has the earlier UNKNOWN hops WORKING and
+       * hop j FAILED; masses multiply left to right along the path */
+      double run = mass;
+      u64 pre[W];
+#pragma unroll
+      for (int w = 0; w < W; ++w) pre[w] = 0;
+      for (int j = k - 1; j >= 0; --j) {
+        const int e = ex.uh[j * ex.stride];
+        const int wi = e >> 6;
+        const u64 bit = 1ull << (e & 63);
+        const double cm = run * ex.q[e];
+        bool queue = true;
+        if (a.mass_mode) {
+          if (cm < a.eps) { /* settled at creation: counted as a truncated node */
+            accE.add(cm);
+            ++c[C_NODES];
+            ++c[C_TRUNC];
+            queue = false;
+          }
+        }
+        if (queue) {
+          if (off < a.seg_cap) {
+            u64 ck[W], cu[W];
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+              ck[w] = known[w] | pre[w] | (w == wi ? bit : 0ull);
+              cu[w] = up[w] | pre[w];
+            }
+            store_item<W>(out, a.stride, out_base + off, ck, cu, cm);
+            ex.store_child(out, a.stride, out_base + off, e);
+            if (a.mass_mode) atomicAdd(&s_hist[mass_class(cm)], 1u);
+          } else { /* segment full: main.cu:245-252 */
+            accE.add(cm);
+            ++c[C_PATHS];
+            ++c[C_OVER];
+          }
+          ++off;
+        }
+        run = run * ex.p[e];
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+          if (w == wi) pre[w] |= bit;
+      }
+      accR.add(run); /* success term, main.cu:254-262 */
+      ++c[C_PATHS];
+    }
+  }
+
+  /* per-workgroup flush; then publish this workgroup's segment count */
+  __syncthreads();
+  if (a.mass_mode)
+    for (int i = threadIdx.x; i < kClasses; i += blockDim.x)
+      a.hist[(u64)blockIdx.x * kClasses + i] = s_hist[i];
+  block_flush(a, accR, accE, accD, c, (unsigned char*)part);
+  if (threadIdx.x == 0) {
+    const u32 n = *s_cnt;
+    a.seg_count[parity ^ 1][blockIdx.x] = n < a.seg_cap ? n : a.seg_cap;
+  }
+}
+
+/* One expansion of each caller-supplied item into fixed output slots (the
+ * parity entry point nr_expand_items). */
+struct ItemsArgs {
+  DevGraph g;
+  Layout lay;
+  const u64* known; /* [count][w] */
+  const u64* up;
+  const double* mass;
+  int count, max_children;
+  int* nchild;
+  double* succ;
+  u64* ck; /* [count][max_children][w] */
+  u64* cu;
+  double* cm;
+};
+
+template <int W, class EX>
+__global__ void __launch_bounds__(256) k_expand_items(ItemsArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  if (EX::kStage) stage_tables(a.g, a.lay, smem);
+  __syncthreads();
+  EX ex;
+  ex.bind_graph(smem, a.lay, a.g, threadIdx.x);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.count) return;
+  u64 known[W], up[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    known[w] = a.known[(u64)i * W + w];
+    up[w] = a.up[(u64)i * W + w];
+  }
+  const double mass = a.mass[i];
+  ex.derive(known, up);
+  int k = ex.find_item(known, up);
+  if (k < 0) {
+    a.nchild[i] = k;
+    a.succ[i] = 0.0;
+    return;
+  }
+  a.nchild[i] = k;
+  double run = mass;
+  u64 pre[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) pre[w] = 0;
+  int c = 0;
+  for (int j = k - 1; j >= 0; --j, ++c) {
+    const int e = ex.uh[j * ex.stride];
+    const int wi = e >> 6;
+    const u64 bit = 1ull << (e & 63);
+    if (c < a.max_children) {
+      u64 base = ((u64)i * a.max_children + c) * W;
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        a.ck[base + w] = known[w] | pre[w] | (w == wi ? bit : 0ull);
+        a.cu[base + w] = up[w] | pre[w];
+      }
+      a.cm[(u64)i * a.max_children + c] = run * ex.q[e];
+    }
+    run = run * ex.p[e];
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+      if (w == wi) pre[w] |= bit;
+  }
+  a.succ[i] = run;
+}
+
+
+typedef void* (*KernelGetter)();
+
+/* Kernels of one width class; batch / refill exist for the row form only. */
+struct Variant {
+  KernelGetter level, items, batch, refill;
+  int ns_bytes; /* bytes per adjacency row carried by an item (0: none) */
+  int nw;       /* CsrExpander node-set words (0: Expander) */
+};
+
+/* CSR form for W (edge-plane words) and node-set words NW (nr_csr.hip). */
+Variant variant_csr(int W, int NW);
+
+}  // namespace nr
+
+#endif

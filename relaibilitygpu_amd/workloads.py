@@ -164,3 +164,27 @@ def write_graph_file(path: str, n: int, edges, probs) -> None:
         for u in range(n):
             rows[u].sort()
             f.write(f"{len(rows[u])} " + " ".join(f"{v} {p!r}" for v, p in rows[u]) + "\n")
+
+
+def random_digraph(n: int, mean_out: float, p_lo: float, p_hi: float, seed: int):
+    """Directed G(n, mean_out/(n-1)) over ordered pairs (u, v), u != v, in
+    lexicographic order; arc i gets id i and p ~ U[p_lo, p_hi]."""
+    rng = Xoshiro256ss(seed)
+    pa = mean_out / (n - 1)
+    arcs = [(u, v) for u in range(n) for v in range(n) if u != v and rng.uniform() < pa]
+    probs = [p_lo + (p_hi - p_lo) * rng.uniform() for _ in arcs]
+    return n, arcs, probs
+
+
+def to_csr_directed(n: int, arcs):
+    """CSR of one-way arcs (row u lists u -> v), rows sorted by head."""
+    rows = [[] for _ in range(n)]
+    for i, (a, b) in enumerate(arcs):
+        rows[a].append((b, i))
+    rp, ci, ei = [0], [], []
+    for u in range(n):
+        for v, i in sorted(rows[u]):
+            ci.append(v)
+            ei.append(i)
+        rp.append(len(ci))
+    return rp, ci, ei
