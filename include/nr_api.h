@@ -49,7 +49,8 @@ typedef enum nr_order {
                          (main.cu:963-1008); reports levels as iterations */
   NR_ORDER_MASS = 1,  /* max-mass first (bucketed device priority queue);
                          the reference's CPU heap order (seq/seq.cpp:49-197) */
-  NR_ORDER_LEARNED = 2 /* learned scorer orders the queue */
+  NR_ORDER_LEARNED = 2 /* learned scorer orders the queue: nr_solve_learned
+                          (nr_solve returns NR_ERR_ARG: it has no model) */
 } nr_order;
 
 typedef struct nr_config {
@@ -170,6 +171,78 @@ int nr_device_release(int device);
 /* Device name / compute units / wavefront size of a HIP device. */
 int nr_device_query(int device, char* name, int name_len, int* compute_units,
                     int* wave_size, long long* total_mem);
+
+/* ------------------------------------------------------------------------
+ * Learned work order.  No counterpart in the reference: a small MLP scores
+ * each child subproblem (predicted log2 of its subtree size under the plain
+ * method) and the class-round queue expands children in order of
+ * log2(mass) - alpha * score.  Any order expands the same exact
+ * decomposition; the score only changes what is resolved before a time or
+ * capacity limit, and R <= R_true <= R + ESP holds whatever it predicts.
+ *
+ * Features (16, see relaibilitygpu_amd/csrc/nr_learn.h child_features): the
+ * child's log2 mass, its UNKNOWN/WORKING/FAILED edge fractions, the failed
+ * hop's position on the parent's path, the path's length and unknown hops,
+ * q of the failed edge, the path suffix's log2 success probability, mass
+ * headroom over eps, allowed degrees of src and dst, terminal incidence,
+ * the parent's log2 mass, remaining unknown hops, and a constant 1.
+ * Model: W1[16][64] (row 15 = hidden bias), ReLU, w2[64], b2: NR_MLP_PARAMS
+ * floats in that order.
+ * ------------------------------------------------------------------------ */
+#define NR_MLP_FEATURES 16
+#define NR_MLP_HIDDEN 64
+#define NR_MLP_PARAMS (16 * 64 + 64 + 1)
+
+typedef struct nr_model nr_model;
+
+/* Solve in level order while logging every child (features at creation,
+ * parent, depth), then derive each child's subtree size bottom-up on the
+ * device and return up to max_records of them, sampled uniformly without
+ * replacement (seeded): features[max_records * 16], labels = log2(subtree
+ * nodes).  The solve's own result goes to *result.  Fails with
+ * NR_ERR_ARG if the queue overflowed (the tree would be incomplete). */
+int nr_log_subproblems(const nr_graph* g, const nr_config* cfg, long long max_records,
+                       unsigned long long seed, float* features, float* labels,
+                       long long* n_out, nr_result* result);
+
+typedef struct nr_train_config {
+  int epochs;         /* default 8 */
+  int batch;          /* samples per Adam step, default 4096 */
+  float lr;           /* default 3e-3 */
+  float weight_decay; /* decoupled, default 1e-5 */
+  unsigned long long seed;
+  int device;
+  float holdout;      /* fraction held out for the holdout loss, default 0.1 */
+  int reserved[4];
+} nr_train_config;
+
+int nr_train_config_init(nr_train_config* cfg);
+
+/* He-style random initial parameters (host). */
+int nr_mlp_init(unsigned long long seed, float* params);
+
+/* Mean loss (y_hat - y)^2 / 2 over n samples and its gradient with respect
+ * to the NR_MLP_PARAMS parameters, computed by the training kernels. */
+int nr_mlp_grad(const float* X, const float* y, long long n, const float* params,
+                float* grad, double* loss, int device);
+
+/* Train on the GPU (FP32 forward / backward kernels, Adam); params is the
+ * initial point on entry and the result on exit; per-epoch mean training
+ * and holdout losses (arrays of cfg->epochs, may be null). */
+int nr_mlp_train(const float* X, const float* y, long long n, const nr_train_config* cfg,
+                 float* params, float* train_loss, float* holdout_loss);
+
+/* Inference form of trained parameters (BF16 weights, FP32 accumulation). */
+int nr_model_create(const float* params, float alpha, nr_model** out);
+void nr_model_destroy(nr_model* model);
+
+/* Scores of n feature rows by the inference kernel (BF16 matrix cores). */
+int nr_model_score(const nr_model* model, const float* features, long long n,
+                   float* scores, int device);
+
+/* nr_solve with the learned order (cfg->order is ignored). */
+int nr_solve_learned(const nr_graph* g, const nr_config* cfg, const nr_model* model,
+                     nr_result* out);
 
 #ifdef __cplusplus
 }

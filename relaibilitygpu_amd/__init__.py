@@ -112,6 +112,23 @@ class GraphInfo(ctypes.Structure):
     ]
 
 
+class TrainConfig(ctypes.Structure):
+    _fields_ = [
+        ("epochs", ctypes.c_int),
+        ("batch", ctypes.c_int),
+        ("lr", ctypes.c_float),
+        ("weight_decay", ctypes.c_float),
+        ("seed", ctypes.c_ulonglong),
+        ("device", ctypes.c_int),
+        ("holdout", ctypes.c_float),
+        ("reserved", ctypes.c_int * 4),
+    ]
+
+
+MLP_FEATURES = 16
+MLP_HIDDEN = 64
+MLP_PARAMS = 16 * 64 + 64 + 1
+
 _lib = None
 
 
@@ -151,6 +168,18 @@ def lib() -> ctypes.CDLL:
     L.nr_status_str.restype = ctypes.c_char_p
     L.nr_device_query.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
                                   P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_longlong)]
+    fp = P(ctypes.c_float)
+    L.nr_log_subproblems.argtypes = [vp, P(Config), ctypes.c_longlong, ctypes.c_ulonglong, fp,
+                                     fp, P(ctypes.c_longlong), P(Result)]
+    L.nr_train_config_init.argtypes = [P(TrainConfig)]
+    L.nr_mlp_init.argtypes = [ctypes.c_ulonglong, fp]
+    L.nr_mlp_grad.argtypes = [fp, fp, ctypes.c_longlong, fp, fp, P(ctypes.c_double), ctypes.c_int]
+    L.nr_mlp_train.argtypes = [fp, fp, ctypes.c_longlong, P(TrainConfig), fp, fp, fp]
+    L.nr_model_create.argtypes = [fp, ctypes.c_float, P(vp)]
+    L.nr_model_destroy.argtypes = [vp]
+    L.nr_model_destroy.restype = None
+    L.nr_model_score.argtypes = [vp, fp, ctypes.c_longlong, fp, ctypes.c_int]
+    L.nr_solve_learned.argtypes = [vp, P(Config), vp, P(Result)]
     _lib = L
     return L
 
@@ -314,3 +343,99 @@ def device_query(device: int = 0) -> dict:
                                  ctypes.byref(mem)), "nr_device_query")
     return {"name": name.value.decode(), "compute_units": cus.value,
             "wave_size": ws.value, "total_mem": mem.value}
+
+
+# ---------------------------------------------------------------- learned order
+def _f32(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if n is not None:
+        a = a.reshape(n, -1) if a.ndim > 1 else a
+    return a
+
+
+def log_subproblems(graph: Graph, max_records: int, seed: int = 1,
+                    config: Config | None = None, **kw):
+    """Tree log of a level-order solve: (features [m,16], labels [m] =
+    log2 subtree nodes, result dict) for m <= max_records sampled children."""
+    cfg = config if config is not None else make_config(**kw)
+    X = np.zeros((max(max_records, 1), MLP_FEATURES), np.float32)
+    Y = np.zeros(max(max_records, 1), np.float32)
+    n = ctypes.c_longlong()
+    res = Result()
+    _check(lib().nr_log_subproblems(graph.handle, ctypes.byref(cfg), int(max_records), int(seed),
+                                    _ptr(X, ctypes.c_float), _ptr(Y, ctypes.c_float),
+                                    ctypes.byref(n), ctypes.byref(res)), "nr_log_subproblems")
+    return X[: n.value], Y[: n.value], res.as_dict()
+
+
+def mlp_init(seed: int = 1) -> np.ndarray:
+    p = np.zeros(MLP_PARAMS, np.float32)
+    _check(lib().nr_mlp_init(int(seed), _ptr(p, ctypes.c_float)), "nr_mlp_init")
+    return p
+
+
+def mlp_grad(X, y, params, device: int = 0):
+    X = _f32(X).reshape(-1, MLP_FEATURES)
+    y = _f32(y).reshape(-1)
+    p = _f32(params).reshape(MLP_PARAMS)
+    g = np.zeros(MLP_PARAMS, np.float32)
+    loss = ctypes.c_double()
+    _check(lib().nr_mlp_grad(_ptr(X, ctypes.c_float), _ptr(y, ctypes.c_float), len(y),
+                             _ptr(p, ctypes.c_float), _ptr(g, ctypes.c_float),
+                             ctypes.byref(loss), int(device)), "nr_mlp_grad")
+    return loss.value, g
+
+
+def mlp_train(X, y, params=None, epochs: int = 8, batch: int = 4096, lr: float = 3e-3,
+              weight_decay: float = 1e-5, seed: int = 1, holdout: float = 0.1, device: int = 0):
+    """Train on the GPU; returns (params, train_loss[epochs], holdout_loss[epochs])."""
+    X = _f32(X).reshape(-1, MLP_FEATURES)
+    y = _f32(y).reshape(-1)
+    p = (mlp_init(seed) if params is None else _f32(params).reshape(MLP_PARAMS)).copy()
+    c = TrainConfig()
+    lib().nr_train_config_init(ctypes.byref(c))
+    c.epochs, c.batch, c.lr, c.weight_decay = int(epochs), int(batch), float(lr), float(weight_decay)
+    c.seed, c.device, c.holdout = int(seed), int(device), float(holdout)
+    tl = np.zeros(epochs, np.float32)
+    hl = np.zeros(epochs, np.float32)
+    _check(lib().nr_mlp_train(_ptr(X, ctypes.c_float), _ptr(y, ctypes.c_float), len(y),
+                              ctypes.byref(c), _ptr(p, ctypes.c_float), _ptr(tl, ctypes.c_float),
+                              _ptr(hl, ctypes.c_float)), "nr_mlp_train")
+    return p, tl, hl
+
+
+class Model:
+    """Inference form of trained parameters (nr_model)."""
+
+    def __init__(self, params, alpha: float = 1.0):
+        self.params = _f32(params).reshape(MLP_PARAMS).copy()
+        self.alpha = float(alpha)
+        h = ctypes.c_void_p()
+        _check(lib().nr_model_create(_ptr(self.params, ctypes.c_float), self.alpha,
+                                     ctypes.byref(h)), "nr_model_create")
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def score(self, X, device: int = 0) -> np.ndarray:
+        X = _f32(X).reshape(-1, MLP_FEATURES)
+        y = np.zeros(len(X), np.float32)
+        _check(lib().nr_model_score(self._h, _ptr(X, ctypes.c_float), len(X),
+                                    _ptr(y, ctypes.c_float), int(device)), "nr_model_score")
+        return y
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.nr_model_destroy(h)
+            self._h = ctypes.c_void_p()
+
+
+def solve_learned(graph: Graph, model: Model, config: Config | None = None, **kw) -> dict:
+    cfg = config if config is not None else make_config(**kw)
+    r = Result()
+    _check(lib().nr_solve_learned(graph.handle, ctypes.byref(cfg), model.handle, ctypes.byref(r)),
+           "nr_solve_learned")
+    return r.as_dict()

@@ -10,7 +10,7 @@ namespace nr {
 
 template <int W, int NW>
 struct KernCsr {
-  static void* level() { return (void*)&k_expand_level<W, CsrExpander<W, NW>>; }
+  static void* level() { return (void*)&k_expand_level<W, CsrExpander<W, NW>, 0>; }
   static void* items() { return (void*)&k_expand_items<W, CsrExpander<W, NW>>; }
 };
 
@@ -19,10 +19,11 @@ static Variant variant_csr_nw(int W) {
   switch (W) {
 #define NR_W(X) \
   case X:      \
-    return Variant{&KernCsr<X, NW>::level, &KernCsr<X, NW>::items, nullptr, nullptr, 0, NW};
+    return Variant{&KernCsr<X, NW>::level, &KernCsr<X, NW>::items, nullptr, nullptr, 0, NW, \
+                   nullptr, nullptr};
     NR_W(1) NR_W(2) NR_W(3) NR_W(4) NR_W(5) NR_W(6) NR_W(7) NR_W(8)
 #undef NR_W
-    default: return Variant{nullptr, nullptr, nullptr, nullptr, 0, 0};
+    default: return Variant{nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr};
   }
 }
 
@@ -32,7 +33,7 @@ Variant variant_csr(int W, int NW) {
     case 2: return variant_csr_nw<2>(W);
     case 3:
     case 4: return variant_csr_nw<4>(W);
-    default: return Variant{nullptr, nullptr, nullptr, nullptr, 0, 0};
+    default: return Variant{nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr};
   }
 }
 

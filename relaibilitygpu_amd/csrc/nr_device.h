@@ -173,6 +173,7 @@ struct Expander {
   NS* lvl;  /* this lane's level sets, stride `stride` */
   u16* uh;  /* this lane's unknown-hop stack, stride `stride` */
   int stride, n, src, dst;
+  mutable int plen; /* hops of the last path found */
 
   __device__ void bind(unsigned char* smem, const Layout& L, int slot, int src_,
                        int dst_) {
@@ -214,6 +215,13 @@ struct Expander {
   __device__ void store_child(const QBuf& out, u64 qstride, u64 slot, int e) const {
     NS* rout = (NS*)out.rows + slot;
     for (int u = 0; u < n; ++u) rout[(u64)u * qstride] = child_row(u, e);
+  }
+  /* allowed degree of u in the loaded item (undirected: out = in) */
+  __device__ int out_degree(int u, const u64 (&)[W], const u64 (&)[W]) const {
+    return sizeof(NS) == 4 ? __popc((u32)rows[u * stride]) : __popcll((u64)rows[u * stride]);
+  }
+  __device__ int in_degree(int u, const u64 (&k)[W], const u64 (&p)[W]) const {
+    return out_degree(u, k, p);
   }
   __device__ void carry(const QBuf& in, u64 slot_in, const QBuf& out, u64 slot_out,
                         u64 qstride) const {
@@ -277,7 +285,10 @@ struct Expander {
    * Returns the number of UNKNOWN hops, -1 if dst is unreachable, -2 on an
    * internal inconsistency (never expected). */
   __device__ int find(const u64 (&known)[W]) const {
-    if (src == dst) return 0;
+    if (src == dst) {
+      plen = 0;
+      return 0;
+    }
     const NS dbit = (NS)1 << dst;
     NS vis = (NS)1 << src;
     NS F = vis;
@@ -300,6 +311,7 @@ struct Expander {
       vis |= nx;
       F = nx;
     }
+    plen = L;
     int v = dst, k = 0;
     for (int l = L; l >= 1; --l) {
       const NS cand = rows[v * stride] & lvl[(l - 1) * stride];
@@ -326,7 +338,10 @@ struct Expander {
   /* find() for an item given by its planes alone; same BFS, same parent
    * rule, same unknown-hop stack. */
   __device__ int find_planes(const u64 (&known)[W], const u64 (&up)[W]) const {
-    if (src == dst) return 0;
+    if (src == dst) {
+      plen = 0;
+      return 0;
+    }
     u64 Dn[W];
 #pragma unroll
     for (int i = 0; i < W; ++i) Dn[i] = known[i] & ~up[i];
@@ -353,6 +368,7 @@ struct Expander {
       vis |= nx;
       F = nx;
     }
+    plen = L;
     int v = dst, k = 0;
     for (int l = L; l >= 1; --l) {
       const NS cand = lazy_row(v, Dn, have) & lvl[(l - 1) * stride];
@@ -435,9 +451,11 @@ struct CsrExpander {
   const u32* arc;
   const int* irp;
   const u32* iarc;
+  const u32* ep;
   unsigned char* lvb; /* this lane's level bytes: node u at (u>>2)*4*stride + (u&3) */
   u16* uh;            /* this lane's unknown-hop stack, stride `stride` */
   int stride, n, src, dst;
+  mutable int plen; /* hops of the last path found */
 
   __device__ void bind_graph(unsigned char* smem, const Layout& L, const DevGraph& g, int slot) {
     p = g.p;
@@ -446,6 +464,7 @@ struct CsrExpander {
     arc = g.arc;
     irp = g.irp;
     iarc = g.iarc;
+    ep = g.ep;
     lvb = smem + L.off_lvl + 4 * slot;
     uh = (u16*)(smem + L.off_uh) + slot;
     stride = L.block;
@@ -458,6 +477,16 @@ struct CsrExpander {
   __device__ void store_child(const QBuf&, u64, u64, int) const {}
   __device__ void carry(const QBuf&, u64, const QBuf&, u64, u64) const {}
 
+  __device__ int out_degree(int u, const u64 (&known)[W], const u64 (&up)[W]) const {
+    int d = 0;
+    for (int j = rp[u]; j < rp[u + 1]; ++j) d += allowed(known, up, (int)(arc[j] >> 16)) ? 1 : 0;
+    return d;
+  }
+  __device__ int in_degree(int u, const u64 (&known)[W], const u64 (&up)[W]) const {
+    int d = 0;
+    for (int j = irp[u]; j < irp[u + 1]; ++j) d += allowed(known, up, (int)(iarc[j] >> 16)) ? 1 : 0;
+    return d;
+  }
   __device__ int level_of(int u) const { return lvb[(u >> 2) * 4 * stride + (u & 3)]; }
   __device__ void set_level(int u, int l) const {
     lvb[(u >> 2) * 4 * stride + (u & 3)] = (unsigned char)l;
@@ -471,7 +500,10 @@ struct CsrExpander {
   /* Pushes the path's UNKNOWN hops on uh[] in dst->src order; returns their
    * number, -1 if dst is unreachable, -2 on an internal inconsistency. */
   __device__ int find_item(const u64 (&known)[W], const u64 (&up)[W]) const {
-    if (src == dst) return 0;
+    if (src == dst) {
+      plen = 0;
+      return 0;
+    }
     for (int i = 0; i < (n + 3) / 4; ++i) *(u32*)(lvb + i * 4 * stride) = 0xffffffffu;
     u64 vis[NW], F[NW];
 #pragma unroll
@@ -520,6 +552,7 @@ struct CsrExpander {
         F[k] = nx[k];
       }
     }
+    plen = L;
     int v = dst, k = 0;
     for (int l = L; l >= 1; --l) {
       int pu = -1, pe = -1;

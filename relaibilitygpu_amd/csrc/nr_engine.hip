@@ -31,6 +31,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <random>
 #include <vector>
 
 #include "nr_device.h"
@@ -118,6 +119,7 @@ __global__ void __launch_bounds__(256) k_init(LevelArgs a, Root root, int W, int
         ((u64*)a.buf[0].rows)[(u64)u * a.stride] = root.rows[u];
     }
     a.buf[0].mass[0] = 1.0;
+    if (a.tag[0]) a.tag[0][0] = a.mlp ? 0u : 0xffffffffu; /* class 0 / no parent record */
   }
 }
 
@@ -781,7 +783,7 @@ int upload_tables(Workspace& ws, const nr_graph* g, int W, DevGraph& d) {
  * bits.  Each class is a separate instantiation of the kernels. */
 template <int W, typename NS>
 struct Kern {
-  static void* level() { return (void*)&k_expand_level<W, Expander<W, NS>>; }
+  static void* level() { return (void*)&k_expand_level<W, Expander<W, NS>, 0>; }
   static void* items() { return (void*)&k_expand_items<W, Expander<W, NS>>; }
   static void* batch() { return (void*)&k_expand_batch<W, NS>; }
   static void* refill() { return (void*)&k_refill_batch<W>; }
@@ -793,10 +795,10 @@ Variant variant_for_w(int W) {
 #define NR_W(X) \
   case X:      \
     return Variant{&Kern<X, NS>::level, &Kern<X, NS>::items, &Kern<X, NS>::batch,               \
-                   &Kern<X, NS>::refill, (int)sizeof(NS), 0};
+                   &Kern<X, NS>::refill, (int)sizeof(NS), 0, nullptr, nullptr};
     NR_W(1) NR_W(2) NR_W(3) NR_W(4) NR_W(5) NR_W(6) NR_W(7) NR_W(8)
 #undef NR_W
-    default: return Variant{nullptr, nullptr, nullptr, nullptr, 0, 0};
+    default: return Variant{nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr};
   }
 }
 
@@ -826,7 +828,9 @@ int pick_variant(const nr_graph* g, Variant& v) {
     v = g->n <= 32 ? variant_for_w<u32>(W) : variant_for_w<u64>(W);
   else
     v = variant_csr(W, (g->n + 63) / 64);
-  return v.level ? NR_OK : NR_ERR_UNSUPPORTED;
+  if (!v.level) return NR_ERR_UNSUPPORTED;
+  add_learn_kernels(v, W);
+  return NR_OK;
 }
 
 Root root_item(const nr_graph* g, int W) {
@@ -851,26 +855,44 @@ Root root_item(const nr_graph* g, int W) {
   return r;
 }
 
-int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_result* out) {
+/* Extras of a level-engine solve: the tree log (mode 1) or a learned
+ * scorer (mode 2). */
+struct LevelExtra {
+  int mode = 0;
+  float* log_feat = nullptr;
+  u32* log_parent = nullptr;
+  u32* log_level = nullptr;
+  u64* log_count = nullptr;
+  u64 log_cap = 0;
+  const MlpDev* mlp = nullptr;
+};
+
+int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_result* out,
+                const LevelExtra* x = nullptr) {
   Variant var;
   int rc = pick_variant(g, var);
   if (rc) return rc;
+  const int mode = x ? x->mode : 0;
+  const int tile_bytes = mode == 2 ? 32 * 256 + 16 : 0; /* per-wave feature tiles */
   const int W = mask_words(g);
   const int cus = ws.prop.multiProcessorCount;
   /* The prefix table of the segment counts (one per workgroup) lives in
    * LDS.  Size the grid from the occupancy without it, then add the table
    * for that grid; occupancy can only drop, so the table stays large
    * enough for the final grid. */
-  const void* kfun = (const void*)var.level();
-  int block = choose_block(var, g, W, cfg->block_size, 4 * (1026 + 256 + kClasses));
+  KernelGetter getter = mode == 1 ? var.level_log : mode == 2 ? var.level_learn : var.level;
+  if (!getter) return NR_ERR_UNSUPPORTED;
+  const void* kfun = (const void*)getter();
+  int block =
+      choose_block(var, g, W, cfg->block_size, 4 * (1026 + 256 + kClasses) + tile_bytes);
   if (block <= 0) return NR_ERR_UNSUPPORTED;
-  Layout lay0 = layout_for(var, g, W, block, 4 * (block + kClasses));
+  Layout lay0 = layout_for(var, g, W, block, 4 * (block + kClasses) + tile_bytes);
   NR_HIP(hipFuncSetAttribute(kfun, hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
   int per_cu0 = 0;
   NR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu0, kfun, block, lay0.total_bytes));
   if (per_cu0 < 1) return NR_ERR_UNSUPPORTED;
   int grid = cfg->grid_blocks > 0 ? cfg->grid_blocks : per_cu0 * cus;
-  const int extra = 4 * (grid + 2 + block + kClasses);
+  const int extra = 4 * (grid + 2 + block + kClasses) + tile_bytes;
   Layout lay = layout_for(var, g, W, block, extra);
   if (lay.total_bytes > 64 * 1024) return NR_ERR_UNSUPPORTED;
   int per_cu = 0;
@@ -879,7 +901,7 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   if (cfg->grid_blocks <= 0) grid = std::min(grid, per_cu * cus);
   const u64 n_slots = (u64)grid; /* partial sums and counters per workgroup */
   const u64 cap = cfg->queue_capacity > 0 ? (u64)cfg->queue_capacity : (1ull << 20);
-  const bool mass = cfg->order == NR_ORDER_MASS;
+  const bool mass = cfg->order == NR_ORDER_MASS || mode == 2;
   /* class rounds trim the pool to `cap` between rounds; the buffers get
    * twice that so a round's children never overflow before the trim */
   const u64 phys = mass ? 2 * cap : cap;
@@ -900,7 +922,8 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   /* SoA buffer: known[W], up[W], mass, rows[n] (each rows plane holds
    * node sets of ns_bytes) */
   const size_t rows_bytes = ((size_t)g->n * slots * var.ns_bytes + 255) / 256 * 256;
-  const size_t per_buf = (size_t)(2 * W + 1) * slots * 8 + rows_bytes;
+  const size_t tag_bytes = mode ? (slots * 4 + 255) / 256 * 256 : 0;
+  const size_t per_buf = (size_t)(2 * W + 1) * slots * 8 + rows_bytes + tag_bytes;
   if ((rc = ws.grow(&ws.q, &ws.q_bytes, 2 * per_buf))) return rc;
   for (int b = 0; b < 2; ++b) {
     u64* base = (u64*)((unsigned char*)ws.q + b * per_buf);
@@ -908,7 +931,16 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
     a.buf[b].up = base + (size_t)W * slots;
     a.buf[b].mass = (double*)(base + (size_t)2 * W * slots);
     a.buf[b].rows = (void*)(base + (size_t)(2 * W + 1) * slots);
+    a.tag[b] = mode ? (u32*)((unsigned char*)a.buf[b].rows + rows_bytes) : nullptr;
   }
+  if (mode == 1) {
+    a.log_feat = x->log_feat;
+    a.log_parent = x->log_parent;
+    a.log_level = x->log_level;
+    a.log_count = x->log_count;
+    a.log_cap = x->log_cap;
+  }
+  if (mode == 2) a.mlp = x->mlp;
   /* per-workgroup slots: 3 DD + C_N counters; segment counts */
   const size_t meta_bytes = n_slots * (3 * sizeof(DD) + C_N * sizeof(u64)) +
                             2 * sizeof(u32) * (2 * (size_t)grid + 4) +
@@ -949,7 +981,7 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   bool finished = false, deadline = false;
   long long timed = 0, rounds = 0;
   u64 max_pool = 1;
-  auto launch_level = [&](int parity) -> int {
+  auto launch_level = [&](int parity, int lv) -> int {
     hipEvent_t kb = nullptr, ke = nullptr;
     if ((size_t)timed < max_timed) {
       int r = ws.event(2 + 2 * timed, &kb);
@@ -957,7 +989,7 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
       if (r) return r;
       NR_HIP(hipEventRecord(kb, st));
     }
-    void* args[] = {&a, (void*)&parity};
+    void* args[] = {&a, (void*)&parity, (void*)&lv};
     NR_HIP(hipLaunchKernel(kfun, dim3(grid), dim3(block), args, lay.total_bytes, st));
     if (kb) {
       NR_HIP(hipEventRecord(ke, st));
@@ -980,7 +1012,7 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
     /* level order: chunks of back-to-back level launches */
     while (!finished) {
       for (int c = 0; c < chunk; ++c, ++level)
-        if ((rc = launch_level(level & 1))) return rc;
+        if ((rc = launch_level(level & 1, level))) return rc;
       NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
       NR_HIP(hipStreamSynchronize(st));
       if (ws.hctl->last_items == 0) finished = true;
@@ -1029,7 +1061,7 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
       a.thr_class = thr;
       a.drop_class = drop;
       const int parity = (int)(rounds & 1);
-      if ((rc = launch_level(parity))) return rc;
+      if ((rc = launch_level(parity, (int)rounds))) return rc;
       void* hargs[] = {&a};
       NR_HIP(hipLaunchKernel((const void*)&k_hist_total, dim3(1), dim3(kClasses), hargs, 0, st));
       NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
@@ -1459,6 +1491,142 @@ int expand_items(Workspace& ws, const nr_graph* g, int count, const u64* known,
   return NR_OK;
 }
 
+/* ---- tree log: subtree sizes of every logged child ---- */
+__global__ void __launch_bounds__(256) k_log_sizes(u64* size, u64 n) {
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (u64)gridDim.x * blockDim.x)
+    size[i] = 1;
+}
+
+/* records are appended launch by launch, so each depth is one contiguous
+ * range; first[d] receives its first record */
+__global__ void __launch_bounds__(256) k_log_bounds(const u32* level, u64 n, u64* first) {
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (u64)gridDim.x * blockDim.x)
+    if (i == 0 || level[i] != level[i - 1]) first[level[i]] = i;
+}
+
+/* one depth, deepest first: each child's finished subtree size is added to
+ * its parent's (the root has no record) */
+__global__ void __launch_bounds__(256) k_log_accumulate(const u32* parent, u64* size, u64 r0,
+                                                        u64 r1) {
+  for (u64 r = r0 + (u64)blockIdx.x * blockDim.x + threadIdx.x; r < r1;
+       r += (u64)gridDim.x * blockDim.x) {
+    const u32 p = parent[r];
+    if (p != 0xffffffffu) atomicAdd((unsigned long long*)&size[p], (unsigned long long)size[r]);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_log_gather(const float* feat, const u64* size,
+                                                    const u64* idx, u64 m, float* X, float* Y) {
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (u64)gridDim.x * blockDim.x) {
+    const u64 r = idx[i];
+    for (int k = 0; k < kFeat; ++k) X[i * kFeat + k] = feat[r * kFeat + k];
+    Y[i] = (float)log2((double)size[r]);
+  }
+}
+
+int log_subproblems(Workspace& ws, const nr_graph* g, const nr_config* cfg, long long max_out,
+                    unsigned long long seed, float* X, float* Y, long long* n_out,
+                    nr_result* res) {
+  size_t free_b = 0, total_b = 0;
+  NR_HIP(hipMemGetInfo(&free_b, &total_b));
+  const size_t rec_bytes = kFeat * 4 + 4 + 4 + 8;
+  const u64 log_cap = std::min<u64>((u64)(free_b / 4 / rec_bytes), 1ull << 31);
+  DevMem mem;
+  LevelExtra x;
+  x.mode = 1;
+  x.log_cap = log_cap;
+  u64* dsize;
+  int rc;
+  if ((rc = mem.alloc(&x.log_feat, (size_t)log_cap * kFeat)) ||
+      (rc = mem.alloc(&x.log_parent, log_cap)) || (rc = mem.alloc(&x.log_level, log_cap)) ||
+      (rc = mem.alloc(&x.log_count, 1)) || (rc = mem.alloc(&dsize, log_cap)))
+    return rc;
+  hipStream_t st = ws.stream;
+  NR_HIP(hipMemsetAsync(x.log_count, 0, sizeof(u64), st));
+  nr_config c2 = *cfg;
+  c2.order = NR_ORDER_LEVEL;
+  c2.time_budget_s = 0.0; /* a log needs whole subtrees */
+  if ((rc = solve_level(ws, g, &c2, res, &x))) return rc;
+  u64 count = 0;
+  NR_HIP(hipMemcpy(&count, x.log_count, sizeof(u64), hipMemcpyDeviceToHost));
+  if (count > log_cap) return NR_ERR_NOMEM;   /* tree larger than the log */
+  if (res->overflowed) return NR_ERR_ARG;     /* capacity too small for a whole tree */
+  const int depth = (int)res->iterations;     /* records have depth 1 .. depth - 1 */
+  std::vector<u64> first((size_t)depth + 2, count);
+  if (count) {
+    u64* dfirst;
+    if ((rc = mem.alloc(&dfirst, first.size()))) return rc;
+    NR_HIP(hipMemcpyAsync(dfirst, first.data(), 8 * first.size(), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_log_sizes, dim3(1024), dim3(256), 0, st, dsize, count);
+    hipLaunchKernelGGL(k_log_bounds, dim3(1024), dim3(256), 0, st, (const u32*)x.log_level,
+                       count, dfirst);
+    NR_HIP(hipMemcpyAsync(first.data(), dfirst, 8 * first.size(), hipMemcpyDeviceToHost, st));
+    NR_HIP(hipStreamSynchronize(st));
+    /* a depth with no record keeps `count`; ends run to the next depth's start */
+    for (int d = depth; d >= 2; --d) {
+      const u64 r0 = first[d];
+      u64 r1 = count;
+      for (int e = d + 1; e <= depth + 1; ++e)
+        if (first[e] < r1 && first[e] >= r0) r1 = first[e];
+      if (r0 >= r1) continue;
+      hipLaunchKernelGGL(k_log_accumulate, dim3(1024), dim3(256), 0, st,
+                         (const u32*)x.log_parent, dsize, r0, r1);
+    }
+  }
+  const u64 m = std::min<u64>((u64)std::max<long long>(max_out, 0), count);
+  std::vector<u64> idx((size_t)m);
+  if (m == count) {
+    for (u64 i = 0; i < m; ++i) idx[i] = i;
+  } else { /* m distinct records, uniformly (Floyd's algorithm), in record order */
+    std::mt19937_64 rng(seed);
+    std::vector<u64> pick;
+    pick.reserve((size_t)m);
+    std::vector<char> taken;
+    taken.assign((size_t)count, 0);
+    for (u64 j = count - m; j < count; ++j) {
+      const u64 t = std::uniform_int_distribution<u64>(0, j)(rng);
+      const u64 v = taken[t] ? j : t;
+      taken[v] = 1;
+    }
+    u64 w = 0;
+    for (u64 i = 0; i < count; ++i)
+      if (taken[i]) idx[w++] = i;
+  }
+  if (m) {
+    u64* didx;
+    float *dX, *dY;
+    if ((rc = mem.alloc(&didx, m)) || (rc = mem.alloc(&dX, (size_t)m * kFeat)) ||
+        (rc = mem.alloc(&dY, m)))
+      return rc;
+    NR_HIP(hipMemcpyAsync(didx, idx.data(), 8 * m, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_log_gather, dim3(1024), dim3(256), 0, st, (const float*)x.log_feat,
+                       (const u64*)dsize, (const u64*)didx, m, dX, dY);
+    NR_HIP(hipMemcpyAsync(X, dX, sizeof(float) * m * kFeat, hipMemcpyDeviceToHost, st));
+    NR_HIP(hipMemcpyAsync(Y, dY, sizeof(float) * m, hipMemcpyDeviceToHost, st));
+    NR_HIP(hipStreamSynchronize(st));
+  }
+  *n_out = (long long)m;
+  return NR_OK;
+}
+
+const MlpDev* model_dev(const nr_model* m); /* nr_learn.hip */
+
+int solve_learned(Workspace& ws, const nr_graph* g, const nr_config* cfg, const nr_model* model,
+                  nr_result* out) {
+  DevMem mem;
+  MlpDev* dm;
+  int rc;
+  if ((rc = mem.alloc(&dm, 1))) return rc;
+  NR_HIP(hipMemcpyAsync(dm, model_dev(model), sizeof(MlpDev), hipMemcpyHostToDevice, ws.stream));
+  LevelExtra x;
+  x.mode = 2;
+  x.mlp = dm;
+  return solve_level(ws, g, cfg, out, &x);
+}
+
 /* Locks and opens the workspace of a device. */
 struct WsLock {
   Workspace* ws = nullptr;
@@ -1489,13 +1657,48 @@ int nr_solve(const nr_graph* g, const nr_config* cfg, nr_result* out) {
   }
   if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0 || cfg->time_budget_s < 0)
     return NR_ERR_ARG;
-  if (cfg->order != NR_ORDER_LEVEL && cfg->order != NR_ORDER_MASS) return NR_ERR_UNSUPPORTED;
+  if (cfg->order == NR_ORDER_LEARNED) return NR_ERR_ARG; /* needs a model: nr_solve_learned */
+  if (cfg->order != NR_ORDER_LEVEL && cfg->order != NR_ORDER_MASS) return NR_ERR_ARG;
   nr::Variant var;
   int rc = nr::pick_variant(g, var);
   if (rc) return rc;
   nr::WsLock L(cfg->device);
   if (L.rc) return L.rc;
   return nr::solve_level(*L.ws, g, cfg, out);
+}
+
+int nr_solve_learned(const nr_graph* g, const nr_config* cfg, const nr_model* model,
+                     nr_result* out) {
+  if (!g || !out || !model) return NR_ERR_ARG;
+  nr_config def;
+  if (!cfg) {
+    nr_config_init(&def);
+    cfg = &def;
+  }
+  if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0 || cfg->time_budget_s < 0)
+    return NR_ERR_ARG;
+  nr::Variant var;
+  int rc = nr::pick_variant(g, var);
+  if (rc) return rc;
+  nr::WsLock L(cfg->device);
+  if (L.rc) return L.rc;
+  return nr::solve_learned(*L.ws, g, cfg, model, out);
+}
+
+int nr_log_subproblems(const nr_graph* g, const nr_config* cfg, long long max_records,
+                       unsigned long long seed, float* features, float* labels,
+                       long long* n_out, nr_result* result) {
+  if (!g || !cfg || max_records < 0 || !n_out || !result ||
+      (max_records > 0 && (!features || !labels)))
+    return NR_ERR_ARG;
+  if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0) return NR_ERR_ARG;
+  nr::Variant var;
+  int rc = nr::pick_variant(g, var);
+  if (rc) return rc;
+  if (!var.level_log) return NR_ERR_UNSUPPORTED;
+  nr::WsLock L(cfg->device);
+  if (L.rc) return L.rc;
+  return nr::log_subproblems(*L.ws, g, cfg, max_records, seed, features, labels, n_out, result);
 }
 
 int nr_solve_batch(const nr_graph* const* graphs, int count, const nr_config* cfg,

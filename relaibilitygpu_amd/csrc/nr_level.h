@@ -13,6 +13,7 @@
 #include <cstring>
 
 #include "nr_device.h"
+#include "nr_learn.h"
 
 namespace nr {
 
@@ -64,6 +65,15 @@ struct LevelArgs {
   int mass_mode;
   int thr_class;
   int drop_class;
+  /* tree log (level order): one record per child, written by its parent */
+  u32* tag[2];        /* per slot: record id (log) or class (learned order) */
+  float* log_feat;    /* [cap][kFeat] child features (bf16-rounded) */
+  u32* log_parent;    /* [cap] parent record, 0xffffffff for the root */
+  u32* log_level;     /* [cap] tree depth of the child */
+  u64* log_count;
+  u64 log_cap;
+  /* learned order */
+  const MlpDev* mlp;
 };
 
 template <int W>
@@ -190,10 +200,41 @@ __device__ inline void block_flush(const LevelArgs& a, Acc2& accR, Acc2& accE, A
   }
 }
 
-/* One tree level: consume buf[parity], append children to the workgroup's
- * segment of buf[parity^1]. */
+/* Parent context of an item's children for the feature function. */
 template <int W, class EX>
-__global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity) {
+__device__ inline void child_ctx(const LevelArgs& a, const EX& ex, const u64 (&known)[W],
+                                 const u64 (&up)[W], double mass, int k, ChildCtx& c) {
+  int pk = 0, pu = 0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    pk += __popcll(known[w]);
+    pu += __popcll(up[w]);
+  }
+  c.inv_e = 1.0f / (float)(a.g.e > 0 ? a.g.e : 1);
+  c.inv_n = 1.0f / (float)a.g.n;
+  c.inv_k = 1.0f / (float)(k > 0 ? k : 1);
+  c.u_p = a.g.e - pk;
+  c.w_p = pu;
+  c.f_p = pk - pu;
+  c.k = k;
+  c.len = ex.plen;
+  c.src = a.g.src;
+  c.dst = a.g.dst;
+  c.deg_src = ex.out_degree(a.g.src, known, up);
+  c.deg_dst = ex.in_degree(a.g.dst, known, up);
+  c.log2_m = flog2(mass);
+  float lp = 0.0f;
+  for (int j = 0; j < k; ++j) lp += __log2f((float)ex.p[ex.uh[j * ex.stride]]);
+  c.lp_total = lp;
+  c.eps = a.eps;
+}
+
+/* One tree level: consume buf[parity], append children to the workgroup's
+ * segment of buf[parity^1].  MODE 0: level order or max-mass class rounds;
+ * MODE 1: level order that also writes the tree log; MODE 2: class rounds
+ * on the learned score (children scored 64 at a time on the matrix cores). */
+template <int W, class EX, int MODE>
+__global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, int level) {
   extern __shared__ __align__(16) unsigned char smem[];
   u32* sp = (u32*)(smem + a.lay.off_extra);
   u32* s_cnt = sp + a.n_seg + 1;
@@ -216,6 +257,9 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity) {
   }
   if (threadIdx.x == 0) *s_cnt = 0;
   u32* s_hist = part + blockDim.x; /* [kClasses], class rounds only */
+  /* per-wave feature tile [64][kFeat] bf16 (learned order), 16-byte aligned */
+  u16* tile = (u16*)(((size_t)(s_hist + kClasses) + 15) & ~(size_t)15) +
+              (threadIdx.x >> 6) * 64 * kFeat;
   if (a.mass_mode)
     for (int i = threadIdx.x; i < kClasses; i += blockDim.x) s_hist[i] = 0;
   if (EX::kStage) stage_tables(a.g, a.lay, smem);
@@ -230,6 +274,17 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity) {
   const u64 wave_id = (u64)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const u64 n_waves = (u64)(gridDim.x * blockDim.x) >> 6;
   const u64 out_base = (u64)blockIdx.x * a.seg_cap;
+
+  bf16x8_t afrag[2];
+  u32 sign = 0;
+  float b2 = 0.0f, alpha = 0.0f;
+  if (MODE == 2) {
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) afrag[mt] = *(const bf16x8_t*)a.mlp->afrag[mt][lane];
+    sign = a.mlp->sign[lane];
+    b2 = a.mlp->b2;
+    alpha = a.mlp->alpha;
+  }
 
   Acc2 accR, accE, accD;
   accR.init();
@@ -248,12 +303,14 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity) {
     int k = 0, stored = 0;
     bool live = false, carry = false;
     u64 slot_in = 0;
+    u32 tag_in = 0xffffffffu;
     if ((u64)lane < chunk && i < n_items) {
       int s = segment_of(sp, a.n_seg, (u32)base);
       while (sp[s + 1] <= i) ++s;
       slot_in = (u64)s * a.seg_cap + (i - sp[s]);
       load_item<W>(in, a.stride, slot_in, known, up, mass);
-      const int cl = a.mass_mode ? mass_class(mass) : 0;
+      if (MODE != 0) tag_in = a.tag[parity][slot_in];
+      const int cl = !a.mass_mode ? 0 : MODE == 2 ? (int)tag_in : mass_class(mass);
       if (a.mass_mode && cl >= a.drop_class) { /* given up: queue over capacity */
         accE.add(mass);
         ++c[C_FOLD];
@@ -293,25 +350,107 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity) {
     const int incl = wave_incl_scan(stored);
     const int total = __shfl(incl, 63, 64);
     u32 wbase = 0;
+    u64 rbase = 0;
     if (total) {
-      if (lane == 63) wbase = atomicAdd(s_cnt, (u32)total);
+      if (lane == 63) {
+        wbase = atomicAdd(s_cnt, (u32)total);
+        if (MODE == 1)
+          rbase = atomicAdd((unsigned long long*)a.log_count, (unsigned long long)total);
+      }
       wbase = __shfl(wbase, 63, 64);
+      if (MODE == 1) rbase = __shfl(rbase, 63, 64);
     }
     u32 off = wbase + (u32)(incl - stored);
     if (carry) {
       if (off < a.seg_cap) {
         store_item<W>(out, a.stride, out_base + off, known, up, mass);
         ex.carry(in, slot_in, out, out_base + off, a.stride);
-        atomicAdd(&s_hist[mass_class(mass)], 1u);
+        if (MODE == 2) a.tag[parity ^ 1][out_base + off] = tag_in;
+        atomicAdd(&s_hist[MODE == 2 ? (int)tag_in : mass_class(mass)], 1u);
       } else {
         accE.add(mass);
         ++c[C_FOLD];
       }
     }
-    if (live) {
+    ChildCtx cx;
+    if (MODE != 0 && live) child_ctx<W, EX>(a, ex, known, up, mass, k, cx);
+    if (MODE == 2) {
+      /* children in rounds, one per lane per round, src side first; every
+       * lane of the wave takes part in each round's tile score */
+      int kmax = live ? k : 0;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) kmax = max(kmax, __shfl_xor(kmax, o, 64));
+      double run = mass;
+      float lp_prefix = 0.0f;
+      u64 pre[W];
+#pragma unroll
+      for (int w = 0; w < W; ++w) pre[w] = 0;
+      for (int t = 0; t < kmax; ++t) {
+        const bool has = live && t < k;
+        int e = 0;
+        double cm = 0.0;
+        float f[kFeat];
+#pragma unroll
+        for (int x = 0; x < kFeat; ++x) f[x] = 0.0f;
+        if (has) {
+          e = ex.uh[(k - 1 - t) * ex.stride];
+          cm = run * ex.q[e];
+          const u32 xy = ex.ep[e];
+          child_features(cx, t, (int)(xy & 0xffffu), (int)(xy >> 16), ex.p[e], ex.q[e], cm,
+                         lp_prefix, f);
+        }
+        u32* trow = (u32*)(tile + lane * kFeat);
+#pragma unroll
+        for (int x = 0; x < kFeat; x += 2)
+          trow[x / 2] = (u32)bf16_bits(f[x]) | ((u32)bf16_bits(f[x + 1]) << 16);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        const float y = mlp_score_tile(afrag, sign, b2, tile);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (has) {
+          const int wi = e >> 6;
+          const u64 bit = 1ull << (e & 63);
+          if (cm < a.eps) { /* settled at creation: counted as a truncated node */
+            accE.add(cm);
+            ++c[C_NODES];
+            ++c[C_TRUNC];
+          } else {
+            if (off < a.seg_cap) {
+              u64 ck[W], cu[W];
+#pragma unroll
+              for (int w = 0; w < W; ++w) {
+                ck[w] = known[w] | pre[w] | (w == wi ? bit : 0ull);
+                cu[w] = up[w] | pre[w];
+              }
+              store_item<W>(out, a.stride, out_base + off, ck, cu, cm);
+              ex.store_child(out, a.stride, out_base + off, e);
+              const int cl = score_class(flog2(cm), y, alpha, kClasses);
+              a.tag[parity ^ 1][out_base + off] = (u32)cl;
+              atomicAdd(&s_hist[cl], 1u);
+            } else { /* segment full: main.cu:245-252 */
+              accE.add(cm);
+              ++c[C_PATHS];
+              ++c[C_OVER];
+            }
+            ++off;
+          }
+          run = run * ex.p[e];
+          lp_prefix += __log2f((float)ex.p[e]);
+#pragma unroll
+          for (int w = 0; w < W; ++w)
+            if (w == wi) pre[w] |= bit;
+        }
+      }
+      if (live) {
+        accR.add(run); /* success term, main.cu:254-262 */
+        ++c[C_PATHS];
+      }
+    } else if (live) {
       /* main.cu:214-266: child j has the earlier UNKNOWN hops WORKING and
        * hop j FAILED; masses multiply left to right along the path */
       double run = mass;
+      float lp_prefix = 0.0f;
       u64 pre[W];
 #pragma unroll
       for (int w = 0; w < W; ++w) pre[w] = 0;
@@ -340,6 +479,21 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity) {
             store_item<W>(out, a.stride, out_base + off, ck, cu, cm);
             ex.store_child(out, a.stride, out_base + off, e);
             if (a.mass_mode) atomicAdd(&s_hist[mass_class(cm)], 1u);
+            if (MODE == 1) {
+              const u64 rid = rbase + (off - wbase);
+              a.tag[parity ^ 1][out_base + off] = (u32)rid;
+              if (rid < a.log_cap) {
+                float f[kFeat];
+                const u32 xy = ex.ep[e];
+                child_features(cx, k - 1 - j, (int)(xy & 0xffffu), (int)(xy >> 16), ex.p[e],
+                               ex.q[e], cm, lp_prefix, f);
+                float* dstf = a.log_feat + rid * kFeat;
+#pragma unroll
+                for (int x = 0; x < kFeat; ++x) dstf[x] = bf16_value(bf16_bits(f[x]));
+                a.log_parent[rid] = tag_in;
+                a.log_level[rid] = (u32)level + 1u;
+              }
+            }
           } else { /* segment full: main.cu:245-252 */
             accE.add(cm);
             ++c[C_PATHS];
@@ -348,6 +502,7 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity) {
           ++off;
         }
         run = run * ex.p[e];
+        if (MODE == 1) lp_prefix += __log2f((float)ex.p[e]);
 #pragma unroll
         for (int w = 0; w < W; ++w)
           if (w == wi) pre[w] |= bit;
@@ -443,7 +598,11 @@ struct Variant {
   KernelGetter level, items, batch, refill;
   int ns_bytes; /* bytes per adjacency row carried by an item (0: none) */
   int nw;       /* CsrExpander node-set words (0: Expander) */
+  KernelGetter level_log, level_learn; /* MODE 1 / MODE 2 of k_expand_level */
 };
+
+/* MODE 1 / MODE 2 kernels of a variant (nr_learn.hip), filled into v. */
+void add_learn_kernels(Variant& v, int W);
 
 /* CSR form for W (edge-plane words) and node-set words NW (nr_csr.hip). */
 Variant variant_csr(int W, int NW);
