@@ -67,8 +67,18 @@ typedef struct nr_config {
   int chunk_items;          /* persistent modes: items claimed per wave refill;
                                0 = auto */
   int flags;                /* NR_FLAG_* */
-  int reserved[6];
+  int rule;                 /* nr_rule: which decomposition each item gets */
+  int reserved[5];
 } nr_config;
+
+/* Decomposition rule (nr_config.rule).  Both are exact; they build
+ * different trees. */
+typedef enum nr_rule {
+  NR_RULE_PATH = 0,    /* k children per path (main.cu:114-283), default */
+  NR_RULE_FORKBEST = 1 /* success on a confirmed WORKING path, else a two-way
+                          branch on the path's most reliable UNKNOWN edge,
+                          nearest dst on ties (main.cu:304-430, 550-567) */
+} nr_rule;
 
 /* nr_config.flags */
 #define NR_FLAG_TIME_LAUNCHES 1 /* time every expand launch with HIP events

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle_nr.so")
 CLI_PATH = os.path.join(HERE, "bin", "cpu_reference")
 
-MODE_PATH, MODE_PIVOT = 0, 1
+MODE_PATH, MODE_PIVOT, MODE_FORKBEST, MODE_MINUNK = 0, 1, 2, 3
 ORDER_LEVEL, ORDER_HEAP, ORDER_DFS = 0, 1, 2
 
 

@@ -29,6 +29,9 @@ NR_ORDER_LEVEL = 0
 NR_ORDER_MASS = 1
 NR_ORDER_LEARNED = 2
 ORDERS = {"level": NR_ORDER_LEVEL, "mass": NR_ORDER_MASS, "learned": NR_ORDER_LEARNED}
+NR_RULE_PATH = 0
+NR_RULE_FORKBEST = 1
+RULES = {"path": NR_RULE_PATH, "forkbest": NR_RULE_FORKBEST}
 
 STATUS = {
     0: "NR_OK",
@@ -61,7 +64,8 @@ class Config(ctypes.Structure):
         ("grid_blocks", ctypes.c_int),
         ("chunk_items", ctypes.c_int),
         ("flags", ctypes.c_int),
-        ("reserved", ctypes.c_int * 6),
+        ("rule", ctypes.c_int),
+        ("reserved", ctypes.c_int * 5),
     ]
 
 
@@ -196,7 +200,7 @@ def _ptr(a: np.ndarray, ctype):
 def make_config(queue_capacity: int = 1 << 20, truncation_eps: float = 1e-15,
                 time_budget_s: float = 0.0, order: str | int = "level", device: int = 0,
                 block_size: int = 0, grid_blocks: int = 0, chunk_items: int = 0,
-                time_launches: bool = False) -> Config:
+                time_launches: bool = False, rule: str | int = "path") -> Config:
     c = Config()
     lib().nr_config_init(ctypes.byref(c))
     c.queue_capacity = int(queue_capacity)
@@ -208,6 +212,7 @@ def make_config(queue_capacity: int = 1 << 20, truncation_eps: float = 1e-15,
     c.grid_blocks = int(grid_blocks)
     c.chunk_items = int(chunk_items)
     c.flags = NR_FLAG_TIME_LAUNCHES if time_launches else 0
+    c.rule = RULES[rule] if isinstance(rule, str) else int(rule)
     return c
 
 

@@ -4,7 +4,8 @@
  *
  *   network_reliability <graph_file> <src> <dst> [--truncation_eps f]
  *       [--queue_capacity n] [--thresh_high f] [--thresh_low f]
- *       [--order level|mass|learned] [--time_budget s] [--device d] [--json]
+ *       [--order level|mass|learned] [--rule path|forkbest] [--model file]
+ *       [--alpha a] [--time_budget s] [--device d] [--json]
  *
  * Argument handling follows workingPad/main.cu:753-790: fewer than three
  * positionals print the usage on stderr and exit 1; unknown flags and flags
@@ -28,6 +29,9 @@ static void usage(const char* prog, long long cap) {
                "  --thresh_high <float>      High-priority log-prob threshold\n"
                "  --thresh_low  <float>      Low-priority  log-prob threshold\n"
                "  --order <level|mass|learned> Work order (default level)\n"
+               "  --rule <path|forkbest>     Decomposition rule (default path)\n"
+               "  --model <file>             Learned order: NR_MLP_PARAMS raw float32\n"
+               "  --alpha <float>            Learned order: score weight (default 0.5)\n"
                "  --time_budget <seconds>    Fold the queue into ESP after this\n"
                "  --device <int>             HIP device ordinal\n"
                "  --json                     Print one JSON line instead\n",
@@ -45,6 +49,8 @@ int main(int argc, char** argv) {
   int src = std::atoi(argv[2]);
   int dst = std::atoi(argv[3]);
   bool json = false;
+  const char* model_file = nullptr;
+  float alpha = 0.5f;
   double thresh_high = -4.60517, thresh_low = -9.21034; /* common.h:35-36 */
   for (int i = 4; i < argc; ++i) {
     const bool has = i + 1 < argc;
@@ -61,6 +67,12 @@ int main(int argc, char** argv) {
       cfg.order = !std::strcmp(o, "mass")      ? NR_ORDER_MASS
                   : !std::strcmp(o, "learned") ? NR_ORDER_LEARNED
                                                : NR_ORDER_LEVEL;
+    } else if (!std::strcmp(argv[i], "--rule") && has) {
+      cfg.rule = !std::strcmp(argv[++i], "forkbest") ? NR_RULE_FORKBEST : NR_RULE_PATH;
+    } else if (!std::strcmp(argv[i], "--model") && has) {
+      model_file = argv[++i];
+    } else if (!std::strcmp(argv[i], "--alpha") && has) {
+      alpha = (float)std::atof(argv[++i]);
     } else if (!std::strcmp(argv[i], "--time_budget") && has) {
       cfg.time_budget_s = std::atof(argv[++i]);
     } else if (!std::strcmp(argv[i], "--device") && has) {
@@ -101,7 +113,24 @@ int main(int argc, char** argv) {
     std::fflush(stdout);
   }
   nr_result r;
-  rc = nr_solve(g, &cfg, &r);
+  if (cfg.order == NR_ORDER_LEARNED) {
+    float params[NR_MLP_PARAMS];
+    FILE* f = model_file ? std::fopen(model_file, "rb") : nullptr;
+    const size_t got = f ? std::fread(params, sizeof(float), NR_MLP_PARAMS, f) : 0;
+    if (f) std::fclose(f);
+    if (got != NR_MLP_PARAMS) {
+      std::fprintf(stderr, "--order learned needs --model with %d float32 parameters\n",
+                   NR_MLP_PARAMS);
+      nr_graph_destroy(g);
+      return 1;
+    }
+    nr_model* m = nullptr;
+    rc = nr_model_create(params, alpha, &m);
+    if (rc == NR_OK) rc = nr_solve_learned(g, &cfg, m, &r);
+    nr_model_destroy(m);
+  } else {
+    rc = nr_solve(g, &cfg, &r);
+  }
   nr_graph_destroy(g);
   if (rc != NR_OK) {
     std::fprintf(stderr, "solve failed: %s\n", nr_status_str(rc));

@@ -216,6 +216,61 @@ struct Expander {
     NS* rout = (NS*)out.rows + slot;
     for (int u = 0; u < n; ++u) rout[(u64)u * qstride] = child_row(u, e);
   }
+  /* neighbours of u over WORKING edges */
+  __device__ NS working_row(int u, const u64 (&up)[W]) const {
+    NS r = 0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+      u64 m = inc[u * W + i] & up[i];
+      while (m) {
+        const int e = i * 64 + __builtin_ctzll(m);
+        m &= m - 1;
+        const u32 x = ep[e];
+        r |= (NS)1 << ((int)((x & 0xffffu) ^ (x >> 16)) ^ u);
+      }
+    }
+    return r;
+  }
+  /* forkBest (main.cu:304-430, 550-567): 0 when a WORKING-only path joins
+   * src to dst (success), -1 dead, -2 inconsistency, else 1 + the edge to
+   * branch on: the path's UNKNOWN hop of highest p, the one nearest dst on
+   * ties */
+  __device__ int fork_item(const u64 (&known)[W], const u64 (&up)[W]) const {
+    if (src == dst) return 0;
+    const NS dbit = (NS)1 << dst;
+    NS vis = (NS)1 << src, F = vis;
+    while (F) {
+      NS nx = 0;
+      NS f = F;
+      while (f) {
+        const int u = ctzs<NS>(f);
+        f &= f - 1;
+        nx |= working_row(u, up);
+      }
+      nx &= ~vis;
+      if (nx & dbit) return 0;
+      vis |= nx;
+      F = nx;
+    }
+    const int k = find(known);
+    if (k <= 0) return k;
+    int best = uh[0];
+    double bp = p[best];
+    for (int j = 1; j < k; ++j) {
+      const int e = uh[j * stride];
+      if (p[e] > bp) {
+        bp = p[e];
+        best = e;
+      }
+    }
+    return 1 + best;
+  }
+  /* rows of a child that keeps the parent's allowed edges */
+  __device__ void store_rows(const QBuf& out, u64 qstride, u64 slot) const {
+    NS* rout = (NS*)out.rows + slot;
+    for (int u = 0; u < n; ++u) rout[(u64)u * qstride] = rows[u * stride];
+  }
+
   /* allowed degree of u in the loaded item (undirected: out = in) */
   __device__ int out_degree(int u, const u64 (&)[W], const u64 (&)[W]) const {
     return sizeof(NS) == 4 ? __popc((u32)rows[u * stride]) : __popcll((u64)rows[u * stride]);
@@ -477,6 +532,53 @@ struct CsrExpander {
   __device__ void store_child(const QBuf&, u64, u64, int) const {}
   __device__ void carry(const QBuf&, u64, const QBuf&, u64, u64) const {}
 
+  __device__ void store_rows(const QBuf&, u64, u64) const {}
+  /* forkBest on the CSR arcs; same contract as Expander::fork_item */
+  __device__ int fork_item(const u64 (&known)[W], const u64 (&up)[W]) const {
+    if (src == dst) return 0;
+    u64 vis[NW], F[NW];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) vis[k] = F[k] = 0;
+    bs_set<NW>(vis, src);
+    bs_set<NW>(F, src);
+    for (bool any = true; any;) {
+      u64 nx[NW];
+#pragma unroll
+      for (int k = 0; k < NW; ++k) nx[k] = 0;
+      any = false;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) {
+        u64 fw = F[k];
+        while (fw) {
+          const int u = k * 64 + __builtin_ctzll(fw);
+          fw &= fw - 1;
+          for (int j = rp[u]; j < rp[u + 1]; ++j) {
+            const u32 x = arc[j];
+            const int v = (int)(x & 0xffffu), e = (int)(x >> 16);
+            if (bs_test<NW>(vis, v) || !((pick<W>(up, e >> 6) >> (e & 63)) & 1ull)) continue;
+            if (v == dst) return 0;
+            bs_set<NW>(vis, v);
+            bs_set<NW>(nx, v);
+            any = true;
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < NW; ++k) F[k] = nx[k];
+    }
+    const int k = find_item(known, up);
+    if (k <= 0) return k;
+    int best = uh[0];
+    double bp = p[best];
+    for (int j = 1; j < k; ++j) {
+      const int e = uh[j * stride];
+      if (p[e] > bp) {
+        bp = p[e];
+        best = e;
+      }
+    }
+    return 1 + best;
+  }
   __device__ int out_degree(int u, const u64 (&known)[W], const u64 (&up)[W]) const {
     int d = 0;
     for (int j = rp[u]; j < rp[u + 1]; ++j) d += allowed(known, up, (int)(arc[j] >> 16)) ? 1 : 0;

@@ -795,10 +795,10 @@ Variant variant_for_w(int W) {
 #define NR_W(X) \
   case X:      \
     return Variant{&Kern<X, NS>::level, &Kern<X, NS>::items, &Kern<X, NS>::batch,               \
-                   &Kern<X, NS>::refill, (int)sizeof(NS), 0, nullptr, nullptr};
+                   &Kern<X, NS>::refill, (int)sizeof(NS), 0, nullptr, nullptr, nullptr};
     NR_W(1) NR_W(2) NR_W(3) NR_W(4) NR_W(5) NR_W(6) NR_W(7) NR_W(8)
 #undef NR_W
-    default: return Variant{nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr};
+    default: return Variant{nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr};
   }
 }
 
@@ -830,6 +830,7 @@ int pick_variant(const nr_graph* g, Variant& v) {
     v = variant_csr(W, (g->n + 63) / 64);
   if (!v.level) return NR_ERR_UNSUPPORTED;
   add_learn_kernels(v, W);
+  add_fork_kernels(v, W);
   return NR_OK;
 }
 
@@ -880,7 +881,10 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
    * LDS.  Size the grid from the occupancy without it, then add the table
    * for that grid; occupancy can only drop, so the table stays large
    * enough for the final grid. */
-  KernelGetter getter = mode == 1 ? var.level_log : mode == 2 ? var.level_learn : var.level;
+  KernelGetter getter = mode == 1   ? var.level_log
+                        : mode == 2 ? var.level_learn
+                        : cfg->rule == NR_RULE_FORKBEST ? var.level_fork
+                                                        : var.level;
   if (!getter) return NR_ERR_UNSUPPORTED;
   const void* kfun = (const void*)getter();
   int block =
@@ -977,7 +981,7 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   auto t_begin = std::chrono::steady_clock::now();
   const bool has_budget = cfg->time_budget_s > 0;
   NR_HIP(hipEventRecord(ev_start, st));
-  int level = 0, chunk = 8;
+  int level = 0, chunk = has_budget ? 1 : 8;
   bool finished = false, deadline = false;
   long long timed = 0, rounds = 0;
   u64 max_pool = 1;
@@ -1002,6 +1006,14 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
            std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count() >=
                cfg->time_budget_s;
   };
+  auto budget_chunk = [&](std::chrono::steady_clock::time_point c0, int launched, int want) {
+    const auto now = std::chrono::steady_clock::now();
+    const double per = std::chrono::duration<double>(now - c0).count() / std::max(launched, 1);
+    const double left =
+        cfg->time_budget_s - std::chrono::duration<double>(now - t_begin).count();
+    const int fit = per > 0 ? (int)(0.25 * left / per) : want;
+    return std::max(1, std::min(want, fit));
+  };
   auto fold = [&](int parity) -> int {
     void* args[] = {&a, (void*)&parity};
     NR_HIP(hipLaunchKernel((const void*)&k_fold, dim3(grid), dim3(block), args,
@@ -1009,8 +1021,12 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
     return NR_OK;
   };
   if (!mass) {
-    /* level order: chunks of back-to-back level launches */
+    /* level order: chunks of back-to-back level launches; under a time
+     * budget a chunk is sized from the measured time per launch so the
+     * deadline is checked at least every quarter of the remaining budget */
     while (!finished) {
+      const auto c0 = std::chrono::steady_clock::now();
+      const int launched = chunk;
       for (int c = 0; c < chunk; ++c, ++level)
         if ((rc = launch_level(level & 1, level))) return rc;
       NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
@@ -1022,6 +1038,7 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
         finished = true;
       }
       chunk = std::min(chunk * 2, 16);
+      if (has_budget && !finished) chunk = budget_chunk(c0, launched, chunk);
     }
   } else {
     /* class rounds: each launch expands every queued item of the heaviest
@@ -1033,8 +1050,12 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
     hist[mass_class(1.0)] = 1;
     u64 pool = 1;
     double ratio = 1.0; /* queued children per expanded item, running estimate */
-    const u64 target = std::max<u64>(1ull << 20, (u64)grid * block);
+    /* a round expands at least a quarter of the pool (heaviest classes
+     * first): carrying the rest then costs at most three times the useful
+     * work, against ~pool/1M times with a fixed-size round */
+    const u64 target0 = std::max<u64>(1ull << 20, (u64)grid * block);
     while (pool > 0) {
+      const u64 target = std::max<u64>(target0, pool / 4);
       /* trim: when the pool exceeds the capacity, give up its lightest
        * classes (at least the heaviest class is kept) */
       int drop = kClasses;
@@ -1659,6 +1680,7 @@ int nr_solve(const nr_graph* g, const nr_config* cfg, nr_result* out) {
     return NR_ERR_ARG;
   if (cfg->order == NR_ORDER_LEARNED) return NR_ERR_ARG; /* needs a model: nr_solve_learned */
   if (cfg->order != NR_ORDER_LEVEL && cfg->order != NR_ORDER_MASS) return NR_ERR_ARG;
+  if (cfg->rule != NR_RULE_PATH && cfg->rule != NR_RULE_FORKBEST) return NR_ERR_ARG;
   nr::Variant var;
   int rc = nr::pick_variant(g, var);
   if (rc) return rc;
@@ -1677,6 +1699,7 @@ int nr_solve_learned(const nr_graph* g, const nr_config* cfg, const nr_model* mo
   }
   if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0 || cfg->time_budget_s < 0)
     return NR_ERR_ARG;
+  if (cfg->rule != NR_RULE_PATH) return NR_ERR_UNSUPPORTED; /* scores path children */
   nr::Variant var;
   int rc = nr::pick_variant(g, var);
   if (rc) return rc;
@@ -1692,6 +1715,7 @@ int nr_log_subproblems(const nr_graph* g, const nr_config* cfg, long long max_re
       (max_records > 0 && (!features || !labels)))
     return NR_ERR_ARG;
   if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0) return NR_ERR_ARG;
+  if (cfg->rule != NR_RULE_PATH) return NR_ERR_UNSUPPORTED;
   nr::Variant var;
   int rc = nr::pick_variant(g, var);
   if (rc) return rc;
@@ -1712,7 +1736,7 @@ int nr_solve_batch(const nr_graph* const* graphs, int count, const nr_config* cf
   }
   if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0 || cfg->time_budget_s < 0)
     return NR_ERR_ARG;
-  if (cfg->order != NR_ORDER_LEVEL) return NR_ERR_UNSUPPORTED;
+  if (cfg->order != NR_ORDER_LEVEL || cfg->rule != NR_RULE_PATH) return NR_ERR_UNSUPPORTED;
   nr::WsLock L(cfg->device);
   if (L.rc) return L.rc;
   return nr::solve_batch(*L.ws, graphs, count, cfg, results);

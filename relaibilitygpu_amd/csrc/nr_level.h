@@ -232,7 +232,8 @@ __device__ inline void child_ctx(const LevelArgs& a, const EX& ex, const u64 (&k
 /* One tree level: consume buf[parity], append children to the workgroup's
  * segment of buf[parity^1].  MODE 0: level order or max-mass class rounds;
  * MODE 1: level order that also writes the tree log; MODE 2: class rounds
- * on the learned score (children scored 64 at a time on the matrix cores). */
+ * on the learned score (children scored 64 at a time on the matrix cores);
+ * MODE 3: the forkBest rule (two-way branch) in either order. */
 template <int W, class EX, int MODE>
 __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, int level) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -304,6 +305,7 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
     bool live = false, carry = false;
     u64 slot_in = 0;
     u32 tag_in = 0xffffffffu;
+    int fork_e = 0;
     if ((u64)lane < chunk && i < n_items) {
       int s = segment_of(sp, a.n_seg, (u32)base);
       while (sp[s + 1] <= i) ++s;
@@ -322,6 +324,24 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
         if (mass < a.eps) { /* main.cu:537-543 */
           accE.add(mass);
           ++c[C_TRUNC];
+        } else if (MODE == 3) { /* forkBest: success, dead, or two children */
+          ex.load(in, a.stride, slot_in);
+          const int r = ex.fork_item(known, up);
+          if (r > 0) {
+            live = true;
+            k = 2;
+            fork_e = r - 1;
+            stored = 2;
+            if (a.mass_mode)
+              stored = (mass * ex.p[fork_e] >= a.eps ? 1 : 0) + (mass * ex.q[fork_e] >= a.eps ? 1 : 0);
+          } else if (r == 0) { /* a WORKING path: main.cu:550-567 */
+            accR.add(mass);
+            ++c[C_PATHS];
+          } else {
+            if (r == -2) ++c[C_ERR];
+            accD.add(mass); /* main.cu:579-582 */
+            ++c[C_DEAD];
+          }
         } else {
           ex.load(in, a.stride, slot_in);
           k = ex.find_item(known, up);
@@ -373,8 +393,42 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
       }
     }
     ChildCtx cx;
-    if (MODE != 0 && live) child_ctx<W, EX>(a, ex, known, up, mass, k, cx);
-    if (MODE == 2) {
+    if ((MODE == 1 || MODE == 2) && live) child_ctx<W, EX>(a, ex, known, up, mass, k, cx);
+    if (MODE == 3) {
+      if (live) { /* WORKING child, then FAILED child (main.cu:355-427) */
+        const int wi = fork_e >> 6;
+        const u64 bit = 1ull << (fork_e & 63);
+#pragma unroll
+        for (int ch = 0; ch < 2; ++ch) {
+          const double cm = ch == 0 ? mass * ex.p[fork_e] : mass * ex.q[fork_e];
+          if (a.mass_mode && cm < a.eps) { /* settled at creation */
+            accE.add(cm);
+            ++c[C_NODES];
+            ++c[C_TRUNC];
+            continue;
+          }
+          if (off < a.seg_cap) {
+            u64 ck[W], cu[W];
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+              ck[w] = known[w] | (w == wi ? bit : 0ull);
+              cu[w] = up[w] | (ch == 0 && w == wi ? bit : 0ull);
+            }
+            store_item<W>(out, a.stride, out_base + off, ck, cu, cm);
+            if (ch == 0)
+              ex.store_rows(out, a.stride, out_base + off);
+            else
+              ex.store_child(out, a.stride, out_base + off, fork_e);
+            if (a.mass_mode) atomicAdd(&s_hist[mass_class(cm)], 1u);
+          } else { /* segment full: main.cu:245-252 */
+            accE.add(cm);
+            ++c[C_PATHS];
+            ++c[C_OVER];
+          }
+          ++off;
+        }
+      }
+    } else if (MODE == 2) {
       /* children in rounds, one per lane per round, src side first; every
        * lane of the wave takes part in each round's tile score */
       int kmax = live ? k : 0;
@@ -599,7 +653,11 @@ struct Variant {
   int ns_bytes; /* bytes per adjacency row carried by an item (0: none) */
   int nw;       /* CsrExpander node-set words (0: Expander) */
   KernelGetter level_log, level_learn; /* MODE 1 / MODE 2 of k_expand_level */
+  KernelGetter level_fork;             /* MODE 3 (forkBest rule) */
 };
+
+/* MODE 3 kernels of a variant (nr_fork.hip), filled into v. */
+void add_fork_kernels(Variant& v, int W);
 
 /* MODE 1 / MODE 2 kernels of a variant (nr_learn.hip), filled into v. */
 void add_learn_kernels(Variant& v, int W);
