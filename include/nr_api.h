@@ -183,6 +183,31 @@ int nr_device_query(int device, char* name, int name_len, int* compute_units,
                     int* wave_size, long long* total_mem);
 
 /* ------------------------------------------------------------------------
+ * Checkpoint / resume of a time-budgeted solve (original/Seq_16.cpp:44,227
+ * folds the pending queue into ESP when the time is up; seq/seq.cpp:189-196
+ * reports the pending mass).  Here the pending subproblems can instead be
+ * kept and the solve continued later, on this or another process or device.
+ * ------------------------------------------------------------------------ */
+typedef struct nr_checkpoint nr_checkpoint;
+
+/* nr_solve that, when cfg->time_budget_s expires first, keeps the pending
+ * subproblems in *kept (else *kept = NULL): the result is then the bracket
+ * so far, ESP including the pending mass (folded stays 0).  With `resume`
+ * (from an earlier call on the same graph, rule and order) the solve starts
+ * from its pending subproblems, sums and counters instead of the root; a
+ * chain of such calls ends with the counts, levels and sums of one
+ * uninterrupted solve.  The queue capacity must hold the pending items. */
+int nr_solve_checkpointed(const nr_graph* g, const nr_config* cfg,
+                          const nr_checkpoint* resume, nr_checkpoint** kept,
+                          nr_result* out);
+int nr_checkpoint_info(const nr_checkpoint* ckpt, long long* pending,
+                       double* pending_mass, long long* nodes_done);
+/* Binary file: magic, shape, graph fingerprint, sums, counters, items. */
+int nr_checkpoint_save(const nr_checkpoint* ckpt, const char* path);
+int nr_checkpoint_load(const char* path, nr_checkpoint** out);
+void nr_checkpoint_destroy(nr_checkpoint* ckpt);
+
+/* ------------------------------------------------------------------------
  * Learned work order.  No counterpart in the reference: a small MLP scores
  * each child subproblem (predicted log2 of its subtree size under the plain
  * method) and the class-round queue expands children in order of

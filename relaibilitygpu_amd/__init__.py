@@ -184,6 +184,13 @@ def lib() -> ctypes.CDLL:
     L.nr_model_destroy.restype = None
     L.nr_model_score.argtypes = [vp, fp, ctypes.c_longlong, fp, ctypes.c_int]
     L.nr_solve_learned.argtypes = [vp, P(Config), vp, P(Result)]
+    L.nr_solve_checkpointed.argtypes = [vp, P(Config), vp, P(vp), P(Result)]
+    L.nr_checkpoint_info.argtypes = [vp, P(ctypes.c_longlong), P(ctypes.c_double),
+                                     P(ctypes.c_longlong)]
+    L.nr_checkpoint_save.argtypes = [vp, ctypes.c_char_p]
+    L.nr_checkpoint_load.argtypes = [ctypes.c_char_p, P(vp)]
+    L.nr_checkpoint_destroy.argtypes = [vp]
+    L.nr_checkpoint_destroy.restype = None
     _lib = L
     return L
 
@@ -444,3 +451,50 @@ def solve_learned(graph: Graph, model: Model, config: Config | None = None, **kw
     _check(lib().nr_solve_learned(graph.handle, ctypes.byref(cfg), model.handle, ctypes.byref(r)),
            "nr_solve_learned")
     return r.as_dict()
+
+
+# ---------------------------------------------------------------- checkpoints
+class Checkpoint:
+    """Pending subproblems of a time-budgeted solve (nr_checkpoint)."""
+
+    def __init__(self, handle):
+        self._h = ctypes.c_void_p(handle)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self) -> dict:
+        n, m, d = ctypes.c_longlong(), ctypes.c_double(), ctypes.c_longlong()
+        _check(lib().nr_checkpoint_info(self._h, ctypes.byref(n), ctypes.byref(m),
+                                        ctypes.byref(d)), "nr_checkpoint_info")
+        return {"pending": n.value, "pending_mass": m.value, "nodes_done": d.value}
+
+    def save(self, path: str) -> None:
+        _check(lib().nr_checkpoint_save(self._h, os.fsencode(path)), "nr_checkpoint_save")
+
+    @classmethod
+    def load(cls, path: str) -> "Checkpoint":
+        h = ctypes.c_void_p()
+        _check(lib().nr_checkpoint_load(os.fsencode(path), ctypes.byref(h)), "nr_checkpoint_load")
+        return cls(h.value)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.nr_checkpoint_destroy(h)
+            self._h = ctypes.c_void_p()
+
+
+def solve_checkpointed(graph: Graph, resume: Checkpoint | None = None,
+                       config: Config | None = None, **kw):
+    """(result, Checkpoint or None): the checkpoint holds the pending
+    subproblems when the time budget expired first."""
+    cfg = config if config is not None else make_config(**kw)
+    r = Result()
+    out = ctypes.c_void_p()
+    _check(lib().nr_solve_checkpointed(graph.handle, ctypes.byref(cfg),
+                                       resume.handle if resume is not None else None,
+                                       ctypes.byref(out), ctypes.byref(r)),
+           "nr_solve_checkpointed")
+    return r.as_dict(), (Checkpoint(out.value) if out.value else None)

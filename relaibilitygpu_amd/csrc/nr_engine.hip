@@ -30,6 +30,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <random>
 #include <vector>
@@ -81,6 +82,14 @@ __global__ void __launch_bounds__(256) k_fold(LevelArgs a, int parity) {
   block_flush(a, zr, e, zd, c, (unsigned char*)part);
 }
 
+/* State a resumed solve starts from (device copy). */
+struct SeedMeta {
+  DD sum[3];       /* R, ESP, D so far */
+  u64 counts[C_N];
+  u64 iterations, max_width;
+  u64 count;       /* pending items */
+};
+
 /* Seed a solve: zero the control block, the partial-sum and counter slots,
  * set the first prefix table to one item, and put the root item
  * (main.cu:853-855) in slot 0 of buffer 0.  A kernel instead of memset /
@@ -91,7 +100,7 @@ struct Root {
 };
 
 __global__ void __launch_bounds__(256) k_init(LevelArgs a, Root root, int W, int n,
-                                              int ns_bytes, u64 n_slots) {
+                                              int ns_bytes, u64 n_slots, const SeedMeta* seed) {
   const u64 tid = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   const u64 stride = (u64)gridDim.x * blockDim.x;
   for (u64 i = tid; i < n_slots; i += stride) {
@@ -100,9 +109,25 @@ __global__ void __launch_bounds__(256) k_init(LevelArgs a, Root root, int W, int
     a.partD[i].hi = a.partD[i].lo = 0.0;
     for (int j = 0; j < C_N; ++j) a.cnt[i * C_N + j] = 0;
   }
-  for (u64 s = tid; s < (u64)a.n_seg; s += stride) {
-    a.seg_count[0][s] = s == 0 ? 1u : 0u;
+  const u64 ns = (u64)a.n_seg;
+  for (u64 s = tid; s < ns; s += stride) {
+    a.seg_count[0][s] =
+        seed ? (u32)(seed->count / ns + (s < seed->count % ns ? 1 : 0)) : (s == 0 ? 1u : 0u);
     a.seg_count[1][s] = 0u;
+  }
+  if (seed) { /* a resumed solve: sums and counters so far in slot 0 */
+    if (tid == 0) {
+      LevelCtl z = {};
+      *a.ctl = z;
+      a.ctl->last_items = seed->count;
+      a.ctl->iterations = seed->iterations;
+      a.ctl->max_width = seed->max_width;
+      a.partR[0] = seed->sum[0];
+      a.partE[0] = seed->sum[1];
+      a.partD[0] = seed->sum[2];
+      for (int j = 0; j < C_N; ++j) a.cnt[j] = seed->counts[j];
+    }
+    return;
   }
   if (tid == 0) {
     LevelCtl z = {};
@@ -169,9 +194,79 @@ __global__ void __launch_bounds__(256) k_finalize(LevelArgs a, u64 n_slots) {
     __syncthreads();
   }
   if (t == 0) {
-    for (int k = 0; k < 3; ++k) a.ctl->sum[k] = ps[k][0] + pc[k][0];
+    for (int k = 0; k < 3; ++k) {
+      double hi, lo;
+      two_sum(ps[k][0], pc[k][0], hi, lo);
+      a.ctl->sum[k] = hi;
+      a.ctl->sum_lo[k] = lo;
+    }
     for (int j = 0; j < C_N; ++j) a.ctl->counts[j] = pn[j][0];
   }
+}
+
+/* ---- checkpoints: pending items out of, and back into, the queue ---- */
+
+
+/* Compact copy of the items of buf[parity] (planes, mass, tag). */
+__global__ void __launch_bounds__(256) k_gather(LevelArgs a, int parity, int W, u64* gk,
+                                                u64* gu, double* gm, u32* gt) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  u32* sp = (u32*)smem;
+  u32* part = sp + a.n_seg + 2;
+  block_prefix(a.seg_count[parity], a.n_seg, sp, part);
+  const u32 n_items = sp[a.n_seg];
+  const QBuf b = a.buf[parity];
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n_items;
+       i += (u64)gridDim.x * blockDim.x) {
+    int s = segment_of(sp, a.n_seg, (u32)i);
+    const u64 slot = (u64)s * a.seg_cap + (i - sp[s]);
+    for (int w = 0; w < W; ++w) {
+      gk[i * W + w] = b.known[(u64)w * a.stride + slot];
+      gu[i * W + w] = b.up[(u64)w * a.stride + slot];
+    }
+    gm[i] = b.mass[slot];
+    if (gt) gt[i] = a.tag[parity] ? a.tag[parity][slot] : 0u;
+  }
+}
+
+/* Seed item i of a resumed solve into buffer 0: segment i % n_seg, slot
+ * i / n_seg (the counts k_init gives every segment). */
+__device__ inline u64 seed_slot(const LevelArgs& a, u64 i) {
+  return (i % (u64)a.n_seg) * a.seg_cap + i / (u64)a.n_seg;
+}
+
+__global__ void __launch_bounds__(256) k_seed_copy(LevelArgs a, int W, const u64* sk,
+                                                   const u64* su, const double* sm,
+                                                   const u32* st, u64 count) {
+  const QBuf b = a.buf[0];
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (u64)gridDim.x * blockDim.x) {
+    const u64 slot = seed_slot(a, i);
+    for (int w = 0; w < W; ++w) {
+      b.known[(u64)w * a.stride + slot] = sk[i * W + w];
+      b.up[(u64)w * a.stride + slot] = su[i * W + w];
+    }
+    b.mass[slot] = sm[i];
+    if (a.tag[0]) a.tag[0][slot] = st ? st[i] : 0u;
+  }
+}
+
+/* Row form: each seeded item's adjacency rows, derived from its planes. */
+template <int W, typename NS>
+__global__ void __launch_bounds__(256) k_seed_rows(LevelArgs a, u64 count) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  stage_tables(a.g, a.lay, smem);
+  __syncthreads();
+  Expander<W, NS> ex;
+  ex.bind_graph(smem, a.lay, a.g, threadIdx.x);
+  const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const u64 slot = seed_slot(a, i);
+  u64 known[W], up[W];
+  double mass;
+  load_item<W>(a.buf[0], a.stride, slot, known, up, mass);
+  ex.derive(known, up);
+  ex.store_rows(a.buf[0], a.stride, slot);
 }
 
 /* ------------------------------------------------------------------ */
@@ -787,6 +882,7 @@ struct Kern {
   static void* items() { return (void*)&k_expand_items<W, Expander<W, NS>>; }
   static void* batch() { return (void*)&k_expand_batch<W, NS>; }
   static void* refill() { return (void*)&k_refill_batch<W>; }
+  static void* seed_rows() { return (void*)&k_seed_rows<W, NS>; }
 };
 
 template <typename NS>
@@ -795,10 +891,11 @@ Variant variant_for_w(int W) {
 #define NR_W(X) \
   case X:      \
     return Variant{&Kern<X, NS>::level, &Kern<X, NS>::items, &Kern<X, NS>::batch,               \
-                   &Kern<X, NS>::refill, (int)sizeof(NS), 0, nullptr, nullptr, nullptr};
+                   &Kern<X, NS>::refill, (int)sizeof(NS), 0, nullptr, nullptr, nullptr,  \
+                   &Kern<X, NS>::seed_rows};
     NR_W(1) NR_W(2) NR_W(3) NR_W(4) NR_W(5) NR_W(6) NR_W(7) NR_W(8)
 #undef NR_W
-    default: return Variant{nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr};
+    default: return Variant{nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr};
   }
 }
 
@@ -856,6 +953,59 @@ Root root_item(const nr_graph* g, int W) {
   return r;
 }
 
+/* RAII bag of device allocations made for one call. */
+struct DevMem {
+  std::vector<void*> ptrs;
+  ~DevMem() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <typename T>
+  int alloc(T** out, size_t count) {
+    void* p = nullptr;
+    if (count == 0) count = 1;
+    if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) return NR_ERR_NOMEM;
+    ptrs.push_back(p);
+    *out = (T*)p;
+    return NR_OK;
+  }
+};
+
+}  // namespace nr
+
+/* A solve's pending subproblems and sums, kept when its time budget
+ * expired (host memory). */
+struct nr_checkpoint {
+  int n = 0, e = 0, w = 0, src = 0, dst = 0, rule = 0, order = 0;
+  double eps = 0.0;
+  unsigned long long graph_hash = 0;
+  double sum_hi[3] = {0, 0, 0}, sum_lo[3] = {0, 0, 0}; /* R, ESP, D */
+  unsigned long long counts[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  unsigned long long iterations = 0, max_width = 0;
+  unsigned long long count = 0;
+  std::vector<unsigned long long> known, up; /* [count][w] */
+  std::vector<double> mass;
+};
+
+namespace nr {
+
+/* FNV-1a over the graph's shape, arcs, probabilities and terminals. */
+u64 graph_hash(const nr_graph* g) {
+  u64 h = 1469598103934665603ull;
+  auto mix = [&](const void* p, size_t bytes) {
+    const unsigned char* c = (const unsigned char*)p;
+    for (size_t i = 0; i < bytes; ++i) h = (h ^ c[i]) * 1099511628211ull;
+  };
+  mix(&g->n, sizeof g->n);
+  mix(&g->e, sizeof g->e);
+  mix(&g->src, sizeof g->src);
+  mix(&g->dst, sizeof g->dst);
+  mix(g->row_ptr.data(), sizeof(int) * g->row_ptr.size());
+  mix(g->col_idx.data(), sizeof(int) * g->col_idx.size());
+  mix(g->edge_id.data(), sizeof(int) * g->edge_id.size());
+  mix(g->p.data(), sizeof(double) * g->p.size());
+  return h;
+}
+
 /* Extras of a level-engine solve: the tree log (mode 1) or a learned
  * scorer (mode 2). */
 struct LevelExtra {
@@ -866,6 +1016,8 @@ struct LevelExtra {
   u64* log_count = nullptr;
   u64 log_cap = 0;
   const MlpDev* mlp = nullptr;
+  const nr_checkpoint* resume = nullptr; /* start from these pending items */
+  nr_checkpoint** save = nullptr;        /* on deadline keep them here */
 };
 
 int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_result* out,
@@ -967,13 +1119,67 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   a.drop_class = kClasses;
 
   hipStream_t st = ws.stream;
-  {
+  const nr_checkpoint* ck = x ? x->resume : nullptr;
+  DevMem seedmem;
+  if (ck) { /* resume: pending items spread over the segments of buffer 0 */
+    if (ck->count > seg_cap64 * (u64)grid) return NR_ERR_ARG; /* capacity below the checkpoint */
+    SeedMeta sm;
+    std::memset(&sm, 0, sizeof sm);
+    for (int k = 0; k < 3; ++k) {
+      sm.sum[k].hi = ck->sum_hi[k];
+      sm.sum[k].lo = ck->sum_lo[k];
+    }
+    for (int j = 0; j < C_N; ++j) sm.counts[j] = ck->counts[j];
+    sm.iterations = mass ? 0 : ck->iterations;
+    sm.max_width = mass ? 0 : ck->max_width;
+    sm.count = ck->count;
+    SeedMeta* dmeta;
+    u64 *dk, *du;
+    double* dm;
+    if ((rc = seedmem.alloc(&dmeta, 1)) || (rc = seedmem.alloc(&dk, ck->count * W)) ||
+        (rc = seedmem.alloc(&du, ck->count * W)) || (rc = seedmem.alloc(&dm, ck->count)))
+      return rc;
+    NR_HIP(hipMemcpyAsync(dmeta, &sm, sizeof sm, hipMemcpyHostToDevice, st));
+    if (ck->count) {
+      NR_HIP(hipMemcpyAsync(dk, ck->known.data(), 8 * ck->count * W, hipMemcpyHostToDevice, st));
+      NR_HIP(hipMemcpyAsync(du, ck->up.data(), 8 * ck->count * W, hipMemcpyHostToDevice, st));
+      NR_HIP(hipMemcpyAsync(dm, ck->mass.data(), 8 * ck->count, hipMemcpyHostToDevice, st));
+    }
+    Root root;
+    std::memset(&root, 0, sizeof root);
+    int Wi = W, ni = g->n, nsb = var.ns_bytes;
+    u64 nw = n_slots;
+    const SeedMeta* dmc = dmeta;
+    void* args[] = {&a, &root, &Wi, &ni, &nsb, &nw, &dmc};
+    NR_HIP(hipLaunchKernel((const void*)&k_init, dim3(64), dim3(256), args, 0, st));
+    if (ck->count) {
+      const u64* dkc = dk;
+      const u64* duc = du;
+      const double* dmcc = dm;
+      const u32* dt = nullptr;
+      u64 cnt = ck->count;
+      void* cargs[] = {&a, &Wi, &dkc, &duc, &dmcc, &dt, &cnt};
+      NR_HIP(hipLaunchKernel((const void*)&k_seed_copy, dim3(1024), dim3(256), cargs, 0, st));
+      if (var.seed_rows) {
+        const void* kseed = (const void*)var.seed_rows();
+        const Layout ls = layout_for(var, g, W, block);
+        NR_HIP(hipFuncSetAttribute(kseed, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   64 * 1024));
+        void* rargs[] = {&a, &cnt};
+        NR_HIP(hipLaunchKernel(kseed, dim3((unsigned)((cnt + block - 1) / block)), dim3(block),
+                               rargs, ls.total_bytes, st));
+      }
+    }
+  } else {
     Root root = root_item(g, W);
     int Wi = W, ni = g->n, nsb = var.ns_bytes;
     u64 nw = n_slots;
-    void* args[] = {&a, &root, &Wi, &ni, &nsb, &nw};
+    const SeedMeta* none = nullptr;
+    void* args[] = {&a, &root, &Wi, &ni, &nsb, &nw, &none};
     NR_HIP(hipLaunchKernel((const void*)&k_init, dim3(64), dim3(256), args, 0, st));
   }
+  nr_checkpoint** save = x ? x->save : nullptr;
+  int pend_parity = -1; /* buffer holding the pending items to keep */
   hipEvent_t ev_start, ev_stop;
   if ((rc = ws.event(0, &ev_start)) || (rc = ws.event(1, &ev_stop))) return rc;
   const size_t max_timed = (cfg->flags & NR_FLAG_TIME_LAUNCHES) ? 2048 : 0;
@@ -1033,7 +1239,10 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
       NR_HIP(hipStreamSynchronize(st));
       if (ws.hctl->last_items == 0) finished = true;
       if (!finished && expired()) {
-        if ((rc = fold(level & 1))) return rc;
+        if (save)
+          pend_parity = level & 1;
+        else if ((rc = fold(level & 1)))
+          return rc;
         deadline = true;
         finished = true;
       }
@@ -1047,8 +1256,16 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
      * up first. */
     const bool trace = std::getenv("NR_TRACE_ROUNDS") != nullptr;
     std::vector<u64> hist(kClasses, 0);
-    hist[mass_class(1.0)] = 1;
     u64 pool = 1;
+    long long launches_here = 0;
+    if (ck) {
+      for (u64 i = 0; i < ck->count; ++i) hist[mass_class(ck->mass[i])] += 1;
+      pool = ck->count;
+      rounds = (long long)ck->iterations; /* reported rounds continue the chain */
+      max_pool = std::max<u64>(ck->max_width, ck->count);
+    } else {
+      hist[mass_class(1.0)] = 1;
+    }
     double ratio = 1.0; /* queued children per expanded item, running estimate */
     /* a round expands at least a quarter of the pool (heaviest classes
      * first): carrying the rest then costs at most three times the useful
@@ -1081,13 +1298,15 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
       const u64 carried = keep - cum;
       a.thr_class = thr;
       a.drop_class = drop;
-      const int parity = (int)(rounds & 1);
+      /* buffer parity counts this call's launches (seeds are in buffer 0) */
+      const int parity = (int)(launches_here & 1);
       if ((rc = launch_level(parity, (int)rounds))) return rc;
       void* hargs[] = {&a};
       NR_HIP(hipLaunchKernel((const void*)&k_hist_total, dim3(1), dim3(kClasses), hargs, 0, st));
       NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
       NR_HIP(hipStreamSynchronize(st));
       ++rounds;
+      ++launches_here;
       pool = 0;
       for (int cl = 0; cl < kClasses; ++cl) {
         hist[cl] = ws.hctl->hist[cl];
@@ -1103,10 +1322,45 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
         ratio = std::max(1.0, 0.5 * ratio + 0.5 * kids / (double)cum);
       }
       if (pool > 0 && expired()) {
-        if ((rc = fold((int)(rounds & 1)))) return rc;
+        if (save)
+          pend_parity = (int)(launches_here & 1);
+        else if ((rc = fold((int)(launches_here & 1))))
+          return rc;
         deadline = true;
         break;
       }
+    }
+  }
+  std::unique_ptr<nr_checkpoint> kept;
+  if (pend_parity >= 0) { /* keep the pending items instead of folding them */
+    std::vector<u32> segs(grid);
+    NR_HIP(hipMemcpyAsync(segs.data(), a.seg_count[pend_parity], 4 * (size_t)grid,
+                          hipMemcpyDeviceToHost, st));
+    NR_HIP(hipStreamSynchronize(st));
+    u64 count = 0;
+    for (u32 c2 : segs) count += c2;
+    kept.reset(new (std::nothrow) nr_checkpoint);
+    if (!kept) return NR_ERR_NOMEM;
+    kept->count = count;
+    kept->known.resize(count * W);
+    kept->up.resize(count * W);
+    kept->mass.resize(count);
+    if (count) {
+      DevMem gm;
+      u64 *gk, *gu;
+      double* gmass;
+      if ((rc = gm.alloc(&gk, count * W)) || (rc = gm.alloc(&gu, count * W)) ||
+          (rc = gm.alloc(&gmass, count)))
+        return rc;
+      int par = pend_parity, Wi = W;
+      u32* gt = nullptr;
+      void* args[] = {&a, &par, &Wi, &gk, &gu, &gmass, &gt};
+      NR_HIP(hipLaunchKernel((const void*)&k_gather, dim3(grid), dim3(block), args,
+                             4 * (grid + 2 + block), st));
+      NR_HIP(hipMemcpyAsync(kept->known.data(), gk, 8 * count * W, hipMemcpyDeviceToHost, st));
+      NR_HIP(hipMemcpyAsync(kept->up.data(), gu, 8 * count * W, hipMemcpyDeviceToHost, st));
+      NR_HIP(hipMemcpyAsync(kept->mass.data(), gmass, 8 * count, hipMemcpyDeviceToHost, st));
+      NR_HIP(hipStreamSynchronize(st));
     }
   }
   {
@@ -1153,6 +1407,33 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   out->lds_bytes = lay.total_bytes;
   out->blocks_per_cu = per_cu;
   out->compute_units = cus;
+  if (save) *save = nullptr;
+  if (kept) { /* the bracket stays certified: pending mass counts in ESP */
+    kept->n = g->n;
+    kept->e = g->e;
+    kept->w = W;
+    kept->src = g->src;
+    kept->dst = g->dst;
+    kept->rule = cfg->rule;
+    kept->order = mass ? NR_ORDER_MASS : NR_ORDER_LEVEL;
+    kept->eps = a.eps;
+    kept->graph_hash = graph_hash(g);
+    for (int k = 0; k < 3; ++k) {
+      kept->sum_hi[k] = h.sum[k];
+      kept->sum_lo[k] = h.sum_lo[k];
+    }
+    for (int j = 0; j < C_N; ++j) kept->counts[j] = h.counts[j];
+    kept->iterations = mass ? (u64)rounds : h.iterations;
+    kept->max_width = mass ? max_pool : h.max_width;
+    double ps = 0.0, pc = 0.0;
+    for (double m : kept->mass) {
+      const double t = ps + m;
+      pc += std::fabs(ps) >= std::fabs(m) ? (ps - t) + m : (m - t) + ps;
+      ps = t;
+    }
+    out->ESP = h.sum[1] + (ps + pc);
+    *save = kept.release();
+  }
   return NR_OK;
 }
 
@@ -1443,22 +1724,6 @@ int solve_batch(Workspace& ws, const nr_graph* const* gs, int count, const nr_co
   return NR_OK;
 }
 
-/* RAII bag of device allocations (test entry point only). */
-struct DevMem {
-  std::vector<void*> ptrs;
-  ~DevMem() {
-    for (void* p : ptrs) (void)hipFree(p);
-  }
-  template <typename T>
-  int alloc(T** out, size_t count) {
-    void* p = nullptr;
-    if (count == 0) count = 1;
-    if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) return NR_ERR_NOMEM;
-    ptrs.push_back(p);
-    *out = (T*)p;
-    return NR_OK;
-  }
-};
 
 int expand_items(Workspace& ws, const nr_graph* g, int count, const u64* known,
                  const u64* up, const double* mass, int max_children, int* nchild,
@@ -1706,6 +1971,109 @@ int nr_solve_learned(const nr_graph* g, const nr_config* cfg, const nr_model* mo
   nr::WsLock L(cfg->device);
   if (L.rc) return L.rc;
   return nr::solve_learned(*L.ws, g, cfg, model, out);
+}
+
+int nr_solve_checkpointed(const nr_graph* g, const nr_config* cfg, const nr_checkpoint* resume,
+                          nr_checkpoint** kept, nr_result* out) {
+  if (!g || !out || !kept) return NR_ERR_ARG;
+  *kept = nullptr;
+  nr_config def;
+  if (!cfg) {
+    nr_config_init(&def);
+    cfg = &def;
+  }
+  if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0 || cfg->time_budget_s < 0)
+    return NR_ERR_ARG;
+  if (cfg->order != NR_ORDER_LEVEL && cfg->order != NR_ORDER_MASS) return NR_ERR_ARG;
+  if (cfg->rule != NR_RULE_PATH && cfg->rule != NR_RULE_FORKBEST) return NR_ERR_ARG;
+  if (resume && (resume->graph_hash != nr::graph_hash(g) || resume->w != nr::mask_words(g) ||
+                 resume->rule != cfg->rule || resume->order != cfg->order))
+    return NR_ERR_ARG; /* another graph, rule or order */
+  nr::Variant var;
+  int rc = nr::pick_variant(g, var);
+  if (rc) return rc;
+  nr::WsLock L(cfg->device);
+  if (L.rc) return L.rc;
+  nr::LevelExtra x;
+  x.resume = resume;
+  x.save = kept;
+  return nr::solve_level(*L.ws, g, cfg, out, &x);
+}
+
+int nr_checkpoint_info(const nr_checkpoint* c, long long* pending, double* pending_mass,
+                       long long* nodes_done) {
+  if (!c) return NR_ERR_ARG;
+  if (pending) *pending = (long long)c->count;
+  if (pending_mass) {
+    double s = 0.0;
+    for (double m : c->mass) s += m;
+    *pending_mass = s;
+  }
+  if (nodes_done) *nodes_done = (long long)c->counts[nr::C_NODES];
+  return NR_OK;
+}
+
+void nr_checkpoint_destroy(nr_checkpoint* c) { delete c; }
+
+static const char kCkptMagic[8] = {'N', 'R', 'C', 'K', 'P', 'T', '0', '1'};
+
+int nr_checkpoint_save(const nr_checkpoint* c, const char* path) {
+  if (!c || !path) return NR_ERR_ARG;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return NR_ERR_IO;
+  const int hdr[8] = {c->n, c->e, c->w, c->src, c->dst, c->rule, c->order, 0};
+  const unsigned long long tail[3] = {c->iterations, c->max_width, c->count};
+  bool ok = std::fwrite(kCkptMagic, 1, 8, f) == 8 && std::fwrite(hdr, sizeof hdr, 1, f) == 1 &&
+            std::fwrite(&c->eps, 8, 1, f) == 1 && std::fwrite(&c->graph_hash, 8, 1, f) == 1 &&
+            std::fwrite(c->sum_hi, 8, 3, f) == 3 && std::fwrite(c->sum_lo, 8, 3, f) == 3 &&
+            std::fwrite(c->counts, 8, 8, f) == 8 && std::fwrite(tail, 8, 3, f) == 3;
+  const size_t nw = (size_t)c->count * c->w;
+  ok = ok && std::fwrite(c->known.data(), 8, nw, f) == nw &&
+       std::fwrite(c->up.data(), 8, nw, f) == nw &&
+       std::fwrite(c->mass.data(), 8, c->count, f) == c->count;
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? NR_OK : NR_ERR_IO;
+}
+
+int nr_checkpoint_load(const char* path, nr_checkpoint** out) {
+  if (!path || !out) return NR_ERR_ARG;
+  *out = nullptr;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return NR_ERR_IO;
+  std::unique_ptr<nr_checkpoint> c(new (std::nothrow) nr_checkpoint);
+  char magic[8];
+  int hdr[8];
+  unsigned long long tail[3];
+  bool ok = c && std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kCkptMagic, 8) == 0 &&
+            std::fread(hdr, sizeof hdr, 1, f) == 1 && std::fread(&c->eps, 8, 1, f) == 1 &&
+            std::fread(&c->graph_hash, 8, 1, f) == 1 && std::fread(c->sum_hi, 8, 3, f) == 3 &&
+            std::fread(c->sum_lo, 8, 3, f) == 3 && std::fread(c->counts, 8, 8, f) == 8 &&
+            std::fread(tail, 8, 3, f) == 3;
+  if (ok) {
+    c->n = hdr[0];
+    c->e = hdr[1];
+    c->w = hdr[2];
+    c->src = hdr[3];
+    c->dst = hdr[4];
+    c->rule = hdr[5];
+    c->order = hdr[6];
+    c->iterations = tail[0];
+    c->max_width = tail[1];
+    c->count = tail[2];
+    ok = c->w >= 1 && c->w <= 16 && c->count < (1ull << 34);
+  }
+  if (ok) {
+    const size_t nw = (size_t)c->count * c->w;
+    c->known.resize(nw);
+    c->up.resize(nw);
+    c->mass.resize(c->count);
+    ok = std::fread(c->known.data(), 8, nw, f) == nw && std::fread(c->up.data(), 8, nw, f) == nw &&
+         std::fread(c->mass.data(), 8, c->count, f) == c->count;
+  }
+  std::fclose(f);
+  if (!ok) return c ? NR_ERR_PARSE : NR_ERR_NOMEM;
+  *out = c.release();
+  return NR_OK;
 }
 
 int nr_log_subproblems(const nr_graph* g, const nr_config* cfg, long long max_records,

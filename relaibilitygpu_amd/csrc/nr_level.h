@@ -41,6 +41,7 @@ struct LevelCtl {
   u64 max_width;  /* widest level */
   u64 counts[C_N];
   double sum[3];  /* R, ESP, D after k_finalize */
+  double sum_lo[3]; /* their compensation terms (double-double low parts) */
 };
 
 
@@ -654,6 +655,7 @@ struct Variant {
   int nw;       /* CsrExpander node-set words (0: Expander) */
   KernelGetter level_log, level_learn; /* MODE 1 / MODE 2 of k_expand_level */
   KernelGetter level_fork;             /* MODE 3 (forkBest rule) */
+  KernelGetter seed_rows;              /* row form: rows of resumed items */
 };
 
 /* MODE 3 kernels of a variant (nr_fork.hip), filled into v. */

@@ -134,3 +134,20 @@ def test_oracle_cli_report_layout():
                        capture_output=True, text=True, check=True)
     assert "Reliability:      R(G) = 0.504000000000" in r.stdout
     assert "Paths enumerated: 1" in r.stdout and "Nodes processed:  4" in r.stdout
+
+
+def test_checkpoint_files_fail_cleanly(tmp_path):
+    """Loading needs no GPU: a missing file is an I/O error, a file that is
+    not a checkpoint (or is cut short) a parse error."""
+    with pytest.raises(nr.NRError) as ei:
+        nr.Checkpoint.load(str(tmp_path / "missing.bin"))
+    assert ei.value.status == 1
+    bad = tmp_path / "bad.bin"
+    bad.write_bytes(b"NRCKPT01" + b"\x00" * 20)
+    with pytest.raises(nr.NRError) as ei:
+        nr.Checkpoint.load(str(bad))
+    assert ei.value.status == 2
+    bad.write_bytes(b"not a checkpoint at all, just text")
+    with pytest.raises(nr.NRError) as ei:
+        nr.Checkpoint.load(str(bad))
+    assert ei.value.status == 2
