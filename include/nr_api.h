@@ -145,7 +145,20 @@ int nr_graph_get_info(const nr_graph* g, nr_graph_info* info);
 int nr_graph_get_csr(const nr_graph* g, int* row_ptr, int* col_idx,
                      int* edge_id, double* p);
 
+/* Two-terminal problem src -> dst (clears a terminal set). */
 int nr_graph_set_terminals(nr_graph* g, int src, int dst);
+
+/* K-terminal reliability (no counterpart in the reference, whose target_set
+ * is declared but unused: original/Rel_16M.cu:43, original/Seq_16.cpp:33):
+ * the probability that every terminal is reached from terminals[0] (on an
+ * undirected graph: that all are connected).  Each subproblem branches on
+ * the UNKNOWN edges of a terminal tree (BFS from terminals[0]; the others,
+ * in list order, walk back to the tree under the lowest-index parent rule)
+ * instead of a path; {s, t} gives the two-terminal tree exactly.  k = 0
+ * returns to the src/dst pair; k = 1 is an error; at most 256 terminals.
+ * Not available to nr_solve_batch. */
+int nr_graph_set_terminal_set(nr_graph* g, const int* terminals, int k);
+int nr_graph_get_terminal_set(const nr_graph* g, int* terminals, int max_k, int* k);
 
 void nr_graph_destroy(nr_graph* g);
 

@@ -214,6 +214,8 @@ struct Engine {
   const orc_graph* g;
   int src, dst, W, NW, mode;
   double eps;
+  const int* terms = nullptr; /* K-terminal mode when nterms >= 2 */
+  int nterms = 0;
 
   bool allowed(const u64* known, const u64* up, int e) const {
     u64 bit = 1ull << (e & 63);
@@ -238,8 +240,84 @@ struct Engine {
    * This is what device_code.cuh:141-171 computes when N <= 32 (one lane
    * scans the frontier in ascending order and the first discoverer wins);
    * for N > 32 the reference is racy and this rule defines it. */
+  /* K-terminal generalisation of find_path (no counterpart in the
+   * reference, whose target_set is declared but unused: original/Rel_16M.cu
+   * :43, original/Seq_16.cpp:33).  BFS from terms[0] over allowed edges,
+   * whole levels, until every terminal is reached (-1 if the frontier dies
+   * first).  Then each other terminal, in list order, is walked back to the
+   * tree built so far under the same parent rule (lowest-index node of the
+   * previous level, lowest edge id on ties); the walks' edges, reversed,
+   * are the tree's hops, so for terms = {src, dst} this is find_path. */
+  int find_tree(const u64* known, const u64* up, bool optimistic, int* hops,
+                std::vector<NodeSet>& levels) const {
+    auto ok = [&](int e) {
+      return optimistic ? allowed(known, up, e) : working(known, up, e);
+    };
+    const int root = terms[0];
+    std::vector<int> lv(g->n, -1);
+    NodeSet vis{}, F{};
+    set(vis, root);
+    set(F, root);
+    lv[root] = 0;
+    if ((int)levels.size() < g->n + 1) levels.resize(g->n + 1);
+    levels[0] = F;
+    auto all_in = [&]() {
+      for (int i = 0; i < nterms; ++i)
+        if (!test(vis, terms[i])) return false;
+      return true;
+    };
+    int L = 0;
+    while (!all_in()) {
+      NodeSet next{};
+      bool any = false;
+      for (int k = 0; k < NW; ++k) {
+        u64 fw = F.w[k];
+        while (fw) {
+          int u = k * 64 + __builtin_ctzll(fw);
+          fw &= fw - 1;
+          for (int j = g->row_ptr[u]; j < g->row_ptr[u + 1]; ++j) {
+            int v = g->col_idx[j];
+            if (test(vis, v) || !ok(g->edge_id[j])) continue;
+            set(vis, v);
+            set(next, v);
+            lv[v] = L + 1;
+            any = true;
+          }
+        }
+      }
+      if (!any) return -1;
+      levels[++L] = next;
+      F = next;
+    }
+    std::vector<char> marked(g->n, 0);
+    marked[root] = 1;
+    std::vector<int> seq;
+    for (int i = 1; i < nterms; ++i) {
+      int v = terms[i];
+      while (!marked[v]) {
+        const int l = lv[v];
+        int pu = -1, pe = -1;
+        for (int k = g->in_ptr[v]; k < g->in_ptr[v + 1]; ++k) {
+          int u = g->in_src[k];
+          if (lv[u] == l - 1 && ok(g->in_eid[k])) {
+            pu = u;
+            pe = g->in_eid[k];
+            break;
+          }
+        }
+        seq.push_back(pe);
+        marked[v] = 1;
+        v = pu;
+      }
+    }
+    const int m = (int)seq.size();
+    for (int h = 0; h < m; ++h) hops[h] = seq[m - 1 - h];
+    return m;
+  }
+
   int find_path(const u64* known, const u64* up, bool optimistic, int* hops,
                 std::vector<NodeSet>& levels) const {
+    if (nterms >= 2) return find_tree(known, up, optimistic, hops, levels);
     if (src == dst) return 0;
     auto ok = [&](int e) {
       return optimistic ? allowed(known, up, e) : working(known, up, e);
@@ -920,6 +998,39 @@ int orc_solve(const orc_graph* g, int src, int dst, const orc_config* cfg,
   if (!cfg || !out) return ORC_ERR_ARG;
   std::memset(out, 0, sizeof(*out));
   Engine en{g, src, dst, g->w, g->nw, cfg->mode, cfg->trunc_eps};
+  Acc acc;
+  auto t0 = Clock::now();
+  switch (cfg->order) {
+    case ORC_ORDER_LEVEL: run_level(en, *cfg, out, acc); break;
+    case ORC_ORDER_HEAP: run_heap(en, *cfg, out, acc); break;
+    case ORC_ORDER_DFS: run_dfs(en, *cfg, out, acc); break;
+    default: return ORC_ERR_ARG;
+  }
+  auto t1 = Clock::now();
+  out->elapsed_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  out->R = acc.R.value();
+  out->ESP = acc.ESP.value();
+  out->D = acc.D.value();
+  out->paths = acc.paths;
+  out->nodes = acc.nodes;
+  out->dead = acc.dead;
+  out->truncated = acc.truncated;
+  out->overflowed = acc.overflowed;
+  out->folded = acc.folded;
+  out->n = g->n;
+  out->e = g->e;
+  return ORC_OK;
+}
+
+int orc_solve_terms(const orc_graph* g, const int* terms, int k, const orc_config* cfg,
+                    orc_result* out) {
+  if (!g || !terms || k < 2 || !cfg || !out) return ORC_ERR_ARG;
+  for (int i = 0; i < k; ++i)
+    if (terms[i] < 0 || terms[i] >= g->n) return ORC_ERR_RANGE;
+  int rc = check_common(g, terms[0], terms[1]);
+  if (rc) return rc;
+  std::memset(out, 0, sizeof(*out));
+  Engine en{g, terms[0], terms[1], g->w, g->nw, cfg->mode, cfg->trunc_eps, terms, k};
   Acc acc;
   auto t0 = Clock::now();
   switch (cfg->order) {

@@ -89,6 +89,12 @@ void orc_graph_arrays(const orc_graph* g, int* row_ptr, int* col_idx,
 int orc_solve(const orc_graph* g, int src, int dst, const orc_config* cfg,
               orc_result* out);
 
+/* K-terminal reliability: probability that every terminal is reached from
+ * terms[0] (undirected: all mutually connected).  The path becomes the
+ * terminal tree of Engine::find_tree; k >= 2, terms = {s, t} is orc_solve. */
+int orc_solve_terms(const orc_graph* g, const int* terms, int k, const orc_config* cfg,
+                    orc_result* out);
+
 /* Expand one item (path mode).  known/up are bit planes of W=ceil(E/64)
  * words.  Children are written in src->dst order of their failed hop:
  * child_known/child_up [max_children*W], child_mass[max_children].

@@ -75,6 +75,8 @@ def lib() -> ctypes.CDLL:
                                        P(ctypes.c_int), P(ctypes.c_double)]
         L.orc_graph_arrays.restype = None
         L.orc_solve.argtypes = [vp, ctypes.c_int, ctypes.c_int, P(OrcConfig), P(OrcResult)]
+        L.orc_solve_terms.argtypes = [vp, P(ctypes.c_int), ctypes.c_int, P(OrcConfig),
+                                      P(OrcResult)]
         U = P(ctypes.c_ulonglong)
         L.orc_expand.argtypes = [vp, ctypes.c_int, ctypes.c_int, U, U, ctypes.c_double,
                                  ctypes.c_int, U, U, P(ctypes.c_double), P(ctypes.c_double),
@@ -143,6 +145,19 @@ class OracleGraph:
         rc = lib().orc_solve(self._h, src, dst, ctypes.byref(cfg), ctypes.byref(r))
         if rc:
             raise RuntimeError(f"orc_solve failed: {rc}")
+        return r.as_dict()
+
+    def solve_terms(self, terms, mode: int = MODE_PATH, order: int = ORDER_LEVEL,
+                    threads: int = 1, trunc_eps: float = 0.0, capacity: int = 0,
+                    time_budget_s: float = 0.0) -> dict:
+        """K-terminal reliability with terms[0] as the BFS root."""
+        t = np.ascontiguousarray(terms, np.int32)
+        cfg = OrcConfig(mode, order, threads, trunc_eps, capacity, time_budget_s)
+        r = OrcResult()
+        rc = lib().orc_solve_terms(self._h, _p(t, ctypes.c_int), len(t), ctypes.byref(cfg),
+                                   ctypes.byref(r))
+        if rc:
+            raise RuntimeError(f"orc_solve_terms failed: {rc}")
         return r.as_dict()
 
     def expand(self, src: int, dst: int, known, up, mass: float):

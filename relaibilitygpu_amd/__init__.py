@@ -159,6 +159,8 @@ def lib() -> ctypes.CDLL:
     L.nr_graph_get_csr.argtypes = [vp, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int),
                                    P(ctypes.c_double)]
     L.nr_graph_set_terminals.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.nr_graph_set_terminal_set.argtypes = [vp, P(ctypes.c_int), ctypes.c_int]
+    L.nr_graph_get_terminal_set.argtypes = [vp, P(ctypes.c_int), ctypes.c_int, P(ctypes.c_int)]
     L.nr_graph_destroy.argtypes = [vp]
     L.nr_graph_destroy.restype = None
     L.nr_solve.argtypes = [vp, P(Config), P(Result)]
@@ -284,6 +286,19 @@ class Graph:
 
     def set_terminals(self, src: int, dst: int) -> None:
         _check(lib().nr_graph_set_terminals(self._h, int(src), int(dst)), "set_terminals")
+
+    def set_terminal_set(self, terminals) -> None:
+        """K-terminal problem rooted at terminals[0] (empty: back to src/dst)."""
+        t = np.ascontiguousarray(list(terminals), dtype=np.int32)
+        _check(lib().nr_graph_set_terminal_set(self._h, _ptr(t, ctypes.c_int), len(t)),
+               "nr_graph_set_terminal_set")
+
+    def terminal_set(self) -> list:
+        k = ctypes.c_int()
+        t = np.zeros(256, np.int32)
+        _check(lib().nr_graph_get_terminal_set(self._h, _ptr(t, ctypes.c_int), 256,
+                                               ctypes.byref(k)), "nr_graph_get_terminal_set")
+        return [int(x) for x in t[: k.value]]
 
     @property
     def handle(self):

@@ -5,7 +5,7 @@
  *   network_reliability <graph_file> <src> <dst> [--truncation_eps f]
  *       [--queue_capacity n] [--thresh_high f] [--thresh_low f]
  *       [--order level|mass|learned] [--rule path|forkbest] [--model file]
- *       [--alpha a] [--time_budget s] [--device d] [--json]
+ *       [--alpha a] [--terminals a,b,c,...] [--time_budget s] [--device d] [--json]
  *
  * Argument handling follows workingPad/main.cu:753-790: fewer than three
  * positionals print the usage on stderr and exit 1; unknown flags and flags
@@ -32,6 +32,7 @@ static void usage(const char* prog, long long cap) {
                "  --rule <path|forkbest>     Decomposition rule (default path)\n"
                "  --model <file>             Learned order: NR_MLP_PARAMS raw float32\n"
                "  --alpha <float>            Learned order: score weight (default 0.5)\n"
+               "  --terminals <a,b,...>      K-terminal reliability (first is the root)\n"
                "  --time_budget <seconds>    Fold the queue into ESP after this\n"
                "  --device <int>             HIP device ordinal\n"
                "  --json                     Print one JSON line instead\n",
@@ -50,6 +51,7 @@ int main(int argc, char** argv) {
   int dst = std::atoi(argv[3]);
   bool json = false;
   const char* model_file = nullptr;
+  const char* terminals = nullptr;
   float alpha = 0.5f;
   double thresh_high = -4.60517, thresh_low = -9.21034; /* common.h:35-36 */
   for (int i = 4; i < argc; ++i) {
@@ -71,6 +73,8 @@ int main(int argc, char** argv) {
       cfg.rule = !std::strcmp(argv[++i], "forkbest") ? NR_RULE_FORKBEST : NR_RULE_PATH;
     } else if (!std::strcmp(argv[i], "--model") && has) {
       model_file = argv[++i];
+    } else if (!std::strcmp(argv[i], "--terminals") && has) {
+      terminals = argv[++i];
     } else if (!std::strcmp(argv[i], "--alpha") && has) {
       alpha = (float)std::atof(argv[++i]);
     } else if (!std::strcmp(argv[i], "--time_budget") && has) {
@@ -89,6 +93,20 @@ int main(int argc, char** argv) {
   if (rc != NR_OK) {
     std::fprintf(stderr, "Cannot load %s: %s\n", file, nr_status_str(rc));
     return 1;
+  }
+  if (terminals) {
+    int t[256], k = 0;
+    for (const char* c = terminals; *c && k < 256;) {
+      t[k++] = std::atoi(c);
+      while (*c && *c != ',') ++c;
+      if (*c == ',') ++c;
+    }
+    rc = nr_graph_set_terminal_set(g, t, k);
+    if (rc != NR_OK) {
+      std::fprintf(stderr, "bad --terminals: %s\n", nr_status_str(rc));
+      nr_graph_destroy(g);
+      return 1;
+    }
   }
   nr_graph_info info;
   nr_graph_get_info(g, &info);

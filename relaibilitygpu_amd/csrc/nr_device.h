@@ -107,6 +107,9 @@ struct DevGraph {
   const u32* arc;   /* out-arcs v | e << 16, each row sorted by (v, e) */
   const int* irp;   /* [n+1] in-arc rows */
   const u32* iarc;  /* in-arcs u | e << 16, each row sorted by (u, e) */
+  const u16* terms; /* K-terminal mode (nterms >= 2): terms[0] is the root */
+  u64 tmask[4];     /* terminal set as a node set */
+  int nterms;
   int n, e, w, src, dst;
 };
 
@@ -174,6 +177,9 @@ struct Expander {
   u16* uh;  /* this lane's unknown-hop stack, stride `stride` */
   int stride, n, src, dst;
   mutable int plen; /* hops of the last path found */
+  const u16* terms;
+  int nterms;
+  NS tmask;
 
   __device__ void bind(unsigned char* smem, const Layout& L, int slot, int src_,
                        int dst_) {
@@ -198,6 +204,9 @@ struct Expander {
   static constexpr int kRowBytes = (int)sizeof(NS);
   __device__ void bind_graph(unsigned char* smem, const Layout& L, const DevGraph& g, int slot) {
     bind(smem, L, slot, g.src, g.dst);
+    terms = g.terms;
+    nterms = g.nterms;
+    tmask = (NS)g.tmask[0];
   }
   /* the item's rows into this lane's slice */
   __device__ void load(const QBuf& in, u64 qstride, u64 slot) const {
@@ -211,7 +220,57 @@ struct Expander {
     for (int i = 0; i < W; ++i) Dn[i] = known[i] & ~up[i];
     build_rows(Dn);
   }
-  __device__ int find_item(const u64 (&known)[W], const u64 (&)[W]) const { return find(known); }
+  __device__ int find_item(const u64 (&known)[W], const u64 (&)[W]) const {
+    return nterms >= 2 ? find_tree(known) : find(known);
+  }
+
+  /* K-terminal form of find(): BFS from terms[0], whole levels, until every
+   * terminal is reached (-1 if the frontier dies first); then each other
+   * terminal, in list order, walks back to the tree built so far under the
+   * same parent rule, pushing the UNKNOWN edges it crosses.  With the
+   * terminals {src, dst} this is find(). */
+  __device__ int find_tree(const u64 (&known)[W]) const {
+    const int root = terms[0];
+    NS vis = (NS)1 << root, F = vis;
+    lvl[0] = F;
+    NS need = tmask & ~vis;
+    int L = 0;
+    while (need) {
+      NS nx = 0, f = F;
+      while (f) {
+        const int u = ctzs<NS>(f);
+        f &= f - 1;
+        nx |= rows[u * stride];
+      }
+      nx &= ~vis;
+      if (!nx) return -1;
+      ++L;
+      lvl[L * stride] = nx;
+      vis |= nx;
+      F = nx;
+      need &= ~nx;
+    }
+    NS marked = (NS)1 << root;
+    int k = 0, len = 0;
+    for (int i = 1; i < nterms; ++i) {
+      int v = terms[i];
+      while (!((marked >> v) & (NS)1)) {
+        int l = 1;
+        while (l <= L && !((lvl[l * stride] >> v) & (NS)1)) ++l;
+        if (l > L) return -2;
+        const NS cand = rows[v * stride] & lvl[(l - 1) * stride];
+        if (!cand) return -2;
+        const int u = ctzs<NS>(cand);
+        const int e = eidm[u * n + v];
+        if (!((pick<W>(known, e >> 6) >> (e & 63)) & 1ull)) uh[(k++) * stride] = (u16)e;
+        ++len;
+        marked |= (NS)1 << v;
+        v = u;
+      }
+    }
+    plen = len;
+    return k;
+  }
   __device__ void store_child(const QBuf& out, u64 qstride, u64 slot, int e) const {
     NS* rout = (NS*)out.rows + slot;
     for (int u = 0; u < n; ++u) rout[(u64)u * qstride] = child_row(u, e);
@@ -236,9 +295,10 @@ struct Expander {
    * branch on: the path's UNKNOWN hop of highest p, the one nearest dst on
    * ties */
   __device__ int fork_item(const u64 (&known)[W], const u64 (&up)[W]) const {
-    if (src == dst) return 0;
-    const NS dbit = (NS)1 << dst;
-    NS vis = (NS)1 << src, F = vis;
+    /* confirmed BFS: every goal node (dst, or all terminals) over WORKING edges */
+    const NS goal = nterms >= 2 ? tmask : (NS)1 << dst;
+    NS vis = (NS)1 << (nterms >= 2 ? terms[0] : src), F = vis;
+    if (!(goal & ~vis)) return 0;
     while (F) {
       NS nx = 0;
       NS f = F;
@@ -248,11 +308,11 @@ struct Expander {
         nx |= working_row(u, up);
       }
       nx &= ~vis;
-      if (nx & dbit) return 0;
       vis |= nx;
+      if (!(goal & ~vis)) return 0;
       F = nx;
     }
-    const int k = find(known);
+    const int k = find_item(known, up);
     if (k <= 0) return k;
     int best = uh[0];
     double bp = p[best];
@@ -511,6 +571,9 @@ struct CsrExpander {
   u16* uh;            /* this lane's unknown-hop stack, stride `stride` */
   int stride, n, src, dst;
   mutable int plen; /* hops of the last path found */
+  const u16* terms;
+  int nterms;
+  u64 tmask[NW];
 
   __device__ void bind_graph(unsigned char* smem, const Layout& L, const DevGraph& g, int slot) {
     p = g.p;
@@ -520,6 +583,10 @@ struct CsrExpander {
     irp = g.irp;
     iarc = g.iarc;
     ep = g.ep;
+    terms = g.terms;
+    nterms = g.nterms;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) tmask[k] = g.tmask[k];
     lvb = smem + L.off_lvl + 4 * slot;
     uh = (u16*)(smem + L.off_uh) + slot;
     stride = L.block;
@@ -535,12 +602,26 @@ struct CsrExpander {
   __device__ void store_rows(const QBuf&, u64, u64) const {}
   /* forkBest on the CSR arcs; same contract as Expander::fork_item */
   __device__ int fork_item(const u64 (&known)[W], const u64 (&up)[W]) const {
-    if (src == dst) return 0;
-    u64 vis[NW], F[NW];
+    /* confirmed BFS: every goal node (dst, or all terminals) over WORKING arcs */
+    u64 vis[NW], F[NW], goal[NW];
 #pragma unroll
-    for (int k = 0; k < NW; ++k) vis[k] = F[k] = 0;
-    bs_set<NW>(vis, src);
-    bs_set<NW>(F, src);
+    for (int k = 0; k < NW; ++k) vis[k] = F[k] = goal[k] = 0;
+    const int root = nterms >= 2 ? terms[0] : src;
+    bs_set<NW>(vis, root);
+    bs_set<NW>(F, root);
+    if (nterms >= 2) {
+#pragma unroll
+      for (int k = 0; k < NW; ++k) goal[k] = tmask[k];
+    } else {
+      bs_set<NW>(goal, dst);
+    }
+    auto reached = [&]() {
+      bool all = true;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) all = all && !(goal[k] & ~vis[k]);
+      return all;
+    };
+    if (reached()) return 0;
     for (bool any = true; any;) {
       u64 nx[NW];
 #pragma unroll
@@ -556,13 +637,13 @@ struct CsrExpander {
             const u32 x = arc[j];
             const int v = (int)(x & 0xffffu), e = (int)(x >> 16);
             if (bs_test<NW>(vis, v) || !((pick<W>(up, e >> 6) >> (e & 63)) & 1ull)) continue;
-            if (v == dst) return 0;
             bs_set<NW>(vis, v);
             bs_set<NW>(nx, v);
             any = true;
           }
         }
       }
+      if (reached()) return 0;
 #pragma unroll
       for (int k = 0; k < NW; ++k) F[k] = nx[k];
     }
@@ -601,7 +682,92 @@ struct CsrExpander {
 
   /* Pushes the path's UNKNOWN hops on uh[] in dst->src order; returns their
    * number, -1 if dst is unreachable, -2 on an internal inconsistency. */
+  /* K-terminal form (see Expander::find_tree) on the CSR arcs */
+  __device__ int find_tree(const u64 (&known)[W], const u64 (&up)[W]) const {
+    for (int i = 0; i < (n + 3) / 4; ++i) *(u32*)(lvb + i * 4 * stride) = 0xffffffffu;
+    const int root = terms[0];
+    u64 vis[NW], F[NW], need[NW];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) vis[k] = F[k] = 0;
+    bs_set<NW>(vis, root);
+    bs_set<NW>(F, root);
+    set_level(root, 0);
+    bool left = false;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      need[k] = tmask[k] & ~vis[k];
+      left |= need[k] != 0;
+    }
+    int L = 0;
+    while (left) {
+      u64 nx[NW];
+#pragma unroll
+      for (int k = 0; k < NW; ++k) nx[k] = 0;
+      bool any = false;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) {
+        u64 fw = F[k];
+        while (fw) {
+          const int u = k * 64 + __builtin_ctzll(fw);
+          fw &= fw - 1;
+          for (int j = rp[u]; j < rp[u + 1]; ++j) {
+            const u32 x = arc[j];
+            const int v = (int)(x & 0xffffu), e = (int)(x >> 16);
+            if (bs_test<NW>(vis, v) || !allowed(known, up, e)) continue;
+            bs_set<NW>(vis, v);
+            bs_set<NW>(nx, v);
+            any = true;
+          }
+        }
+      }
+      if (!any) return -1;
+      ++L;
+      if (L >= 255) return -2;
+      left = false;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) {
+        u64 m = nx[k];
+        while (m) {
+          set_level(k * 64 + __builtin_ctzll(m), L);
+          m &= m - 1;
+        }
+        F[k] = nx[k];
+        need[k] &= ~nx[k];
+        left |= need[k] != 0;
+      }
+    }
+    u64 marked[NW];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) marked[k] = 0;
+    bs_set<NW>(marked, root);
+    int k = 0, len = 0;
+    for (int i = 1; i < nterms; ++i) {
+      int v = terms[i];
+      while (!bs_test<NW>(marked, v)) {
+        const int l = level_of(v);
+        int pu = -1, pe = -1;
+        for (int j = irp[v]; j < irp[v + 1]; ++j) {
+          const u32 x = iarc[j];
+          const int u = (int)(x & 0xffffu), e = (int)(x >> 16);
+          if (level_of(u) == l - 1 && allowed(known, up, e)) {
+            pu = u;
+            pe = e;
+            break;
+          }
+        }
+        if (pu < 0) return -2;
+        if (!((pick<W>(known, pe >> 6) >> (pe & 63)) & 1ull)) uh[(k++) * stride] = (u16)pe;
+        ++len;
+        bs_set<NW>(marked, v);
+        v = pu;
+      }
+    }
+    plen = len;
+    return k;
+  }
+
   __device__ int find_item(const u64 (&known)[W], const u64 (&up)[W]) const {
+    if (nterms >= 2) return find_tree(known, up);
     if (src == dst) {
       plen = 0;
       return 0;

@@ -504,6 +504,7 @@ __global__ void __launch_bounds__(256) k_expand_batch(BatchArgs a, int parity) {
       ex.n = d.n;
       ex.src = d.src;
       ex.dst = d.dst;
+      ex.nterms = 0;
       cnt = 1;
       if (mass < a.eps) { /* main.cu:537-543 */
         fx_limbs_add(e2, e1, e0, mass);
@@ -764,7 +765,7 @@ Workspace g_ws[kMaxDevices];
 struct TableBlob {
   std::vector<unsigned char> bytes;
   size_t off_p, off_q, off_sadj, off_inc, off_ep, off_deg, off_eidm;
-  size_t off_rp, off_arc, off_irp, off_iarc;
+  size_t off_rp, off_arc, off_irp, off_iarc, off_terms;
 };
 
 void pack_tables(const nr_graph* g, int W, TableBlob& b) {
@@ -785,6 +786,7 @@ void pack_tables(const nr_graph* g, int W, TableBlob& b) {
   b.off_ep = take(4 * (size_t)e);
   b.off_deg = take(4 * (size_t)ns + 4);
   b.off_eidm = take(2 * (size_t)ns * ns + 4);
+  b.off_terms = take(2 * (size_t)std::max<size_t>(g->terms.size(), 1));
   b.off_rp = take(4 * (size_t)(n + 1));
   b.off_arc = take(4 * (size_t)std::max(g->nnz, 1));
   b.off_irp = take(4 * (size_t)(n + 1));
@@ -845,6 +847,8 @@ void pack_tables(const nr_graph* g, int W, TableBlob& b) {
   }
   rp[n] = k;
   irp[n] = ik;
+  u16* tm = (u16*)&b.bytes[b.off_terms];
+  for (size_t i = 0; i < g->terms.size(); ++i) tm[i] = (u16)g->terms[i];
 }
 
 int upload_tables(Workspace& ws, const nr_graph* g, int W, DevGraph& d) {
@@ -866,6 +870,10 @@ int upload_tables(Workspace& ws, const nr_graph* g, int W, DevGraph& d) {
   d.arc = (const u32*)(base + b.off_arc);
   d.irp = (const int*)(base + b.off_irp);
   d.iarc = (const u32*)(base + b.off_iarc);
+  d.terms = (const u16*)(base + b.off_terms);
+  d.nterms = (int)g->terms.size();
+  for (int k = 0; k < 4; ++k) d.tmask[k] = 0;
+  for (int t : g->terms) d.tmask[t >> 6] |= 1ull << (t & 63);
   d.n = g->n;
   d.e = g->e;
   d.w = W;
@@ -1003,6 +1011,7 @@ u64 graph_hash(const nr_graph* g) {
   mix(g->col_idx.data(), sizeof(int) * g->col_idx.size());
   mix(g->edge_id.data(), sizeof(int) * g->edge_id.size());
   mix(g->p.data(), sizeof(double) * g->p.size());
+  mix(g->terms.data(), sizeof(int) * g->terms.size());
   return h;
 }
 
@@ -1456,6 +1465,7 @@ int solve_batch(Workspace& ws, const nr_graph* const* gs, int count, const nr_co
     if (!g) return NR_ERR_ARG;
     if (!g->symmetric || g->n > 64 || mask_words(g) > 8) return NR_ERR_UNSUPPORTED;
     if ((u32)g->e >= kMaxDepth) return NR_ERR_UNSUPPORTED; /* depth <= E fits the tag */
+    if (!g->terms.empty()) return NR_ERR_UNSUPPORTED;     /* two-terminal instances only */
     Wmax = std::max(Wmax, mask_words(g));
     Nmax = std::max(Nmax, g->n);
   }

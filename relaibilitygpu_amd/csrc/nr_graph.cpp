@@ -237,6 +237,7 @@ int nr_graph_set_terminals(nr_graph* g, int src, int dst) {
   if (src < 0 || src >= g->n || dst < 0 || dst >= g->n) return NR_ERR_RANGE;
   g->src = src;
   g->dst = dst;
+  g->terms.clear(); /* back to the two-terminal problem */
   return NR_OK;
 }
 
@@ -267,6 +268,26 @@ int nr_graph_get_csr(const nr_graph* g, int* row_ptr, int* col_idx, int* edge_id
 }
 
 void nr_graph_destroy(nr_graph* g) { delete g; }
+
+int nr_graph_set_terminal_set(nr_graph* g, const int* terminals, int k) {
+  if (!g || k < 0 || k == 1 || (k > 0 && !terminals)) return NR_ERR_ARG;
+  if (k > 256) return NR_ERR_UNSUPPORTED;
+  for (int i = 0; i < k; ++i)
+    if (terminals[i] < 0 || terminals[i] >= g->n) return NR_ERR_RANGE;
+  g->terms.assign(terminals, terminals + k);
+  if (k >= 2) {
+    g->src = terminals[0];
+    g->dst = terminals[1];
+  }
+  return NR_OK;
+}
+
+int nr_graph_get_terminal_set(const nr_graph* g, int* terminals, int max_k, int* k) {
+  if (!g || !k) return NR_ERR_ARG;
+  *k = (int)g->terms.size();
+  for (int i = 0; i < *k && i < max_k && terminals; ++i) terminals[i] = g->terms[i];
+  return NR_OK;
+}
 
 int nr_config_init(nr_config* cfg) {
   if (!cfg) return NR_ERR_ARG;

@@ -25,6 +25,7 @@ struct nr_graph {
   std::vector<uint8_t> dirmask; /* bit0: ea->eb in CSR, bit1: eb->ea */
   bool symmetric = true;
   int precontracted = 0, predeleted = 0;
+  std::vector<int> terms; /* K-terminal set (empty: the src/dst pair) */
 };
 
 namespace nr {

@@ -11,11 +11,12 @@ Pins, each chosen so that a plausible mistake in the oracle fails one:
 import math
 import random
 
+import numpy as np
 import pytest
 
 from conftest import closed_form, graph_path
-from oracle_ffi import (MODE_PATH, MODE_PIVOT, ORDER_DFS, ORDER_HEAP, ORDER_LEVEL,
-                        OracleGraph)
+from oracle_ffi import (MODE_FORKBEST, MODE_PATH, MODE_PIVOT, ORDER_DFS, ORDER_HEAP,
+                        ORDER_LEVEL, OracleGraph)
 
 
 def load(name):
@@ -256,3 +257,79 @@ def test_minunk_never_longer_in_unknowns():
     a = g.solve(0, 4)
     b = g.solve(0, 4, mode=MODE_MINUNK)
     assert b["nodes"] <= a["nodes"] and abs(a["R"] - b["R"]) <= 1e-15
+
+
+# ------------------------------------------------------------ K-terminal
+def _brute_k(n, edges, probs, K):
+    """Probability that every terminal of K is connected (undirected), by
+    enumerating all 2^E edge states with a union-find."""
+    import itertools
+    R = 0.0
+    for st in itertools.product([0, 1], repeat=len(edges)):
+        par = list(range(n))
+
+        def f(x):
+            while par[x] != x:
+                par[x] = par[par[x]]
+                x = par[x]
+            return x
+        for (a, b), s in zip(edges, st):
+            if s:
+                par[f(a)] = f(b)
+        if len({f(k) for k in K}) == 1:
+            R += float(np.prod([p if s else 1 - p for p, s in zip(probs, st)]))
+    return R
+
+
+def test_kterminal_pair_is_two_terminal():
+    """K = {s, t} builds exactly the two-terminal tree (same counts, R)."""
+    g = load("COMP10-05.txt")
+    a = g.solve(0, 9)
+    b = g.solve_terms([0, 9])
+    assert (a["paths"], a["nodes"], a["iterations"], a["max_width"]) == (
+        b["paths"], b["nodes"], b["iterations"], b["max_width"])
+    assert a["R"] == b["R"]
+
+
+@pytest.mark.parametrize("n", [4, 5, 6, 8])
+def test_all_terminal_cycle_closed_form(n):
+    """All-terminal reliability of the cycle C_n: p^n + n p^(n-1) q."""
+    from relaibilitygpu_amd import workloads as wl
+    p = 0.85
+    edges = sorted((min(i, (i + 1) % n), max(i, (i + 1) % n)) for i in range(n))
+    rp, ci, ei = wl.to_csr(n, edges)
+    og = OracleGraph.from_csr(rp, ci, ei, [p] * n)
+    r = og.solve_terms(list(range(n)))
+    assert abs(r["R"] - (p ** n + n * p ** (n - 1) * (1 - p))) <= 1e-14
+    assert abs(r["R"] + r["D"] - 1.0) <= 1e-14
+
+
+@pytest.mark.parametrize("seed,K", [(3, [0, 3, 5]), (4, [6, 1, 2, 0]), (5, [2, 4]),
+                                    (6, [0, 1, 2, 3, 4, 5, 6])])
+def test_kterminal_brute_force(seed, K):
+    """Random graphs on 7 nodes: path rule, forkBest rule, a permuted
+    terminal list and depth-first order all give the brute-force R."""
+    from relaibilitygpu_amd import workloads as wl
+    rng = np.random.default_rng(seed)
+    n = 7
+    edges = sorted({(int(min(x, y)), int(max(x, y))) for x, y in rng.integers(0, n, (14, 2))
+                    if x != y})
+    probs = list(rng.uniform(0.3, 0.9, len(edges)))
+    rp, ci, ei = wl.to_csr(n, edges)
+    og = OracleGraph.from_csr(rp, ci, ei, probs)
+    R = _brute_k(n, edges, probs, K)
+    for r in (og.solve_terms(K), og.solve_terms(K[::-1]), og.solve_terms(K, mode=MODE_FORKBEST),
+              og.solve_terms(K, order=ORDER_DFS, threads=2)):
+        assert abs(r["R"] - R) <= 1e-13
+        assert abs(r["R"] + r["D"] - 1.0) <= 1e-13
+
+
+def test_kterminal_detects_a_wrong_tree():
+    """A plausible slip, branching only on the path to the last terminal,
+    changes R: on C4 with K = all nodes it would give two-terminal R."""
+    from relaibilitygpu_amd import workloads as wl
+    p = 0.85
+    edges = [(0, 1), (0, 3), (1, 2), (2, 3)]
+    rp, ci, ei = wl.to_csr(4, edges)
+    og = OracleGraph.from_csr(rp, ci, ei, [p] * 4)
+    assert abs(og.solve_terms([0, 1, 2, 3])["R"] - og.solve(0, 3)["R"]) > 1e-3
