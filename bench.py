@@ -8,8 +8,11 @@ library caches its per-device workspace), so a timed step is the solve itself.
 
 Prints one JSON line with the metric, a roofline object for the expand
 kernel (HBM bound: algorithmic bytes per subproblem = 2 * (16*W + 8), one
-write when the item is created and one read when it is expanded) and the CPU
-baseline (the oracle on the host cores, same workload).
+write when the item is created and one read when it is expanded), the CPU
+baseline (the oracle on the host cores, same workload) and the metric's
+second part: the certified gap ESP at a time budget (60 s by default) for
+the learned order against the plain level and max-mass orders on K15 and
+K25 (esp_at_budget).
 """
 from __future__ import annotations
 
@@ -30,7 +33,7 @@ WORKLOADS = {
             "desc": "K10 (N=10, E=45, p=0.5) src 0 -> dst 9, full enumeration, level order"},
     "k8p09": {"graph": "COMP8-09.txt", "src": 0, "dst": 7, "eps": 0.0, "cap": 1 << 16,
               "desc": "K8 (N=8, E=28, p=0.9) src 0 -> dst 7, full enumeration, level order"},
-    "er4096": {"batch": 4096, "eps": 1e-12, "cap": 1 << 29,
+    "er4096": {"batch": 4096, "eps": 1e-12, "cap": 1 << 27,
                "desc": ("4096 seeded Erdos-Renyi instances (n 12..40, mean degree 4..8, "
                         "p 0.80..0.99, terminals at distance >= 3), truncation 1e-12, "
                         "one batched solve")},
@@ -189,6 +192,49 @@ def run_batch(args, wl: dict) -> dict:
     }
 
 
+def esp_leg(budget_s: float = 60.0) -> dict:
+    """Certified gap ESP at a time budget, learned order vs the plain orders,
+    on K15 and K25 (p = 0.5, delta = 1e-15, capacity 96 777 216 items).  The
+    scorer is trained here, on the GPU, on 2e6 children logged from plain K10
+    solves (delta 0, 1e-6, 1e-9); labels are log2 subtree sizes."""
+    import numpy as np
+
+    import relaibilitygpu_amd as nr
+
+    G = os.path.join(ROOT, "tests", "golden", "graphs")
+    t0 = time.perf_counter()
+    k10 = nr.Graph.load(os.path.join(G, "COMP10-05.txt"), 0, 9)
+    Xs, Ys = [], []
+    for eps, m in ((0.0, 1000000), (1e-6, 500000), (1e-9, 500000)):
+        X, Y, _ = nr.log_subproblems(k10, m, seed=7, truncation_eps=eps, queue_capacity=1 << 22)
+        Xs.append(X)
+        Ys.append(Y)
+    X, Y = np.concatenate(Xs), np.concatenate(Ys)
+    t_log = time.perf_counter() - t0
+    params, tl, hl = nr.mlp_train(X, Y, None, epochs=10, batch=4096, lr=3e-3, seed=3)
+    t_train = time.perf_counter() - t0 - t_log
+    model = nr.Model(params, alpha=0.5)
+    out = {"budget_s": budget_s, "delta": 1e-15, "queue_capacity": 96777216,
+           "training": {"records": int(len(Y)), "label_var": float(Y.var()),
+                        "holdout_mse": float(hl[-1]), "log_s": t_log, "train_s": t_train,
+                        "alpha": 0.5}}
+    for name, n in (("K15", 15), ("K25", 25)):
+        g = nr.Graph.load(os.path.join(G, f"COMP{n}-05.txt"), 0, n - 1)
+        kw = dict(truncation_eps=1e-15, queue_capacity=96777216, time_budget_s=budget_s)
+        rows = {}
+        for order in ("level", "mass"):
+            r = nr.solve(g, order=order, **kw)
+            rows[order] = r
+        rows["learned"] = nr.solve_learned(g, model, **kw)
+        out[name] = {k: {"ESP": v["ESP"], "R": v["R"], "nodes": v["nodes"],
+                         "elapsed_ms": v["elapsed_ms"], "deadline_hit": v["deadline_hit"]}
+                     for k, v in rows.items()}
+        out[name]["ESP_mass_over_learned"] = rows["mass"]["ESP"] / rows["learned"]["ESP"]
+        print(f"esp {name}: " + ", ".join(f"{k} {v['ESP']:.3e}" for k, v in rows.items()),
+              file=sys.stderr, flush=True)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -198,6 +244,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-sample", type=int, default=512,
                     help="batched workloads: instances timed on the CPU")
+    ap.add_argument("--no-esp", action="store_true",
+                    help="skip the learned-vs-plain certified-gap leg")
+    ap.add_argument("--esp-budget", type=float, default=60.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -303,6 +352,8 @@ def main() -> int:
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(wl)
+    if not args.no_esp:
+        line["esp_at_budget"] = esp_leg(args.esp_budget)
     print(json.dumps(line), flush=True)
     return 0
 
