@@ -84,6 +84,11 @@ typedef enum nr_rule {
 #define NR_FLAG_TIME_LAUNCHES 1 /* time every expand launch with HIP events
                                    (fills kernel_ms / kernel_launches; adds a
                                    few microseconds between launches) */
+#define NR_FLAG_ROWS 4          /* undirected graphs of <= 64 nodes: items always
+                                   carry their adjacency rows (by default they
+                                   do only while the rows take <= 48 bytes, and
+                                   are derived from the planes otherwise; same
+                                   results) */
 
 typedef struct nr_result {
   double R;                     /* reliability lower bound (exact when ESP=0) */

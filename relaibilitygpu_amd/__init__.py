@@ -24,6 +24,7 @@ LIB_PATH = os.path.join(_HERE, "libnrgpu.so")
 CLI_PATH = os.path.join(_HERE, "bin", "network_reliability")
 
 NR_FLAG_TIME_LAUNCHES = 1
+NR_FLAG_ROWS = 4
 
 NR_ORDER_LEVEL = 0
 NR_ORDER_MASS = 1
@@ -209,7 +210,8 @@ def _ptr(a: np.ndarray, ctype):
 def make_config(queue_capacity: int = 1 << 20, truncation_eps: float = 1e-15,
                 time_budget_s: float = 0.0, order: str | int = "level", device: int = 0,
                 block_size: int = 0, grid_blocks: int = 0, chunk_items: int = 0,
-                time_launches: bool = False, rule: str | int = "path") -> Config:
+                time_launches: bool = False, rule: str | int = "path",
+                rows: bool = False) -> Config:
     c = Config()
     lib().nr_config_init(ctypes.byref(c))
     c.queue_capacity = int(queue_capacity)
@@ -220,7 +222,7 @@ def make_config(queue_capacity: int = 1 << 20, truncation_eps: float = 1e-15,
     c.block_size = int(block_size)
     c.grid_blocks = int(grid_blocks)
     c.chunk_items = int(chunk_items)
-    c.flags = NR_FLAG_TIME_LAUNCHES if time_launches else 0
+    c.flags = (NR_FLAG_TIME_LAUNCHES if time_launches else 0) | (NR_FLAG_ROWS if rows else 0)
     c.rule = RULES[rule] if isinstance(rule, str) else int(rule)
     return c
 

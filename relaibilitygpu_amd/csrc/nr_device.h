@@ -496,6 +496,68 @@ struct Expander {
     return k;
   }
 
+  /* find_planes() with the smallest LDS slice: adjacency derived on every
+   * use (no cache) and each node's BFS level kept as a byte (lvb: node u at
+   * (u>>2)*4*stride + (u&3) of this lane's slice); the parent of v is the
+   * lowest-index neighbour on the previous level, as in find(). */
+  __device__ int find_lean(const u64 (&known)[W], const u64 (&up)[W], unsigned char* lvb) const {
+    if (src == dst) {
+      plen = 0;
+      return 0;
+    }
+    u64 Dn[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i) Dn[i] = known[i] & ~up[i];
+    for (int i = 0; i < (n + 3) / 4; ++i) *(u32*)(lvb + i * 4 * stride) = 0xffffffffu;
+    auto setl = [&](int u, int l) { lvb[(u >> 2) * 4 * stride + (u & 3)] = (unsigned char)l; };
+    auto getl = [&](int u) { return (int)lvb[(u >> 2) * 4 * stride + (u & 3)]; };
+    const NS dbit = (NS)1 << dst;
+    NS vis = (NS)1 << src;
+    NS F = vis;
+    setl(src, 0);
+    int L = 0;
+    for (;;) {
+      NS nx = 0;
+      NS f = F;
+      while (f) {
+        const int u = ctzs<NS>(f);
+        f &= f - 1;
+        nx |= adj_of(u, Dn);
+        if (nx & dbit) break;
+      }
+      nx &= ~vis;
+      if (!nx) return -1;
+      ++L;
+      if (nx & dbit) break;
+      NS m = nx;
+      while (m) {
+        setl(ctzs<NS>(m), L);
+        m &= m - 1;
+      }
+      vis |= nx;
+      F = nx;
+    }
+    plen = L;
+    int v = dst, k = 0;
+    for (int l = L; l >= 1; --l) {
+      NS cand = adj_of(v, Dn);
+      int u = -1;
+      while (cand) {
+        const int c = ctzs<NS>(cand);
+        cand &= cand - 1;
+        if (getl(c) == l - 1) {
+          u = c;
+          break;
+        }
+      }
+      if (u < 0) return -2;
+      const int e = eidm[u * n + v];
+      if (!((pick<W>(known, e >> 6) >> (e & 63)) & 1ull)) uh[(k++) * stride] = (u16)e;
+      v = u;
+    }
+    return k;
+  }
+
   /* Row u of a child whose FAILED hop is edge e: the parent's row with the
    * ends of e removed from each other's rows. */
   __device__ NS child_row(int u, int e) const {
@@ -839,6 +901,48 @@ struct CsrExpander {
       v = pu;
     }
     return k;
+  }
+};
+
+/* LDS layout of the lean row-form expander: staged graph tables, then the
+ * kernel's extra region, per-lane node levels (bytes) and hop stack. */
+inline Layout make_layout_lean(int n, int e, int w, int block, int extra_bytes = 0) {
+  Layout L = make_layout(n, e, w, block, 0, extra_bytes, true);
+  int o = L.off_extra + extra_bytes;
+  o = (o + 15) / 16 * 16;
+  L.off_lvl = o;
+  L.off_rows = o;
+  o += 4 * ((n + 3) / 4) * block;
+  o = (o + 15) / 16 * 16;
+  L.off_uh = o;
+  o += 2 * n * block;
+  L.total_bytes = (o + 15) / 16 * 16;
+  return L;
+}
+
+/* Row-form graphs (undirected, N <= 64) with items that carry no rows: the
+ * graph's tables are staged in LDS and each node's allowed neighbours are
+ * derived from the item's FAILED plane when the search reaches it
+ * (Expander::adj_of); node levels are bytes.  Used for the plain path rule;
+ * the same search as Expander::find (Expander::find_lean). */
+template <int W, typename NS>
+struct LeanExpander : Expander<W, NS> {
+  static constexpr bool kStage = true;
+  static constexpr int kRowBytes = 0;
+  unsigned char* lvb;
+  __device__ void bind_graph(unsigned char* smem, const Layout& L, const DevGraph& g, int slot) {
+    Expander<W, NS>::bind_graph(smem, L, g, slot);
+    this->rows = nullptr;
+    this->lvl = nullptr;
+    lvb = smem + L.off_lvl + 4 * slot;
+  }
+  __device__ void load(const QBuf&, u64, u64) const {}
+  __device__ void derive(const u64 (&)[W], const u64 (&)[W]) const {}
+  __device__ void store_child(const QBuf&, u64, u64, int) const {}
+  __device__ void store_rows(const QBuf&, u64, u64) const {}
+  __device__ void carry(const QBuf&, u64, const QBuf&, u64, u64) const {}
+  __device__ int find_item(const u64 (&known)[W], const u64 (&up)[W]) const {
+    return this->find_lean(known, up, lvb);
   }
 };
 

@@ -41,13 +41,21 @@
 
 namespace nr {
 
-/* Sum the per-segment class histograms into the control block (one
- * thread per class). */
-__global__ void __launch_bounds__(kClasses) k_hist_total(LevelArgs a) {
-  const int cl = threadIdx.x;
+/* Sum the per-segment class histograms into the control block: one
+ * workgroup per class, a strided sum over the segments and a tree in LDS
+ * (fixed order, so the totals do not depend on timing). */
+__global__ void __launch_bounds__(256) k_hist_total(LevelArgs a) {
+  __shared__ u64 part[256];
+  const int cl = blockIdx.x;
   u64 sum = 0;
-  for (int b = 0; b < a.n_seg; ++b) sum += a.hist[(u64)b * kClasses + cl];
-  a.ctl->hist[cl] = sum;
+  for (int b = threadIdx.x; b < a.n_seg; b += blockDim.x) sum += a.hist[(u64)b * kClasses + cl];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) a.ctl->hist[cl] = part[0];
 }
 
 /* Fold every queued item of buf[parity] into ESP (time budget expiry;
@@ -456,10 +464,11 @@ __global__ void __launch_bounds__(256) k_expand_batch(BatchArgs a, int parity) {
   __syncthreads();
 
   Expander<W, NS> ex;
-  ex.rows = (NS*)(smem + a.lay.off_rows) + threadIdx.x;
-  ex.lvl = (NS*)(smem + a.lay.off_lvl) + threadIdx.x;
+  ex.rows = nullptr;
+  ex.lvl = nullptr;
   ex.uh = (u16*)(smem + a.lay.off_uh) + threadIdx.x;
   ex.stride = a.lay.block;
+  unsigned char* lvb = smem + a.lay.off_lvl + 4 * threadIdx.x;
 
   const QBuf in = a.buf[parity];
   const QBuf out = a.buf[parity ^ 1];
@@ -510,7 +519,7 @@ __global__ void __launch_bounds__(256) k_expand_batch(BatchArgs a, int parity) {
         fx_limbs_add(e2, e1, e0, mass);
         cnt |= 1ull << 16;
       } else {
-        k = ex.find_planes(known, up);
+        k = ex.find_lean(known, up, lvb);
         if (k >= 0) {
           live = true;
         } else {
@@ -900,16 +909,17 @@ Variant variant_for_w(int W) {
   case X:      \
     return Variant{&Kern<X, NS>::level, &Kern<X, NS>::items, &Kern<X, NS>::batch,               \
                    &Kern<X, NS>::refill, (int)sizeof(NS), 0, nullptr, nullptr, nullptr,  \
-                   &Kern<X, NS>::seed_rows};
+                   &Kern<X, NS>::seed_rows, 0};
     NR_W(1) NR_W(2) NR_W(3) NR_W(4) NR_W(5) NR_W(6) NR_W(7) NR_W(8)
 #undef NR_W
-    default: return Variant{nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr};
+    default: return Variant{nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr, 0};
   }
 }
 
 /* LDS layout of a variant's expand kernels for graph g. */
 Layout layout_for(const Variant& v, const nr_graph* g, int W, int block, int extra = 0) {
   if (v.nw) return make_layout_csr(g->n, block, extra);
+  if (v.lean) return make_layout_lean(g->n, std::max(g->e, 1), W, block, extra);
   return make_layout(g->n, std::max(g->e, 1), W, block, v.ns_bytes, extra);
 }
 
@@ -1035,6 +1045,13 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   int rc = pick_variant(g, var);
   if (rc) return rc;
   const int mode = x ? x->mode : 0;
+  /* the plain path rule on a row-form graph whose rows would exceed 48 bytes
+   * per item runs with items that carry none (measured on MI355X: K10, 40 B
+   * of rows, 1.42 ms with rows vs 1.94 ms without; K15 23.1 vs 21.5 ms; K25
+   * 64.0 vs 31.4 ms), unless the caller asks for the row-carrying kernel */
+  if (mode == 0 && cfg->rule == NR_RULE_PATH && g->terms.empty() && !var.nw &&
+      g->n * var.ns_bytes > 48 && !(cfg->flags & NR_FLAG_ROWS))
+    var = variant_lean(mask_words(g), var.ns_bytes);
   const int tile_bytes = mode == 2 ? 32 * 256 + 16 : 0; /* per-wave feature tiles */
   const int W = mask_words(g);
   const int cus = ws.prop.multiProcessorCount;
@@ -1311,7 +1328,7 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
       const int parity = (int)(launches_here & 1);
       if ((rc = launch_level(parity, (int)rounds))) return rc;
       void* hargs[] = {&a};
-      NR_HIP(hipLaunchKernel((const void*)&k_hist_total, dim3(1), dim3(kClasses), hargs, 0, st));
+      NR_HIP(hipLaunchKernel((const void*)&k_hist_total, dim3(kClasses), dim3(256), hargs, 0, st));
       NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
       NR_HIP(hipStreamSynchronize(st));
       ++rounds;
@@ -1480,8 +1497,8 @@ int solve_batch(Workspace& ws, const nr_graph* const* gs, int count, const nr_co
    * leave room for 2 workgroups per CU */
   int block = cfg->block_size > 0 ? cfg->block_size : 256;
   if (block % 64 || block > 256) return NR_ERR_ARG;
-  auto lay_for = [&](int blk, int grid) {
-    return make_layout(Nmax, 1, W, blk, var.ns_bytes, 4 * (grid + 3 + blk), false);
+  auto lay_for = [&](int blk, int grid) { /* level bytes + hop stack per lane */
+    return make_layout_csr(Nmax, blk, 4 * (grid + 3 + blk));
   };
   if (cfg->block_size <= 0)
     while (block > 64 && lay_for(block, 4 * cus).total_bytes > lds_max) block /= 2;

@@ -656,7 +656,12 @@ struct Variant {
   KernelGetter level_log, level_learn; /* MODE 1 / MODE 2 of k_expand_level */
   KernelGetter level_fork;             /* MODE 3 (forkBest rule) */
   KernelGetter seed_rows;              /* row form: rows of resumed items */
+  int lean;                            /* items carry no rows (LeanExpander) */
 };
+
+/* Lean row form (plain path rule, two terminals) for W words, node sets of
+ * NS bytes (nr_lean.hip). */
+Variant variant_lean(int W, int ns_bytes);
 
 /* MODE 3 kernels of a variant (nr_fork.hip), filled into v. */
 void add_fork_kernels(Variant& v, int W);

@@ -65,9 +65,9 @@ def test_k10_level_chain(tmp_path, expected):
 def test_k15_mass_chain(tmp_path, expected):
     print("case checkpoint K15 mass chain", flush=True)
     g = nr.Graph.load(graph_path("COMP15-05.txt"), 0, 14)
-    kw = dict(truncation_eps=1e-9, queue_capacity=1 << 26, order="mass")
+    kw = dict(truncation_eps=1e-10, queue_capacity=1 << 26, order="mass")
     full = nr.solve(g, **kw)
-    r, links, mids = chain(g, tmp_path, 2e-3, **kw)
+    r, links, mids = chain(g, tmp_path, 3e-4, **kw)
     assert links >= 2
     same(r, full, levels=False)
     R = closed_form(expected, 15, "0.5")
