@@ -20,10 +20,10 @@ static Variant variant_csr_nw(int W) {
 #define NR_W(X) \
   case X:      \
     return Variant{&KernCsr<X, NW>::level, &KernCsr<X, NW>::items, nullptr, nullptr, 0, NW, \
-                   nullptr, nullptr, nullptr, nullptr, 0};
+                   nullptr, nullptr, nullptr, nullptr, 0, nullptr};
     NR_W(1) NR_W(2) NR_W(3) NR_W(4) NR_W(5) NR_W(6) NR_W(7) NR_W(8)
 #undef NR_W
-    default: return Variant{nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr, 0};
+    default: return Variant{nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr, 0, nullptr};
   }
 }
 

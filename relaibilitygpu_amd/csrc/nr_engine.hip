@@ -909,10 +909,10 @@ Variant variant_for_w(int W) {
   case X:      \
     return Variant{&Kern<X, NS>::level, &Kern<X, NS>::items, &Kern<X, NS>::batch,               \
                    &Kern<X, NS>::refill, (int)sizeof(NS), 0, nullptr, nullptr, nullptr,  \
-                   &Kern<X, NS>::seed_rows, 0};
+                   &Kern<X, NS>::seed_rows, 0, nullptr};
     NR_W(1) NR_W(2) NR_W(3) NR_W(4) NR_W(5) NR_W(6) NR_W(7) NR_W(8)
 #undef NR_W
-    default: return Variant{nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr, 0};
+    default: return Variant{nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr, 0, nullptr};
   }
 }
 
@@ -946,6 +946,7 @@ int pick_variant(const nr_graph* g, Variant& v) {
   if (!v.level) return NR_ERR_UNSUPPORTED;
   add_learn_kernels(v, W);
   add_fork_kernels(v, W);
+  add_mass_kernels(v, W);
   return NR_OK;
 }
 
@@ -1050,8 +1051,10 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
    * of rows, 1.42 ms with rows vs 1.94 ms without; K15 23.1 vs 21.5 ms; K25
    * 64.0 vs 31.4 ms), unless the caller asks for the row-carrying kernel */
   if (mode == 0 && cfg->rule == NR_RULE_PATH && g->terms.empty() && !var.nw &&
-      g->n * var.ns_bytes > 48 && !(cfg->flags & NR_FLAG_ROWS))
+      g->n * var.ns_bytes > 48 && !(cfg->flags & NR_FLAG_ROWS)) {
     var = variant_lean(mask_words(g), var.ns_bytes);
+    add_mass_kernels(var, mask_words(g));
+  }
   const int tile_bytes = mode == 2 ? 32 * 256 + 16 : 0; /* per-wave feature tiles */
   const int W = mask_words(g);
   const int cus = ws.prop.multiProcessorCount;
@@ -1059,9 +1062,11 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
    * LDS.  Size the grid from the occupancy without it, then add the table
    * for that grid; occupancy can only drop, so the table stays large
    * enough for the final grid. */
-  KernelGetter getter = mode == 1   ? var.level_log
-                        : mode == 2 ? var.level_learn
+  const bool mass_order = cfg->order == NR_ORDER_MASS;
+  KernelGetter getter = mode == 1                       ? var.level_log
+                        : mode == 2                     ? var.level_learn
                         : cfg->rule == NR_RULE_FORKBEST ? var.level_fork
+                        : mass_order                    ? var.level_mass
                                                         : var.level;
   if (!getter) return NR_ERR_UNSUPPORTED;
   const void* kfun = (const void*)getter();

@@ -14,7 +14,7 @@ struct KernLean {
 
 template <typename NS>
 static Variant lean_for_w(int W) {
-  Variant v{nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr, 1};
+  Variant v{nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr, 1, nullptr};
   switch (W) {
 #define NR_W(X)                            \
   case X:                                  \
@@ -24,6 +24,7 @@ static Variant lean_for_w(int W) {
 #undef NR_W
     default: break;
   }
+  v.ns_bytes = (int)sizeof(NS); /* node-set width (items carry no rows) */
   return v;
 }
 

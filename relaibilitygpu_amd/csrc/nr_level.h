@@ -231,13 +231,16 @@ __device__ inline void child_ctx(const LevelArgs& a, const EX& ex, const u64 (&k
 }
 
 /* One tree level: consume buf[parity], append children to the workgroup's
- * segment of buf[parity^1].  MODE 0: level order or max-mass class rounds;
- * MODE 1: level order that also writes the tree log; MODE 2: class rounds
- * on the learned score (children scored 64 at a time on the matrix cores);
- * MODE 3: the forkBest rule (two-way branch) in either order. */
+ * segment of buf[parity^1].  MODE 0: level order; MODE 4: max-mass class
+ * rounds; MODE 1: level order that also writes the tree log; MODE 2: class
+ * rounds on the learned score (children scored 64 at a time on the matrix
+ * cores); MODE 3: the forkBest rule (two-way branch) in either order. */
 template <int W, class EX, int MODE>
 __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, int level) {
   extern __shared__ __align__(16) unsigned char smem[];
+  /* class rounds: always in MODE 4 (max-mass order of the path rule), never
+   * in MODE 0 (level order) and MODE 1, by the argument in MODE 2 and 3 */
+  const bool mass_on = MODE == 4 || ((MODE == 2 || MODE == 3) && a.mass_mode);
   u32* sp = (u32*)(smem + a.lay.off_extra);
   u32* s_cnt = sp + a.n_seg + 1;
   u32* part = s_cnt + 1;
@@ -252,7 +255,7 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
   }
   if (n_items == 0) {
     if (threadIdx.x == 0) a.seg_count[parity ^ 1][blockIdx.x] = 0;
-    if (a.mass_mode)
+    if (mass_on)
       for (int i = threadIdx.x; i < kClasses; i += blockDim.x)
         a.hist[(u64)blockIdx.x * kClasses + i] = 0;
     return;
@@ -262,7 +265,7 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
   /* per-wave feature tile [64][kFeat] bf16 (learned order), 16-byte aligned */
   u16* tile = (u16*)(((size_t)(s_hist + kClasses) + 15) & ~(size_t)15) +
               (threadIdx.x >> 6) * 64 * kFeat;
-  if (a.mass_mode)
+  if (mass_on)
     for (int i = threadIdx.x; i < kClasses; i += blockDim.x) s_hist[i] = 0;
   if (EX::kStage) stage_tables(a.g, a.lay, smem);
   __syncthreads();
@@ -313,11 +316,11 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
       slot_in = (u64)s * a.seg_cap + (i - sp[s]);
       load_item<W>(in, a.stride, slot_in, known, up, mass);
       if (MODE != 0) tag_in = a.tag[parity][slot_in];
-      const int cl = !a.mass_mode ? 0 : MODE == 2 ? (int)tag_in : mass_class(mass);
-      if (a.mass_mode && cl >= a.drop_class) { /* given up: queue over capacity */
+      const int cl = !mass_on ? 0 : MODE == 2 ? (int)tag_in : mass_class(mass);
+      if (mass_on && cl >= a.drop_class) { /* given up: queue over capacity */
         accE.add(mass);
         ++c[C_FOLD];
-      } else if (a.mass_mode && cl > a.thr_class) { /* lighter: carried */
+      } else if (mass_on && cl > a.thr_class) { /* lighter: carried */
         carry = true;
         stored = 1;
       } else {
@@ -333,7 +336,7 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
             k = 2;
             fork_e = r - 1;
             stored = 2;
-            if (a.mass_mode)
+            if (mass_on)
               stored = (mass * ex.p[fork_e] >= a.eps ? 1 : 0) + (mass * ex.q[fork_e] >= a.eps ? 1 : 0);
           } else if (r == 0) { /* a WORKING path: main.cu:550-567 */
             accR.add(mass);
@@ -349,7 +352,7 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
           if (k >= 0) {
             live = true;
             stored = k;
-            if (a.mass_mode) { /* children below eps or over capacity are not queued */
+            if (mass_on) { /* children below eps or over capacity are not queued */
               stored = 0;
               double run = mass;
               for (int j = k - 1; j >= 0; --j) {
@@ -402,7 +405,7 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
 #pragma unroll
         for (int ch = 0; ch < 2; ++ch) {
           const double cm = ch == 0 ? mass * ex.p[fork_e] : mass * ex.q[fork_e];
-          if (a.mass_mode && cm < a.eps) { /* settled at creation */
+          if (mass_on && cm < a.eps) { /* settled at creation */
             accE.add(cm);
             ++c[C_NODES];
             ++c[C_TRUNC];
@@ -420,7 +423,7 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
               ex.store_rows(out, a.stride, out_base + off);
             else
               ex.store_child(out, a.stride, out_base + off, fork_e);
-            if (a.mass_mode) atomicAdd(&s_hist[mass_class(cm)], 1u);
+            if (mass_on) atomicAdd(&s_hist[mass_class(cm)], 1u);
           } else { /* segment full: main.cu:245-252 */
             accE.add(cm);
             ++c[C_PATHS];
@@ -515,7 +518,7 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
         const u64 bit = 1ull << (e & 63);
         const double cm = run * ex.q[e];
         bool queue = true;
-        if (a.mass_mode) {
+        if (mass_on) {
           if (cm < a.eps) { /* settled at creation: counted as a truncated node */
             accE.add(cm);
             ++c[C_NODES];
@@ -533,7 +536,7 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
             }
             store_item<W>(out, a.stride, out_base + off, ck, cu, cm);
             ex.store_child(out, a.stride, out_base + off, e);
-            if (a.mass_mode) atomicAdd(&s_hist[mass_class(cm)], 1u);
+            if (mass_on) atomicAdd(&s_hist[mass_class(cm)], 1u);
             if (MODE == 1) {
               const u64 rid = rbase + (off - wbase);
               a.tag[parity ^ 1][out_base + off] = (u32)rid;
@@ -569,7 +572,7 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
 
   /* per-workgroup flush; then publish this workgroup's segment count */
   __syncthreads();
-  if (a.mass_mode)
+  if (mass_on)
     for (int i = threadIdx.x; i < kClasses; i += blockDim.x)
       a.hist[(u64)blockIdx.x * kClasses + i] = s_hist[i];
   block_flush(a, accR, accE, accD, c, (unsigned char*)part);
@@ -657,7 +660,11 @@ struct Variant {
   KernelGetter level_fork;             /* MODE 3 (forkBest rule) */
   KernelGetter seed_rows;              /* row form: rows of resumed items */
   int lean;                            /* items carry no rows (LeanExpander) */
+  KernelGetter level_mass;             /* MODE 4 (max-mass class rounds) */
 };
+
+/* MODE 4 kernel of a variant (nr_mass.hip), filled into v. */
+void add_mass_kernels(Variant& v, int W);
 
 /* Lean row form (plain path rule, two terminals) for W words, node sets of
  * NS bytes (nr_lean.hip). */
