@@ -900,16 +900,20 @@ struct Kern {
   static void* batch() { return (void*)&k_expand_batch<W, NS>; }
   static void* refill() { return (void*)&k_refill_batch<W>; }
   static void* seed_rows() { return (void*)&k_seed_rows<W, NS>; }
+  static void* small() { return (void*)&k_small_levels<W, Expander<W, NS>>; }
 };
 
 template <typename NS>
 Variant variant_for_w(int W) {
   switch (W) {
-#define NR_W(X) \
-  case X:      \
-    return Variant{&Kern<X, NS>::level, &Kern<X, NS>::items, &Kern<X, NS>::batch,               \
-                   &Kern<X, NS>::refill, (int)sizeof(NS), 0, nullptr, nullptr, nullptr,  \
-                   &Kern<X, NS>::seed_rows, 0, nullptr};
+#define NR_W(X)                                                                           \
+  case X: {                                                                               \
+    Variant v{&Kern<X, NS>::level, &Kern<X, NS>::items, &Kern<X, NS>::batch,              \
+              &Kern<X, NS>::refill, (int)sizeof(NS), 0, nullptr, nullptr, nullptr,        \
+              &Kern<X, NS>::seed_rows, 0, nullptr};                                       \
+    v.small = &Kern<X, NS>::small;                                                        \
+    return v;                                                                             \
+  }
     NR_W(1) NR_W(2) NR_W(3) NR_W(4) NR_W(5) NR_W(6) NR_W(7) NR_W(8)
 #undef NR_W
     default: return Variant{nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr, 0, nullptr};
@@ -1257,6 +1261,30 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
                            4 * (grid + 2 + block), st));
     return NR_OK;
   };
+  /* the first, small levels in one workgroup (fresh level-order solves of
+   * the path rule): a level of at most max_in items, whose children fit
+   * segment 0 (k <= n - 1 hops each), never needs a second workgroup */
+  if (var.small && !ck && mode == 0 && !mass && cfg->rule == NR_RULE_PATH) {
+    const u32 per = (u32)std::max(1, g->n - 1);
+    const u32 max_in = std::min<u32>(256u, std::min<u32>((u32)block, seg_cap / per));
+    if (max_in >= 1) {
+      const void* ks = (const void*)var.small();
+      NR_HIP(hipFuncSetAttribute(ks, hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
+      hipEvent_t kb = nullptr, ke = nullptr;
+      if ((size_t)timed < max_timed) {
+        if ((rc = ws.event(2 + 2 * timed, &kb)) || (rc = ws.event(3 + 2 * timed, &ke))) return rc;
+        NR_HIP(hipEventRecord(kb, st));
+      }
+      u32 mi = max_in;
+      int ml = 64;
+      void* sargs[] = {&a, &mi, &ml};
+      NR_HIP(hipLaunchKernel(ks, dim3(1), dim3(block), sargs, lay.total_bytes, st));
+      if (kb) {
+        NR_HIP(hipEventRecord(ke, st));
+        ++timed;
+      }
+    }
+  }
   if (!mass) {
     /* level order: chunks of back-to-back level launches; under a time
      * budget a chunk is sized from the measured time per launch so the

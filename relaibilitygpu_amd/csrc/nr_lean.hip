@@ -10,6 +10,7 @@ namespace nr {
 template <int W, typename NS>
 struct KernLean {
   static void* level() { return (void*)&k_expand_level<W, LeanExpander<W, NS>, 0>; }
+  static void* small() { return (void*)&k_small_levels<W, LeanExpander<W, NS>>; }
 };
 
 template <typename NS>
@@ -19,6 +20,7 @@ static Variant lean_for_w(int W) {
 #define NR_W(X)                            \
   case X:                                  \
     v.level = &KernLean<X, NS>::level;     \
+    v.small = &KernLean<X, NS>::small;     \
     break;
     NR_W(1) NR_W(2) NR_W(3) NR_W(4) NR_W(5) NR_W(6) NR_W(7) NR_W(8)
 #undef NR_W

@@ -582,6 +582,132 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
   }
 }
 
+/* The first levels of a level-order solve (path rule) in one workgroup:
+ * while a level holds at most max_in items (one lane each), the workgroup
+ * expands it from segment 0 of one buffer into segment 0 of the other and
+ * goes straight on to the next, with no launch, prefix scan or flush in
+ * between.  It stops before the first larger level (or after max_levels),
+ * leaves that level in buffer 0, segment 0, and flushes its sums to slot 0;
+ * the level launches take over from there.  Same counts, levels and widest
+ * level as one launch per level. */
+template <int W, class EX>
+__global__ void __launch_bounds__(256) k_small_levels(LevelArgs a, u32 max_in, int max_levels) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  u32* s_cnt = (u32*)(smem + a.lay.off_extra);
+  u32* part = s_cnt + 4;
+  if (EX::kStage) stage_tables(a.g, a.lay, smem);
+  EX ex;
+  ex.bind_graph(smem, a.lay, a.g, threadIdx.x);
+  Acc2 accR, accE, accD;
+  accR.init();
+  accE.init();
+  accD.init();
+  u32 c[C_N] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const int lane = threadIdx.x & 63;
+  int p = 0;
+  u32 n = a.seg_count[0][0];
+  for (int lv = 0; n > 0 && n <= max_in && lv < max_levels; ++lv) {
+    if (threadIdx.x == 0) {
+      *s_cnt = 0;
+      a.ctl->iterations += 1; /* main.cu:970-973 */
+      if (n > a.ctl->max_width) a.ctl->max_width = n;
+    }
+    __syncthreads();
+    const QBuf in = a.buf[p];
+    const QBuf out = a.buf[p ^ 1];
+    for (u32 base = 0; base < n; base += blockDim.x) {
+      const u32 i = base + threadIdx.x;
+      u64 known[W], up[W];
+      double mass = 0.0;
+      int k = 0;
+      bool live = false;
+      if (i < n) {
+        load_item<W>(in, a.stride, i, known, up, mass);
+        ++c[C_NODES];
+        if (mass < a.eps) { /* main.cu:537-543 */
+          accE.add(mass);
+          ++c[C_TRUNC];
+        } else {
+          ex.load(in, a.stride, i);
+          k = ex.find_item(known, up);
+          if (k >= 0) {
+            live = true;
+          } else {
+            if (k == -2) ++c[C_ERR];
+            accD.add(mass); /* main.cu:579-582 */
+            ++c[C_DEAD];
+            k = 0;
+          }
+        }
+      }
+      const int incl = wave_incl_scan(k);
+      const int total = __shfl(incl, 63, 64);
+      u32 wbase = 0;
+      if (total) {
+        if (lane == 63) wbase = atomicAdd(s_cnt, (u32)total);
+        wbase = __shfl(wbase, 63, 64);
+      }
+      if (live) { /* main.cu:214-266 */
+        u32 off = wbase + (u32)(incl - k);
+        double run = mass;
+        u64 pre[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) pre[w] = 0;
+        for (int j = k - 1; j >= 0; --j, ++off) {
+          const int e = ex.uh[j * ex.stride];
+          const int wi = e >> 6;
+          const u64 bit = 1ull << (e & 63);
+          const double cm = run * ex.q[e];
+          if (off < a.seg_cap) {
+            u64 ck[W], cu[W];
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+              ck[w] = known[w] | pre[w] | (w == wi ? bit : 0ull);
+              cu[w] = up[w] | pre[w];
+            }
+            store_item<W>(out, a.stride, off, ck, cu, cm);
+            ex.store_child(out, a.stride, off, e);
+          } else { /* segment full: main.cu:245-252 */
+            accE.add(cm);
+            ++c[C_PATHS];
+            ++c[C_OVER];
+          }
+          run = run * ex.p[e];
+#pragma unroll
+          for (int w = 0; w < W; ++w)
+            if (w == wi) pre[w] |= bit;
+        }
+        accR.add(run); /* success term, main.cu:254-262 */
+        ++c[C_PATHS];
+      }
+    }
+    __syncthreads();
+    const u32 made = *s_cnt;
+    n = made < a.seg_cap ? made : a.seg_cap;
+    p ^= 1;
+    __syncthreads();
+  }
+  if (p == 1) { /* hand the level over in buffer 0 */
+    const QBuf from = a.buf[1];
+    const QBuf to = a.buf[0];
+    for (u32 i = threadIdx.x; i < n; i += blockDim.x) {
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        to.known[(u64)w * a.stride + i] = from.known[(u64)w * a.stride + i];
+        to.up[(u64)w * a.stride + i] = from.up[(u64)w * a.stride + i];
+      }
+      to.mass[i] = from.mass[i];
+      ex.carry(from, i, to, i, a.stride);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    a.seg_count[0][0] = n;
+    a.ctl->last_items = n;
+  }
+  block_flush(a, accR, accE, accD, c, (unsigned char*)part);
+}
+
 /* One expansion of each caller-supplied item into fixed output slots (the
  * parity entry point nr_expand_items). */
 struct ItemsArgs {
@@ -661,6 +787,7 @@ struct Variant {
   KernelGetter seed_rows;              /* row form: rows of resumed items */
   int lean;                            /* items carry no rows (LeanExpander) */
   KernelGetter level_mass;             /* MODE 4 (max-mass class rounds) */
+  KernelGetter small;                  /* k_small_levels of this form */
 };
 
 /* MODE 4 kernel of a variant (nr_mass.hip), filled into v. */
