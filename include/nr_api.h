@@ -86,7 +86,7 @@ typedef enum nr_rule {
                                    few microseconds between launches) */
 #define NR_FLAG_ROWS 4          /* undirected graphs of <= 64 nodes: items always
                                    carry their adjacency rows (by default they
-                                   do only while the rows take <= 48 bytes, and
+                                   do only while the rows take <= 64 bytes, and
                                    are derived from the planes otherwise; same
                                    results) */
 
@@ -250,10 +250,13 @@ typedef struct nr_model nr_model;
 
 /* Solve in level order while logging every child (features at creation,
  * parent, depth), then derive each child's subtree size bottom-up on the
- * device and return up to max_records of them, sampled uniformly without
- * replacement (seeded): features[max_records * 16], labels = log2(subtree
- * nodes).  The solve's own result goes to *result.  Fails with
- * NR_ERR_ARG if the queue overflowed (the tree would be incomplete). */
+ * device and return up to max_records of them: the records with the
+ * smallest seeded hashes of their content (features and size), in hash
+ * order — a uniform sample without replacement that is the same for the
+ * same tree and seed whatever order the device logged it in.
+ * features[max_records * 16], labels = log2(subtree nodes).  The solve's
+ * own result goes to *result.  Fails with NR_ERR_ARG if the queue
+ * overflowed (the tree would be incomplete). */
 int nr_log_subproblems(const nr_graph* g, const nr_config* cfg, long long max_records,
                        unsigned long long seed, float* features, float* labels,
                        long long* n_out, nr_result* result);

@@ -325,6 +325,21 @@ struct Expander {
     }
     return 1 + best;
   }
+  /* store_child() for the item of another lane of the same workgroup
+   * (slot offset `delta`): reads that lane's rows slice */
+  __device__ void store_child_of(const QBuf& out, u64 qstride, u64 slot, int e, int delta) const {
+    const u32 x = ep[e];
+    const int a0 = (int)(x & 0xffffu), b0 = (int)(x >> 16);
+    const NS* r = rows + delta;
+    NS* rout = (NS*)out.rows + slot;
+    for (int u = 0; u < n; ++u) {
+      NS v = r[u * stride];
+      if (u == a0) v &= ~((NS)1 << b0);
+      if (u == b0) v &= ~((NS)1 << a0);
+      rout[(u64)u * qstride] = v;
+    }
+  }
+
   /* rows of a child that keeps the parent's allowed edges */
   __device__ void store_rows(const QBuf& out, u64 qstride, u64 slot) const {
     NS* rout = (NS*)out.rows + slot;
@@ -659,6 +674,7 @@ struct CsrExpander {
   __device__ void load(const QBuf&, u64, u64) const {}
   __device__ void derive(const u64 (&)[W], const u64 (&)[W]) const {}
   __device__ void store_child(const QBuf&, u64, u64, int) const {}
+  __device__ void store_child_of(const QBuf&, u64, u64, int, int) const {}
   __device__ void carry(const QBuf&, u64, const QBuf&, u64, u64) const {}
 
   __device__ void store_rows(const QBuf&, u64, u64) const {}
@@ -939,6 +955,7 @@ struct LeanExpander : Expander<W, NS> {
   __device__ void load(const QBuf&, u64, u64) const {}
   __device__ void derive(const u64 (&)[W], const u64 (&)[W]) const {}
   __device__ void store_child(const QBuf&, u64, u64, int) const {}
+  __device__ void store_child_of(const QBuf&, u64, u64, int, int) const {}
   __device__ void store_rows(const QBuf&, u64, u64) const {}
   __device__ void carry(const QBuf&, u64, const QBuf&, u64, u64) const {}
   __device__ int find_item(const u64 (&known)[W], const u64 (&up)[W]) const {

@@ -1050,12 +1050,13 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   int rc = pick_variant(g, var);
   if (rc) return rc;
   const int mode = x ? x->mode : 0;
-  /* the plain path rule on a row-form graph whose rows would exceed 48 bytes
-   * per item runs with items that carry none (measured on MI355X: K10, 40 B
-   * of rows, 1.42 ms with rows vs 1.94 ms without; K15 23.1 vs 21.5 ms; K25
-   * 64.0 vs 31.4 ms), unless the caller asks for the row-carrying kernel */
+  /* the plain path rule on a row-form graph whose rows would exceed 64 bytes
+   * per item runs with items that carry none (measured on MI355X, level
+   * order: K10, 40 B of rows, 1.33 ms with rows vs 1.34 ms without; K15,
+   * 60 B, 19.8 vs 20.4 ms; K25, 100 B, 54.9 vs 30.6 ms), unless the caller
+   * asks for the row-carrying kernel */
   if (mode == 0 && cfg->rule == NR_RULE_PATH && g->terms.empty() && !var.nw &&
-      g->n * var.ns_bytes > 48 && !(cfg->flags & NR_FLAG_ROWS)) {
+      g->n * var.ns_bytes > 64 && !(cfg->flags & NR_FLAG_ROWS)) {
     var = variant_lean(mask_words(g), var.ns_bytes);
     add_mass_kernels(var, mask_words(g));
   }
@@ -1873,6 +1874,33 @@ __global__ void __launch_bounds__(256) k_log_gather(const float* feat, const u64
   }
 }
 
+/* Seeded hash of a record's content (features and subtree size): the
+ * sample of a log depends only on the tree, not on the order in which the
+ * device appended its records (splitmix64 finalizer over the words). */
+__device__ inline u64 mix64(u64 z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__global__ void __launch_bounds__(256) k_log_select(const float* feat, const u64* size, u64 n,
+                                                    u64 seed, u64 thr, u64* keys, u64* ids,
+                                                    u64* found, u64 cap) {
+  for (u64 r = (u64)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (u64)gridDim.x * blockDim.x) {
+    u64 h = mix64(seed ^ mix64(size[r]));
+    const u32* f = (const u32*)(feat + r * kFeat);
+    for (int k = 0; k < kFeat; ++k) h = mix64(h ^ (u64)f[k] ^ ((u64)k << 32));
+    if (h < thr) {
+      const u64 at = atomicAdd((unsigned long long*)found, 1ull);
+      if (at < cap) {
+        keys[at] = h;
+        ids[at] = r;
+      }
+    }
+  }
+}
+
 int log_subproblems(Workspace& ws, const nr_graph* g, const nr_config* cfg, long long max_out,
                     unsigned long long seed, float* X, float* Y, long long* n_out,
                     nr_result* res) {
@@ -1922,25 +1950,47 @@ int log_subproblems(Workspace& ws, const nr_graph* g, const nr_config* cfg, long
                          (const u32*)x.log_parent, dsize, r0, r1);
     }
   }
-  const u64 m = std::min<u64>((u64)std::max<long long>(max_out, 0), count);
-  std::vector<u64> idx((size_t)m);
-  if (m == count) {
-    for (u64 i = 0; i < m; ++i) idx[i] = i;
-  } else { /* m distinct records, uniformly (Floyd's algorithm), in record order */
-    std::mt19937_64 rng(seed);
-    std::vector<u64> pick;
-    pick.reserve((size_t)m);
-    std::vector<char> taken;
-    taken.assign((size_t)count, 0);
-    for (u64 j = count - m; j < count; ++j) {
-      const u64 t = std::uniform_int_distribution<u64>(0, j)(rng);
-      const u64 v = taken[t] ? j : t;
-      taken[v] = 1;
+  /* the sample: records whose seeded content hash falls below a threshold
+   * set for ~m hits, the m smallest hashes kept, in hash order; identical
+   * for identical trees whatever the order the device wrote them in */
+  const u64 want = std::min<u64>((u64)std::max<long long>(max_out, 0), count);
+  std::vector<u64> idx;
+  if (want > 0) {
+    /* identical records share a hash, so a threshold aimed at ~want hits
+     * can fall short: widen it until it catches want records (records of
+     * equal content are interchangeable, so which copies come is moot) */
+    double frac = std::min(1.0, ((double)want + 6.0 * std::sqrt((double)want) + 16.0) /
+                                    (double)count);
+    u64 *dkeys = nullptr, *dids = nullptr, *dfound;
+    if ((rc = mem.alloc(&dkeys, count)) || (rc = mem.alloc(&dids, count)) ||
+        (rc = mem.alloc(&dfound, 1)))
+      return rc;
+    u64 found = 0;
+    for (;;) {
+      const u64 thr = frac >= 1.0 ? ~0ull : (u64)(frac * 18446744073709551615.0);
+      NR_HIP(hipMemsetAsync(dfound, 0, sizeof(u64), st));
+      hipLaunchKernelGGL(k_log_select, dim3(1024), dim3(256), 0, st, (const float*)x.log_feat,
+                         (const u64*)dsize, count, (u64)seed, thr, dkeys, dids, dfound, count);
+      NR_HIP(hipMemcpyAsync(&found, dfound, sizeof(u64), hipMemcpyDeviceToHost, st));
+      NR_HIP(hipStreamSynchronize(st));
+      if (found >= want || frac >= 1.0) break;
+      frac = std::min(1.0, frac * 2.0);
     }
-    u64 w = 0;
-    for (u64 i = 0; i < count; ++i)
-      if (taken[i]) idx[w++] = i;
+    found = std::min(found, count);
+    std::vector<u64> keys(found), ids(found);
+    NR_HIP(hipMemcpyAsync(keys.data(), dkeys, 8 * found, hipMemcpyDeviceToHost, st));
+    NR_HIP(hipMemcpyAsync(ids.data(), dids, 8 * found, hipMemcpyDeviceToHost, st));
+    NR_HIP(hipStreamSynchronize(st));
+    std::vector<u64> order(found);
+    for (u64 i = 0; i < found; ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](u64 x1, u64 x2) {
+      return keys[x1] != keys[x2] ? keys[x1] < keys[x2] : ids[x1] < ids[x2];
+    });
+    const u64 m0 = std::min<u64>(want, found);
+    idx.resize((size_t)m0);
+    for (u64 i = 0; i < m0; ++i) idx[i] = ids[order[i]];
   }
+  const u64 m = (u64)idx.size();
   if (m) {
     u64* didx;
     float *dX, *dY;

@@ -201,6 +201,93 @@ __device__ inline void block_flush(const LevelArgs& a, Acc2& accR, Acc2& accE, A
   }
 }
 
+/* A wave's children written child-major (path rule): child c of the wave,
+ * counted in lane-prefix order, goes to slot wbase + c of the workgroup's
+ * segment and is written by lane c % 64, which finds its owner lane by a
+ * binary search over the wave's inclusive prefix, takes the owner's planes
+ * and mass by shuffles and its hops and rows from the owner's LDS slice, and
+ * forms the child mass with the same left-to-right products as the owner
+ * would (main.cu:214-266): the queue receives the same items in the same
+ * slots, every store instruction covers consecutive slots, and no lane idles
+ * while another works through a long path.  Every lane of the wave calls
+ * it. */
+template <int W, class EX>
+__device__ inline void emit_children_wave(const EX& ex, const QBuf& out, u64 qstride,
+                                          u64 out_base, u32 seg_cap, u32 wbase, int incl, int k,
+                                          const u64 (&known)[W], const u64 (&up)[W], double mass,
+                                          Acc2& accE, u32 (&c)[C_N]) {
+  const int lane = threadIdx.x & 63;
+  const int total = __shfl(incl, 63, 64);
+  for (int c0 = 0; c0 < total; c0 += 64) {
+    const int cc = c0 + lane;
+    const int target = cc < total ? cc : total - 1;
+    int lo = 0, hi = 63; /* first lane whose inclusive prefix exceeds target */
+#pragma unroll
+    for (int step = 0; step < 6; ++step) {
+      const int mid = (lo + hi) >> 1;
+      const int im = __shfl(incl, mid, 64);
+      if (im > target)
+        hi = mid;
+      else
+        lo = mid + 1;
+    }
+    const int o = lo;
+    const int ko = __shfl(k, o, 64);
+    const int io = __shfl(incl, o, 64);
+    u64 ok[W], ou[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      ok[w] = __shfl(known[w], o, 64);
+      ou[w] = __shfl(up[w], o, 64);
+    }
+    const double om = __shfl(mass, o, 64);
+    if (cc < total) {
+      const int j = cc - (io - ko); /* child index from the src side */
+      const int delta = o - lane;
+      const u16* uho = ex.uh + delta;
+      double run = om;
+      u64 pre[W];
+#pragma unroll
+      for (int w = 0; w < W; ++w) pre[w] = 0;
+      for (int i = 0; i < j; ++i) {
+        const int e = uho[(ko - 1 - i) * ex.stride];
+        run = run * ex.p[e];
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+          if (w == (e >> 6)) pre[w] |= 1ull << (e & 63);
+      }
+      const int e = uho[(ko - 1 - j) * ex.stride];
+      const int wi = e >> 6;
+      const u64 bit = 1ull << (e & 63);
+      const double cm = run * ex.q[e];
+      const u32 off = wbase + (u32)cc;
+      if (off < seg_cap) {
+        u64 ck[W], cu[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          ck[w] = ok[w] | pre[w] | (w == wi ? bit : 0ull);
+          cu[w] = ou[w] | pre[w];
+        }
+        store_item<W>(out, qstride, out_base + off, ck, cu, cm);
+        ex.store_child_of(out, qstride, out_base + off, e, delta);
+      } else { /* segment full: main.cu:245-252 */
+        accE.add(cm);
+        ++c[C_PATHS];
+        ++c[C_OVER];
+      }
+    }
+  }
+}
+
+/* Success term of a live item: mass times the p of all its UNKNOWN hops,
+ * src side first (main.cu:254-262). */
+template <class EX>
+__device__ inline double success_mass(const EX& ex, double mass, int k) {
+  double run = mass;
+  for (int j = k - 1; j >= 0; --j) run = run * ex.p[ex.uh[j * ex.stride]];
+  return run;
+}
+
 /* Parent context of an item's children for the feature function. */
 template <int W, class EX>
 __device__ inline void child_ctx(const LevelArgs& a, const EX& ex, const u64 (&known)[W],
@@ -504,6 +591,13 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
         accR.add(run); /* success term, main.cu:254-262 */
         ++c[C_PATHS];
       }
+    } else if (MODE == 0 && EX::kRowBytes > 0) { /* children with rows: child-major */
+      emit_children_wave<W, EX>(ex, out, a.stride, out_base, a.seg_cap, wbase, incl, stored, known,
+                                up, mass, accE, c);
+      if (live) {
+        accR.add(success_mass(ex, mass, k));
+        ++c[C_PATHS];
+      }
     } else if (live) {
       /* main.cu:214-266: child j has the earlier UNKNOWN hops WORKING and
        * hop j FAILED; masses multiply left to right along the path */
@@ -647,7 +741,14 @@ __global__ void __launch_bounds__(256) k_small_levels(LevelArgs a, u32 max_in, i
         if (lane == 63) wbase = atomicAdd(s_cnt, (u32)total);
         wbase = __shfl(wbase, 63, 64);
       }
-      if (live) { /* main.cu:214-266 */
+      if (EX::kRowBytes > 0) {
+        emit_children_wave<W, EX>(ex, out, a.stride, 0, a.seg_cap, wbase, incl, k, known, up,
+                                  mass, accE, c);
+        if (live) {
+          accR.add(success_mass(ex, mass, k));
+          ++c[C_PATHS];
+        }
+      } else if (live) { /* main.cu:214-266 */
         u32 off = wbase + (u32)(incl - k);
         double run = mass;
         u64 pre[W];
@@ -666,7 +767,6 @@ __global__ void __launch_bounds__(256) k_small_levels(LevelArgs a, u32 max_in, i
               cu[w] = up[w] | pre[w];
             }
             store_item<W>(out, a.stride, off, ck, cu, cm);
-            ex.store_child(out, a.stride, off, e);
           } else { /* segment full: main.cu:245-252 */
             accE.add(cm);
             ++c[C_PATHS];
