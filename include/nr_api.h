@@ -68,16 +68,29 @@ typedef struct nr_config {
                                0 = auto */
   int flags;                /* NR_FLAG_* */
   int rule;                 /* nr_rule: which decomposition each item gets */
-  int reserved[5];
+  int sweep_period;         /* NR_RULE_ALTERNATE: the sweep direction flips
+                               every this many FAILED edges (tree levels);
+                               0 = 2 */
+  int reserved[4];
 } nr_config;
 
 /* Decomposition rule (nr_config.rule).  Both are exact; they build
  * different trees. */
 typedef enum nr_rule {
   NR_RULE_PATH = 0,    /* k children per path (main.cu:114-283), default */
-  NR_RULE_FORKBEST = 1 /* success on a confirmed WORKING path, else a two-way
+  NR_RULE_FORKBEST = 1, /* success on a confirmed WORKING path, else a two-way
                           branch on the path's most reliable UNKNOWN edge,
                           nearest dst on ties (main.cu:304-430, 550-567) */
+  NR_RULE_ALTERNATE = 2 /* no counterpart in the reference: k children per
+                          path as NR_RULE_PATH, on a path with the fewest
+                          UNKNOWN edges (WORKING edges contracted; walk back
+                          to the lowest-index node of the previous level,
+                          then the lowest-index node of the component it
+                          joins), children swept from the src end when
+                          (FAILED edges / sweep_period) is even and from the
+                          dst end otherwise.  Undirected graphs of up to 64
+                          nodes, two terminals; NR_ERR_UNSUPPORTED
+                          otherwise */
 } nr_rule;
 
 /* nr_config.flags */
@@ -187,6 +200,16 @@ int nr_expand_items(const nr_graph* g, int device, int count,
                     int max_children, int* nchild, double* succ,
                     unsigned long long* child_known,
                     unsigned long long* child_up, double* child_mass);
+
+/* nr_expand_items() under cfg->rule (NR_RULE_PATH or NR_RULE_ALTERNATE with
+ * cfg->sweep_period) on device cfg->device; children in sweep order.  No
+ * counterpart in the reference beyond nr_expand_items'. */
+int nr_expand_items_rule(const nr_graph* g, const nr_config* cfg, int count,
+                         const unsigned long long* known,
+                         const unsigned long long* up, const double* mass,
+                         int max_children, int* nchild, double* succ,
+                         unsigned long long* child_known,
+                         unsigned long long* child_up, double* child_mass);
 
 const char* nr_status_str(int status);
 

@@ -216,6 +216,7 @@ struct Engine {
   double eps;
   const int* terms = nullptr; /* K-terminal mode when nterms >= 2 */
   int nterms = 0;
+  int period = 2; /* ORC_MODE_ALTERNATE: sweep flips every `period` FAILED edges */
 
   bool allowed(const u64* known, const u64* up, int e) const {
     u64 bit = 1ull << (e & 63);
@@ -591,6 +592,129 @@ int expand_minunk_impl(const Engine& en, const u64* known, const u64* up, double
   return k;
 }
 
+/* Alternating-sweep rule (ORC_MODE_ALTERNATE; no counterpart in the
+ * reference).  Any src->dst path, and any order of its UNKNOWN hops, splits
+ * an item into disjoint children plus the all-WORKING success term, so this
+ * rule keeps the decomposition exact; it only changes the tree's shape.
+ *
+ * Path: fewest UNKNOWN hops, WORKING edges contracted.  Level 0 is the set
+ * of nodes joined to src by WORKING edges; level l+1 is every node without a
+ * level that has an allowed edge from level l, together with the nodes
+ * joined to those by WORKING edges.  Every level is closed under WORKING
+ * edges, so an allowed edge between consecutive levels is UNKNOWN.  The
+ * walk back starts from K = dst's WORKING component; at level l the parent
+ * is the lowest-index node u of level l-1 with an allowed edge to K, the hop
+ * is u's edge to the lowest-index node of K it reaches, and the walk goes on
+ * from u's WORKING component.
+ *
+ * Sweep: with f the item's FAILED edge count, the children are formed from
+ * the src end (child i: the first i-1 hops WORKING, hop i FAILED) when
+ * (f / period) is even and from the dst end otherwise; the masses multiply
+ * along the sweep.  Each child has one more FAILED edge than its parent, so
+ * the direction flips every `period` tree levels. */
+int expand_alt_impl(const Engine& en, const u64* known, const u64* up, double mass,
+                    u64* ck, u64* cu, double* cm, double* succ, int* hops) {
+  const orc_graph* g = en.g;
+  const int n = g->n, W = en.W;
+  if (en.src == en.dst) {
+    *succ = mass;
+    return 0;
+  }
+  std::vector<int> lev(n, -1);
+  /* extend `set` (nodes just given level l) over WORKING edges */
+  auto close = [&](std::vector<int>& set, int l) {
+    for (size_t i = 0; i < set.size(); ++i) {
+      const int u = set[i];
+      for (int j = g->row_ptr[u]; j < g->row_ptr[u + 1]; ++j) {
+        const int v = g->col_idx[j];
+        if (lev[v] < 0 && en.working(known, up, g->edge_id[j])) {
+          lev[v] = l;
+          set.push_back(v);
+        }
+      }
+    }
+  };
+  std::vector<int> cur{en.src};
+  lev[en.src] = 0;
+  close(cur, 0);
+  int L = 0;
+  while (lev[en.dst] < 0) {
+    std::vector<int> nxt;
+    for (int u : cur)
+      for (int j = g->row_ptr[u]; j < g->row_ptr[u + 1]; ++j) {
+        const int v = g->col_idx[j];
+        if (lev[v] < 0 && en.allowed(known, up, g->edge_id[j])) {
+          lev[v] = L + 1;
+          nxt.push_back(v);
+        }
+      }
+    if (nxt.empty()) return -1;
+    ++L;
+    close(nxt, L);
+    cur.swap(nxt);
+  }
+  /* WORKING component of x as flags */
+  auto comp = [&](int x) {
+    std::vector<char> in(n, 0);
+    std::vector<int> st{x};
+    in[x] = 1;
+    while (!st.empty()) {
+      const int u = st.back();
+      st.pop_back();
+      for (int j = g->row_ptr[u]; j < g->row_ptr[u + 1]; ++j) {
+        const int v = g->col_idx[j];
+        if (!in[v] && en.working(known, up, g->edge_id[j])) {
+          in[v] = 1;
+          st.push_back(v);
+        }
+      }
+    }
+    return in;
+  };
+  std::vector<char> K = comp(en.dst);
+  for (int l = L; l >= 1; --l) {
+    int bu = -1, be = -1;
+    for (int u = 0; u < n && bu < 0; ++u) {
+      if (lev[u] != l - 1) continue;
+      int bv = n;
+      for (int j = g->row_ptr[u]; j < g->row_ptr[u + 1]; ++j) {
+        const int v = g->col_idx[j], e = g->edge_id[j];
+        if (K[v] && en.allowed(known, up, e) && (v < bv || (v == bv && e < be))) {
+          bv = v;
+          be = e;
+        }
+      }
+      if (bv < n) bu = u;
+    }
+    if (bu < 0) return -1; /* not reached: every level joins the previous one */
+    hops[l - 1] = be;
+    K = comp(bu);
+  }
+  int f = 0;
+  for (int i = 0; i < W; ++i) f += __builtin_popcountll(known[i] & ~up[i]);
+  const int per = en.period > 0 ? en.period : 2;
+  if ((f / per) & 1) std::reverse(hops, hops + L);
+  u64 pre[kMaxW] = {0};
+  double run = mass;
+  for (int h = 0; h < L; ++h) {
+    const int e = hops[h];
+    const u64 bit = 1ull << (e & 63);
+    const int wi = e >> 6;
+    u64* kk = ck + (size_t)h * W;
+    u64* uu = cu + (size_t)h * W;
+    for (int i = 0; i < W; ++i) {
+      kk[i] = known[i] | pre[i];
+      uu[i] = up[i] | pre[i];
+    }
+    kk[wi] |= bit;
+    cm[h] = run * g->q[e];
+    run = run * g->p[e];
+    pre[wi] |= bit;
+  }
+  *succ = run;
+  return L;
+}
+
 /* Flat item store: planes (known, up) of W words + FP64 mass. */
 struct Items {
   int W = 1;
@@ -667,6 +791,9 @@ struct Worker {
     else if (en.mode == ORC_MODE_MINUNK)
       k = expand_minunk_impl(en, known, up, mass, ck.data(), cu.data(), cm.data(), &succ,
                              hops.data());
+    else if (en.mode == ORC_MODE_ALTERNATE)
+      k = expand_alt_impl(en, known, up, mass, ck.data(), cu.data(), cm.data(), &succ,
+                          hops.data());
     else
       k = en.expand_path(known, up, mass, ck.data(), cu.data(), cm.data(), &succ,
                          hops.data(), nullptr, levels);
@@ -675,7 +802,8 @@ struct Worker {
       acc.dead++;
       return 0;
     }
-    if (en.mode == ORC_MODE_PATH || en.mode == ORC_MODE_MINUNK || k == 0) {
+    if (en.mode == ORC_MODE_PATH || en.mode == ORC_MODE_MINUNK ||
+        en.mode == ORC_MODE_ALTERNATE || k == 0) {
       acc.R.add(succ);
       acc.paths++;
     }
@@ -998,6 +1126,7 @@ int orc_solve(const orc_graph* g, int src, int dst, const orc_config* cfg,
   if (!cfg || !out) return ORC_ERR_ARG;
   std::memset(out, 0, sizeof(*out));
   Engine en{g, src, dst, g->w, g->nw, cfg->mode, cfg->trunc_eps};
+  en.period = cfg->sweep_period;
   Acc acc;
   auto t0 = Clock::now();
   switch (cfg->order) {
@@ -1030,6 +1159,7 @@ int orc_solve_terms(const orc_graph* g, const int* terms, int k, const orc_confi
   int rc = check_common(g, terms[0], terms[1]);
   if (rc) return rc;
   std::memset(out, 0, sizeof(*out));
+  if (cfg->mode == ORC_MODE_ALTERNATE) return ORC_ERR_ARG; /* two terminals only */
   Engine en{g, terms[0], terms[1], g->w, g->nw, cfg->mode, cfg->trunc_eps, terms, k};
   Acc acc;
   auto t0 = Clock::now();
@@ -1075,6 +1205,35 @@ int orc_expand(const orc_graph* g, int src, int dst, const uint64_t* known,
   if (path_eids)
     for (int i = 0; i < nh; ++i) path_eids[i] = hops[i];
   int m = std::min(k, max_children);
+  if (child_known) std::copy(ck.begin(), ck.begin() + (size_t)m * g->w, child_known);
+  if (child_up) std::copy(cu.begin(), cu.begin() + (size_t)m * g->w, child_up);
+  if (child_mass) std::copy(cm.begin(), cm.begin() + m, child_mass);
+  return k;
+}
+
+int orc_expand_rule(const orc_graph* g, int src, int dst, int mode, int period,
+                    const uint64_t* known, const uint64_t* up, double mass, int max_children,
+                    uint64_t* child_known, uint64_t* child_up, double* child_mass, double* succ,
+                    int* path_eids) {
+  if (check_common(g, src, dst)) return -2;
+  if (mode != ORC_MODE_PATH && mode != ORC_MODE_ALTERNATE) return -2;
+  Engine en{g, src, dst, g->w, g->nw, mode, 0.0};
+  en.period = period;
+  std::vector<NodeSet> levels(g->n + 1);
+  std::vector<int> hops(g->n + 1);
+  std::vector<u64> ck((size_t)(g->n + 1) * g->w), cu((size_t)(g->n + 1) * g->w);
+  std::vector<double> cm(g->n + 1);
+  double s = 0.0;
+  int k = mode == ORC_MODE_ALTERNATE
+              ? expand_alt_impl(en, known, up, mass, ck.data(), cu.data(), cm.data(), &s,
+                                hops.data())
+              : en.expand_path(known, up, mass, ck.data(), cu.data(), cm.data(), &s,
+                               hops.data(), nullptr, levels);
+  if (succ) *succ = (k < 0) ? 0.0 : s;
+  if (k < 0) return -1;
+  if (path_eids && mode == ORC_MODE_ALTERNATE)
+    for (int i = 0; i < k; ++i) path_eids[i] = hops[i];
+  const int m = std::min(k, max_children);
   if (child_known) std::copy(ck.begin(), ck.begin() + (size_t)m * g->w, child_known);
   if (child_up) std::copy(cu.begin(), cu.begin() + (size_t)m * g->w, child_up);
   if (child_mass) std::copy(cm.begin(), cm.begin() + m, child_mass);

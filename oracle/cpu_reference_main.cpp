@@ -2,9 +2,9 @@
  * cpu_reference_main.cpp - command line for the CPU oracle (test
  * infrastructure; see oracle.h).
  *
- *   cpu_reference <graph_file> <src> <dst> [--mode path|pivot|forkbest]
+ *   cpu_reference <graph_file> <src> <dst> [--mode path|pivot|forkbest|minunk|alternate]
  *       [--order level|heap|dfs] [--threads T] [--truncation_eps f]
- *       [--queue_capacity n] [--time_budget s] [--json]
+ *       [--queue_capacity n] [--time_budget s] [--sweep_period n] [--json]
  *
  * The text report follows the layout of workingPad/main.cu:1037-1056 so the
  * oracle and the engine can be diffed line by line (minus the timing line).
@@ -18,9 +18,9 @@
 int main(int argc, char** argv) {
   if (argc < 4) {
     std::fprintf(stderr,
-                 "Usage: %s <graph_file> <src> <dst> [--mode path|pivot|forkbest] "
+                 "Usage: %s <graph_file> <src> <dst> [--mode path|pivot|forkbest|minunk|alternate] "
                  "[--order level|heap|dfs] [--threads T] [--truncation_eps f] "
-                 "[--queue_capacity n] [--time_budget s] [--json]\n",
+                 "[--queue_capacity n] [--time_budget s] [--sweep_period n] [--json]\n",
                  argv[0]);
     return 1;
   }
@@ -38,6 +38,7 @@ int main(int argc, char** argv) {
       cfg.mode = !std::strcmp(argv[i], "pivot")      ? ORC_MODE_PIVOT
                  : !std::strcmp(argv[i], "forkbest") ? ORC_MODE_FORKBEST
                  : !std::strcmp(argv[i], "minunk")   ? ORC_MODE_MINUNK
+                 : !std::strcmp(argv[i], "alternate") ? ORC_MODE_ALTERNATE
                                                      : ORC_MODE_PATH;
     } else if (!std::strcmp(a, "--order") && has) {
       ++i;
@@ -52,6 +53,8 @@ int main(int argc, char** argv) {
       cfg.capacity = std::atoll(argv[++i]);
     } else if (!std::strcmp(a, "--time_budget") && has) {
       cfg.time_budget_s = std::atof(argv[++i]);
+    } else if (!std::strcmp(a, "--sweep_period") && has) {
+      cfg.sweep_period = std::atoi(argv[++i]);
     } else if (!std::strcmp(a, "--json")) {
       json = true;
     }

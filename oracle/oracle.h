@@ -44,6 +44,9 @@ extern "C" {
                                (workingPad/main.cu:304-430, 550-567) */
 #define ORC_MODE_MINUNK 3   /* path decomposition on a path with the fewest
                                UNKNOWN edges (BFS over WORKING components) */
+#define ORC_MODE_ALTERNATE 4 /* fewest-UNKNOWN path (lowest-index walk back over
+                                WORKING components), children swept from the src
+                                end or the dst end by the FAILED edge count */
 
 /* status codes */
 #define ORC_OK 0
@@ -60,6 +63,8 @@ typedef struct orc_config {
   double trunc_eps;      /* items with mass < trunc_eps go to ESP; 0 = off */
   long long capacity;    /* queue capacity in items; 0 = unbounded */
   double time_budget_s;  /* 0 = none; on expiry the queue folds into ESP */
+  int sweep_period;      /* ORC_MODE_ALTERNATE: flip every this many FAILED
+                            edges (<= 0: 2) */
 } orc_config;
 
 typedef struct orc_result {
@@ -105,6 +110,15 @@ int orc_expand(const orc_graph* g, int src, int dst, const uint64_t* known,
                const uint64_t* up, double mass, int max_children,
                uint64_t* child_known, uint64_t* child_up, double* child_mass,
                double* succ, int* nhops, int* path_eids);
+
+/* Expand one item under ORC_MODE_PATH or ORC_MODE_ALTERNATE (with its
+ * sweep period): children in sweep order, *succ the success mass, path_eids
+ * (alternate rule only) the hops in sweep order.  Returns the number of
+ * children, -1 if dead, -2 on bad arguments. */
+int orc_expand_rule(const orc_graph* g, int src, int dst, int mode, int period,
+                    const uint64_t* known, const uint64_t* up, double mass, int max_children,
+                    uint64_t* child_known, uint64_t* child_up, double* child_mass, double* succ,
+                    int* path_eids);
 
 /* Per-instance exact counts for a list of items processed to completion in
  * DFS order (full subtree under each item, with truncation eps). */

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle_nr.so")
 CLI_PATH = os.path.join(HERE, "bin", "cpu_reference")
 
-MODE_PATH, MODE_PIVOT, MODE_FORKBEST, MODE_MINUNK = 0, 1, 2, 3
+MODE_PATH, MODE_PIVOT, MODE_FORKBEST, MODE_MINUNK, MODE_ALTERNATE = 0, 1, 2, 3, 4
 ORDER_LEVEL, ORDER_HEAP, ORDER_DFS = 0, 1, 2
 
 
@@ -26,6 +26,7 @@ class OrcConfig(ctypes.Structure):
         ("trunc_eps", ctypes.c_double),
         ("capacity", ctypes.c_longlong),
         ("time_budget_s", ctypes.c_double),
+        ("sweep_period", ctypes.c_int),
     ]
 
 
@@ -81,6 +82,9 @@ def lib() -> ctypes.CDLL:
         L.orc_expand.argtypes = [vp, ctypes.c_int, ctypes.c_int, U, U, ctypes.c_double,
                                  ctypes.c_int, U, U, P(ctypes.c_double), P(ctypes.c_double),
                                  P(ctypes.c_int), P(ctypes.c_int)]
+        L.orc_expand_rule.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, U, U, ctypes.c_double, ctypes.c_int, U, U,
+                                      P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_int)]
         L.orc_subtree_stats.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, U, U,
                                         P(ctypes.c_double), ctypes.c_double, U,
                                         P(ctypes.c_double)]
@@ -139,8 +143,8 @@ class OracleGraph:
 
     def solve(self, src: int, dst: int, mode: int = MODE_PATH, order: int = ORDER_LEVEL,
               threads: int = 1, trunc_eps: float = 0.0, capacity: int = 0,
-              time_budget_s: float = 0.0) -> dict:
-        cfg = OrcConfig(mode, order, threads, trunc_eps, capacity, time_budget_s)
+              time_budget_s: float = 0.0, sweep_period: int = 0) -> dict:
+        cfg = OrcConfig(mode, order, threads, trunc_eps, capacity, time_budget_s, sweep_period)
         r = OrcResult()
         rc = lib().orc_solve(self._h, src, dst, ctypes.byref(cfg), ctypes.byref(r))
         if rc:
@@ -180,6 +184,32 @@ class OracleGraph:
                              ctypes.byref(succ), ctypes.byref(nh), _p(path, ctypes.c_int))
         kk = max(k, 0)
         return k, succ.value, ck[:kk], cu[:kk], cm[:kk], path[: max(nh.value, 0)]
+
+    def expand_rule(self, src: int, dst: int, known, up, mass: float, mode: int = MODE_PATH,
+                    period: int = 0):
+        """One expansion under MODE_PATH or MODE_ALTERNATE: (k, succ,
+        children_known, children_up, children_mass, hops in sweep order or
+        None); k = -1 when dead."""
+        W = self.mask_words
+        n = self.dims()["n"]
+        known = np.ascontiguousarray(known, np.uint64).reshape(W)
+        up = np.ascontiguousarray(up, np.uint64).reshape(W)
+        mc = n + 1
+        ck = np.zeros((mc, W), np.uint64)
+        cu = np.zeros((mc, W), np.uint64)
+        cm = np.zeros(mc, np.float64)
+        succ = ctypes.c_double()
+        path = np.zeros(n + 1, np.int32)
+        U = ctypes.c_ulonglong
+        k = lib().orc_expand_rule(self._h, src, dst, mode, period, _p(known, U), _p(up, U),
+                                  float(mass), mc, _p(ck, U), _p(cu, U),
+                                  _p(cm, ctypes.c_double), ctypes.byref(succ),
+                                  _p(path, ctypes.c_int))
+        if k < -1:
+            raise RuntimeError("orc_expand_rule: bad arguments")
+        kk = max(k, 0)
+        hops = path[:kk] if mode == MODE_ALTERNATE else None
+        return k, succ.value, ck[:kk], cu[:kk], cm[:kk], hops
 
     def subtree_stats(self, src, dst, known, up, mass, trunc_eps=0.0):
         W = self.mask_words

@@ -32,7 +32,8 @@ NR_ORDER_LEARNED = 2
 ORDERS = {"level": NR_ORDER_LEVEL, "mass": NR_ORDER_MASS, "learned": NR_ORDER_LEARNED}
 NR_RULE_PATH = 0
 NR_RULE_FORKBEST = 1
-RULES = {"path": NR_RULE_PATH, "forkbest": NR_RULE_FORKBEST}
+NR_RULE_ALTERNATE = 2
+RULES = {"path": NR_RULE_PATH, "forkbest": NR_RULE_FORKBEST, "alternate": NR_RULE_ALTERNATE}
 
 STATUS = {
     0: "NR_OK",
@@ -66,7 +67,8 @@ class Config(ctypes.Structure):
         ("chunk_items", ctypes.c_int),
         ("flags", ctypes.c_int),
         ("rule", ctypes.c_int),
-        ("reserved", ctypes.c_int * 5),
+        ("sweep_period", ctypes.c_int),
+        ("reserved", ctypes.c_int * 4),
     ]
 
 
@@ -171,6 +173,11 @@ def lib() -> ctypes.CDLL:
         P(ctypes.c_double), ctypes.c_int, P(ctypes.c_int), P(ctypes.c_double),
         P(ctypes.c_ulonglong), P(ctypes.c_ulonglong), P(ctypes.c_double),
     ]
+    L.nr_expand_items_rule.argtypes = [
+        vp, P(Config), ctypes.c_int, P(ctypes.c_ulonglong), P(ctypes.c_ulonglong),
+        P(ctypes.c_double), ctypes.c_int, P(ctypes.c_int), P(ctypes.c_double),
+        P(ctypes.c_ulonglong), P(ctypes.c_ulonglong), P(ctypes.c_double),
+    ]
     L.nr_status_str.argtypes = [ctypes.c_int]
     L.nr_status_str.restype = ctypes.c_char_p
     L.nr_device_query.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
@@ -211,7 +218,7 @@ def make_config(queue_capacity: int = 1 << 20, truncation_eps: float = 1e-15,
                 time_budget_s: float = 0.0, order: str | int = "level", device: int = 0,
                 block_size: int = 0, grid_blocks: int = 0, chunk_items: int = 0,
                 time_launches: bool = False, rule: str | int = "path",
-                rows: bool = False) -> Config:
+                rows: bool = False, sweep_period: int = 0) -> Config:
     c = Config()
     lib().nr_config_init(ctypes.byref(c))
     c.queue_capacity = int(queue_capacity)
@@ -224,6 +231,7 @@ def make_config(queue_capacity: int = 1 << 20, truncation_eps: float = 1e-15,
     c.chunk_items = int(chunk_items)
     c.flags = (NR_FLAG_TIME_LAUNCHES if time_launches else 0) | (NR_FLAG_ROWS if rows else 0)
     c.rule = RULES[rule] if isinstance(rule, str) else int(rule)
+    c.sweep_period = int(sweep_period)
     return c
 
 
@@ -344,8 +352,10 @@ class Expansion:
 
 
 def expand_items(graph: Graph, known: np.ndarray, up: np.ndarray, mass: np.ndarray,
-                 max_children: int, device: int = 0) -> Expansion:
-    """One GPU expansion of each item (nr_expand_items)."""
+                 max_children: int, device: int = 0, rule: str | int = "path",
+                 sweep_period: int = 0) -> Expansion:
+    """One GPU expansion of each item (nr_expand_items; nr_expand_items_rule
+    for another rule): children in sweep order."""
     W = graph.info().mask_words
     known = np.ascontiguousarray(known, dtype=np.uint64).reshape(-1, W)
     up = np.ascontiguousarray(up, dtype=np.uint64).reshape(-1, W)
@@ -357,11 +367,20 @@ def expand_items(graph: Graph, known: np.ndarray, up: np.ndarray, mass: np.ndarr
     cu = np.zeros((n, max_children, W), np.uint64)
     cm = np.zeros((n, max_children), np.float64)
     U = ctypes.c_ulonglong
-    _check(lib().nr_expand_items(graph.handle, int(device), n, _ptr(known, U), _ptr(up, U),
-                                 _ptr(mass, ctypes.c_double), int(max_children),
-                                 _ptr(nchild, ctypes.c_int), _ptr(succ, ctypes.c_double),
-                                 _ptr(ck, U), _ptr(cu, U), _ptr(cm, ctypes.c_double)),
-           "nr_expand_items")
+    if rule in ("path", NR_RULE_PATH) and not sweep_period:
+        _check(lib().nr_expand_items(graph.handle, int(device), n, _ptr(known, U), _ptr(up, U),
+                                     _ptr(mass, ctypes.c_double), int(max_children),
+                                     _ptr(nchild, ctypes.c_int), _ptr(succ, ctypes.c_double),
+                                     _ptr(ck, U), _ptr(cu, U), _ptr(cm, ctypes.c_double)),
+               "nr_expand_items")
+    else:
+        cfg = make_config(device=device, rule=rule, sweep_period=sweep_period)
+        _check(lib().nr_expand_items_rule(graph.handle, ctypes.byref(cfg), n, _ptr(known, U),
+                                          _ptr(up, U), _ptr(mass, ctypes.c_double),
+                                          int(max_children), _ptr(nchild, ctypes.c_int),
+                                          _ptr(succ, ctypes.c_double), _ptr(ck, U),
+                                          _ptr(cu, U), _ptr(cm, ctypes.c_double)),
+               "nr_expand_items_rule")
     return Expansion(nchild, succ, ck, cu, cm)
 
 

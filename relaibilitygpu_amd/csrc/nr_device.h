@@ -111,6 +111,7 @@ struct DevGraph {
   u64 tmask[4];     /* terminal set as a node set */
   int nterms;
   int n, e, w, src, dst;
+  int period; /* alternating sweep: flip every `period` FAILED edges */
 };
 
 /* One queue buffer in structure-of-arrays form: plane w of item i at
@@ -960,6 +961,140 @@ struct LeanExpander : Expander<W, NS> {
   __device__ void carry(const QBuf&, u64, const QBuf&, u64, u64) const {}
   __device__ int find_item(const u64 (&known)[W], const u64 (&up)[W]) const {
     return this->find_lean(known, up, lvb);
+  }
+};
+
+/* The alternating-sweep rule on the lean row form (items carry no rows).
+ * Path: fewest UNKNOWN edges.  Node sets are closed under WORKING edges
+ * (close()): level 0 is src's WORKING component, level l+1 the unlevelled
+ * allowed neighbours of level l with their WORKING components, until dst
+ * is levelled; an allowed edge between consecutive levels is therefore
+ * UNKNOWN.  The walk back starts from dst's component K: the parent is the
+ * lowest-index node u on the previous level with an allowed edge into K, the
+ * hop is u's edge to the lowest-index node of K it reaches, and K becomes
+ * u's component.  Hops are pushed dst side first, as by find(); the stack is
+ * then reversed when (FAILED edges / period) is odd, so the children are
+ * formed from the dst end.  Node levels are bytes in the lane's slice, as in
+ * find_lean(). */
+template <int W, typename NS>
+struct AltExpander : LeanExpander<W, NS> {
+  int period;
+  __device__ void bind_graph(unsigned char* smem, const Layout& L, const DevGraph& g, int slot) {
+    LeanExpander<W, NS>::bind_graph(smem, L, g, slot);
+    period = g.period > 0 ? g.period : 2;
+  }
+  __device__ int find_item(const u64 (&known)[W], const u64 (&up)[W]) const {
+    return find_alt(known, up);
+  }
+  /* allowed degree of u (items carry no rows) */
+  __device__ int out_degree(int u, const u64 (&known)[W], const u64 (&up)[W]) const {
+    u64 Dn[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i) Dn[i] = known[i] & ~up[i];
+    const NS a = this->adj_of(u, Dn);
+    return sizeof(NS) == 4 ? __popc((u32)a) : __popcll((u64)a);
+  }
+  __device__ int in_degree(int u, const u64 (&k)[W], const u64 (&p)[W]) const {
+    return out_degree(u, k, p);
+  }
+
+  __device__ int find_alt(const u64 (&known)[W], const u64 (&up)[W]) const {
+    const int n = this->n, stride = this->stride;
+    if (this->src == this->dst) {
+      this->plen = 0;
+      return 0;
+    }
+    u64 Dn[W];
+    int f = 0;
+    NS wn = 0; /* nodes with a WORKING edge */
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+      Dn[i] = known[i] & ~up[i];
+      f += __popcll(Dn[i]);
+      u64 m = up[i];
+      while (m) {
+        const u32 x = this->ep[i * 64 + __builtin_ctzll(m)];
+        m &= m - 1;
+        wn |= ((NS)1 << (x & 0xffffu)) | ((NS)1 << (x >> 16));
+      }
+    }
+    unsigned char* lv = this->lvb;
+    for (int i = 0; i < (n + 3) / 4; ++i) *(u32*)(lv + i * 4 * stride) = 0xffffffffu;
+    auto setl = [&](NS m, int l) {
+      while (m) {
+        const int u = ctzs<NS>(m);
+        m &= m - 1;
+        lv[(u >> 2) * 4 * stride + (u & 3)] = (unsigned char)l;
+      }
+    };
+    auto getl = [&](int u) { return (int)lv[(u >> 2) * 4 * stride + (u & 3)]; };
+    auto close = [&](NS x) {
+      NS todo = x & wn;
+      while (todo) {
+        const int u = ctzs<NS>(todo);
+        todo &= todo - 1;
+        const NS add = this->working_row(u, up) & ~x;
+        x |= add;
+        todo |= add;
+      }
+      return x;
+    };
+    const NS dbit = (NS)1 << this->dst;
+    NS vis = close((NS)1 << this->src);
+    setl(vis, 0);
+    int L = 0;
+    if (!(vis & dbit)) {
+      NS F = vis;
+      for (;;) {
+        NS nx = 0, fr = F;
+        while (fr) {
+          const int u = ctzs<NS>(fr);
+          fr &= fr - 1;
+          nx |= this->adj_of(u, Dn);
+        }
+        nx &= ~vis;
+        if (!nx) return -1;
+        ++L;
+        nx = close(nx);
+        setl(nx, L);
+        vis |= nx;
+        if (nx & dbit) break;
+        F = nx;
+      }
+    }
+    this->plen = L;
+    NS K = close(dbit);
+    for (int l = L; l >= 1; --l) {
+      int ub = 64;
+      NS kk = K;
+      while (kk) {
+        const int v = ctzs<NS>(kk);
+        kk &= kk - 1;
+        NS a = this->adj_of(v, Dn) & ~K;
+        while (a) {
+          const int c = ctzs<NS>(a);
+          if (c >= ub) break;
+          a &= a - 1;
+          if (getl(c) == l - 1) {
+            ub = c;
+            break;
+          }
+        }
+      }
+      if (ub >= n) return -2;
+      const NS into = this->adj_of(ub, Dn) & K;
+      if (!into) return -2;
+      const int v = ctzs<NS>(into);
+      this->uh[(L - l) * stride] = this->eidm[ub * n + v];
+      K = close((NS)1 << ub);
+    }
+    if ((f / period) & 1)
+      for (int i = 0, j = L - 1; i < j; ++i, --j) {
+        const u16 t = this->uh[i * stride];
+        this->uh[i * stride] = this->uh[j * stride];
+        this->uh[j * stride] = t;
+      }
+    return L;
   }
 };
 

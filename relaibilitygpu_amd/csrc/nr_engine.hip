@@ -888,6 +888,7 @@ int upload_tables(Workspace& ws, const nr_graph* g, int W, DevGraph& d) {
   d.w = W;
   d.src = g->src;
   d.dst = g->dst;
+  d.period = 2;
   return NR_OK;
 }
 
@@ -999,6 +1000,7 @@ struct DevMem {
  * expired (host memory). */
 struct nr_checkpoint {
   int n = 0, e = 0, w = 0, src = 0, dst = 0, rule = 0, order = 0;
+  int period = 0; /* sweep period of NR_RULE_ALTERNATE (0 otherwise) */
   double eps = 0.0;
   unsigned long long graph_hash = 0;
   double sum_hi[3] = {0, 0, 0}, sum_lo[3] = {0, 0, 0}; /* R, ESP, D */
@@ -1060,6 +1062,10 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
     var = variant_lean(mask_words(g), var.ns_bytes);
     add_mass_kernels(var, mask_words(g));
   }
+  if (cfg->rule == NR_RULE_ALTERNATE) {
+    if (!g->symmetric || g->n > 64 || !g->terms.empty()) return NR_ERR_UNSUPPORTED;
+    var = variant_alt(mask_words(g), var.ns_bytes);
+  }
   const int tile_bytes = mode == 2 ? 32 * 256 + 16 : 0; /* per-wave feature tiles */
   const int W = mask_words(g);
   const int cus = ws.prop.multiProcessorCount;
@@ -1106,6 +1112,7 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   LevelArgs a;
   std::memset(&a, 0, sizeof a);
   if ((rc = upload_tables(ws, g, W, a.g))) return rc;
+  a.g.period = cfg->sweep_period > 0 ? cfg->sweep_period : 2;
   a.lay = lay;
   a.seg_cap = seg_cap;
   a.n_seg = grid;
@@ -1113,7 +1120,8 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   a.eps = cfg->truncation_eps > 0 ? cfg->truncation_eps : 0.0;
   /* SoA buffer: known[W], up[W], mass, rows[n] (each rows plane holds
    * node sets of ns_bytes) */
-  const size_t rows_bytes = ((size_t)g->n * slots * var.ns_bytes + 255) / 256 * 256;
+  const int item_ns = var.lean ? 0 : var.ns_bytes; /* lean items carry no rows */
+  const size_t rows_bytes = ((size_t)g->n * slots * item_ns + 255) / 256 * 256;
   const size_t tag_bytes = mode ? (slots * 4 + 255) / 256 * 256 : 0;
   const size_t per_buf = (size_t)(2 * W + 1) * slots * 8 + rows_bytes + tag_bytes;
   if ((rc = ws.grow(&ws.q, &ws.q_bytes, 2 * per_buf))) return rc;
@@ -1183,7 +1191,7 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
     }
     Root root;
     std::memset(&root, 0, sizeof root);
-    int Wi = W, ni = g->n, nsb = var.ns_bytes;
+    int Wi = W, ni = g->n, nsb = item_ns;
     u64 nw = n_slots;
     const SeedMeta* dmc = dmeta;
     void* args[] = {&a, &root, &Wi, &ni, &nsb, &nw, &dmc};
@@ -1208,7 +1216,7 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
     }
   } else {
     Root root = root_item(g, W);
-    int Wi = W, ni = g->n, nsb = var.ns_bytes;
+    int Wi = W, ni = g->n, nsb = item_ns;
     u64 nw = n_slots;
     const SeedMeta* none = nullptr;
     void* args[] = {&a, &root, &Wi, &ni, &nsb, &nw, &none};
@@ -1265,7 +1273,8 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   /* the first, small levels in one workgroup (fresh level-order solves of
    * the path rule): a level of at most max_in items, whose children fit
    * segment 0 (k <= n - 1 hops each), never needs a second workgroup */
-  if (var.small && !ck && mode == 0 && !mass && cfg->rule == NR_RULE_PATH) {
+  if (var.small && !ck && mode == 0 && !mass &&
+      (cfg->rule == NR_RULE_PATH || cfg->rule == NR_RULE_ALTERNATE)) {
     const u32 per = (u32)std::max(1, g->n - 1);
     const u32 max_in = std::min<u32>(256u, std::min<u32>((u32)block, seg_cap / per));
     if (max_in >= 1) {
@@ -1475,6 +1484,7 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
     kept->src = g->src;
     kept->dst = g->dst;
     kept->rule = cfg->rule;
+    kept->period = cfg->rule == NR_RULE_ALTERNATE ? a.g.period : 0;
     kept->order = mass ? NR_ORDER_MASS : NR_ORDER_LEVEL;
     kept->eps = a.eps;
     kept->graph_hash = graph_hash(g);
@@ -1786,12 +1796,18 @@ int solve_batch(Workspace& ws, const nr_graph* const* gs, int count, const nr_co
 }
 
 
-int expand_items(Workspace& ws, const nr_graph* g, int count, const u64* known,
-                 const u64* up, const double* mass, int max_children, int* nchild,
-                 double* succ, u64* ck, u64* cu, double* cm) {
+int expand_items(Workspace& ws, const nr_graph* g, int rule, int period, int count,
+                 const u64* known, const u64* up, const double* mass, int max_children,
+                 int* nchild, double* succ, u64* ck, u64* cu, double* cm) {
   Variant var;
   int rc = pick_variant(g, var);
   if (rc) return rc;
+  if (rule == NR_RULE_ALTERNATE) {
+    if (!g->symmetric || g->n > 64 || !g->terms.empty()) return NR_ERR_UNSUPPORTED;
+    var = variant_alt(mask_words(g), var.ns_bytes);
+  } else if (rule != NR_RULE_PATH) {
+    return NR_ERR_UNSUPPORTED;
+  }
   if (count <= 0) return NR_OK;
   const int W = mask_words(g);
   const int block = choose_block(var, g, W, 0);
@@ -1805,6 +1821,7 @@ int expand_items(Workspace& ws, const nr_graph* g, int count, const u64* known,
   ItemsArgs a;
   std::memset(&a, 0, sizeof a);
   if ((rc = upload_tables(ws, g, W, a.g))) return rc;
+  a.g.period = period > 0 ? period : 2;
   a.lay = lay;
   a.count = count;
   a.max_children = std::max(max_children, 1);
@@ -2055,7 +2072,10 @@ int nr_solve(const nr_graph* g, const nr_config* cfg, nr_result* out) {
     return NR_ERR_ARG;
   if (cfg->order == NR_ORDER_LEARNED) return NR_ERR_ARG; /* needs a model: nr_solve_learned */
   if (cfg->order != NR_ORDER_LEVEL && cfg->order != NR_ORDER_MASS) return NR_ERR_ARG;
-  if (cfg->rule != NR_RULE_PATH && cfg->rule != NR_RULE_FORKBEST) return NR_ERR_ARG;
+  if (cfg->rule != NR_RULE_PATH && cfg->rule != NR_RULE_FORKBEST &&
+      cfg->rule != NR_RULE_ALTERNATE)
+    return NR_ERR_ARG;
+  if (cfg->sweep_period < 0) return NR_ERR_ARG;
   nr::Variant var;
   int rc = nr::pick_variant(g, var);
   if (rc) return rc;
@@ -2074,7 +2094,9 @@ int nr_solve_learned(const nr_graph* g, const nr_config* cfg, const nr_model* mo
   }
   if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0 || cfg->time_budget_s < 0)
     return NR_ERR_ARG;
-  if (cfg->rule != NR_RULE_PATH) return NR_ERR_UNSUPPORTED; /* scores path children */
+  if (cfg->rule != NR_RULE_PATH && cfg->rule != NR_RULE_ALTERNATE)
+    return NR_ERR_UNSUPPORTED; /* scores path children */
+  if (cfg->sweep_period < 0) return NR_ERR_ARG;
   nr::Variant var;
   int rc = nr::pick_variant(g, var);
   if (rc) return rc;
@@ -2095,10 +2117,15 @@ int nr_solve_checkpointed(const nr_graph* g, const nr_config* cfg, const nr_chec
   if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0 || cfg->time_budget_s < 0)
     return NR_ERR_ARG;
   if (cfg->order != NR_ORDER_LEVEL && cfg->order != NR_ORDER_MASS) return NR_ERR_ARG;
-  if (cfg->rule != NR_RULE_PATH && cfg->rule != NR_RULE_FORKBEST) return NR_ERR_ARG;
+  if (cfg->rule != NR_RULE_PATH && cfg->rule != NR_RULE_FORKBEST &&
+      cfg->rule != NR_RULE_ALTERNATE)
+    return NR_ERR_ARG;
+  if (cfg->sweep_period < 0) return NR_ERR_ARG;
   if (resume && (resume->graph_hash != nr::graph_hash(g) || resume->w != nr::mask_words(g) ||
-                 resume->rule != cfg->rule || resume->order != cfg->order))
-    return NR_ERR_ARG; /* another graph, rule or order */
+                 resume->rule != cfg->rule || resume->order != cfg->order ||
+                 (cfg->rule == NR_RULE_ALTERNATE &&
+                  resume->period != (cfg->sweep_period > 0 ? cfg->sweep_period : 2))))
+    return NR_ERR_ARG; /* another graph, rule, sweep period or order */
   nr::Variant var;
   int rc = nr::pick_variant(g, var);
   if (rc) return rc;
@@ -2131,7 +2158,7 @@ int nr_checkpoint_save(const nr_checkpoint* c, const char* path) {
   if (!c || !path) return NR_ERR_ARG;
   FILE* f = std::fopen(path, "wb");
   if (!f) return NR_ERR_IO;
-  const int hdr[8] = {c->n, c->e, c->w, c->src, c->dst, c->rule, c->order, 0};
+  const int hdr[8] = {c->n, c->e, c->w, c->src, c->dst, c->rule, c->order, c->period};
   const unsigned long long tail[3] = {c->iterations, c->max_width, c->count};
   bool ok = std::fwrite(kCkptMagic, 1, 8, f) == 8 && std::fwrite(hdr, sizeof hdr, 1, f) == 1 &&
             std::fwrite(&c->eps, 8, 1, f) == 1 && std::fwrite(&c->graph_hash, 8, 1, f) == 1 &&
@@ -2167,6 +2194,7 @@ int nr_checkpoint_load(const char* path, nr_checkpoint** out) {
     c->dst = hdr[4];
     c->rule = hdr[5];
     c->order = hdr[6];
+    c->period = hdr[7];
     c->iterations = tail[0];
     c->max_width = tail[1];
     c->count = tail[2];
@@ -2192,8 +2220,9 @@ int nr_log_subproblems(const nr_graph* g, const nr_config* cfg, long long max_re
   if (!g || !cfg || max_records < 0 || !n_out || !result ||
       (max_records > 0 && (!features || !labels)))
     return NR_ERR_ARG;
-  if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0) return NR_ERR_ARG;
-  if (cfg->rule != NR_RULE_PATH) return NR_ERR_UNSUPPORTED;
+  if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0 || cfg->sweep_period < 0)
+    return NR_ERR_ARG;
+  if (cfg->rule != NR_RULE_PATH && cfg->rule != NR_RULE_ALTERNATE) return NR_ERR_UNSUPPORTED;
   nr::Variant var;
   int rc = nr::pick_variant(g, var);
   if (rc) return rc;
@@ -2232,8 +2261,25 @@ int nr_expand_items(const nr_graph* g, int device, int count,
   if (rc) return rc;
   nr::WsLock L(device);
   if (L.rc) return L.rc;
-  return nr::expand_items(*L.ws, g, count, known, up, mass, max_children, nchild, succ,
-                          child_known, child_up, child_mass);
+  return nr::expand_items(*L.ws, g, NR_RULE_PATH, 0, count, known, up, mass, max_children,
+                          nchild, succ, child_known, child_up, child_mass);
+}
+
+int nr_expand_items_rule(const nr_graph* g, const nr_config* cfg, int count,
+                         const unsigned long long* known, const unsigned long long* up,
+                         const double* mass, int max_children, int* nchild, double* succ,
+                         unsigned long long* child_known, unsigned long long* child_up,
+                         double* child_mass) {
+  if (!g || !cfg || count < 0 || (count > 0 && (!known || !up || !mass || !nchild || !succ)))
+    return NR_ERR_ARG;
+  if (cfg->rule != NR_RULE_PATH && cfg->rule != NR_RULE_ALTERNATE) return NR_ERR_UNSUPPORTED;
+  nr::Variant var;
+  int rc = nr::pick_variant(g, var);
+  if (rc) return rc;
+  nr::WsLock L(cfg->device);
+  if (L.rc) return L.rc;
+  return nr::expand_items(*L.ws, g, cfg->rule, cfg->sweep_period, count, known, up, mass,
+                          max_children, nchild, succ, child_known, child_up, child_mass);
 }
 
 int nr_device_release(int device) {

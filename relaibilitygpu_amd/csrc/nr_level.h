@@ -897,6 +897,11 @@ void add_mass_kernels(Variant& v, int W);
  * NS bytes (nr_lean.hip). */
 Variant variant_lean(int W, int ns_bytes);
 
+/* Alternating-sweep rule (AltExpander) for W words and node sets of
+ * NS bytes: level, max-mass, tree-log, learned, small-level and
+ * single-expansion kernels (nr_alt.hip). */
+Variant variant_alt(int W, int ns_bytes);
+
 /* MODE 3 kernels of a variant (nr_fork.hip), filled into v. */
 void add_fork_kernels(Variant& v, int W);
 

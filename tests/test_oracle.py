@@ -333,3 +333,181 @@ def test_kterminal_detects_a_wrong_tree():
     rp, ci, ei = wl.to_csr(4, edges)
     og = OracleGraph.from_csr(rp, ci, ei, [p] * 4)
     assert abs(og.solve_terms([0, 1, 2, 3])["R"] - og.solve(0, 3)["R"]) > 1e-3
+
+
+# ------------------------------------------------- alternating-sweep rule
+from oracle_ffi import MODE_ALTERNATE  # noqa: E402
+
+
+def _states(rnd, e, pf=0.3, pw=0.2):
+    known = up = 0
+    for i in range(e):
+        s = rnd.random()
+        if s < pf:
+            known |= 1 << i
+        elif s < pf + pw:
+            known |= 1 << i
+            up |= 1 << i
+    return known, up
+
+
+def _zero_one_distance(n, ends, src, dst, known, up):
+    """Fewest UNKNOWN edges on a src->dst walk that avoids FAILED edges
+    (WORKING edges cost 0), by 0-1 BFS; None when dst is unreachable."""
+    import collections
+    adj = [[] for _ in range(n)]
+    for i, (a, b) in enumerate(ends):
+        if (known >> i) & 1 and not (up >> i) & 1:
+            continue
+        w = 0 if (known >> i) & 1 else 1
+        adj[a].append((b, w))
+        adj[b].append((a, w))
+    dist = [None] * n
+    dist[src] = 0
+    dq = collections.deque([src])
+    while dq:
+        u = dq.popleft()
+        for v, w in adj[u]:
+            if dist[v] is None or dist[u] + w < dist[v]:
+                dist[v] = dist[u] + w
+                (dq.appendleft if w == 0 else dq.append)(v)
+    return dist[dst]
+
+
+def _components(n, ends, up):
+    lab = list(range(n))
+
+    def find(x):
+        while lab[x] != x:
+            x = lab[x]
+        return x
+    for i, (a, b) in enumerate(ends):
+        if (up >> i) & 1:
+            lab[find(a)] = find(b)
+    return [find(v) for v in range(n)]
+
+
+@pytest.mark.parametrize("period", [1, 2, 3])
+def test_alternate_rule_exact(expected, period):
+    """Any path and any hop order split an item exactly: R equals brute
+    force and the closed form, R + D = 1, for several sweep periods."""
+    for bf in expected["brute_force"]:
+        r = load(bf["graph"]).solve(bf["src"], bf["dst"], mode=MODE_ALTERNATE,
+                                    sweep_period=period)
+        assert abs(r["R"] - bf["R"]) <= 1e-15, bf
+        assert abs(r["R"] + r["D"] - 1.0) <= 1e-15
+    for name, n, p in [("COMP8.txt", 8, "0.5"), ("COMP8-09.txt", 8, "0.9"),
+                       ("COMP8-03.txt", 8, "0.3")]:
+        r = load(name).solve(0, n - 1, mode=MODE_ALTERNATE, sweep_period=period)
+        assert abs(r["R"] - closed_form(expected, n, p)) <= 1e-14
+
+
+def test_alternate_k10_closed_form_and_fewer_nodes(expected):
+    g = load("COMP10-05.txt")
+    a = g.solve(0, 9, mode=MODE_ALTERNATE, order=ORDER_DFS, threads=8)
+    assert abs(a["R"] - closed_form(expected, 10, "0.5")) <= 1e-13
+    assert a["ESP"] == 0 and abs(a["R"] + a["D"] - 1.0) <= 1e-14
+    # fewer than half the plain rule's 14 866 064 subproblems
+    assert a["nodes"] < 14866064 // 2
+
+
+def test_alternate_counts_independent_of_order_and_threads():
+    g = load("COMP8.txt")
+    ref = g.solve(0, 7, mode=MODE_ALTERNATE)
+    for order, threads in [(ORDER_HEAP, 1), (ORDER_DFS, 1), (ORDER_DFS, 4)]:
+        r = g.solve(0, 7, mode=MODE_ALTERNATE, order=order, threads=threads)
+        assert (r["paths"], r["nodes"], r["dead"]) == (ref["paths"], ref["nodes"], ref["dead"])
+        assert abs(r["R"] - ref["R"]) <= 1e-15
+    # leaves (successes + dead) and internal nodes: every live item adds
+    # exactly one success, so paths = live items and nodes = paths + dead
+    assert ref["nodes"] == ref["paths"] + ref["dead"]
+
+
+@pytest.mark.parametrize("eps", [1e-6, 1e-9])
+def test_alternate_brackets_under_truncation(expected, eps):
+    g = load("COMP15-05.txt")
+    r = g.solve(0, 14, mode=MODE_ALTERNATE, order=ORDER_DFS, threads=8, trunc_eps=eps)
+    R = closed_form(expected, 15, "0.5")
+    assert r["R"] <= R <= r["R"] + r["ESP"] * (1 + 1e-12)
+    assert abs(r["R"] + r["D"] + r["ESP"] - 1.0) <= 1e-13
+
+
+@pytest.mark.parametrize("name,src,dst,period", [("COMP10-05.txt", 0, 9, 1),
+                                                  ("COMP10-05.txt", 0, 9, 2),
+                                                  ("test_kite.txt", 0, 4, 1),
+                                                  ("temp4x4.txt", 0, 3, 3)])
+def test_alternate_single_expansion(name, src, dst, period):
+    """One expansion on random edge states: the path has the fewest UNKNOWN
+    hops (0-1 BFS distance, WORKING edges free), its hops chain src's
+    WORKING component to dst's, the sweep starts at the src end exactly
+    when (FAILED count / period) is even, and child i has hops 0..i-1
+    WORKING, hop i FAILED and mass m * p_0 ... p_{i-1} * q_i."""
+    g = load(name)
+    rp, ci, ei, p = g.arrays()
+    n, E = len(rp) - 1, len(p)
+    ends = [None] * E
+    for u in range(n):
+        for j in range(rp[u], rp[u + 1]):
+            ends[ei[j]] = (min(u, ci[j]), max(u, ci[j]))
+    rnd = random.Random(11)
+    seen_dirs = set()
+    for _ in range(400):
+        known, up = _states(rnd, E)
+        m = rnd.uniform(0.1, 1.0)
+        kw = [(known >> (64 * w)) & ((1 << 64) - 1) for w in range(g.mask_words)]
+        uw = [(up >> (64 * w)) & ((1 << 64) - 1) for w in range(g.mask_words)]
+        k, succ, ck, cu, cm, hops = g.expand_rule(src, dst, kw, uw, m, MODE_ALTERNATE, period)
+        d = _zero_one_distance(n, ends, src, dst, known, up)
+        if d is None:
+            assert k == -1
+            continue
+        assert k == d
+        lab = _components(n, ends, up & known)
+        hops = [int(h) for h in hops]
+        # orientation: the sweep starts at src's component or at dst's
+        f = bin(known & ~up).count("1")
+        from_src = (f // period) % 2 == 0
+        chain = hops if from_src else hops[::-1]
+        reached = {lab[src]}
+        for e in chain:
+            a, b = ends[e]
+            assert not (known >> e) & 1  # every hop is UNKNOWN
+            assert (lab[a] in reached) != (lab[b] in reached)
+            reached |= {lab[a], lab[b]}
+        assert lab[dst] in reached
+        if k:
+            seen_dirs.add(from_src)
+        run = m
+        pre = 0
+        for i, e in enumerate(hops):
+            ckv = sum(int(x) << (64 * w) for w, x in enumerate(ck[i]))
+            cuv = sum(int(x) << (64 * w) for w, x in enumerate(cu[i]))
+            assert ckv == known | pre | (1 << e) and cuv == up | pre
+            assert cm[i] == run * (1.0 - p[e])
+            run = run * p[e]
+            pre |= 1 << e
+        assert succ == run
+        assert abs(sum(cm) + succ - m) <= 1e-15 * (k + 1)
+    assert seen_dirs == {True, False}
+
+
+def test_alternate_detects_a_longer_path():
+    """Planted mistake: the plain rule's path (hop-shortest, WORKING edges
+    counted) is not the fewest-UNKNOWN path, and the 0-1 distance check
+    above tells them apart on the kite graph."""
+    g = load("test_kite.txt")
+    rp, ci, ei, p = g.arrays()
+    n, E = len(rp) - 1, len(p)
+    ends = [None] * E
+    for u in range(n):
+        for j in range(rp[u], rp[u + 1]):
+            ends[ei[j]] = (min(u, ci[j]), max(u, ci[j]))
+    rnd = random.Random(5)
+    differs = 0
+    for _ in range(300):
+        known, up = _states(rnd, E, 0.1, 0.5)
+        d = _zero_one_distance(n, ends, 0, 4, known, up)
+        k, *_ = g.expand(0, 4, [known], [up], 1.0)
+        if d is not None and k >= 0 and k != d:
+            differs += 1
+    assert differs > 0
