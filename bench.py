@@ -10,9 +10,10 @@ Prints one JSON line with the metric, a roofline object for the expand
 kernel (HBM bound: algorithmic bytes per subproblem = 2 * (16*W + 8), one
 write when the item is created and one read when it is expanded), the CPU
 baseline (the oracle on the host cores, same workload) and the metric's
-second part: the certified gap ESP at a time budget (60 s by default) for
-the learned order against the plain level and max-mass orders on K15 and
-K25 (esp_at_budget).
+second part: the certified gap ESP at a time budget (60 s by default) on
+K15 and K25 at delta = 1e-12, the plain method (path rule, level and
+max-mass order) against the alternating-sweep rule in the learned order
+(esp_at_budget).
 """
 from __future__ import annotations
 
@@ -193,10 +194,16 @@ def run_batch(args, wl: dict) -> dict:
 
 
 def esp_leg(budget_s: float = 60.0) -> dict:
-    """Certified gap ESP at a time budget, learned order vs the plain orders,
-    on K15 and K25 (p = 0.5, delta = 1e-15, capacity 96 777 216 items).  The
-    scorer is trained here, on the GPU, on 2e6 children logged from plain K10
-    solves (delta 0, 1e-6, 1e-9); labels are log2 subtree sizes."""
+    """Certified gap ESP at a time budget on K15 and K25 (p = 0.5, capacity
+    96 777 216 items).  At delta = 1e-12 (the configured workload): the
+    plain GPU method (path rule) in level and max-mass order against the
+    alternating-sweep rule (sweep period picked by pilot solves at delta
+    1e-9) in level order and in the learned order.  At delta = 1e-15, where
+    the capacity binds and the order decides what is trimmed: the
+    alternating rule in level, max-mass and learned order.  The scorer is
+    trained here, on the GPU, on 2e6 children logged from K10 solves under
+    the alternating rule (delta 0, 1e-6, 1e-9); labels are log2 subtree
+    sizes."""
     import numpy as np
 
     import relaibilitygpu_amd as nr
@@ -205,8 +212,9 @@ def esp_leg(budget_s: float = 60.0) -> dict:
     t0 = time.perf_counter()
     k10 = nr.Graph.load(os.path.join(G, "COMP10-05.txt"), 0, 9)
     Xs, Ys = [], []
-    for eps, m in ((0.0, 1000000), (1e-6, 500000), (1e-9, 500000)):
-        X, Y, _ = nr.log_subproblems(k10, m, seed=7, truncation_eps=eps, queue_capacity=1 << 22)
+    for eps, m in ((0.0, 1600000), (1e-6, 200000), (1e-9, 200000)):
+        X, Y, _ = nr.log_subproblems(k10, m, seed=7, truncation_eps=eps, queue_capacity=1 << 22,
+                                     rule="alternate")
         Xs.append(X)
         Ys.append(Y)
     X, Y = np.concatenate(Xs), np.concatenate(Ys)
@@ -214,24 +222,40 @@ def esp_leg(budget_s: float = 60.0) -> dict:
     params, tl, hl = nr.mlp_train(X, Y, None, epochs=10, batch=4096, lr=3e-3, seed=3)
     t_train = time.perf_counter() - t0 - t_log
     model = nr.Model(params, alpha=0.5)
-    out = {"budget_s": budget_s, "delta": 1e-15, "queue_capacity": 96777216,
+    cap = 96777216
+    out = {"budget_s": budget_s, "queue_capacity": cap,
            "training": {"records": int(len(Y)), "label_var": float(Y.var()),
                         "holdout_mse": float(hl[-1]), "log_s": t_log, "train_s": t_train,
-                        "alpha": 0.5}}
+                        "alpha": 0.5, "rule": "alternate"}}
+
+    def row(r):
+        return {"ESP": r["ESP"], "R": r["R"], "nodes": r["nodes"],
+                "elapsed_ms": r["elapsed_ms"], "deadline_hit": r["deadline_hit"]}
+
     for name, n in (("K15", 15), ("K25", 25)):
         g = nr.Graph.load(os.path.join(G, f"COMP{n}-05.txt"), 0, n - 1)
-        kw = dict(truncation_eps=1e-15, queue_capacity=96777216, time_budget_s=budget_s)
-        rows = {}
-        for order in ("level", "mass"):
-            r = nr.solve(g, order=order, **kw)
-            rows[order] = r
-        rows["learned"] = nr.solve_learned(g, model, **kw)
-        out[name] = {k: {"ESP": v["ESP"], "R": v["R"], "nodes": v["nodes"],
-                         "elapsed_ms": v["elapsed_ms"], "deadline_hit": v["deadline_hit"]}
-                     for k, v in rows.items()}
-        out[name]["ESP_mass_over_learned"] = rows["mass"]["ESP"] / rows["learned"]["ESP"]
-        print(f"esp {name}: " + ", ".join(f"{k} {v['ESP']:.3e}" for k, v in rows.items()),
-              file=sys.stderr, flush=True)
+        period, pilot = nr.tune_sweep_period(g, pilot_eps=1e-9, max_period=6)
+        res = {"sweep_period": period, "pilot_esp": pilot}
+        for delta in (1e-12, 1e-15):
+            kw = dict(truncation_eps=delta, queue_capacity=cap, time_budget_s=budget_s)
+            alt = dict(kw, rule="alternate", sweep_period=period)
+            rows = {}
+            if delta == 1e-12:
+                rows["plain_level"] = nr.solve(g, **kw)
+                rows["plain_mass"] = nr.solve(g, order="mass", **kw)
+            rows["alternate_level"] = nr.solve(g, **alt)
+            if delta == 1e-15:
+                rows["alternate_mass"] = nr.solve(g, order="mass", **alt)
+            rows["alternate_learned"] = nr.solve_learned(g, model, **alt)
+            res[f"delta_{delta:g}"] = {k: row(v) for k, v in rows.items()}
+            print(f"esp {name} delta {delta:g} (period {period}): "
+                  + ", ".join(f"{k} {v['ESP']:.3e} ({v['elapsed_ms']:.0f} ms)"
+                              for k, v in rows.items()), file=sys.stderr, flush=True)
+        d12 = res["delta_1e-12"]
+        res["ESP_plain_over_learned"] = d12["plain_level"]["ESP"] / d12["alternate_learned"]["ESP"]
+        res["ESP_plain_heap_over_learned"] = (d12["plain_mass"]["ESP"]
+                                              / d12["alternate_learned"]["ESP"])
+        out[name] = res
     return out
 
 

@@ -201,6 +201,15 @@ int nr_expand_items(const nr_graph* g, int device, int count,
                     unsigned long long* child_known,
                     unsigned long long* child_up, double* child_mass);
 
+/* Sweep period of NR_RULE_ALTERNATE for graph g, picked by pilot solves:
+ * level order at truncation pilot_eps (> 0) for periods 1..max_period (at
+ * most 64), the other settings from cfg (or the defaults).  *best = the
+ * period with the smallest certified gap, the fewest subproblems on ties;
+ * pilot_esp[0..max_period-1] (optional) receives each pilot's ESP.  No
+ * counterpart in the reference. */
+int nr_tune_sweep_period(const nr_graph* g, const nr_config* cfg, double pilot_eps,
+                         int max_period, int* best, double* pilot_esp);
+
 /* nr_expand_items() under cfg->rule (NR_RULE_PATH or NR_RULE_ALTERNATE with
  * cfg->sweep_period) on device cfg->device; children in sweep order.  No
  * counterpart in the reference beyond nr_expand_items'. */

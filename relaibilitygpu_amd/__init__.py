@@ -178,6 +178,8 @@ def lib() -> ctypes.CDLL:
         P(ctypes.c_double), ctypes.c_int, P(ctypes.c_int), P(ctypes.c_double),
         P(ctypes.c_ulonglong), P(ctypes.c_ulonglong), P(ctypes.c_double),
     ]
+    L.nr_tune_sweep_period.argtypes = [vp, P(Config), ctypes.c_double, ctypes.c_int,
+                                       P(ctypes.c_int), P(ctypes.c_double)]
     L.nr_status_str.argtypes = [ctypes.c_int]
     L.nr_status_str.restype = ctypes.c_char_p
     L.nr_device_query.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
@@ -332,6 +334,18 @@ def solve(graph: Graph, config: Config | None = None, **kw) -> dict:
     r = Result()
     _check(lib().nr_solve(graph.handle, ctypes.byref(cfg), ctypes.byref(r)), "nr_solve")
     return r.as_dict()
+
+
+def tune_sweep_period(graph: Graph, pilot_eps: float = 1e-9, max_period: int = 6,
+                      config: Config | None = None, **kw) -> tuple[int, list[float]]:
+    """(period, pilot ESP per period 1..max_period): nr_tune_sweep_period."""
+    cfg = config if config is not None else make_config(**kw)
+    best = ctypes.c_int()
+    esp = np.zeros(max_period, np.float64)
+    _check(lib().nr_tune_sweep_period(graph.handle, ctypes.byref(cfg), float(pilot_eps),
+                                      int(max_period), ctypes.byref(best),
+                                      _ptr(esp, ctypes.c_double)), "nr_tune_sweep_period")
+    return best.value, [float(x) for x in esp]
 
 
 def solve_batch(graphs, config: Config | None = None, **kw) -> list[dict]:

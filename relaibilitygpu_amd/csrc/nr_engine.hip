@@ -2084,6 +2084,36 @@ int nr_solve(const nr_graph* g, const nr_config* cfg, nr_result* out) {
   return nr::solve_level(*L.ws, g, cfg, out);
 }
 
+int nr_tune_sweep_period(const nr_graph* g, const nr_config* cfg, double pilot_eps,
+                         int max_period, int* best, double* pilot_esp) {
+  if (!g || !best || max_period < 1 || max_period > 64 || !(pilot_eps > 0.0)) return NR_ERR_ARG;
+  nr_config c;
+  if (cfg)
+    c = *cfg;
+  else
+    nr_config_init(&c);
+  c.rule = NR_RULE_ALTERNATE;
+  c.order = NR_ORDER_LEVEL;
+  c.truncation_eps = pilot_eps;
+  c.time_budget_s = 0.0;
+  *best = 0;
+  double best_esp = 0.0;
+  unsigned long long best_nodes = 0;
+  for (int per = 1; per <= max_period; ++per) {
+    c.sweep_period = per;
+    nr_result r;
+    const int rc = nr_solve(g, &c, &r);
+    if (rc) return rc;
+    if (pilot_esp) pilot_esp[per - 1] = r.ESP;
+    if (*best == 0 || r.ESP < best_esp || (r.ESP == best_esp && r.nodes < best_nodes)) {
+      *best = per;
+      best_esp = r.ESP;
+      best_nodes = r.nodes;
+    }
+  }
+  return NR_OK;
+}
+
 int nr_solve_learned(const nr_graph* g, const nr_config* cfg, const nr_model* model,
                      nr_result* out) {
   if (!g || !out || !model) return NR_ERR_ARG;
