@@ -921,6 +921,149 @@ struct CsrExpander {
   }
 };
 
+/* The alternating-sweep rule (AltExpander's search and sweep) on the CSR
+ * arcs of an undirected graph of up to 256 nodes: node sets of NW words in
+ * VGPRs, levels as bytes.  Closures follow WORKING arcs from a work set; the
+ * walk back takes, over the nodes v of K in ascending order, the first
+ * in-arc (rows sorted by (u, e)) from the previous level, and keeps the
+ * lowest u (the lowest v on ties): lowest u, then the lowest node of K it
+ * reaches, then the lowest edge id. */
+template <int W, int NW>
+struct CsrAltExpander : CsrExpander<W, NW> {
+  int period;
+  __device__ void bind_graph(unsigned char* smem, const Layout& L, const DevGraph& g, int slot) {
+    CsrExpander<W, NW>::bind_graph(smem, L, g, slot);
+    period = g.period > 0 ? g.period : 2;
+  }
+  __device__ static bool working(const u64 (&known)[W], const u64 (&up)[W], int e) {
+    const int wi = e >> 6;
+    const u64 b = 1ull << (e & 63);
+    return (pick<W>(known, wi) & pick<W>(up, wi) & b) != 0;
+  }
+  /* extend x (and vis) over WORKING arcs from the nodes of x */
+  __device__ __forceinline__ void close(u64 (&x)[NW], u64 (&vis)[NW], const u64 (&known)[W],
+                                        const u64 (&up)[W]) const {
+    u64 todo[NW];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) todo[k] = x[k];
+    for (;;) {
+      int u = -1;
+#pragma unroll
+      for (int k = 0; k < NW; ++k)
+        if (u < 0 && todo[k]) {
+          u = k * 64 + __builtin_ctzll(todo[k]);
+          todo[k] &= todo[k] - 1;
+        }
+      if (u < 0) break;
+      const int j1 = this->rp[u + 1];
+      for (int j = this->rp[u]; j < j1; ++j) {
+        const u32 a = this->arc[j];
+        const int v = (int)(a & 0xffffu), e = (int)(a >> 16);
+        if (bs_test<NW>(vis, v) || !working(known, up, e)) continue;
+        bs_set<NW>(vis, v);
+        bs_set<NW>(x, v);
+        bs_set<NW>(todo, v);
+      }
+    }
+  }
+  __device__ int find_item(const u64 (&known)[W], const u64 (&up)[W]) const {
+    const int n = this->n, src = this->src, dst = this->dst;
+    if (src == dst) {
+      this->plen = 0;
+      return 0;
+    }
+    for (int i = 0; i < (n + 3) / 4; ++i) *(u32*)(this->lvb + i * 4 * this->stride) = 0xffffffffu;
+    u64 vis[NW], F[NW];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) vis[k] = F[k] = 0;
+    bs_set<NW>(vis, src);
+    bs_set<NW>(F, src);
+    close(F, vis, known, up);
+#pragma unroll
+    for (int k = 0; k < NW; ++k)
+      for (u64 w = F[k]; w; w &= w - 1) this->set_level(k * 64 + __builtin_ctzll(w), 0);
+    int L = 0;
+    while (!bs_test<NW>(vis, dst)) {
+      u64 nx[NW];
+      bool any = false;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) nx[k] = 0;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) {
+        u64 fw = F[k];
+        while (fw) {
+          const int u = k * 64 + __builtin_ctzll(fw);
+          fw &= fw - 1;
+          const int j1 = this->rp[u + 1];
+          for (int j = this->rp[u]; j < j1; ++j) {
+            const u32 a = this->arc[j];
+            const int v = (int)(a & 0xffffu), e = (int)(a >> 16);
+            if (bs_test<NW>(vis, v) || !this->allowed(known, up, e)) continue;
+            bs_set<NW>(vis, v);
+            bs_set<NW>(nx, v);
+            any = true;
+          }
+        }
+      }
+      if (!any) return -1;
+      ++L;
+      if (L >= 255) return -2;
+      close(nx, vis, known, up);
+#pragma unroll
+      for (int k = 0; k < NW; ++k)
+        for (u64 w = nx[k]; w; w &= w - 1) this->set_level(k * 64 + __builtin_ctzll(w), L);
+#pragma unroll
+      for (int k = 0; k < NW; ++k) F[k] = nx[k];
+    }
+    this->plen = L;
+    u64 K[NW], none[NW];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) K[k] = none[k] = 0;
+    bs_set<NW>(K, dst);
+    bs_set<NW>(none, dst);
+    close(K, none, known, up);
+    for (int l = L; l >= 1; --l) {
+      int bu = n, be = -1;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) {
+        u64 kw = K[k];
+        while (kw) {
+          const int v = k * 64 + __builtin_ctzll(kw);
+          kw &= kw - 1;
+          const int j1 = this->irp[v + 1];
+          for (int j = this->irp[v]; j < j1; ++j) {
+            const u32 a = this->iarc[j];
+            const int u = (int)(a & 0xffffu), e = (int)(a >> 16);
+            if (u >= bu) break;
+            if (this->level_of(u) == l - 1 && this->allowed(known, up, e)) {
+              bu = u;
+              be = e;
+              break;
+            }
+          }
+        }
+      }
+      if (be < 0) return -2;
+      this->uh[(L - l) * this->stride] = (u16)be;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) K[k] = none[k] = 0;
+      bs_set<NW>(K, bu);
+      bs_set<NW>(none, bu);
+      close(K, none, known, up);
+    }
+    int f = 0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) f += __popcll(known[i] & ~up[i]);
+    if ((f / period) & 1)
+      for (int i = 0, j = L - 1; i < j; ++i, --j) {
+        const u16 t = this->uh[i * this->stride];
+        this->uh[i * this->stride] = this->uh[j * this->stride];
+        this->uh[j * this->stride] = t;
+      }
+    return L;
+  }
+};
+
 /* LDS layout of the lean row-form expander: staged graph tables, then the
  * kernel's extra region, per-lane node levels (bytes) and hop stack. */
 inline Layout make_layout_lean(int n, int e, int w, int block, int extra_bytes = 0) {

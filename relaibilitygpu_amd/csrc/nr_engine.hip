@@ -1063,8 +1063,10 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
     add_mass_kernels(var, mask_words(g));
   }
   if (cfg->rule == NR_RULE_ALTERNATE) {
-    if (!g->symmetric || g->n > 64 || !g->terms.empty()) return NR_ERR_UNSUPPORTED;
-    var = variant_alt(mask_words(g), var.ns_bytes);
+    if (!g->symmetric || !g->terms.empty()) return NR_ERR_UNSUPPORTED;
+    var = g->n <= 64 ? variant_alt(mask_words(g), var.ns_bytes)
+                     : variant_alt_csr(mask_words(g), (g->n + 63) / 64);
+    if (!var.level) return NR_ERR_UNSUPPORTED;
   }
   const int tile_bytes = mode == 2 ? 32 * 256 + 16 : 0; /* per-wave feature tiles */
   const int W = mask_words(g);
@@ -1803,8 +1805,10 @@ int expand_items(Workspace& ws, const nr_graph* g, int rule, int period, int cou
   int rc = pick_variant(g, var);
   if (rc) return rc;
   if (rule == NR_RULE_ALTERNATE) {
-    if (!g->symmetric || g->n > 64 || !g->terms.empty()) return NR_ERR_UNSUPPORTED;
-    var = variant_alt(mask_words(g), var.ns_bytes);
+    if (!g->symmetric || !g->terms.empty()) return NR_ERR_UNSUPPORTED;
+    var = g->n <= 64 ? variant_alt(mask_words(g), var.ns_bytes)
+                     : variant_alt_csr(mask_words(g), (g->n + 63) / 64);
+    if (!var.items) return NR_ERR_UNSUPPORTED;
   } else if (rule != NR_RULE_PATH) {
     return NR_ERR_UNSUPPORTED;
   }

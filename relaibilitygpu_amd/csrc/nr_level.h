@@ -902,6 +902,10 @@ Variant variant_lean(int W, int ns_bytes);
  * single-expansion kernels (nr_alt.hip). */
 Variant variant_alt(int W, int ns_bytes);
 
+/* The alternating-sweep rule on the CSR arcs (CsrAltExpander) for W words and
+ * NW node-set words (nr_alt_csr.hip); undirected graphs only. */
+Variant variant_alt_csr(int W, int NW);
+
 /* MODE 3 kernels of a variant (nr_fork.hip), filled into v. */
 void add_fork_kernels(Variant& v, int W);
 

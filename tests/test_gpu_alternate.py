@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 
 nr = pytest.importorskip("relaibilitygpu_amd")
 from relaibilitygpu_amd import workloads as wl  # noqa: E402
-from oracle_ffi import (MODE_ALTERNATE, MODE_PATH, ORDER_DFS, ORDER_LEVEL,  # noqa: E402
+from oracle_ffi import (MODE_ALTERNATE, ORDER_DFS, ORDER_LEVEL,  # noqa: E402
                         OracleGraph)
 
 REL = 1e-12
@@ -201,6 +201,80 @@ def test_learned_order_on_alternate_rule(expected):
     r = nr.solve_learned(g15, m, truncation_eps=1e-12, queue_capacity=1 << 24, rule="alternate")
     o = nr.solve(g15, truncation_eps=1e-12, queue_capacity=1 << 24, rule="alternate")
     assert (r["paths"], r["nodes"], r["truncated"]) == (o["paths"], o["nodes"], o["truncated"])
+
+
+def _grid10():
+    rng = wl.Xoshiro256ss(0x6121D)
+    n, edges, _ = wl.grid_graph(10, 10, [0.0] * 1000)
+    probs = [0.85 + 0.10 * rng.uniform() for _ in edges]
+    return n, edges, probs
+
+
+@pytest.mark.parametrize("kind,eps", [("grid", 1e-6), ("grid", 1e-8), ("ba", 1e-12),
+                                      ("ba", 1e-15)])
+def test_csr_form_vs_oracle(kind, eps):
+    """Undirected graphs of more than 64 nodes run the rule on the CSR arcs
+    (CsrAltExpander): same counts as the oracle, and a smaller certified gap
+    than the plain rule at the same delta."""
+    print(f"case alternate CSR {kind} eps={eps}", flush=True)
+    if kind == "grid":
+        n, edges, probs = _grid10()
+        s, t = 0, 99
+    else:
+        n, edges, probs = wl.barabasi_albert(150, 2, 0.95, 0xBA150)
+        s, t = 0, 149
+    rp, ci, ei = wl.to_csr(n, edges)
+    g = nr.Graph.from_csr(rp, ci, ei, probs, s, t)
+    og = OracleGraph.from_csr(rp, ci, ei, probs)
+    for period in (1, 2):
+        r = nr.solve(g, truncation_eps=eps, queue_capacity=1 << 24, rule="alternate",
+                     sweep_period=period)
+        o = og.solve(s, t, mode=MODE_ALTERNATE, order=ORDER_LEVEL, trunc_eps=eps,
+                     sweep_period=period)
+        check(r, o)
+    plain = nr.solve(g, truncation_eps=eps, queue_capacity=1 << 24)
+    assert plain["ESP"] > r["ESP"]
+
+
+@pytest.mark.parametrize("kind", ["grid", "ba"])
+def test_csr_single_expansions(kind):
+    print(f"case alternate CSR expansions {kind}", flush=True)
+    if kind == "grid":
+        n, edges, probs = _grid10()
+        s, t = 0, 99
+    else:
+        n, edges, probs = wl.barabasi_albert(150, 2, 0.95, 0xBA150)
+        s, t = 5, 140
+    rp, ci, ei = wl.to_csr(n, edges)
+    g = nr.Graph.from_csr(rp, ci, ei, probs, s, t)
+    og = OracleGraph.from_csr(rp, ci, ei, probs)
+    W = og.mask_words
+    E = len(edges)
+    rng = np.random.default_rng(31)
+    count = 300
+    known = np.zeros((count, W), np.uint64)
+    up = np.zeros((count, W), np.uint64)
+    for i in range(count):
+        state = rng.choice(3, size=E, p=[0.6, 0.15, 0.25])
+        for e in range(E):
+            if state[e]:
+                known[i, e >> 6] |= np.uint64(1 << (e & 63))
+            if state[e] == 2:
+                up[i, e >> 6] |= np.uint64(1 << (e & 63))
+    mass = rng.uniform(0.1, 1.0, count)
+    x = nr.expand_items(g, known, up, mass, max_children=n, rule="alternate", sweep_period=1)
+    live = 0
+    for i in range(count):
+        k, succ, ck, cu, cm, _ = og.expand_rule(s, t, known[i], up[i], mass[i], MODE_ALTERNATE, 1)
+        assert x.nchild[i] == k
+        if k < 0:
+            continue
+        live += 1
+        assert x.succ[i] == succ
+        assert np.array_equal(x.child_known[i, :k], ck[:k])
+        assert np.array_equal(x.child_up[i, :k], cu[:k])
+        assert np.array_equal(x.child_mass[i, :k], cm[:k])
+    assert live > 10
 
 
 def test_unsupported_forms():
