@@ -127,6 +127,7 @@ __global__ void __launch_bounds__(256) k_init(LevelArgs a, Root root, int W, int
     if (tid == 0) {
       LevelCtl z = {};
       *a.ctl = z;
+      a.ctl->start_tick = wall_clock64();
       a.ctl->last_items = seed->count;
       a.ctl->iterations = seed->iterations;
       a.ctl->max_width = seed->max_width;
@@ -140,6 +141,7 @@ __global__ void __launch_bounds__(256) k_init(LevelArgs a, Root root, int W, int
   if (tid == 0) {
     LevelCtl z = {};
     *a.ctl = z;
+    a.ctl->start_tick = wall_clock64();
     a.ctl->last_items = 1;
     for (int w = 0; w < W; ++w) {
       a.buf[0].known[(u64)w * a.stride] = root.known[w];
@@ -1120,6 +1122,17 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   a.n_seg = grid;
   a.stride = slots;
   a.eps = cfg->truncation_eps > 0 ? cfg->truncation_eps : 0.0;
+  /* Under a time budget (and no checkpoint to keep whole levels for) the
+   * kernels compare the GPU wall clock with the deadline before each batch
+   * of items, so a long level cannot overrun the budget; the host folds what
+   * is left once it sees the deadline. */
+  a.budget_ticks = 0;
+  if (cfg->time_budget_s > 0 && !(x && x->save)) {
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ws.device) == hipSuccess &&
+        khz > 0)
+      a.budget_ticks = std::max<u64>(1, (u64)(cfg->time_budget_s * 1e3 * (double)khz));
+  }
   /* SoA buffer: known[W], up[W], mass, rows[n] (each rows plane holds
    * node sets of ns_bytes) */
   const int item_ns = var.lean ? 0 : var.ns_bytes; /* lean items carry no rows */

@@ -42,6 +42,7 @@ struct LevelCtl {
   u64 counts[C_N];
   double sum[3];  /* R, ESP, D after k_finalize */
   double sum_lo[3]; /* their compensation terms (double-double low parts) */
+  u64 start_tick;   /* wall clock (s_memrealtime) when the solve was seeded */
 };
 
 
@@ -75,6 +76,9 @@ struct LevelArgs {
   u64 log_cap;
   /* learned order */
   const MlpDev* mlp;
+  /* time budget in wall-clock ticks from ctl->start_tick (0: none): items a
+   * wave reaches after it are carried to the output unexpanded */
+  u64 budget_ticks;
 };
 
 template <int W>
@@ -333,9 +337,10 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
   u32* part = s_cnt + 1;
   block_prefix(a.seg_count[parity], a.n_seg, sp, part);
   const u32 n_items = sp[a.n_seg];
+  const u64 t_end = a.budget_ticks ? a.ctl->start_tick + a.budget_ticks : 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) { /* count the level (main.cu:970-973) */
     a.ctl->last_items = n_items;
-    if (n_items) {
+    if (n_items && !(t_end && wall_clock64() >= t_end)) {
       a.ctl->iterations += 1;
       if (n_items > a.ctl->max_width) a.ctl->max_width = n_items;
     }
@@ -397,6 +402,8 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
     u64 slot_in = 0;
     u32 tag_in = 0xffffffffu;
     int fork_e = 0;
+    /* past the time budget: the wave's items stay pending (carried) */
+    const bool late = t_end && wall_clock64() >= t_end;
     if ((u64)lane < chunk && i < n_items) {
       int s = segment_of(sp, a.n_seg, (u32)base);
       while (sp[s + 1] <= i) ++s;
@@ -404,7 +411,10 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
       load_item<W>(in, a.stride, slot_in, known, up, mass);
       if (MODE != 0) tag_in = a.tag[parity][slot_in];
       const int cl = !mass_on ? 0 : MODE == 2 ? (int)tag_in : mass_class(mass);
-      if (mass_on && cl >= a.drop_class) { /* given up: queue over capacity */
+      if (late) {
+        carry = true;
+        stored = 1;
+      } else if (mass_on && cl >= a.drop_class) { /* given up: queue over capacity */
         accE.add(mass);
         ++c[C_FOLD];
       } else if (mass_on && cl > a.thr_class) { /* lighter: carried */
@@ -477,7 +487,7 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
         store_item<W>(out, a.stride, out_base + off, known, up, mass);
         ex.carry(in, slot_in, out, out_base + off, a.stride);
         if (MODE == 2) a.tag[parity ^ 1][out_base + off] = tag_in;
-        atomicAdd(&s_hist[MODE == 2 ? (int)tag_in : mass_class(mass)], 1u);
+        if (mass_on) atomicAdd(&s_hist[MODE == 2 ? (int)tag_in : mass_class(mass)], 1u);
       } else {
         accE.add(mass);
         ++c[C_FOLD];
@@ -592,8 +602,10 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
         ++c[C_PATHS];
       }
     } else if (MODE == 0 && EX::kRowBytes > 0) { /* children with rows: child-major */
-      emit_children_wave<W, EX>(ex, out, a.stride, out_base, a.seg_cap, wbase, incl, stored, known,
-                                up, mass, accE, c);
+      /* `late` is wave-uniform: a late wave only carries, and has no children */
+      if (!late)
+        emit_children_wave<W, EX>(ex, out, a.stride, out_base, a.seg_cap, wbase, incl, stored,
+                                  known, up, mass, accE, c);
       if (live) {
         accR.add(success_mass(ex, mass, k));
         ++c[C_PATHS];
