@@ -13,7 +13,8 @@ baseline (the oracle on the host cores, same workload) and the metric's
 second part: the certified gap ESP at a time budget (60 s by default) on
 K15 and K25 at delta = 1e-12, the plain method (path rule, level and
 max-mass order) against the alternating-sweep rule in the learned order
-(esp_at_budget).
+(esp_at_budget), and the same comparison on a 10x10 grid and a
+Barabasi-Albert graph at a 1 s budget (esp_sparse_graphs).
 """
 from __future__ import annotations
 
@@ -193,17 +194,10 @@ def run_batch(args, wl: dict) -> dict:
     }
 
 
-def esp_leg(budget_s: float = 60.0) -> dict:
-    """Certified gap ESP at a time budget on K15 and K25 (p = 0.5, capacity
-    96 777 216 items).  At delta = 1e-12 (the configured workload): the
-    plain GPU method (path rule) in level and max-mass order against the
-    alternating-sweep rule (sweep period picked by pilot solves at delta
-    1e-9) in level order and in the learned order.  At delta = 1e-15, where
-    the capacity binds and the order decides what is trimmed: the
-    alternating rule in level, max-mass and learned order.  The scorer is
-    trained here, on the GPU, on 2e6 children logged from K10 solves under
-    the alternating rule (delta 0, 1e-6, 1e-9); labels are log2 subtree
-    sizes."""
+def train_scorer():
+    """The learned scorer, trained here on the GPU on 2e6 children logged from
+    K10 solves under the alternating rule (delta 0, 1e-6, 1e-9); labels are
+    log2 subtree sizes.  Returns (model, training summary)."""
     import numpy as np
 
     import relaibilitygpu_amd as nr
@@ -221,17 +215,29 @@ def esp_leg(budget_s: float = 60.0) -> dict:
     t_log = time.perf_counter() - t0
     params, tl, hl = nr.mlp_train(X, Y, None, epochs=10, batch=4096, lr=3e-3, seed=3)
     t_train = time.perf_counter() - t0 - t_log
-    model = nr.Model(params, alpha=0.5)
+    info = {"records": int(len(Y)), "label_var": float(Y.var()), "holdout_mse": float(hl[-1]),
+            "log_s": t_log, "train_s": t_train, "alpha": 0.5, "rule": "alternate"}
+    return nr.Model(params, alpha=0.5), info
+
+
+def _row(r):
+    return {"ESP": r["ESP"], "R": r["R"], "nodes": r["nodes"], "elapsed_ms": r["elapsed_ms"],
+            "deadline_hit": r["deadline_hit"]}
+
+
+def esp_leg(model, budget_s: float = 60.0) -> dict:
+    """Certified gap ESP at a time budget on K15 and K25 (p = 0.5, capacity
+    96 777 216 items).  At delta = 1e-12 (the configured workload): the
+    plain GPU method (path rule) in level and max-mass order against the
+    alternating-sweep rule (sweep period picked by pilot solves at delta
+    1e-9) in level order and in the learned order.  At delta = 1e-15, where
+    the order could decide what the capacity trims: the alternating rule in
+    level, max-mass and learned order."""
+    import relaibilitygpu_amd as nr
+
+    G = os.path.join(ROOT, "tests", "golden", "graphs")
     cap = 96777216
-    out = {"budget_s": budget_s, "queue_capacity": cap,
-           "training": {"records": int(len(Y)), "label_var": float(Y.var()),
-                        "holdout_mse": float(hl[-1]), "log_s": t_log, "train_s": t_train,
-                        "alpha": 0.5, "rule": "alternate"}}
-
-    def row(r):
-        return {"ESP": r["ESP"], "R": r["R"], "nodes": r["nodes"],
-                "elapsed_ms": r["elapsed_ms"], "deadline_hit": r["deadline_hit"]}
-
+    out = {"budget_s": budget_s, "queue_capacity": cap}
     for name, n in (("K15", 15), ("K25", 25)):
         g = nr.Graph.load(os.path.join(G, f"COMP{n}-05.txt"), 0, n - 1)
         period, pilot = nr.tune_sweep_period(g, pilot_eps=1e-9, max_period=6)
@@ -247,7 +253,7 @@ def esp_leg(budget_s: float = 60.0) -> dict:
             if delta == 1e-15:
                 rows["alternate_mass"] = nr.solve(g, order="mass", **alt)
             rows["alternate_learned"] = nr.solve_learned(g, model, **alt)
-            res[f"delta_{delta:g}"] = {k: row(v) for k, v in rows.items()}
+            res[f"delta_{delta:g}"] = {k: _row(v) for k, v in rows.items()}
             print(f"esp {name} delta {delta:g} (period {period}): "
                   + ", ".join(f"{k} {v['ESP']:.3e} ({v['elapsed_ms']:.0f} ms)"
                               for k, v in rows.items()), file=sys.stderr, flush=True)
@@ -256,6 +262,48 @@ def esp_leg(budget_s: float = 60.0) -> dict:
         res["ESP_plain_heap_over_learned"] = (d12["plain_mass"]["ESP"]
                                               / d12["alternate_learned"]["ESP"])
         out[name] = res
+    return out
+
+
+def sparse_leg(model, budget_s: float = 1.0) -> dict:
+    """The scorer evaluated off its training graph: a 10x10 grid (180 edges,
+    p ~ U[0.85, 0.95] seeded, corner to corner) and Barabasi-Albert n = 150,
+    m = 2, p = 0.95 (0 -> 149), at delta = 1e-12 and an equal time budget:
+    plain path rule (level, max-mass) against the alternating rule (period
+    from pilot solves) in level, max-mass and learned order.  No closed form
+    exists; every bracket must contain R, so all brackets must intersect."""
+    import relaibilitygpu_amd as nr
+    from relaibilitygpu_amd import workloads as wl
+
+    rng = wl.Xoshiro256ss(0x6121D)
+    n, edges, _ = wl.grid_graph(10, 10, [0.0] * 1000)
+    probs = [0.85 + 0.10 * rng.uniform() for _ in edges]
+    graphs = [("grid10x10", n, edges, probs, 0, 99, 1e-6)]
+    n, edges, probs = wl.barabasi_albert(150, 2, 0.95, 0xBA150)
+    graphs.append(("ba150", n, edges, probs, 0, 149, 1e-9))
+    out = {"budget_s": budget_s, "delta": 1e-12, "queue_capacity": 96777216}
+    for name, n, edges, probs, s, t, pilot_eps in graphs:
+        rp, ci, ei = wl.to_csr(n, edges)
+        g = nr.Graph.from_csr(rp, ci, ei, probs, s, t)
+        period, _ = nr.tune_sweep_period(g, pilot_eps=pilot_eps, max_period=6)
+        kw = dict(truncation_eps=1e-12, queue_capacity=96777216, time_budget_s=budget_s)
+        alt = dict(kw, rule="alternate", sweep_period=period)
+        rows = {"plain_level": nr.solve(g, **kw), "plain_mass": nr.solve(g, order="mass", **kw),
+                "alternate_level": nr.solve(g, **alt),
+                "alternate_mass": nr.solve(g, order="mass", **alt),
+                "alternate_learned": nr.solve_learned(g, model, **alt)}
+        lo = max(r["R"] for r in rows.values())
+        hi = min(r["R"] + r["ESP"] for r in rows.values())
+        res = {"sweep_period": period, "rows": {k: _row(v) for k, v in rows.items()},
+               "brackets_intersect": bool(lo <= hi * (1 + 1e-12)),
+               "ESP_plain_over_learned": rows["plain_level"]["ESP"]
+               / rows["alternate_learned"]["ESP"],
+               "ESP_plain_heap_over_learned": rows["plain_mass"]["ESP"]
+               / rows["alternate_learned"]["ESP"]}
+        out[name] = res
+        print(f"sparse {name} (period {period}): "
+              + ", ".join(f"{k} {v['ESP']:.3e} ({v['elapsed_ms']:.0f} ms)"
+                          for k, v in rows.items()), file=sys.stderr, flush=True)
     return out
 
 
@@ -271,6 +319,8 @@ def main() -> int:
     ap.add_argument("--no-esp", action="store_true",
                     help="skip the learned-vs-plain certified-gap leg")
     ap.add_argument("--esp-budget", type=float, default=60.0)
+    ap.add_argument("--sparse-budget", type=float, default=1.0,
+                    help="time budget of the grid / Barabasi-Albert certified-gap leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -377,7 +427,10 @@ def main() -> int:
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(wl)
     if not args.no_esp:
-        line["esp_at_budget"] = esp_leg(args.esp_budget)
+        model, info = train_scorer()
+        line["esp_at_budget"] = esp_leg(model, args.esp_budget)
+        line["esp_at_budget"]["training"] = info
+        line["esp_sparse_graphs"] = sparse_leg(model, args.sparse_budget)
     print(json.dumps(line), flush=True)
     return 0
 
