@@ -62,7 +62,8 @@ typedef struct nr_config {
                                folded into ESP, keeping R <= R_true <= R+ESP */
   int order;                /* nr_order */
   int device;               /* HIP device ordinal */
-  int block_size;           /* threads per block; 0 = auto (256) */
+  int block_size;           /* threads per block: 64, 128 or 256; other
+                               values (0) = auto, the largest that fits LDS */
   int grid_blocks;          /* blocks per launch; 0 = auto from occupancy */
   int chunk_items;          /* persistent modes: items claimed per wave refill;
                                0 = auto */

@@ -114,6 +114,20 @@ struct DevGraph {
   int period; /* alternating sweep: flip every `period` FAILED edges */
 };
 
+/* The workgroup size as the stride of a lane's LDS slices ([slot][lane]):
+ * always a power of two, so i * stride is a shift (a 32-bit multiply is a
+ * multi-pass VALU instruction on CDNA). */
+struct Pow2 {
+  int sh = 0;
+  __host__ __device__ Pow2& operator=(int v) {
+    sh = __builtin_ctz((unsigned)v);
+    return *this;
+  }
+  __host__ __device__ operator int() const { return 1 << sh; }
+};
+__host__ __device__ inline int operator*(int i, Pow2 p) { return i << p.sh; }
+__host__ __device__ inline int operator*(Pow2 p, int i) { return i << p.sh; }
+
 /* One queue buffer in structure-of-arrays form: plane w of item i at
  * known[w * stride + i]; rows (when items carry them) likewise per node. */
 struct QBuf {
@@ -176,7 +190,8 @@ struct Expander {
   NS* rows; /* this lane's allowed-adjacency rows, stride `stride` */
   NS* lvl;  /* this lane's level sets, stride `stride` */
   u16* uh;  /* this lane's unknown-hop stack, stride `stride` */
-  int stride, n, src, dst;
+  Pow2 stride;
+  int n, src, dst;
   mutable int plen; /* hops of the last path found */
   const u16* terms;
   int nterms;
@@ -428,10 +443,12 @@ struct Expander {
     for (;;) {
       NS nx = 0;
       NS f = F;
-      while (f) {
-        const int u = ctzs<NS>(f);
+      while (f) { /* two rows per step: two LDS loads in flight */
+        const int u0 = ctzs<NS>(f);
         f &= f - 1;
-        nx |= rows[u * stride];
+        const int u1 = f ? ctzs<NS>(f) : u0;
+        f &= f - 1;
+        nx |= rows[u0 * stride] | rows[u1 * stride];
         if (nx & dbit) break;
       }
       nx &= ~vis;
@@ -647,7 +664,8 @@ struct CsrExpander {
   const u32* ep;
   unsigned char* lvb; /* this lane's level bytes: node u at (u>>2)*4*stride + (u&3) */
   u16* uh;            /* this lane's unknown-hop stack, stride `stride` */
-  int stride, n, src, dst;
+  Pow2 stride;
+  int n, src, dst;
   mutable int plen; /* hops of the last path found */
   const u16* terms;
   int nterms;
@@ -1142,7 +1160,8 @@ struct AltExpander : LeanExpander<W, NS> {
   }
 
   __device__ int find_alt(const u64 (&known)[W], const u64 (&up)[W]) const {
-    const int n = this->n, stride = this->stride;
+    const int n = this->n;
+    const Pow2 stride = this->stride;
     if (this->src == this->dst) {
       this->plen = 0;
       return 0;

@@ -934,8 +934,8 @@ Layout layout_for(const Variant& v, const nr_graph* g, int W, int block, int ext
  * a requested size is honoured when it fits. */
 int choose_block(const Variant& v, const nr_graph* g, int W, int requested, int extra = 0) {
   const int cands[4] = {requested, 256, 128, 64};
-  for (int b : cands) {
-    if (b <= 0 || b > 256 || b % 64) continue;
+  for (int b : cands) { /* 64, 128 or 256: LDS slices are strided by a power of two */
+    if (b != 64 && b != 128 && b != 256) continue;
     if (layout_for(v, g, W, b, extra).total_bytes <= 64 * 1024) return b;
   }
   return 0;
