@@ -326,68 +326,23 @@ __device__ inline void child_ctx(const LevelArgs& a, const EX& ex, const u64 (&k
  * rounds; MODE 1: level order that also writes the tree log; MODE 2: class
  * rounds on the learned score (children scored 64 at a time on the matrix
  * cores); MODE 3: the forkBest rule (two-way branch) in either order. */
+/* The items of one level (buf[parity], n_items of them, segment prefix sp)
+ * expanded into the workgroup's segment of buf[parity ^ 1]: the body of
+ * k_expand_level.  Sums and counts go to the caller's accumulators. */
 template <int W, class EX, int MODE>
-__global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, int level) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  /* class rounds: always in MODE 4 (max-mass order of the path rule), never
-   * in MODE 0 (level order) and MODE 1, by the argument in MODE 2 and 3 */
+__device__ __forceinline__ void expand_level_items(const LevelArgs& a, const EX& ex, int parity,
+                                                   int level, const u32* sp, u32 n_items,
+                                                   u32* s_cnt, u32* s_hist, u16* tile, u64 t_end,
+                                                   const bf16x8_t (&afrag)[2], u32 sign, float b2,
+                                                   float alpha, Acc2& accR, Acc2& accE,
+                                                   Acc2& accD, u32 (&c)[C_N]) {
   const bool mass_on = MODE == 4 || ((MODE == 2 || MODE == 3) && a.mass_mode);
-  u32* sp = (u32*)(smem + a.lay.off_extra);
-  u32* s_cnt = sp + a.n_seg + 1;
-  u32* part = s_cnt + 1;
-  block_prefix(a.seg_count[parity], a.n_seg, sp, part);
-  const u32 n_items = sp[a.n_seg];
-  const u64 t_end = a.budget_ticks ? a.ctl->start_tick + a.budget_ticks : 0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) { /* count the level (main.cu:970-973) */
-    a.ctl->last_items = n_items;
-    if (n_items && !(t_end && wall_clock64() >= t_end)) {
-      a.ctl->iterations += 1;
-      if (n_items > a.ctl->max_width) a.ctl->max_width = n_items;
-    }
-  }
-  if (n_items == 0) {
-    if (threadIdx.x == 0) a.seg_count[parity ^ 1][blockIdx.x] = 0;
-    if (mass_on)
-      for (int i = threadIdx.x; i < kClasses; i += blockDim.x)
-        a.hist[(u64)blockIdx.x * kClasses + i] = 0;
-    return;
-  }
-  if (threadIdx.x == 0) *s_cnt = 0;
-  u32* s_hist = part + blockDim.x; /* [kClasses], class rounds only */
-  /* per-wave feature tile [64][kFeat] bf16 (learned order), 16-byte aligned */
-  u16* tile = (u16*)(((size_t)(s_hist + kClasses) + 15) & ~(size_t)15) +
-              (threadIdx.x >> 6) * 64 * kFeat;
-  if (mass_on)
-    for (int i = threadIdx.x; i < kClasses; i += blockDim.x) s_hist[i] = 0;
-  if (EX::kStage) stage_tables(a.g, a.lay, smem);
-  __syncthreads();
-
-  EX ex;
-  ex.bind_graph(smem, a.lay, a.g, threadIdx.x);
-
   const QBuf in = a.buf[parity];
   const QBuf out = a.buf[parity ^ 1];
   const int lane = threadIdx.x & 63;
   const u64 wave_id = (u64)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const u64 n_waves = (u64)(gridDim.x * blockDim.x) >> 6;
   const u64 out_base = (u64)blockIdx.x * a.seg_cap;
-
-  bf16x8_t afrag[2];
-  u32 sign = 0;
-  float b2 = 0.0f, alpha = 0.0f;
-  if (MODE == 2) {
-#pragma unroll
-    for (int mt = 0; mt < 2; ++mt) afrag[mt] = *(const bf16x8_t*)a.mlp->afrag[mt][lane];
-    sign = a.mlp->sign[lane];
-    b2 = a.mlp->b2;
-    alpha = a.mlp->alpha;
-  }
-
-  Acc2 accR, accE, accD;
-  accR.init();
-  accE.init();
-  accD.init();
-  u32 c[C_N] = {0, 0, 0, 0, 0, 0, 0, 0};
 
   /* Small levels are spread over every wave (chunk < 64 lanes) so each
    * workgroup receives, and appends to its segment, a fair share. */
@@ -675,6 +630,68 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
       ++c[C_PATHS];
     }
   }
+
+}
+
+template <int W, class EX, int MODE>
+__global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, int level) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  /* class rounds: always in MODE 4 (max-mass order of the path rule), never
+   * in MODE 0 (level order) and MODE 1, by the argument in MODE 2 and 3 */
+  const bool mass_on = MODE == 4 || ((MODE == 2 || MODE == 3) && a.mass_mode);
+  u32* sp = (u32*)(smem + a.lay.off_extra);
+  u32* s_cnt = sp + a.n_seg + 1;
+  u32* part = s_cnt + 1;
+  block_prefix(a.seg_count[parity], a.n_seg, sp, part);
+  const u32 n_items = sp[a.n_seg];
+  const u64 t_end = a.budget_ticks ? a.ctl->start_tick + a.budget_ticks : 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) { /* count the level (main.cu:970-973) */
+    a.ctl->last_items = n_items;
+    if (n_items && !(t_end && wall_clock64() >= t_end)) {
+      a.ctl->iterations += 1;
+      if (n_items > a.ctl->max_width) a.ctl->max_width = n_items;
+    }
+  }
+  if (n_items == 0) {
+    if (threadIdx.x == 0) a.seg_count[parity ^ 1][blockIdx.x] = 0;
+    if (mass_on)
+      for (int i = threadIdx.x; i < kClasses; i += blockDim.x)
+        a.hist[(u64)blockIdx.x * kClasses + i] = 0;
+    return;
+  }
+  if (threadIdx.x == 0) *s_cnt = 0;
+  u32* s_hist = part + blockDim.x; /* [kClasses], class rounds only */
+  /* per-wave feature tile [64][kFeat] bf16 (learned order), 16-byte aligned */
+  u16* tile = (u16*)(((size_t)(s_hist + kClasses) + 15) & ~(size_t)15) +
+              (threadIdx.x >> 6) * 64 * kFeat;
+  if (mass_on)
+    for (int i = threadIdx.x; i < kClasses; i += blockDim.x) s_hist[i] = 0;
+  if (EX::kStage) stage_tables(a.g, a.lay, smem);
+  __syncthreads();
+
+  EX ex;
+  ex.bind_graph(smem, a.lay, a.g, threadIdx.x);
+
+  const int lane = threadIdx.x & 63;
+  bf16x8_t afrag[2];
+  u32 sign = 0;
+  float b2 = 0.0f, alpha = 0.0f;
+  if (MODE == 2) {
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) afrag[mt] = *(const bf16x8_t*)a.mlp->afrag[mt][lane];
+    sign = a.mlp->sign[lane];
+    b2 = a.mlp->b2;
+    alpha = a.mlp->alpha;
+  }
+
+  Acc2 accR, accE, accD;
+  accR.init();
+  accE.init();
+  accD.init();
+  u32 c[C_N] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+  expand_level_items<W, EX, MODE>(a, ex, parity, level, sp, n_items, s_cnt, s_hist, tile, t_end,
+                                  afrag, sign, b2, alpha, accR, accE, accD, c);
 
   /* per-workgroup flush; then publish this workgroup's segment count */
   __syncthreads();
