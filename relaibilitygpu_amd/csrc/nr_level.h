@@ -156,8 +156,26 @@ __device__ inline void flush_counts(u64* slot, const u32 (&c)[C_N]) {
  * block's double-double and counter slots (owned by this workgroup for the
  * launch; launches on one stream do not overlap).  `scratch` is LDS of at
  * least 80 bytes per wave. */
+/* The workgroup's slot values (three double-doubles, then C_N u64 counters:
+ * 28 dwords) loaded one dword per lane by lanes 0..27 of wave 0 when the
+ * launch starts, so block_flush() need not wait for them at the end. */
+__device__ inline u32 prefetch_slot(const LevelArgs& a) {
+  const int t = threadIdx.x;
+  if (t >= 4 * 3 + 2 * C_N) return 0u;
+  const u32* src = t < 4    ? (const u32*)&a.partR[blockIdx.x] + t
+                   : t < 8  ? (const u32*)&a.partE[blockIdx.x] + (t - 4)
+                   : t < 12 ? (const u32*)&a.partD[blockIdx.x] + (t - 8)
+                            : (const u32*)&a.cnt[(u64)blockIdx.x * C_N] + (t - 12);
+  return *src;
+}
+
+__device__ inline double dword_pair(u32 lo, u32 hi) {
+  return __longlong_as_double((long long)(((u64)hi << 32) | lo));
+}
+
 __device__ inline void block_flush(const LevelArgs& a, Acc2& accR, Acc2& accE, Acc2& accD,
-                                   const u32 (&c)[C_N], unsigned char* scratch) {
+                                   const u32 (&c)[C_N], unsigned char* scratch,
+                                   const u32* pre = nullptr) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int n_w = (blockDim.x + 63) >> 6;
   wave_sum_dd(accR.s, accR.c);
@@ -178,9 +196,16 @@ __device__ inline void block_flush(const LevelArgs& a, Acc2& accR, Acc2& accE, A
 #pragma unroll
     for (int j = 0; j < C_N; ++j) sc[wave * C_N + j] = t[j];
   }
+  /* prefetched slot values of wave 0, as wave-uniform scalars */
+  u32 pv[4 * 3 + 2 * C_N];
+  if (pre && wave == 0) {
+#pragma unroll
+    for (int i = 0; i < 4 * 3 + 2 * C_N; ++i) pv[i] = __builtin_amdgcn_readlane(*pre, i);
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     DD* dst[3] = {&a.partR[blockIdx.x], &a.partE[blockIdx.x], &a.partD[blockIdx.x]};
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
       double s = 0.0, cc = 0.0;
       for (int w = 0; w < n_w; ++w) {
@@ -189,7 +214,21 @@ __device__ inline void block_flush(const LevelArgs& a, Acc2& accR, Acc2& accE, A
         s = x;
         cc += sd[w * 6 + 2 * k + 1] + e;
       }
-      if (s != 0.0 || cc != 0.0) dd_merge_slot(dst[k], s, cc);
+      if (s != 0.0 || cc != 0.0) {
+        if (pre) { /* dd_merge_slot() on the prefetched value */
+          const double ohi = dword_pair(pv[4 * k], pv[4 * k + 1]);
+          const double olo = dword_pair(pv[4 * k + 2], pv[4 * k + 3]);
+          double t, e;
+          two_sum(ohi, s, t, e);
+          const double lo = olo + cc + e;
+          double hi2, e2;
+          two_sum(t, lo, hi2, e2);
+          dst[k]->hi = hi2;
+          dst[k]->lo = e2;
+        } else {
+          dd_merge_slot(dst[k], s, cc);
+        }
+      }
     }
     u32 tot[C_N];
     bool any = false;
@@ -200,7 +239,9 @@ __device__ inline void block_flush(const LevelArgs& a, Acc2& accR, Acc2& accE, A
     }
     if (any) {
       u64* slot = &a.cnt[(u64)blockIdx.x * C_N];
-      for (int j = 0; j < C_N; ++j) slot[j] += tot[j];
+#pragma unroll
+      for (int j = 0; j < C_N; ++j)
+        slot[j] = (pre ? ((u64)pv[12 + 2 * j + 1] << 32 | pv[12 + 2 * j]) : slot[j]) + tot[j];
     }
   }
 }
@@ -659,6 +700,7 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
         a.hist[(u64)blockIdx.x * kClasses + i] = 0;
     return;
   }
+  const u32 pre = prefetch_slot(a);
   if (threadIdx.x == 0) *s_cnt = 0;
   u32* s_hist = part + blockDim.x; /* [kClasses], class rounds only */
   /* per-wave feature tile [64][kFeat] bf16 (learned order), 16-byte aligned */
@@ -698,7 +740,7 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
   if (mass_on)
     for (int i = threadIdx.x; i < kClasses; i += blockDim.x)
       a.hist[(u64)blockIdx.x * kClasses + i] = s_hist[i];
-  block_flush(a, accR, accE, accD, c, (unsigned char*)part);
+  block_flush(a, accR, accE, accD, c, (unsigned char*)part, &pre);
   if (threadIdx.x == 0) {
     const u32 n = *s_cnt;
     a.seg_count[parity ^ 1][blockIdx.x] = n < a.seg_cap ? n : a.seg_cap;
