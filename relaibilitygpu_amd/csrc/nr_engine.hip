@@ -117,6 +117,11 @@ __global__ void __launch_bounds__(256) k_init(LevelArgs a, Root root, int W, int
     a.partD[i].hi = a.partD[i].lo = 0.0;
     for (int j = 0; j < C_N; ++j) a.cnt[i * C_N + j] = 0;
   }
+  if (blockIdx.x == 0) { /* the control block, zeroed by the whole workgroup */
+    static_assert(sizeof(LevelCtl) % 8 == 0, "LevelCtl is a whole number of words");
+    for (u32 i = threadIdx.x; i < sizeof(LevelCtl) / 8; i += blockDim.x) ((u64*)a.ctl)[i] = 0;
+    __syncthreads();
+  }
   const u64 ns = (u64)a.n_seg;
   for (u64 s = tid; s < ns; s += stride) {
     a.seg_count[0][s] =
@@ -125,8 +130,6 @@ __global__ void __launch_bounds__(256) k_init(LevelArgs a, Root root, int W, int
   }
   if (seed) { /* a resumed solve: sums and counters so far in slot 0 */
     if (tid == 0) {
-      LevelCtl z = {};
-      *a.ctl = z;
       a.ctl->start_tick = wall_clock64();
       a.ctl->last_items = seed->count;
       a.ctl->iterations = seed->iterations;
@@ -139,8 +142,6 @@ __global__ void __launch_bounds__(256) k_init(LevelArgs a, Root root, int W, int
     return;
   }
   if (tid == 0) {
-    LevelCtl z = {};
-    *a.ctl = z;
     a.ctl->start_tick = wall_clock64();
     a.ctl->last_items = 1;
     for (int w = 0; w < W; ++w) {
@@ -1322,6 +1323,10 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
       NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
       NR_HIP(hipStreamSynchronize(st));
       if (ws.hctl->last_items == 0) finished = true;
+      /* past the widest level: shorter chunks as the levels thin out, so
+       * few launches run after the last level */
+      const u64 w = ws.hctl->last_items, max_w = ws.hctl->max_width;
+      const bool shrinking = w < max_w;
       if (!finished && expired()) {
         if (save)
           pend_parity = level & 1;
@@ -1331,6 +1336,9 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
         finished = true;
       }
       chunk = std::min(chunk * 2, 16);
+      if (shrinking) chunk = std::min(chunk, 8);
+      if (shrinking && 4 * w < max_w) chunk = std::min(chunk, 4);
+      if (shrinking && 64 * w < max_w) chunk = std::min(chunk, 2);
       if (has_budget && !finished) chunk = budget_chunk(c0, launched, chunk);
     }
   } else {
