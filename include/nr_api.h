@@ -190,6 +190,16 @@ int nr_solve(const nr_graph* g, const nr_config* cfg, nr_result* out);
 int nr_solve_batch(const nr_graph* const* graphs, int count,
                    const nr_config* cfg, nr_result* results);
 
+/* Launch schedule of nr_solve_batch for these instances, picked by timing
+ * trial batches of the first `sample` instances (all of them when 0):
+ * workgroups of 256, 128 or 64 threads, 1, 2 or 4 workgroups per CU or the
+ * occupancy limit, the rest of cfg (or the defaults) unchanged.  *tuned
+ * receives cfg with block_size and grid_blocks of the fastest trial, and
+ * *best_rate (optional) its subproblems per second.  No counterpart in the
+ * reference (its launch shape is fixed, main.cu:882-948). */
+int nr_tune_batch(const nr_graph* const* graphs, int count, const nr_config* cfg,
+                  int sample, nr_config* tuned, double* best_rate);
+
 /* Expand `count` caller-supplied items once (one subproblem each) on the
  * GPU.  Item i has planes known[i*W..], up[i*W..] (W = mask_words) and mass
  * mass[i].  Outputs: nchild[i] (-1 dead), succ[i], and children in slots

@@ -178,6 +178,8 @@ def lib() -> ctypes.CDLL:
         P(ctypes.c_double), ctypes.c_int, P(ctypes.c_int), P(ctypes.c_double),
         P(ctypes.c_ulonglong), P(ctypes.c_ulonglong), P(ctypes.c_double),
     ]
+    L.nr_tune_batch.argtypes = [P(vp), ctypes.c_int, P(Config), ctypes.c_int, P(Config),
+                                P(ctypes.c_double)]
     L.nr_tune_sweep_period.argtypes = [vp, P(Config), ctypes.c_double, ctypes.c_int,
                                        P(ctypes.c_int), P(ctypes.c_double)]
     L.nr_status_str.argtypes = [ctypes.c_int]
@@ -346,6 +348,35 @@ def tune_sweep_period(graph: Graph, pilot_eps: float = 1e-9, max_period: int = 6
                                       int(max_period), ctypes.byref(best),
                                       _ptr(esp, ctypes.c_double)), "nr_tune_sweep_period")
     return best.value, [float(x) for x in esp]
+
+
+def tune_batch(graphs, sample: int = 0, config: Config | None = None, **kw):
+    """(tuned Config, subproblems/s of the best trial): nr_tune_batch."""
+    cfg = config if config is not None else make_config(**kw)
+    hs = (ctypes.c_void_p * len(graphs))(*[g.handle.value for g in graphs])
+    out = Config()
+    rate = ctypes.c_double()
+    _check(lib().nr_tune_batch(hs, len(graphs), ctypes.byref(cfg), int(sample), ctypes.byref(out),
+                               ctypes.byref(rate)), "nr_tune_batch")
+    return out, rate.value
+
+
+class Batch:
+    """A fixed list of instances solved repeatedly (nr_solve_batch): the
+    handle array and result buffer are built once, the library keeps the
+    instances' tables on the device between calls, and run() returns the
+    results as a numpy structured array (fields of nr_result)."""
+
+    def __init__(self, graphs, config: Config | None = None, **kw):
+        self.graphs = list(graphs)
+        self.config = config if config is not None else make_config(**kw)
+        self._arr = (ctypes.c_void_p * len(self.graphs))(*[g.handle.value for g in self.graphs])
+        self._res = (Result * len(self.graphs))()
+
+    def run(self) -> np.ndarray:
+        _check(lib().nr_solve_batch(self._arr, len(self.graphs), ctypes.byref(self.config),
+                                    self._res), "nr_solve_batch")
+        return np.ctypeslib.as_array(self._res).copy()
 
 
 def solve_batch(graphs, config: Config | None = None, **kw) -> list[dict]:

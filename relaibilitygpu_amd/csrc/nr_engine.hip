@@ -714,6 +714,14 @@ struct Workspace {
   size_t meta_bytes = 0;
   LevelCtl* ctl = nullptr;
   LevelCtl* hctl = nullptr; /* pinned */
+  /* device copy of the last batch's instance tables, roots and admission
+   * order, reused while the same graphs (pointer and stamp) come back */
+  void* btab = nullptr;
+  size_t btab_bytes = 0;
+  std::vector<std::pair<const nr_graph*, unsigned long long>> bkey;
+  int bW = 0;
+  std::vector<u32> bperm; /* the admission order on the host */
+  size_t bo_desc = 0, bo_tabs = 0, bo_roots = 0, bo_perm = 0, bo_perm2 = 0;
   std::vector<hipEvent_t> events;
   std::mutex mu;
 
@@ -757,6 +765,10 @@ struct Workspace {
     events.clear();
     if (q) (void)hipFree(q);
     if (tab) (void)hipFree(tab);
+    if (btab) (void)hipFree(btab);
+    btab = nullptr;
+    btab_bytes = 0;
+    bkey.clear();
     if (meta) (void)hipFree(meta);
     if (ctl) (void)hipFree(ctl);
     if (hctl) (void)hipHostFree(hctl);
@@ -1586,66 +1598,85 @@ int solve_batch(Workspace& ws, const nr_graph* const* gs, int count, const nr_co
   const u32 seg_cap = (u32)seg_cap64;
   if ((u64)count > kMaxBatch) return NR_ERR_ARG;
 
-  /* one buffer: desc | instance tables | roots | perm | perm2 */
-  std::vector<InstDesc> desc(count);
-  std::vector<unsigned char> tabs;
-  std::vector<u64> roots((size_t)count * 2 * W, 0);
-  for (int i = 0; i < count; ++i) {
-    const nr_graph* g = gs[i];
-    TableBlob tb;
-    pack_tables(g, W, tb);
-    const size_t base = (tabs.size() + 255) / 256 * 256;
-    if (base + tb.bytes.size() >= (1ull << 32)) return NR_ERR_ARG;
-    tabs.resize(base + tb.bytes.size(), 0);
-    std::memcpy(&tabs[base], tb.bytes.data(), tb.bytes.size());
-    InstDesc d;
-    std::memset(&d, 0, sizeof d);
-    d.n = g->n;
-    d.e = g->e;
-    d.src = g->src;
-    d.dst = g->dst;
-    d.off_p = (u32)(base + tb.off_p);
-    d.off_q = (u32)(base + tb.off_q);
-    d.off_sadj = (u32)(base + tb.off_sadj);
-    d.off_inc = (u32)(base + tb.off_inc);
-    d.off_ep = (u32)(base + tb.off_ep);
-    d.off_deg = (u32)(base + tb.off_deg);
-    d.off_eidm = (u32)(base + tb.off_eidm);
-    desc[i] = d;
-    Root r = root_item(g, mask_words(g));
-    u64* dst = &roots[(size_t)i * 2 * W];
-    for (int w = 0; w < W; ++w) {
-      dst[w] = w < mask_words(g) ? r.known[w] : 0;
-      dst[W + w] = w < mask_words(g) ? r.up[w] : 0;
-    }
-  }
-  /* admission order: larger instances first, so the longest subtrees start
-   * early and the tail of the stream holds the small ones */
-  std::vector<u32> perm(count);
-  for (int i = 0; i < count; ++i) perm[i] = (u32)i;
-  std::stable_sort(perm.begin(), perm.end(), [&](u32 x, u32 y) {
-    return (long long)gs[x]->e * gs[x]->n > (long long)gs[y]->e * gs[y]->n;
-  });
-  size_t o = 0;
-  auto take = [&](size_t bytes) {
-    o = (o + 255) / 256 * 256;
-    size_t at = o;
-    o += bytes;
-    return at;
-  };
-  const size_t o_desc = take(sizeof(InstDesc) * count), o_tabs = take(tabs.size()),
-               o_roots = take(8 * roots.size()), o_perm = take(4 * (size_t)count),
-               o_perm2 = take(4 * (size_t)count);
-  std::vector<unsigned char> blob(o, 0);
-  std::memcpy(&blob[o_desc], desc.data(), sizeof(InstDesc) * count);
-  std::memcpy(&blob[o_tabs], tabs.data(), tabs.size());
-  std::memcpy(&blob[o_roots], roots.data(), 8 * roots.size());
-  std::memcpy(&blob[o_perm], perm.data(), 4 * perm.size());
+  /* one buffer: desc | instance tables | roots | perm | perm2, packed and
+   * uploaded once for a given list of graphs */
   int rc;
-  if ((rc = ws.grow(&ws.tab, &ws.tab_bytes, blob.size()))) return rc;
   hipStream_t st = ws.stream;
-  NR_HIP(hipMemcpyAsync(ws.tab, blob.data(), blob.size(), hipMemcpyHostToDevice, st));
-  unsigned char* tb = (unsigned char*)ws.tab;
+  std::vector<std::pair<const nr_graph*, unsigned long long>> key((size_t)count);
+  for (int i = 0; i < count; ++i) key[(size_t)i] = {gs[i], gs[i]->stamp};
+  if (!(ws.btab && ws.bW == W && ws.bkey == key)) {
+    ws.bkey.clear();
+    /* one buffer: desc | instance tables | roots | perm | perm2 */
+    std::vector<InstDesc> desc(count);
+    std::vector<unsigned char> tabs;
+    std::vector<u64> roots((size_t)count * 2 * W, 0);
+    for (int i = 0; i < count; ++i) {
+      const nr_graph* g = gs[i];
+      TableBlob tb;
+      pack_tables(g, W, tb);
+      const size_t base = (tabs.size() + 255) / 256 * 256;
+      if (base + tb.bytes.size() >= (1ull << 32)) return NR_ERR_ARG;
+      tabs.resize(base + tb.bytes.size(), 0);
+      std::memcpy(&tabs[base], tb.bytes.data(), tb.bytes.size());
+      InstDesc d;
+      std::memset(&d, 0, sizeof d);
+      d.n = g->n;
+      d.e = g->e;
+      d.src = g->src;
+      d.dst = g->dst;
+      d.off_p = (u32)(base + tb.off_p);
+      d.off_q = (u32)(base + tb.off_q);
+      d.off_sadj = (u32)(base + tb.off_sadj);
+      d.off_inc = (u32)(base + tb.off_inc);
+      d.off_ep = (u32)(base + tb.off_ep);
+      d.off_deg = (u32)(base + tb.off_deg);
+      d.off_eidm = (u32)(base + tb.off_eidm);
+      desc[i] = d;
+      Root r = root_item(g, mask_words(g));
+      u64* dst = &roots[(size_t)i * 2 * W];
+      for (int w = 0; w < W; ++w) {
+        dst[w] = w < mask_words(g) ? r.known[w] : 0;
+        dst[W + w] = w < mask_words(g) ? r.up[w] : 0;
+      }
+    }
+    /* admission order: larger instances first, so the longest subtrees start
+     * early and the tail of the stream holds the small ones */
+    std::vector<u32> perm(count);
+    for (int i = 0; i < count; ++i) perm[i] = (u32)i;
+    std::stable_sort(perm.begin(), perm.end(), [&](u32 x, u32 y) {
+      return (long long)gs[x]->e * gs[x]->n > (long long)gs[y]->e * gs[y]->n;
+    });
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+      o = (o + 255) / 256 * 256;
+      size_t at = o;
+      o += bytes;
+      return at;
+    };
+    const size_t o_desc = take(sizeof(InstDesc) * count), o_tabs = take(tabs.size()),
+                 o_roots = take(8 * roots.size()), o_perm = take(4 * (size_t)count),
+                 o_perm2 = take(4 * (size_t)count);
+    std::vector<unsigned char> blob(o, 0);
+    std::memcpy(&blob[o_desc], desc.data(), sizeof(InstDesc) * count);
+    std::memcpy(&blob[o_tabs], tabs.data(), tabs.size());
+    std::memcpy(&blob[o_roots], roots.data(), 8 * roots.size());
+    std::memcpy(&blob[o_perm], perm.data(), 4 * perm.size());
+
+    if ((rc = ws.grow(&ws.btab, &ws.btab_bytes, blob.size()))) return rc;
+    NR_HIP(hipMemcpyAsync(ws.btab, blob.data(), blob.size(), hipMemcpyHostToDevice, st));
+    NR_HIP(hipStreamSynchronize(st)); /* the host blob goes out of scope */
+    ws.bkey = key;
+    ws.bW = W;
+    ws.bperm = perm;
+    ws.bo_desc = o_desc;
+    ws.bo_tabs = o_tabs;
+    ws.bo_roots = o_roots;
+    ws.bo_perm = o_perm;
+    ws.bo_perm2 = o_perm2;
+  }
+  const size_t o_desc = ws.bo_desc, o_tabs = ws.bo_tabs, o_roots = ws.bo_roots,
+               o_perm = ws.bo_perm, o_perm2 = ws.bo_perm2;
+  unsigned char* tb = (unsigned char*)ws.btab;
 
   BatchArgs a;
   std::memset(&a, 0, sizeof a);
@@ -1704,7 +1735,7 @@ int solve_batch(Workspace& ws, const nr_graph* const* gs, int count, const nr_co
   std::vector<char> admitted(count, 0), rerun(count, 0);
   bool deadline = false;
   long long launches = 0, max_width = 0;
-  std::vector<u32> todo(perm);
+  std::vector<u32> todo(ws.bperm);
   /* pass 0 streams every instance; an instance that still overflowed (spill
    * stack full) is solved again in a pass of its own */
   for (int pass = 0; !todo.empty() && !deadline; ++pass) {
@@ -2285,6 +2316,50 @@ int nr_log_subproblems(const nr_graph* g, const nr_config* cfg, long long max_re
   nr::WsLock L(cfg->device);
   if (L.rc) return L.rc;
   return nr::log_subproblems(*L.ws, g, cfg, max_records, seed, features, labels, n_out, result);
+}
+
+int nr_tune_batch(const nr_graph* const* graphs, int count, const nr_config* cfg, int sample,
+                  nr_config* tuned, double* best_rate) {
+  if (!graphs || count <= 0 || !tuned || sample < 0) return NR_ERR_ARG;
+  nr_config c;
+  if (cfg)
+    c = *cfg;
+  else
+    nr_config_init(&c);
+  const int n = sample > 0 && sample < count ? sample : count;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, c.device) != hipSuccess) return NR_ERR_DEVICE;
+  const int cus = prop.multiProcessorCount;
+  std::vector<nr_result> res((size_t)n);
+  *tuned = c;
+  double best = -1.0;
+  const int blocks[3] = {256, 128, 64};
+  const int per_cu[4] = {0, 1, 2, 4}; /* 0: the occupancy limit */
+  for (int b : blocks)
+    for (int m : per_cu) {
+      nr_config t = c;
+      t.block_size = b;
+      t.grid_blocks = m * cus;
+      double rate = 0.0;
+      int rc = NR_OK;
+      for (int rep = 0; rep < 2 && rc == NR_OK; ++rep) { /* the first run warms up */
+        rc = nr_solve_batch(graphs, n, &t, res.data());
+        if (rc == NR_OK && rep == 1) {
+          double nodes = 0.0;
+          for (const nr_result& r : res) nodes += (double)r.nodes;
+          rate = res[0].elapsed_ms > 0 ? nodes / (res[0].elapsed_ms * 1e-3) : 0.0;
+        }
+      }
+      if (rc == NR_ERR_UNSUPPORTED || rc == NR_ERR_NOMEM) continue; /* does not fit */
+      if (rc) return rc;
+      if (rate > best) {
+        best = rate;
+        *tuned = t;
+      }
+    }
+  if (best < 0) return NR_ERR_UNSUPPORTED;
+  if (best_rate) *best_rate = best;
+  return NR_OK;
 }
 
 int nr_solve_batch(const nr_graph* const* graphs, int count, const nr_config* cfg,

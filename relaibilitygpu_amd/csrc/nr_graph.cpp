@@ -14,6 +14,7 @@
  * (graph.h:132-140 keeps float logs instead).
  */
 #include <algorithm>
+#include <atomic>
 #include <cerrno>
 #include <cstdio>
 #include <cstdlib>
@@ -24,6 +25,11 @@
 #include "nr_internal.h"
 
 namespace nr {
+
+unsigned long long next_stamp() {
+  static std::atomic<unsigned long long> counter{0};
+  return ++counter;
+}
 
 namespace {
 
@@ -238,6 +244,7 @@ int nr_graph_set_terminals(nr_graph* g, int src, int dst) {
   g->src = src;
   g->dst = dst;
   g->terms.clear(); /* back to the two-terminal problem */
+  g->stamp = nr::next_stamp();
   return NR_OK;
 }
 
@@ -279,6 +286,7 @@ int nr_graph_set_terminal_set(nr_graph* g, const int* terminals, int k) {
     g->src = terminals[0];
     g->dst = terminals[1];
   }
+  g->stamp = nr::next_stamp();
   return NR_OK;
 }
 

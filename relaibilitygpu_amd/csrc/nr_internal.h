@@ -26,6 +26,9 @@ struct nr_graph {
   bool symmetric = true;
   int precontracted = 0, predeleted = 0;
   std::vector<int> terms; /* K-terminal set (empty: the src/dst pair) */
+  /* unique per state: new at creation and at every change of terminals
+   * (keys the device copy of a batch's instance tables) */
+  unsigned long long stamp = 0;
 };
 
 namespace nr {
@@ -33,6 +36,7 @@ namespace nr {
 int load_graph_text(const char* text, size_t len, nr_graph* g);
 int finish_graph(nr_graph* g);
 inline int mask_words(const nr_graph* g) { return g->e > 0 ? (g->e + 63) / 64 : 1; }
+unsigned long long next_stamp();
 
 }  // namespace nr
 

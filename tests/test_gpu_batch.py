@@ -148,3 +148,26 @@ def test_batch_rejects():
     with pytest.raises(nr.NRError):
         nr.solve_batch([g], order="mass")
     assert nr.solve_batch([], truncation_eps=0.0) == []
+
+
+def test_batch_reuses_device_tables_until_a_graph_changes():
+    """The library keeps a batch's instance tables on the device between calls
+    with the same graphs; changing an instance's terminals (or passing other
+    graphs) must rebuild them: every result equals a fresh single solve."""
+    print("case batch table cache", flush=True)
+    pairs = [er_pair(i) for i in range(40)]
+    graphs = [p[0] for p in pairs]
+    batch = nr.Batch(graphs, truncation_eps=1e-9, queue_capacity=1 << 22)
+    first = batch.run()
+    again = batch.run()
+    assert (first["nodes"] == again["nodes"]).all() and (first["paths"] == again["paths"]).all()
+    g0, o0, s0, t0 = pairs[0]
+    n0 = g0.info().n
+    g0.set_terminals(t0, (s0 + 1) % n0 if (s0 + 1) % n0 != t0 else (s0 + 2) % n0)
+    changed = batch.run()
+    single = nr.solve(g0, truncation_eps=1e-9, queue_capacity=1 << 22)
+    assert changed["nodes"][0] == single["nodes"] and changed["paths"][0] == single["paths"]
+    assert close(changed["R"][0], single["R"])
+    assert (changed["nodes"][1:] == first["nodes"][1:]).all()
+    other = nr.solve_batch(graphs[5:25], truncation_eps=1e-9, queue_capacity=1 << 22)
+    assert [r["nodes"] for r in other] == [int(x) for x in first["nodes"][5:25]]
