@@ -1112,6 +1112,160 @@ void orc_graph_arrays(const orc_graph* g, int* row_ptr, int* col_idx, int* edge_
   if (p) std::copy(g->p.begin(), g->p.end(), p);
 }
 
+/* ------------------------------------------------------------------ */
+/* Graphs beyond the fixed node-set and plane sizes (more than 256 nodes */
+/* or 1024 edges): the same path rule and level order with per-node      */
+/* level arrays and planes of any width.  Path mode, level order only.   */
+/* ------------------------------------------------------------------ */
+namespace {
+
+struct BigEngine {
+  const orc_graph* g;
+  int src, dst, W;
+  std::vector<int> lev, cur, nxt;
+  bool allowed(const u64* known, const u64* up, int e) const {
+    const u64 b = 1ull << (e & 63);
+    return !((known[e >> 6] & b) && !(up[e >> 6] & b));
+  }
+  /* BFS from src over allowed arcs, whole levels, until the level holding
+   * dst (device_code.cuh:80-194); the walk back takes the lowest-index
+   * node of the previous level with an allowed arc, lowest edge id on ties
+   * (in-arc rows are sorted by (u, eid)).  Returns the path length (hops in
+   * src->dst order) or -1 when dst is unreachable. */
+  int find(const u64* known, const u64* up, std::vector<int>& hops) {
+    const int n = g->n;
+    if (src == dst) return 0;
+    lev.assign(n, -1);
+    cur.assign(1, src);
+    lev[src] = 0;
+    int L = 0;
+    while (lev[dst] < 0) {
+      nxt.clear();
+      for (int u : cur)
+        for (int j = g->row_ptr[u]; j < g->row_ptr[u + 1]; ++j) {
+          const int v = g->col_idx[j];
+          if (lev[v] < 0 && allowed(known, up, g->edge_id[j])) {
+            lev[v] = L + 1;
+            nxt.push_back(v);
+          }
+        }
+      if (nxt.empty()) return -1;
+      ++L;
+      cur.swap(nxt);
+    }
+    hops.assign(L, -1);
+    int v = dst;
+    for (int l = L; l >= 1; --l) {
+      int pu = -1, pe = -1;
+      for (int k = g->in_ptr[v]; k < g->in_ptr[v + 1]; ++k)
+        if (lev[g->in_src[k]] == l - 1 && allowed(known, up, g->in_eid[k])) {
+          pu = g->in_src[k];
+          pe = g->in_eid[k];
+          break;
+        }
+      if (pu < 0) return -1; /* not reached: every levelled node has a parent */
+      hops[l - 1] = pe;
+      v = pu;
+    }
+    return L;
+  }
+};
+
+void run_level_big(const orc_graph* g, int src, int dst, const orc_config& cfg, orc_result* r,
+                   Acc& acc) {
+  const int W = g->w;
+  BigEngine en{g, src, dst, W, {}, {}, {}};
+  std::vector<u64> cur, nxt; /* items: known[W], up[W] */
+  std::vector<double> cm, nm;
+  cur.resize(2 * (size_t)W);
+  {
+    std::vector<u64> k0, u0;
+    initial_item(g, k0, u0);
+    std::copy(k0.begin(), k0.end(), cur.begin());
+    std::copy(u0.begin(), u0.end(), cur.begin() + W);
+  }
+  cm.assign(1, 1.0);
+  const size_t cap = cfg.capacity > 0 ? (size_t)cfg.capacity : (size_t)-1;
+  const double eps = cfg.trunc_eps;
+  Deadline dl(cfg.time_budget_s);
+  std::vector<int> hops;
+  std::vector<u64> pre(W);
+  long long iters = 0, maxw = 0;
+  bool stop = false;
+  while (!cm.empty() && !stop) {
+    ++iters;
+    maxw = std::max(maxw, (long long)cm.size());
+    nxt.clear();
+    nm.clear();
+    size_t i = 0;
+    for (; i < cm.size(); ++i) {
+      if ((i & 255) == 0 && dl.hit()) {
+        stop = true;
+        break;
+      }
+      const u64* known = &cur[i * 2 * W];
+      const u64* up = known + W;
+      const double mass = cm[i];
+      acc.nodes++;
+      if (mass < eps) {
+        acc.ESP.add(mass);
+        acc.truncated++;
+        continue;
+      }
+      const int L = en.find(known, up, hops);
+      if (L < 0) {
+        acc.D.add(mass);
+        acc.dead++;
+        continue;
+      }
+      std::fill(pre.begin(), pre.end(), 0ull);
+      double run = mass;
+      for (int h = 0; h < L; ++h) {
+        const int e = hops[h];
+        const u64 bit = 1ull << (e & 63);
+        if (known[e >> 6] & bit) continue; /* a WORKING hop */
+        const double c = run * g->q[e];
+        if (nm.size() < cap) {
+          const size_t o = nxt.size();
+          nxt.resize(o + 2 * (size_t)W);
+          for (int w = 0; w < W; ++w) {
+            nxt[o + w] = known[w] | pre[w];
+            nxt[o + W + w] = up[w] | pre[w];
+          }
+          nxt[o + (e >> 6)] |= bit;
+          nm.push_back(c);
+        } else { /* main.cu:245-252 */
+          acc.ESP.add(c);
+          acc.paths++;
+          acc.overflowed++;
+        }
+        run = run * g->p[e];
+        pre[e >> 6] |= bit;
+      }
+      acc.R.add(run);
+      acc.paths++;
+    }
+    if (stop) { /* fold what is left of this level and the next */
+      for (size_t j = i; j < cm.size(); ++j) {
+        acc.ESP.add(cm[j]);
+        acc.folded++;
+      }
+      for (double m : nm) {
+        acc.ESP.add(m);
+        acc.folded++;
+      }
+      r->deadline_hit = 1;
+      break;
+    }
+    cur.swap(nxt);
+    cm.swap(nm);
+  }
+  r->iterations = iters;
+  r->max_width = maxw;
+}
+
+}  // namespace
+
 static int check_common(const orc_graph* g, int src, int dst) {
   if (!g) return ORC_ERR_ARG;
   if (src < 0 || src >= g->n || dst < 0 || dst >= g->n) return ORC_ERR_RANGE;
@@ -1122,6 +1276,27 @@ static int check_common(const orc_graph* g, int src, int dst) {
 int orc_solve(const orc_graph* g, int src, int dst, const orc_config* cfg,
               orc_result* out) {
   int rc = check_common(g, src, dst);
+  if (rc == ORC_ERR_SIZE && cfg && out && cfg->mode == ORC_MODE_PATH &&
+      cfg->order == ORC_ORDER_LEVEL) { /* beyond the fixed sizes */
+    std::memset(out, 0, sizeof(*out));
+    Acc acc;
+    auto t0 = Clock::now();
+    run_level_big(g, src, dst, *cfg, out, acc);
+    auto t1 = Clock::now();
+    out->elapsed_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    out->R = acc.R.value();
+    out->ESP = acc.ESP.value();
+    out->D = acc.D.value();
+    out->paths = acc.paths;
+    out->nodes = acc.nodes;
+    out->dead = acc.dead;
+    out->truncated = acc.truncated;
+    out->overflowed = acc.overflowed;
+    out->folded = acc.folded;
+    out->n = g->n;
+    out->e = g->e;
+    return ORC_OK;
+  }
   if (rc) return rc;
   if (!cfg || !out) return ORC_ERR_ARG;
   std::memset(out, 0, sizeof(*out));

@@ -37,6 +37,7 @@
 
 #include "nr_device.h"
 #include "nr_internal.h"
+#include "nr_big.h"
 #include "nr_level.h"
 
 namespace nr {
@@ -1060,6 +1061,207 @@ struct LevelExtra {
   const nr_checkpoint* resume = nullptr; /* start from these pending items */
   nr_checkpoint** save = nullptr;        /* on deadline keep them here */
 };
+
+/* The path rule in level order on graphs past the per-lane forms (more than
+ * 256 nodes or 512 edges): one wave per subproblem (nr_big.hip).  Same
+ * counting, capacity, truncation and time-budget semantics as the level
+ * kernel; items are planes of W words stored item-major. */
+int solve_big(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_result* out) {
+  if (g->n > 65535 || g->e > 65535) return NR_ERR_UNSUPPORTED; /* arcs pack node | edge << 16 */
+  if (!g->terms.empty() || cfg->rule != NR_RULE_PATH || cfg->order != NR_ORDER_LEVEL)
+    return NR_ERR_UNSUPPORTED;
+  const int W = mask_words(g);
+  const int cus = ws.prop.multiProcessorCount;
+  const void* kfun = big_expand_kernel();
+  NR_HIP(hipFuncSetAttribute(kfun, hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
+  /* workgroups of 4, 2 or 1 waves, the most whose slices fit in 64 KiB
+   * beside the segment prefix table */
+  int block = 0, grid = 0, per_cu = 0;
+  Layout lay;
+  BigShape bs;
+  for (int waves = 4; waves >= 1 && !block; waves /= 2) {
+    const int b = 64 * waves;
+    for (int pass = 0; pass < 2; ++pass) { /* size the grid, then fit its table */
+      const int g0 = pass ? grid : 4 * cus;
+      const int extra = 4 * (g0 + 2 + b + kClasses);
+      bs = big_shape(g->n, W, extra);
+      const int total = bs.off_waves + waves * bs.wave_bytes;
+      if (total > 64 * 1024) break;
+      int pc = 0;
+      NR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, kfun, b, total));
+      if (pc < 1) break;
+      if (pass == 0) {
+        grid = cfg->grid_blocks > 0 ? cfg->grid_blocks : pc * cus;
+        continue;
+      }
+      block = b;
+      per_cu = pc;
+      std::memset(&lay, 0, sizeof lay);
+      lay.n = g->n;
+      lay.e = g->e;
+      lay.w = W;
+      lay.block = b;
+      lay.off_extra = 0;
+      lay.total_bytes = total;
+    }
+  }
+  if (!block) return NR_ERR_UNSUPPORTED;
+  const u64 n_slots = (u64)grid;
+  const u64 cap = cfg->queue_capacity > 0 ? (u64)cfg->queue_capacity : (1ull << 20);
+  const u64 seg_cap64 = std::max<u64>(1, (cap + grid - 1) / grid);
+  if (seg_cap64 * (u64)grid >= (1ull << 32)) return NR_ERR_ARG;
+  const u32 seg_cap = (u32)seg_cap64;
+  const u64 slots = seg_cap64 * (u64)grid;
+  int rc;
+  LevelArgs a;
+  std::memset(&a, 0, sizeof a);
+  if ((rc = upload_tables(ws, g, W, a.g))) return rc;
+  a.lay = lay;
+  a.seg_cap = seg_cap;
+  a.n_seg = grid;
+  a.stride = slots;
+  a.eps = cfg->truncation_eps > 0 ? cfg->truncation_eps : 0.0;
+  a.budget_ticks = 0;
+  if (cfg->time_budget_s > 0) {
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ws.device) == hipSuccess &&
+        khz > 0)
+      a.budget_ticks = std::max<u64>(1, (u64)(cfg->time_budget_s * 1e3 * (double)khz));
+  }
+  /* buffers: planes [slots][2W] u64, then masses [slots] */
+  const size_t per_buf = ((size_t)2 * W * slots + slots) * 8;
+  if ((rc = ws.grow(&ws.q, &ws.q_bytes, 2 * per_buf + 16 * (size_t)W))) return rc;
+  for (int b = 0; b < 2; ++b) {
+    u64* base = (u64*)((unsigned char*)ws.q + b * per_buf);
+    a.buf[b].known = base;
+    a.buf[b].up = nullptr;
+    a.buf[b].mass = (double*)(base + (size_t)2 * W * slots);
+    a.buf[b].rows = nullptr;
+  }
+  u64* droot = (u64*)((unsigned char*)ws.q + 2 * per_buf);
+  const size_t meta_bytes = n_slots * (3 * sizeof(DD) + C_N * sizeof(u64)) +
+                            2 * sizeof(u32) * (2 * (size_t)grid + 4);
+  if ((rc = ws.grow(&ws.meta, &ws.meta_bytes, meta_bytes))) return rc;
+  {
+    unsigned char* m = (unsigned char*)ws.meta;
+    a.partR = (DD*)m;
+    a.partE = a.partR + n_slots;
+    a.partD = a.partE + n_slots;
+    a.cnt = (u64*)(a.partD + n_slots);
+    u32* q = (u32*)(a.cnt + n_slots * C_N);
+    a.seg_count[0] = q;
+    a.seg_count[1] = q + grid;
+  }
+  a.ctl = ws.ctl;
+  hipStream_t st = ws.stream;
+  {
+    /* the root (main.cu:717-748): p <= 0 FAILED, p >= 1 WORKING */
+    std::vector<u64> root((size_t)2 * W, 0);
+    for (int e = 0; e < g->e; ++e) {
+      const u64 bit = 1ull << (e & 63);
+      if (g->p[e] <= 0.0) {
+        root[e >> 6] |= bit;
+      } else if (g->p[e] >= 1.0) {
+        root[e >> 6] |= bit;
+        root[W + (e >> 6)] |= bit;
+      }
+    }
+    NR_HIP(hipMemcpyAsync(droot, root.data(), 16 * (size_t)W, hipMemcpyHostToDevice, st));
+    NR_HIP(hipStreamSynchronize(st));
+  }
+  hipEvent_t ev_start, ev_stop;
+  if ((rc = ws.event(0, &ev_start)) || (rc = ws.event(1, &ev_stop))) return rc;
+  const size_t max_timed = (cfg->flags & NR_FLAG_TIME_LAUNCHES) ? 2048 : 0;
+  auto t_begin = std::chrono::steady_clock::now();
+  NR_HIP(hipEventRecord(ev_start, st));
+  {
+    const u64* dr = droot;
+    int Wi = W;
+    u64 ns = n_slots;
+    void* args[] = {&a, &dr, &Wi, &ns};
+    NR_HIP(hipLaunchKernel(big_init_kernel(), dim3(64), dim3(256), args, 0, st));
+  }
+  long long timed = 0;
+  int level = 0, chunk = cfg->time_budget_s > 0 ? 1 : 8;
+  bool finished = false, deadline = false;
+  auto expired = [&]() {
+    return cfg->time_budget_s > 0 &&
+           std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count() >=
+               cfg->time_budget_s;
+  };
+  while (!finished) {
+    for (int c = 0; c < chunk; ++c, ++level) {
+      hipEvent_t kb = nullptr, ke = nullptr;
+      if ((size_t)timed < max_timed) {
+        if ((rc = ws.event(2 + 2 * timed, &kb)) || (rc = ws.event(3 + 2 * timed, &ke))) return rc;
+        NR_HIP(hipEventRecord(kb, st));
+      }
+      int par = level & 1;
+      void* args[] = {&a, &par, &bs};
+      NR_HIP(hipLaunchKernel(kfun, dim3(grid), dim3(block), args, lay.total_bytes, st));
+      if (kb) {
+        NR_HIP(hipEventRecord(ke, st));
+        ++timed;
+      }
+    }
+    NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
+    NR_HIP(hipStreamSynchronize(st));
+    if (ws.hctl->counts[C_ERR] || ws.hctl->last_items == 0) finished = true;
+    if (!finished && expired()) {
+      int par = level & 1;
+      void* args[] = {&a, &par};
+      NR_HIP(hipLaunchKernel((const void*)&k_fold, dim3(grid), dim3(256), args,
+                             4 * (grid + 2 + 256), st));
+      deadline = true;
+      finished = true;
+    }
+    chunk = std::min(chunk * 2, 16);
+    if (ws.hctl->last_items < ws.hctl->max_width) chunk = std::min(chunk, 4);
+    if (cfg->time_budget_s > 0) chunk = 1;
+  }
+  {
+    u64 nw = n_slots;
+    void* args[] = {&a, &nw};
+    NR_HIP(hipLaunchKernel((const void*)&k_finalize, dim3(1), dim3(256), args, 0, st));
+  }
+  NR_HIP(hipEventRecord(ev_stop, st));
+  NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
+  NR_HIP(hipStreamSynchronize(st));
+  float ms = 0.f;
+  NR_HIP(hipEventElapsedTime(&ms, ev_start, ev_stop));
+  const LevelCtl& h = *ws.hctl;
+  double kms = 0.0;
+  for (long long i = 0; i < timed; ++i) {
+    float xm = 0.f;
+    NR_HIP(hipEventElapsedTime(&xm, ws.events[2 + 2 * i], ws.events[3 + 2 * i]));
+    kms += xm;
+  }
+  if (h.counts[C_ERR]) return NR_ERR_UNSUPPORTED; /* a path longer than 254 hops */
+  std::memset(out, 0, sizeof(*out));
+  out->R = h.sum[0];
+  out->ESP = h.sum[1];
+  out->D = h.sum[2];
+  out->paths = h.counts[C_PATHS];
+  out->nodes = h.counts[C_NODES];
+  out->dead = h.counts[C_DEAD];
+  out->truncated = h.counts[C_TRUNC];
+  out->overflowed = h.counts[C_OVER];
+  out->folded = h.counts[C_FOLD];
+  out->iterations = (long long)h.iterations;
+  out->max_queue_items = (long long)h.max_width;
+  out->item_bytes = 16LL * W + 8;
+  out->elapsed_ms = ms;
+  out->kernel_ms = kms;
+  out->kernel_launches = timed;
+  out->deadline_hit = deadline ? 1 : 0;
+  out->overflowed_flag = h.counts[C_OVER] ? 1 : 0;
+  out->grid_blocks = grid;
+  out->block_size = block;
+  out->lds_bytes = lay.total_bytes;
+  out->blocks_per_cu = per_cu;
+  out->compute_units = cus;
+  return NR_OK;
+}
 
 int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_result* out,
                 const LevelExtra* x = nullptr) {
@@ -2134,6 +2336,11 @@ int nr_solve(const nr_graph* g, const nr_config* cfg, nr_result* out) {
   if (cfg->sweep_period < 0) return NR_ERR_ARG;
   nr::Variant var;
   int rc = nr::pick_variant(g, var);
+  if (rc == NR_ERR_UNSUPPORTED) { /* past the per-lane forms: one wave per item */
+    nr::WsLock L(cfg->device);
+    if (L.rc) return L.rc;
+    return nr::solve_big(*L.ws, g, cfg, out);
+  }
   if (rc) return rc;
   nr::WsLock L(cfg->device);
   if (L.rc) return L.rc;
