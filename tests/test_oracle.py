@@ -511,3 +511,66 @@ def test_alternate_detects_a_longer_path():
         if d is not None and k >= 0 and k != d:
             differs += 1
     assert differs > 0
+
+
+# ------------------------------------------- graphs past the fixed sizes
+def test_big_graph_oracle_matches_the_fixed_size_engine(expected):
+    """K8 embedded in a 300-node graph (292 isolated extra nodes, so the
+    any-size restatement runs): the same counts, levels, widest level and R
+    as the fixed-size engine on K8, whose counts the reference logs pin."""
+    small = load("COMP8.txt")
+    rp, ci, ei, p = small.arrays()
+    n = 300
+    rp_big = list(rp) + [rp[-1]] * (n - 8)
+    g = OracleGraph.from_csr(rp_big, ci, ei, p)
+    assert g.dims()["n"] == n
+    a = g.solve(0, 7)
+    b = small.solve(0, 7)
+    for k in ("paths", "nodes", "dead", "truncated", "iterations", "max_width"):
+        assert a[k] == b[k], k
+    assert abs(a["R"] - closed_form(expected, 8, "0.5")) <= 1e-14
+    # a renumbering that puts the terminals far apart in index order
+    perm = list(range(n))
+    perm[0], perm[250] = 250, 0
+    perm[7], perm[299] = 299, 7
+    edges = []
+    for u in range(8):
+        for j in range(rp[u], rp[u + 1]):
+            v = ci[j]
+            if u < v:
+                edges.append((perm[u], perm[v], p[ei[j]]))
+    rows = [[] for _ in range(n)]
+    for i, (x, y, _) in enumerate(edges):
+        rows[x].append((y, i))
+        rows[y].append((x, i))
+    rp2, ci2, ei2 = [0], [], []
+    for u in range(n):
+        for v, i in sorted(rows[u]):
+            ci2.append(v)
+            ei2.append(i)
+        rp2.append(len(ci2))
+    g2 = OracleGraph.from_csr(rp2, ci2, ei2, [e[2] for e in edges])
+    c = g2.solve(250, 299)
+    assert abs(c["R"] - b["R"]) <= 1e-14 and c["paths"] + c["dead"] == c["nodes"]
+
+
+def test_big_graph_oracle_series_and_bundle():
+    """Closed forms on 300-node graphs: a series path (R = prod p, one
+    success, n nodes) and a parallel bundle of ten 2-hop paths with isolated
+    padding (R = 1 - prod(1 - p_a p_b))."""
+    rnd = random.Random(9)
+    n = 300
+    edges = [(i, i + 1) for i in range(n - 1)]
+    p = [rnd.uniform(0.97, 0.999) for _ in edges]
+    g = OracleGraph.from_csr(*_csr(n, edges), p)
+    r = g.solve(0, n - 1)
+    assert abs(r["R"] - math.prod(p)) <= 1e-14 and r["paths"] == 1 and r["nodes"] == n
+    k = 10  # ten 2-hop paths; 288 isolated nodes make the graph big
+    edges = []
+    for m in range(1, k + 1):
+        edges += [(0, m), (m, k + 1)]
+    p = [rnd.uniform(0.3, 0.6) for _ in edges]
+    g = OracleGraph.from_csr(*_csr(n, edges), p)
+    r = g.solve(0, k + 1)
+    want = 1.0 - math.prod(1 - p[2 * i] * p[2 * i + 1] for i in range(k))
+    assert abs(r["R"] - want) <= 1e-14 and r["ESP"] == 0
