@@ -163,10 +163,23 @@ def test_single_expansions_bit_exact(name, s, t, count):
     assert alive > count // 10
 
 
-def test_unsupported_shapes_fail_loudly():
-    g = nr.Graph.load(graph_path("COMP40.txt"), 0, 39)  # 780 edges > 512
+def test_wide_graph_runs_one_wave_per_item(expected):
+    """K40 (780 edges, past the per-lane forms) runs in the wave-per-item
+    form: counts, levels and widest level equal the oracle's fixed-size
+    engine; orders the form lacks fail loudly."""
+    print("case K40 wave-per-item form", flush=True)
+    g, og = pair("COMP40.txt", 0, 39)
+    for eps in (1e-3, 1e-4):
+        r = nr.solve(g, truncation_eps=eps, queue_capacity=1 << 20)
+        o = og.solve(0, 39, trunc_eps=eps, capacity=1 << 20)
+        assert (r["paths"], r["nodes"], r["dead"], r["truncated"]) == (
+            o["paths"], o["nodes"], o["dead"], o["truncated"])
+        assert (r["iterations"], r["max_queue_items"]) == (o["iterations"], o["max_width"])
+        assert abs(r["R"] - o["R"]) <= REL * o["R"] and abs(r["ESP"] - o["ESP"]) <= REL * o["ESP"]
+        Rt = closed_form(expected, 40, "0.1")
+        assert r["R"] <= Rt <= r["R"] + r["ESP"] + 1e-14
     with pytest.raises(nr.NRError) as ei:
-        nr.solve(g, truncation_eps=1e-3, queue_capacity=1024)
+        nr.solve(g, truncation_eps=1e-3, queue_capacity=1024, order="mass")
     assert ei.value.status == 5
 
 
