@@ -574,3 +574,23 @@ def test_big_graph_oracle_series_and_bundle():
     r = g.solve(0, k + 1)
     want = 1.0 - math.prod(1 - p[2 * i] * p[2 * i + 1] for i in range(k))
     assert abs(r["R"] - want) <= 1e-14 and r["ESP"] == 0
+
+
+# ------------------------------------ edge pivot against the reference program
+def test_pivot_mode_matches_the_reference_program():
+    """tests/golden/pivot_ref.json holds the reference's own edge-pivot program
+    (workingPad/cpu_reference.cpp, built from its source into oracle/_ref)
+    run on these graphs: the oracle's pivot mode must give the same node and
+    path counts; R agrees to the float32 parsing of p that program uses
+    (exactly where p is a float, e.g. 0.5)."""
+    import json
+    import os
+    vecs = json.load(open(os.path.join(os.path.dirname(graph_path("COMP8.txt")), "..",
+                                       "pivot_ref.json")))
+    assert len(vecs) >= 9
+    for v in vecs:
+        r = load(v["graph"]).solve(v["src"], v["dst"], mode=MODE_PIVOT, order=ORDER_DFS,
+                                   threads=4)
+        assert (r["paths"], r["nodes"]) == (v["paths"], v["nodes"]), v
+        tol = 1e-12 if v["graph"] in ("COMP8.txt", "cycle4.txt", "temp4x4.txt") else 1e-7
+        assert abs(r["R"] - v["R_float_p"]) <= tol * max(r["R"], 1e-3) + 5e-13, v
