@@ -82,16 +82,22 @@ typedef enum nr_rule {
   NR_RULE_FORKBEST = 1, /* success on a confirmed WORKING path, else a two-way
                           branch on the path's most reliable UNKNOWN edge,
                           nearest dst on ties (main.cu:304-430, 550-567) */
-  NR_RULE_ALTERNATE = 2 /* no counterpart in the reference: k children per
+  NR_RULE_ALTERNATE = 2, /* no counterpart in the reference: k children per
                           path as NR_RULE_PATH, on a path with the fewest
                           UNKNOWN edges (WORKING edges contracted; walk back
                           to the lowest-index node of the previous level,
                           then the lowest-index node of the component it
                           joins), children swept from the src end when
                           (FAILED edges / sweep_period) is even and from the
-                          dst end otherwise.  Undirected graphs of up to 64
+                          dst end otherwise.  Undirected graphs of up to 256
                           nodes, two terminals; NR_ERR_UNSUPPORTED
                           otherwise */
+  NR_RULE_PIVOT = 3     /* edge-pivot factoring (workingPad/cpu_reference.cpp
+                          :107-145): success on a confirmed WORKING path,
+                          dead when dst is unreachable, else a two-way branch
+                          on the UNKNOWN edge of highest p over the whole
+                          graph, lowest id on ties (WORKING child first);
+                          graphs of up to 256 nodes, orders as forkBest */
 } nr_rule;
 
 /* nr_config.flags */

@@ -33,7 +33,9 @@ ORDERS = {"level": NR_ORDER_LEVEL, "mass": NR_ORDER_MASS, "learned": NR_ORDER_LE
 NR_RULE_PATH = 0
 NR_RULE_FORKBEST = 1
 NR_RULE_ALTERNATE = 2
-RULES = {"path": NR_RULE_PATH, "forkbest": NR_RULE_FORKBEST, "alternate": NR_RULE_ALTERNATE}
+NR_RULE_PIVOT = 3
+RULES = {"path": NR_RULE_PATH, "forkbest": NR_RULE_FORKBEST, "alternate": NR_RULE_ALTERNATE,
+         "pivot": NR_RULE_PIVOT}
 
 STATUS = {
     0: "NR_OK",

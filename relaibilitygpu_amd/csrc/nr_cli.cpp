@@ -29,7 +29,7 @@ static void usage(const char* prog, long long cap) {
                "  --thresh_high <float>      High-priority log-prob threshold\n"
                "  --thresh_low  <float>      Low-priority  log-prob threshold\n"
                "  --order <level|mass|learned> Work order (default level)\n"
-               "  --rule <path|forkbest|alternate> Decomposition rule (default path)\n"
+               "  --rule <path|forkbest|alternate|pivot> Decomposition rule (default path)\n"
                "  --sweep_period <int>       Alternate rule: flip every n levels (default 2)\n"
                "  --model <file>             Learned order: NR_MLP_PARAMS raw float32\n"
                "  --alpha <float>            Learned order: score weight (default 0.5)\n"
@@ -74,6 +74,7 @@ int main(int argc, char** argv) {
       const char* r = argv[++i];
       cfg.rule = !std::strcmp(r, "forkbest")    ? NR_RULE_FORKBEST
                  : !std::strcmp(r, "alternate") ? NR_RULE_ALTERNATE
+                 : !std::strcmp(r, "pivot")     ? NR_RULE_PIVOT
                                                 : NR_RULE_PATH;
     } else if (!std::strcmp(argv[i], "--sweep_period") && has) {
       cfg.sweep_period = std::atoi(argv[++i]);

@@ -112,6 +112,7 @@ struct DevGraph {
   int nterms;
   int n, e, w, src, dst;
   int period; /* alternating sweep: flip every `period` FAILED edges */
+  int pivot;  /* two-way rule: branch on the best UNKNOWN edge of the whole graph */
 };
 
 /* The workgroup size as the stride of a lane's LDS slices ([slot][lane]):
@@ -196,6 +197,7 @@ struct Expander {
   const u16* terms;
   int nterms;
   NS tmask;
+  int ne, pivot; /* edges; edge-pivot rule instead of forkBest (fork_item) */
 
   __device__ void bind(unsigned char* smem, const Layout& L, int slot, int src_,
                        int dst_) {
@@ -223,6 +225,8 @@ struct Expander {
     terms = g.terms;
     nterms = g.nterms;
     tmask = (NS)g.tmask[0];
+    ne = g.e;
+    pivot = g.pivot;
   }
   /* the item's rows into this lane's slice */
   __device__ void load(const QBuf& in, u64 qstride, u64 slot) const {
@@ -330,6 +334,7 @@ struct Expander {
     }
     const int k = find_item(known, up);
     if (k <= 0) return k;
+    if (pivot) return 1 + pivot_edge(known);
     int best = uh[0];
     double bp = p[best];
     for (int j = 1; j < k; ++j) {
@@ -340,6 +345,27 @@ struct Expander {
       }
     }
     return 1 + best;
+  }
+  /* the edge-pivot rule (workingPad/cpu_reference.cpp:107-145): the first
+   * UNKNOWN edge of highest p over the whole graph (strict >, so the lowest
+   * id on ties, cpu_reference.cpp:112) */
+  __device__ int pivot_edge(const u64 (&known)[W]) const {
+    int best = -1;
+    double bp = -1.0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+      u64 m = ~known[i];
+      while (m) {
+        const int e = i * 64 + __builtin_ctzll(m);
+        m &= m - 1;
+        if (e >= ne) break;
+        if (p[e] > bp) {
+          bp = p[e];
+          best = e;
+        }
+      }
+    }
+    return best;
   }
   /* store_child() for the item of another lane of the same workgroup
    * (slot offset `delta`): reads that lane's rows slice */
@@ -670,8 +696,11 @@ struct CsrExpander {
   const u16* terms;
   int nterms;
   u64 tmask[NW];
+  int ne, pivot; /* edges; edge-pivot rule instead of forkBest (fork_item) */
 
   __device__ void bind_graph(unsigned char* smem, const Layout& L, const DevGraph& g, int slot) {
+    ne = g.e;
+    pivot = g.pivot;
     p = g.p;
     q = g.q;
     rp = g.rp;
@@ -746,6 +775,7 @@ struct CsrExpander {
     }
     const int k = find_item(known, up);
     if (k <= 0) return k;
+    if (pivot) return 1 + pivot_edge(known);
     int best = uh[0];
     double bp = p[best];
     for (int j = 1; j < k; ++j) {
@@ -756,6 +786,27 @@ struct CsrExpander {
       }
     }
     return 1 + best;
+  }
+  /* the edge-pivot rule (workingPad/cpu_reference.cpp:107-145): the first
+   * UNKNOWN edge of highest p over the whole graph (strict >, so the lowest
+   * id on ties, cpu_reference.cpp:112) */
+  __device__ int pivot_edge(const u64 (&known)[W]) const {
+    int best = -1;
+    double bp = -1.0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+      u64 m = ~known[i];
+      while (m) {
+        const int e = i * 64 + __builtin_ctzll(m);
+        m &= m - 1;
+        if (e >= ne) break;
+        if (p[e] > bp) {
+          bp = p[e];
+          best = e;
+        }
+      }
+    }
+    return best;
   }
   __device__ int out_degree(int u, const u64 (&known)[W], const u64 (&up)[W]) const {
     int d = 0;

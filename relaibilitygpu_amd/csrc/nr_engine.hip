@@ -905,6 +905,7 @@ int upload_tables(Workspace& ws, const nr_graph* g, int W, DevGraph& d) {
   d.src = g->src;
   d.dst = g->dst;
   d.period = 2;
+  d.pivot = 0;
   return NR_OK;
 }
 
@@ -1295,7 +1296,8 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   const bool mass_order = cfg->order == NR_ORDER_MASS;
   KernelGetter getter = mode == 1                       ? var.level_log
                         : mode == 2                     ? var.level_learn
-                        : cfg->rule == NR_RULE_FORKBEST ? var.level_fork
+                        : cfg->rule == NR_RULE_FORKBEST || cfg->rule == NR_RULE_PIVOT
+                            ? var.level_fork
                         : mass_order                    ? var.level_mass
                                                         : var.level;
   if (!getter) return NR_ERR_UNSUPPORTED;
@@ -1332,6 +1334,7 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   std::memset(&a, 0, sizeof a);
   if ((rc = upload_tables(ws, g, W, a.g))) return rc;
   a.g.period = cfg->sweep_period > 0 ? cfg->sweep_period : 2;
+  a.g.pivot = cfg->rule == NR_RULE_PIVOT ? 1 : 0;
   a.lay = lay;
   a.seg_cap = seg_cap;
   a.n_seg = grid;
@@ -2331,7 +2334,7 @@ int nr_solve(const nr_graph* g, const nr_config* cfg, nr_result* out) {
   if (cfg->order == NR_ORDER_LEARNED) return NR_ERR_ARG; /* needs a model: nr_solve_learned */
   if (cfg->order != NR_ORDER_LEVEL && cfg->order != NR_ORDER_MASS) return NR_ERR_ARG;
   if (cfg->rule != NR_RULE_PATH && cfg->rule != NR_RULE_FORKBEST &&
-      cfg->rule != NR_RULE_ALTERNATE)
+      cfg->rule != NR_RULE_ALTERNATE && cfg->rule != NR_RULE_PIVOT)
     return NR_ERR_ARG;
   if (cfg->sweep_period < 0) return NR_ERR_ARG;
   nr::Variant var;
@@ -2411,7 +2414,7 @@ int nr_solve_checkpointed(const nr_graph* g, const nr_config* cfg, const nr_chec
     return NR_ERR_ARG;
   if (cfg->order != NR_ORDER_LEVEL && cfg->order != NR_ORDER_MASS) return NR_ERR_ARG;
   if (cfg->rule != NR_RULE_PATH && cfg->rule != NR_RULE_FORKBEST &&
-      cfg->rule != NR_RULE_ALTERNATE)
+      cfg->rule != NR_RULE_ALTERNATE && cfg->rule != NR_RULE_PIVOT)
     return NR_ERR_ARG;
   if (cfg->sweep_period < 0) return NR_ERR_ARG;
   if (resume && (resume->graph_hash != nr::graph_hash(g) || resume->w != nr::mask_words(g) ||
