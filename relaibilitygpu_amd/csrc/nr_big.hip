@@ -151,12 +151,18 @@ __global__ void __launch_bounds__(256) k_expand_big(LevelArgs a, int parity, Big
           if ((word >> lane) & 1ull) {
             const int u = w * 64 + lane;
             const int j1 = a.g.rp[u + 1];
-            for (int j = a.g.rp[u]; j < j1; ++j) {
-              const u32 x = a.g.arc[j];
-              const int v = (int)(x & 0xffffu), e = (int)(x >> 16);
-              if (lev[v] != 0xff || !allowed(e)) continue;
-              lev[v] = (unsigned char)(L + 1);
-              atomicOr((unsigned long long*)&nxt[v >> 6], 1ull << (v & 63));
+            for (int j = a.g.rp[u]; j < j1; j += 8) { /* eight arc loads in flight */
+              u32 xs[8];
+#pragma unroll
+              for (int t = 0; t < 8; ++t) xs[t] = j + t < j1 ? a.g.arc[j + t] : 0u;
+#pragma unroll
+              for (int t = 0; t < 8; ++t) {
+                if (j + t >= j1) break;
+                const int v = (int)(xs[t] & 0xffffu), e = (int)(xs[t] >> 16);
+                if (lev[v] != 0xff || !allowed(e)) continue;
+                lev[v] = (unsigned char)(L + 1);
+                atomicOr((unsigned long long*)&nxt[v >> 6], 1ull << (v & 63));
+              }
             }
           }
         }
