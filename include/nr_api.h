@@ -80,8 +80,9 @@ typedef struct nr_config {
 typedef enum nr_rule {
   NR_RULE_PATH = 0,    /* k children per path (main.cu:114-283), default */
   NR_RULE_FORKBEST = 1, /* success on a confirmed WORKING path, else a two-way
-                          branch on the path's most reliable UNKNOWN edge,
-                          nearest dst on ties (main.cu:304-430, 550-567) */
+                          branch on the path's most reliable UNKNOWN edge
+                          (FP32 logf keys), ties settled as the reference's
+                          warp argmax settles them (main.cu:304-430, 550-567) */
   NR_RULE_ALTERNATE = 2, /* no counterpart in the reference: k children per
                           path as NR_RULE_PATH, on a path with the fewest
                           UNKNOWN edges (WORKING edges contracted; walk back

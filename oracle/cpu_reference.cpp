@@ -450,11 +450,13 @@ struct Engine {
 /* forkBest expansion, restating workingPad/main.cu:550-567 and 304-430:
  * a confirmed (WORKING-only) BFS success ends the item; otherwise the
  * optimistic path (lowest-index parent rule) is walked from dst towards src
- * and the UNKNOWN hop of highest p is the pivot (ties: the first met from
- * the dst side — main.cu keeps the path dst-first, main.cu:332-341, and its
- * warp argmax keeps the lowest hop index on ties, 379-388).  Children: the
- * pivot WORKING (mass * p) then FAILED (mass * q).  Returns -1 dead, 0
- * success, 2 children. */
+ * (the reference keeps it dst-first, main.cu:332-341) and the pivot is the
+ * UNKNOWN hop that the reference's warp argmax returns (main.cu:355-388):
+ * keys are logf of p in FP32 (graph.h:138), and ties are settled by the
+ * lane layout and the shuffle tree, restated step by step below.  With
+ * terminal sets the hops are indexed in the order the tree walk meets them.
+ * Children: the pivot WORKING (mass * p) then FAILED (mass * q).  Returns
+ * -1 dead, 0 success, 2 children. */
 int expand_forkbest_impl(const Engine& en, const u64* known, const u64* up, double mass,
                          u64* ck, u64* cu, double* cm, double* succ, int* hops,
                          std::vector<NodeSet>& levels) {
@@ -464,16 +466,36 @@ int expand_forkbest_impl(const Engine& en, const u64* known, const u64* up, doub
   }
   const int L = en.find_path(known, up, true, hops, levels);
   if (L < 0) return -1;
-  int best = -1;
-  double bp = -1.0;
-  for (int h = L - 1; h >= 0; --h) {
-    const int e = hops[h];
+  /* per-lane argmax over hops r = lane, lane + 32, ... counted from the dst
+   * end (strict >, so a lane keeps its first hop on ties), then the
+   * shfl_down tree over 32 lanes, where position i keeps its own value on
+   * ties with position i + off: among lanes holding the best key the winner
+   * is the one whose 5-bit reversed index is smallest. */
+  float lane_key[32];
+  int lane_edge[32];
+  for (int l = 0; l < 32; ++l) { /* -1e30 in main.cu:356; -inf keeps a lane
+                                    without an UNKNOWN hop below every key */
+    lane_key[l] = -INFINITY;
+    lane_edge[l] = -1;
+  }
+  for (int r = 0; r < L; ++r) {
+    const int e = hops[L - 1 - r];
     const u64 bit = 1ull << (e & 63);
-    if (!(known[e >> 6] & bit) && en.g->p[e] > bp) {
-      bp = en.g->p[e];
-      best = e;
+    if (known[e >> 6] & bit) continue;
+    const float pf = (float)en.g->p[e]; /* graph.h:138 */
+    const float key = pf > 0.0f ? logf(pf) : -1e30f;
+    if (key > lane_key[r & 31]) {
+      lane_key[r & 31] = key;
+      lane_edge[r & 31] = e;
     }
   }
+  for (int off = 16; off > 0; off >>= 1)
+    for (int i = 0; i < off; ++i)
+      if (lane_key[i + off] > lane_key[i]) {
+        lane_key[i] = lane_key[i + off];
+        lane_edge[i] = lane_edge[i + off];
+      }
+  const int best = lane_edge[0];
   if (best < 0) { /* all-WORKING path: unreachable after the confirmed BFS */
     *succ = mass;
     return 0;

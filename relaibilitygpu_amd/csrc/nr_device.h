@@ -98,6 +98,7 @@ inline Layout make_layout(int n, int e, int w, int block, int ns_bytes,
 struct DevGraph {
   const double* p;
   const double* q;
+  const float* lk;  /* [e] forkBest's pivot key: logf of p rounded to FP32 (graph.h:138) */
   const u64* sadj;  /* [n] static neighbour set (n <= 64) */
   const u64* inc;   /* [n*w] incident edge set (n <= 64) */
   const u32* ep;    /* [e] endpoints a | b << 16 */
@@ -179,6 +180,26 @@ __device__ inline u64 pick(const u64 (&a)[W], int wi) {
 }
 
 /* The per-lane expander: BFS + lowest-index parent path + unknown hops. */
+/* forkBest's pivot among a path's UNKNOWN hops (main.cu:356-392).  The
+ * reference gives hop r of the path (counted from the dst end, every hop
+ * counted) to lane r mod 32; a lane keeps the first hop of highest key
+ * (strict >), then a shfl_down tree over the 32 lanes keeps position i's
+ * value on ties with position i + off.  So the pivot is the hop of highest
+ * key, ties going to the smallest 5-bit reversal of r mod 32, then to the
+ * smallest r.  Keys are logf of p in FP32, as the reference stores them. */
+struct ForkPick {
+  int best = -1, tie = 0;
+  float key = 0.0f;
+  __device__ void offer(int e, int r, float k) {
+    const int t = (int)((__builtin_bitreverse32((u32)(r & 31)) >> 27) << 16) | r;
+    if (best < 0 || k > key || (k == key && t < tie)) {
+      best = e;
+      key = k;
+      tie = t;
+    }
+  }
+};
+
 template <int W, typename NS>
 struct Expander {
   const double* p;
@@ -188,6 +209,7 @@ struct Expander {
   const u32* ep;
   const u32* deg;
   const u16* eidm;
+  const float* lk; /* forkBest keys (global memory) */
   NS* rows; /* this lane's allowed-adjacency rows, stride `stride` */
   NS* lvl;  /* this lane's level sets, stride `stride` */
   u16* uh;  /* this lane's unknown-hop stack, stride `stride` */
@@ -227,6 +249,7 @@ struct Expander {
     tmask = (NS)g.tmask[0];
     ne = g.e;
     pivot = g.pivot;
+    lk = g.lk;
   }
   /* the item's rows into this lane's slice */
   __device__ void load(const QBuf& in, u64 qstride, u64 slot) const {
@@ -335,16 +358,41 @@ struct Expander {
     const int k = find_item(known, up);
     if (k <= 0) return k;
     if (pivot) return 1 + pivot_edge(known);
-    int best = uh[0];
-    double bp = p[best];
-    for (int j = 1; j < k; ++j) {
-      const int e = uh[j * stride];
-      if (p[e] > bp) {
-        bp = p[e];
-        best = e;
+    const int b = fork_pick(known);
+    return b < 0 ? -2 : 1 + b;
+  }
+  /* forkBest's pivot (ForkPick) over the path just found: the walk of
+   * find() / find_tree() again, with each hop's index along the walk */
+  __device__ int fork_pick(const u64 (&known)[W]) const {
+    ForkPick fp;
+    auto hop = [&](int e, int r) {
+      if (!((pick<W>(known, e >> 6) >> (e & 63)) & 1ull)) fp.offer(e, r, lk[e]);
+    };
+    if (nterms >= 2) {
+      NS marked = (NS)1 << terms[0];
+      int r = 0;
+      for (int i = 1; i < nterms; ++i) {
+        int v = terms[i];
+        while (!((marked >> v) & (NS)1)) {
+          int l = 1;
+          while (l < n && !((lvl[l * stride] >> v) & (NS)1)) ++l;
+          const NS cand = l < n ? rows[v * stride] & lvl[(l - 1) * stride] : (NS)0;
+          if (!cand) return -1;
+          const int u = ctzs<NS>(cand);
+          hop(eidm[u * n + v], r++);
+          marked |= (NS)1 << v;
+          v = u;
+        }
+      }
+    } else {
+      int v = dst;
+      for (int l = plen; l >= 1; --l) {
+        const int u = ctzs<NS>(rows[v * stride] & lvl[(l - 1) * stride]);
+        hop(eidm[u * n + v], plen - l);
+        v = u;
       }
     }
-    return 1 + best;
+    return fp.best;
   }
   /* the edge-pivot rule (workingPad/cpu_reference.cpp:107-145): the first
    * UNKNOWN edge of highest p over the whole graph (strict >, so the lowest
@@ -688,6 +736,7 @@ struct CsrExpander {
   const int* irp;
   const u32* iarc;
   const u32* ep;
+  const float* lk;    /* forkBest keys */
   unsigned char* lvb; /* this lane's level bytes: node u at (u>>2)*4*stride + (u&3) */
   u16* uh;            /* this lane's unknown-hop stack, stride `stride` */
   Pow2 stride;
@@ -708,6 +757,7 @@ struct CsrExpander {
     irp = g.irp;
     iarc = g.iarc;
     ep = g.ep;
+    lk = g.lk;
     terms = g.terms;
     nterms = g.nterms;
 #pragma unroll
@@ -776,16 +826,66 @@ struct CsrExpander {
     const int k = find_item(known, up);
     if (k <= 0) return k;
     if (pivot) return 1 + pivot_edge(known);
-    int best = uh[0];
-    double bp = p[best];
-    for (int j = 1; j < k; ++j) {
-      const int e = uh[j * stride];
-      if (p[e] > bp) {
-        bp = p[e];
-        best = e;
+    const int b = fork_pick(known, up);
+    return b < 0 ? -2 : 1 + b;
+  }
+  /* forkBest's pivot (ForkPick) over the path just found: the walk of
+   * find_item() / find_tree() again, with each hop's index along the walk */
+  __device__ int fork_pick(const u64 (&known)[W], const u64 (&up)[W]) const {
+    ForkPick fp;
+    auto parent = [&](int v, int& pe) {
+      const int l = level_of(v);
+      for (int j = irp[v]; j < irp[v + 1]; ++j) {
+        const u32 x = iarc[j];
+        const int u = (int)(x & 0xffffu), e = (int)(x >> 16);
+        if (level_of(u) == l - 1 && allowed(known, up, e)) {
+          pe = e;
+          return u;
+        }
+      }
+      return -1;
+    };
+    auto hop = [&](int e, int r) {
+      if (!((pick<W>(known, e >> 6) >> (e & 63)) & 1ull)) fp.offer(e, r, lk[e]);
+    };
+    if (nterms >= 2) {
+      u64 marked[NW];
+#pragma unroll
+      for (int k = 0; k < NW; ++k) marked[k] = 0;
+      bs_set<NW>(marked, terms[0]);
+      int r = 0;
+      for (int i = 1; i < nterms; ++i) {
+        int v = terms[i];
+        while (!bs_test<NW>(marked, v)) {
+          int pe = -1;
+          const int u = parent(v, pe);
+          if (u < 0) return -1;
+          hop(pe, r++);
+          bs_set<NW>(marked, v);
+          v = u;
+        }
+      }
+    } else {
+      /* dst's own level byte is not written by find_item (it stops at the
+       * level that reaches dst): its parents lie on level plen - 1 */
+      int v = dst;
+      for (int l = plen; l >= 1; --l) {
+        int pu = -1, pe = -1;
+        for (int j = irp[v]; j < irp[v + 1]; ++j) {
+          const u32 x = iarc[j];
+          const int u = (int)(x & 0xffffu), e = (int)(x >> 16);
+          if (level_of(u) == l - 1 && allowed(known, up, e)) {
+            pu = u;
+            pe = e;
+            break;
+          }
+        }
+        if (pu < 0) return -1;
+        hop(pe, plen - l);
+        v = pu;
       }
     }
-    return 1 + best;
+    return fp.best;
   }
   /* the edge-pivot rule (workingPad/cpu_reference.cpp:107-145): the first
    * UNKNOWN edge of highest p over the whole graph (strict >, so the lowest

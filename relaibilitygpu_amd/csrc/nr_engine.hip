@@ -790,7 +790,7 @@ Workspace g_ws[kMaxDevices];
 struct TableBlob {
   std::vector<unsigned char> bytes;
   size_t off_p, off_q, off_sadj, off_inc, off_ep, off_deg, off_eidm;
-  size_t off_rp, off_arc, off_irp, off_iarc, off_terms;
+  size_t off_rp, off_arc, off_irp, off_iarc, off_terms, off_lk;
 };
 
 void pack_tables(const nr_graph* g, int W, TableBlob& b) {
@@ -816,6 +816,7 @@ void pack_tables(const nr_graph* g, int W, TableBlob& b) {
   b.off_arc = take(4 * (size_t)std::max(g->nnz, 1));
   b.off_irp = take(4 * (size_t)(n + 1));
   b.off_iarc = take(4 * (size_t)std::max(g->nnz, 1));
+  b.off_lk = take(4 * (size_t)e);
   b.bytes.assign((o + 15) / 16 * 16, 0);
   double* p = (double*)&b.bytes[b.off_p];
   double* q = (double*)&b.bytes[b.off_q];
@@ -824,11 +825,15 @@ void pack_tables(const nr_graph* g, int W, TableBlob& b) {
   u32* ep = (u32*)&b.bytes[b.off_ep];
   u32* deg = (u32*)&b.bytes[b.off_deg];
   u16* eidm = (u16*)&b.bytes[b.off_eidm];
+  float* lk = (float*)&b.bytes[b.off_lk];
   for (size_t i = 0; i < (size_t)ns * ns; ++i) eidm[i] = 0xffff;
   for (int id = 0; id < g->e; ++id) {
     const int a = g->ea[id], c = g->eb[id];
     p[id] = g->p[id];
     q[id] = g->q[id];
+    /* forkBest's key, computed on the host as graph.h:138 does */
+    const float pf = (float)g->p[id];
+    lk[id] = pf > 0.0f ? logf(pf) : -1e30f;
     ep[id] = (u32)a | ((u32)c << 16);
     if (!ns) continue;
     inc[(size_t)a * W + (id >> 6)] |= 1ull << (id & 63);
@@ -896,6 +901,7 @@ int upload_tables(Workspace& ws, const nr_graph* g, int W, DevGraph& d) {
   d.irp = (const int*)(base + b.off_irp);
   d.iarc = (const u32*)(base + b.off_iarc);
   d.terms = (const u16*)(base + b.off_terms);
+  d.lk = (const float*)(base + b.off_lk);
   d.nterms = (int)g->terms.size();
   for (int k = 0; k < 4; ++k) d.tmask[k] = 0;
   for (int t : g->terms) d.tmask[t >> 6] |= 1ull << (t & 63);

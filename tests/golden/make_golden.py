@@ -122,6 +122,23 @@ def main() -> None:
          "iterations": 23, "max_width": 8160,
          "source": "workingPad/output_maxMem_forkPath.txt:41-44"},
     ]
+    # Reference run logs of the forkBest rule (FORK_BEST_EDGE): the same
+    # fields.  The K15/K25 runs of that log filled the queue (overflow into
+    # ESP, placement order set by atomics) and are not reproducible counts.
+    out["logged_forkbest_counts"] = [
+        {"graph": "COMP8.txt", "src": 0, "dst": 7, "paths": 22560, "nodes": 78251,
+         "iterations": 29, "max_width": 11820,
+         "source": "workingPad/output_maxMem_forBest.txt:18-21"},
+        {"graph": "COMP8-03.txt", "src": 0, "dst": 7, "paths": 22560, "nodes": 78251,
+         "iterations": 29, "max_width": 11820,
+         "source": "workingPad/output_maxMem_forBest.txt:41-44"},
+        {"graph": "COMP9.txt", "src": 0, "dst": 8, "paths": 327203, "nodes": 1136093,
+         "iterations": 37, "max_width": 147000,
+         "source": "workingPad/output_maxMem_forBest.txt:64-67"},
+        {"graph": "COMP10-05.txt", "src": 0, "dst": 9, "paths": 5660730,
+         "nodes": 19686583, "iterations": 46, "max_width": 2249520,
+         "source": "workingPad/output_maxMem_forBest.txt:87-90"},
+    ]
     # Small-graph counts of the same rule, from an independent scratch
     # restatement of it that reproduced the K8/K9 logs exactly.
     out["derived_counts"] = [

@@ -1,15 +1,13 @@
 """GPU parity of the forkBest rule (NR_RULE_FORKBEST) against the oracle's
-forkbest mode: a confirmed WORKING path is a success, otherwise a two-way
-branch on the path's most reliable UNKNOWN edge (nearest dst on ties).
+forkbest mode and the reference's own logged runs: a confirmed WORKING path
+is a success, otherwise a two-way branch on the path's UNKNOWN edge of
+highest FP32 log p, ties settled as the reference's warp argmax settles them
+(lane r mod 32 for hop r from the dst end, then its shuffle tree).
 
 Counts bit-exact, levels and widest level in level order, R to 1e-12; in
-full enumeration R also equals the closed form (the rule builds a
-different, still exact, decomposition).  The K10 counts are the oracle's
-(oracle/cpu_reference.cpp, forkbest mode, level order; 10 s on one core,
-so stored here): the reference's logged forkBest run (output_maxMem_
-forBest.txt) has the same 46 levels but other counts, its tie-break among
-equal-p edges not being recoverable from the source; the counts are
-therefore pinned to the oracle, not to the log.
+full enumeration R also equals the closed form.  K8, K8 at p = 0.3, K9 and
+K10 reproduce workingPad/output_maxMem_forBest.txt exactly (paths, nodes,
+kernel iterations, max queue items).
 """
 import numpy as np
 import pytest
@@ -48,14 +46,50 @@ def test_files_vs_oracle(name, s, t):
     check(r, o)
 
 
-def test_k10_forkbest(expected):
-    print("case forkbest K10 full", flush=True)
-    g = nr.Graph.load(graph_path("COMP10-05.txt"), 0, 9)
+@pytest.mark.parametrize("idx", [0, 1, 2, 3])
+def test_reference_forkbest_logs(expected, idx):
+    """The reference's logged forkBest runs (K8, K8 p = 0.3, K9, K10)."""
+    lg = expected["logged_forkbest_counts"][idx]
+    print(f"case forkbest log {lg['graph']}", flush=True)
+    g = nr.Graph.load(graph_path(lg["graph"]), lg["src"], lg["dst"])
     r = nr.solve(g, truncation_eps=0.0, queue_capacity=1 << 23, rule="forkbest")
     assert (r["paths"], r["nodes"], r["iterations"], r["max_queue_items"]) == (
-        7433032, 29732127, 46, 3018960)
-    assert close(r["R"], closed_form(expected, 10, "0.5"))
+        lg["paths"], lg["nodes"], lg["iterations"], lg["max_width"]), lg
     assert abs(r["R"] + r["D"] - 1.0) <= 1e-13
+    if lg["graph"] == "COMP10-05.txt":
+        assert close(r["R"], closed_form(expected, 10, "0.5"))
+
+
+@pytest.mark.parametrize("chords", [0, 3])
+def test_forkbest_long_paths(chords):
+    """Rings of 80 nodes (CSR form), uniform p, src and dst 40 hops apart:
+    paths of more than 32 hops put several hops on one of the reference's
+    lanes, which exercises both stages of its tie-break."""
+    print(f"case forkbest ring80 chords={chords}", flush=True)
+    n = 80
+    edges = [(i, (i + 1) % n) for i in range(n)]
+    edges = sorted((min(a, b), max(a, b)) for a, b in edges)
+    for a, b in [(5, 70), (12, 33), (50, 61)][:chords]:
+        edges.append((a, b))
+    probs = [0.9] * len(edges)
+    rp, ci, ei = wl.to_csr(n, edges)
+    g = nr.Graph.from_csr(rp, ci, ei, probs, 0, 40)
+    og = OracleGraph.from_csr(rp, ci, ei, probs)
+    r = nr.solve(g, truncation_eps=0.0, queue_capacity=1 << 22, rule="forkbest")
+    check(r, og.solve(0, 40, mode=MODE_FORKBEST, order=ORDER_LEVEL))
+
+
+def test_forkbest_keys_are_fp32_logs(expected):
+    """p = 0.5 plus perturbations below FP32 resolution: the keys tie as in
+    the reference (FP32 logf), so the tree is the logged uniform-p K8 one."""
+    print("case forkbest fp32 keys", flush=True)
+    n, edges = 8, [(a, b) for a in range(8) for b in range(a + 1, 8)]
+    probs = [0.5 + 1e-12 * (i % 7) for i in range(len(edges))]
+    rp, ci, ei = wl.to_csr(n, edges)
+    g = nr.Graph.from_csr(rp, ci, ei, probs, 0, 7)
+    r = nr.solve(g, truncation_eps=0.0, queue_capacity=1 << 22, rule="forkbest")
+    lg = expected["logged_forkbest_counts"][0]
+    assert (r["paths"], r["nodes"], r["iterations"]) == (lg["paths"], lg["nodes"], lg["iterations"])
 
 
 def test_forkbest_mass_order_same_tree(expected):

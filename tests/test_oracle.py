@@ -240,15 +240,33 @@ def test_alternative_rules_exact(expected, mode):
         assert abs(r["R"] - closed_form(expected, n, p)) <= 1e-14
 
 
-def test_forkbest_levels_match_logs(expected):
-    """forkBest restated from the source reproduces the logged level count
-    (workingPad/output_maxMem_forBest.txt:20) and binary-tree shape
-    (nodes = 2 * internal + 1); its logged paths/nodes (22560 / 78251) are
-    not reproduced by any tie rule and are not used as pins."""
-    r = load("COMP8.txt").solve(0, 7, mode=MODE_FORKBEST)
-    assert r["iterations"] == 29
-    leaves = r["paths"] + r["dead"]
-    assert r["nodes"] == 2 * leaves - 1
+@pytest.mark.parametrize("idx", [0, 1, 2])
+def test_forkbest_matches_logs(expected, idx):
+    """forkBest reproduces the reference's logged runs exactly: paths, nodes,
+    levels and widest level (workingPad/output_maxMem_forBest.txt).  With a
+    uniform p every hop ties, so these counts pin the tie-break of the
+    reference's warp argmax (lane r mod 32, shuffle tree); taking the first
+    UNKNOWN hop from either end instead gives other counts (K8: 26830 paths,
+    107319 nodes).  The tree is binary: nodes = 2 * leaves - 1."""
+    lg = expected["logged_forkbest_counts"][idx]
+    r = load(lg["graph"]).solve(lg["src"], lg["dst"], mode=MODE_FORKBEST, threads=4)
+    assert (r["paths"], r["nodes"], r["iterations"], r["max_width"]) == (
+        lg["paths"], lg["nodes"], lg["iterations"], lg["max_width"]), lg
+    assert r["nodes"] == 2 * (r["paths"] + r["dead"]) - 1
+
+
+def test_forkbest_keys_are_fp32_logs(expected):
+    """Keys are logf of p in FP32 (graph.h:138): K8 with p = 0.5 plus
+    perturbations below FP32 resolution ties exactly as the uniform K8 does
+    and reproduces its logged tree; comparing the FP64 p would not."""
+    from relaibilitygpu_amd import workloads as wl
+    n, edges = 8, [(a, b) for a in range(8) for b in range(a + 1, 8)]
+    probs = [0.5 + 1e-12 * (i % 7) for i in range(len(edges))]
+    rp, ci, ei = wl.to_csr(n, edges)
+    r = OracleGraph.from_csr(rp, ci, ei, probs).solve(0, 7, mode=MODE_FORKBEST)
+    lg = expected["logged_forkbest_counts"][0]
+    assert (r["paths"], r["nodes"], r["iterations"], r["max_width"]) == (
+        lg["paths"], lg["nodes"], lg["iterations"], lg["max_width"])
 
 
 def test_minunk_never_longer_in_unknowns():
