@@ -78,13 +78,15 @@ inline Layout make_layout(int n, int e, int w, int block, int ns_bytes,
   };
   const int ee = e > 0 ? e : 1;
   const int t = tables ? 1 : 0; /* batched instances read tables from HBM */
+  /* the table part is also the head of the device table blob (the host
+   * packs it with these offsets), so staging it is one flat copy */
   L.off_p = take(t * 8 * ee, 16);
   L.off_q = take(t * 8 * ee, 16);
-  L.off_sadj = take(t * 8 * n, 16);
-  L.off_inc = take(t * 8 * n * w, 16);
+  L.off_sadj = take(t * (8 * n + 8), 16);
+  L.off_inc = take(t * (8 * n * w + 8), 16);
   L.off_ep = take(t * 4 * ee, 16);
-  L.off_deg = take(t * 4 * n, 16);
-  L.off_eidm = take(t * 2 * n * n, 16);
+  L.off_deg = take(t * (4 * n + 4), 16);
+  L.off_eidm = take(t * (2 * n * n + 4), 16);
   L.table_bytes = (o + 15) / 16 * 16;
   L.off_extra = take(extra_bytes, 16);
   L.off_rows = take(ns_bytes * n * block, 16);
@@ -140,23 +142,14 @@ struct QBuf {
 };
 
 /* Cooperative copy of the tables into LDS (all threads of the block). */
+/* The graph tables into LDS: the device blob starts with the table part of
+ * the LDS layout, byte for byte (p at offset 0), so this is one copy of
+ * table_bytes in 16-byte vectors, all loads issued before any store waits. */
 __device__ inline void stage_tables(const DevGraph& g, const Layout& L,
                                     unsigned char* smem) {
-  auto cp = [&](const void* src, int off, int bytes) {
-    const u32* s = (const u32*)src;
-    u32* d = (u32*)(smem + off);
-    for (int i = threadIdx.x; i < bytes / 4; i += blockDim.x) d[i] = s[i];
-  };
-  cp(g.p, L.off_p, 8 * g.e);
-  cp(g.q, L.off_q, 8 * g.e);
-  cp(g.sadj, L.off_sadj, 8 * g.n);
-  cp(g.inc, L.off_inc, 8 * g.n * g.w);
-  cp(g.ep, L.off_ep, 4 * g.e);
-  cp(g.deg, L.off_deg, 4 * g.n);
-  /* eidm is 2-byte; copy as 2-byte to stay in bounds for odd n*n */
-  const u16* s = g.eidm;
-  u16* d = (u16*)(smem + L.off_eidm);
-  for (int i = threadIdx.x; i < g.n * g.n; i += blockDim.x) d[i] = s[i];
+  const uint4* s = (const uint4*)g.p;
+  uint4* d = (uint4*)smem;
+  for (int i = threadIdx.x; i < L.table_bytes / 16; i += blockDim.x) d[i] = s[i];
 }
 
 template <typename NS>

@@ -797,20 +797,23 @@ void pack_tables(const nr_graph* g, int W, TableBlob& b) {
   const int n = g->n, e = std::max(g->e, 1);
   /* node-set tables exist for the row form only (n <= 64) */
   const int ns = n <= 64 ? n : 0;
-  size_t o = 0;
+  /* the head of the blob is the table part of the row form's LDS layout,
+   * offset for offset, so a workgroup stages it with one flat copy */
+  const Layout T = make_layout(ns, e, W, 64, 0, 0, true);
+  size_t o = (size_t)T.table_bytes;
   auto take = [&](size_t bytes) {
     o = (o + 15) / 16 * 16;
     size_t at = o;
     o += bytes;
     return at;
   };
-  b.off_p = take(8 * (size_t)e);
-  b.off_q = take(8 * (size_t)e);
-  b.off_sadj = take(8 * (size_t)ns + 8);
-  b.off_inc = take(8 * (size_t)ns * W + 8);
-  b.off_ep = take(4 * (size_t)e);
-  b.off_deg = take(4 * (size_t)ns + 4);
-  b.off_eidm = take(2 * (size_t)ns * ns + 4);
+  b.off_p = (size_t)T.off_p;
+  b.off_q = (size_t)T.off_q;
+  b.off_sadj = (size_t)T.off_sadj;
+  b.off_inc = (size_t)T.off_inc;
+  b.off_ep = (size_t)T.off_ep;
+  b.off_deg = (size_t)T.off_deg;
+  b.off_eidm = (size_t)T.off_eidm;
   b.off_terms = take(2 * (size_t)std::max<size_t>(g->terms.size(), 1));
   b.off_rp = take(4 * (size_t)(n + 1));
   b.off_arc = take(4 * (size_t)std::max(g->nnz, 1));

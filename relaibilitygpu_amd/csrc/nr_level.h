@@ -119,27 +119,56 @@ __device__ inline int segment_of(const u32* sp, int n_seg, u32 base) {
 
 /* Exclusive prefix of the per-segment counts into sp[0..n_seg] (LDS), by
  * the whole workgroup; `part` is blockDim.x words of LDS scratch. */
-__device__ inline void block_prefix(const u32* counts, int n_seg, u32* sp, u32* part) {
-  const int t = threadIdx.x, T = blockDim.x;
+/* Exclusive prefix of the segment counts into sp[0..n_seg] (sp[n_seg] is
+ * the total).  The counts are loaded first and `between` runs while they are
+ * in flight (the level kernels stage their graph tables there); then a
+ * shuffle scan per wave and one barrier to combine the waves. */
+template <class F>
+__device__ inline void block_prefix(const u32* counts, int n_seg, u32* sp, u32* part,
+                                    F&& between) {
+  const int t = threadIdx.x, T = blockDim.x, lane = t & 63, wave = t >> 6;
+  const int n_w = (T + 63) >> 6;
   const int per = (n_seg + T - 1) / T;
   const int b0 = min(t * per, n_seg), b1 = min(b0 + per, n_seg);
+  constexpr int kReg = 8; /* counts held in registers; more are read twice */
+  u32 v[kReg];
+#pragma unroll
+  for (int j = 0; j < kReg; ++j) v[j] = b0 + j < b1 ? counts[b0 + j] : 0u;
+  between();
   u32 sum = 0;
-  for (int b = b0; b < b1; ++b) sum += counts[b];
-  part[t] = sum;
-  __syncthreads();
-  for (int off = 1; off < T; off <<= 1) {
-    const u32 v = t >= off ? part[t - off] : 0u;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kReg; ++j) sum += v[j];
+  for (int b = b0 + kReg; b < b1; ++b) sum += counts[b];
+  u32 incl = sum;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const u32 x = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += x;
   }
-  u32 run = t ? part[t - 1] : 0u;
-  for (int b = b0; b < b1; ++b) {
+  if (lane == 63) part[wave] = incl;
+  __syncthreads();
+  u32 run = incl - sum, total = 0;
+  for (int w = 0; w < n_w; ++w) {
+    const u32 x = part[w];
+    if (w < wave) run += x;
+    total += x;
+  }
+#pragma unroll
+  for (int j = 0; j < kReg; ++j)
+    if (b0 + j < b1) {
+      sp[b0 + j] = run;
+      run += v[j];
+    }
+  for (int b = b0 + kReg; b < b1; ++b) {
     sp[b] = run;
     run += counts[b];
   }
-  if (t == T - 1) sp[n_seg] = part[T - 1];
+  if (t == 0) sp[n_seg] = total;
   __syncthreads();
+}
+
+__device__ inline void block_prefix(const u32* counts, int n_seg, u32* sp, u32* part) {
+  block_prefix(counts, n_seg, sp, part, [] {});
 }
 
 __device__ inline void flush_counts(u64* slot, const u32 (&c)[C_N]) {
@@ -405,6 +434,8 @@ __device__ __forceinline__ void expand_level_items(const LevelArgs& a, const EX&
       while (sp[s + 1] <= i) ++s;
       slot_in = (u64)s * a.seg_cap + (i - sp[s]);
       load_item<W>(in, a.stride, slot_in, known, up, mass);
+      /* level order: the rows travel with the planes (one memory round) */
+      if (MODE == 0 && !late) ex.load(in, a.stride, slot_in);
       if (MODE != 0) tag_in = a.tag[parity][slot_in];
       const int cl = !mass_on ? 0 : MODE == 2 ? (int)tag_in : mass_class(mass);
       if (late) {
@@ -440,7 +471,7 @@ __device__ __forceinline__ void expand_level_items(const LevelArgs& a, const EX&
             ++c[C_DEAD];
           }
         } else {
-          ex.load(in, a.stride, slot_in);
+          if (MODE != 0) ex.load(in, a.stride, slot_in);
           k = ex.find_item(known, up);
           if (k >= 0) {
             live = true;
@@ -683,7 +714,11 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
   u32* sp = (u32*)(smem + a.lay.off_extra);
   u32* s_cnt = sp + a.n_seg + 1;
   u32* part = s_cnt + 1;
-  block_prefix(a.seg_count[parity], a.n_seg, sp, part);
+  const u32 pre = prefetch_slot(a);
+  /* the graph tables are staged while the segment counts are in flight */
+  block_prefix(a.seg_count[parity], a.n_seg, sp, part, [&] {
+    if (EX::kStage) stage_tables(a.g, a.lay, smem);
+  });
   const u32 n_items = sp[a.n_seg];
   const u64 t_end = a.budget_ticks ? a.ctl->start_tick + a.budget_ticks : 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) { /* count the level (main.cu:970-973) */
@@ -700,7 +735,6 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
         a.hist[(u64)blockIdx.x * kClasses + i] = 0;
     return;
   }
-  const u32 pre = prefetch_slot(a);
   if (threadIdx.x == 0) *s_cnt = 0;
   u32* s_hist = part + blockDim.x; /* [kClasses], class rounds only */
   /* per-wave feature tile [64][kFeat] bf16 (learned order), 16-byte aligned */
@@ -708,7 +742,6 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
               (threadIdx.x >> 6) * 64 * kFeat;
   if (mass_on)
     for (int i = threadIdx.x; i < kClasses; i += blockDim.x) s_hist[i] = 0;
-  if (EX::kStage) stage_tables(a.g, a.lay, smem);
   __syncthreads();
 
   EX ex;
@@ -788,12 +821,12 @@ __global__ void __launch_bounds__(256) k_small_levels(LevelArgs a, u32 max_in, i
       bool live = false;
       if (i < n) {
         load_item<W>(in, a.stride, i, known, up, mass);
+        ex.load(in, a.stride, i); /* with the planes: one memory round */
         ++c[C_NODES];
         if (mass < a.eps) { /* main.cu:537-543 */
           accE.add(mass);
           ++c[C_TRUNC];
         } else {
-          ex.load(in, a.stride, i);
           k = ex.find_item(known, up);
           if (k >= 0) {
             live = true;
