@@ -171,6 +171,62 @@ __device__ inline void block_prefix(const u32* counts, int n_seg, u32* sp, u32* 
   block_prefix(counts, n_seg, sp, part, [] {});
 }
 
+/* Wave totals for block_flush without LDS traffic: DPP moves sum within
+ * each 16-lane row (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row half
+ * mirror, row mirror: after each step every lane holds its group's sum),
+ * then the four row sums are read with readlane.  All 64 lanes must be
+ * active; the result is wave-uniform. */
+template <int CTRL>
+__device__ inline u32 dpp_u32(u32 x) {
+  return (u32)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xF, 0xF, false);
+}
+
+template <int CTRL>
+__device__ inline double dpp_f64(double x) {
+  const u64 b = (u64)__double_as_longlong(x);
+  const u64 lo = dpp_u32<CTRL>((u32)b), hi = dpp_u32<CTRL>((u32)(b >> 32));
+  return __longlong_as_double((long long)(hi << 32 | lo));
+}
+
+__device__ inline double readlane_f64(double x, int l) {
+  const u64 b = (u64)__double_as_longlong(x);
+  const u64 lo = (u32)__builtin_amdgcn_readlane((int)(u32)b, l);
+  const u64 hi = (u32)__builtin_amdgcn_readlane((int)(u32)(b >> 32), l);
+  return __longlong_as_double((long long)(hi << 32 | lo));
+}
+
+__device__ inline u32 wave_total_u32(u32 x) {
+  x += dpp_u32<0xB1>(x);
+  x += dpp_u32<0x4E>(x);
+  x += dpp_u32<0x141>(x);
+  x += dpp_u32<0x140>(x);
+  return (u32)__builtin_amdgcn_readlane((int)x, 0) + (u32)__builtin_amdgcn_readlane((int)x, 16) +
+         (u32)__builtin_amdgcn_readlane((int)x, 32) + (u32)__builtin_amdgcn_readlane((int)x, 48);
+}
+
+/* (s, c) pairs combined with error-free additions, as wave_sum_dd() does;
+ * lanes paired by a step hold equal pairs afterwards (two_sum is symmetric) */
+__device__ inline void wave_total_dd(double& s, double& c) {
+  auto add = [&](double os, double oc) {
+    double t, e;
+    two_sum(s, os, t, e);
+    s = t;
+    c = c + oc + e;
+  };
+  add(dpp_f64<0xB1>(s), dpp_f64<0xB1>(c));
+  add(dpp_f64<0x4E>(s), dpp_f64<0x4E>(c));
+  add(dpp_f64<0x141>(s), dpp_f64<0x141>(c));
+  add(dpp_f64<0x140>(s), dpp_f64<0x140>(c));
+  const double s1 = readlane_f64(s, 16), c1 = readlane_f64(c, 16);
+  const double s2 = readlane_f64(s, 32), c2 = readlane_f64(c, 32);
+  const double s3 = readlane_f64(s, 48), c3 = readlane_f64(c, 48);
+  s = readlane_f64(s, 0);
+  c = readlane_f64(c, 0);
+  add(s1, c1);
+  add(s2, c2);
+  add(s3, c3);
+}
+
 __device__ inline void flush_counts(u64* slot, const u32 (&c)[C_N]) {
   bool any = false;
 #pragma unroll
@@ -207,14 +263,14 @@ __device__ inline void block_flush(const LevelArgs& a, Acc2& accR, Acc2& accE, A
                                    const u32* pre = nullptr) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int n_w = (blockDim.x + 63) >> 6;
-  wave_sum_dd(accR.s, accR.c);
-  wave_sum_dd(accE.s, accE.c);
-  wave_sum_dd(accD.s, accD.c);
+  wave_total_dd(accR.s, accR.c);
+  wave_total_dd(accE.s, accE.c);
+  wave_total_dd(accD.s, accD.c);
   double* sd = (double*)scratch;      /* [n_w][6] */
   u32* sc = (u32*)(sd + 6 * n_w);     /* [n_w][C_N] */
   u32 t[C_N];
 #pragma unroll
-  for (int j = 0; j < C_N; ++j) t[j] = (u32)wave_sum_u64(c[j]);
+  for (int j = 0; j < C_N; ++j) t[j] = wave_total_u32(c[j]);
   if (lane == 0) {
     sd[wave * 6 + 0] = accR.s;
     sd[wave * 6 + 1] = accR.c;
