@@ -289,15 +289,28 @@ __device__ inline void block_flush(const LevelArgs& a, Acc2& accR, Acc2& accE, A
   }
   __syncthreads();
   if (threadIdx.x == 0) {
+    /* every wave's totals read at once (blocks have at most 4 waves); the
+     * missing waves of a smaller block count as exact zeros */
+    constexpr int kMaxW = 4;
+    double v[kMaxW][6];
+    u32 cv[kMaxW][C_N];
+#pragma unroll
+    for (int w = 0; w < kMaxW; ++w) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) v[w][k] = w < n_w ? sd[w * 6 + k] : 0.0;
+#pragma unroll
+      for (int j = 0; j < C_N; ++j) cv[w][j] = w < n_w ? sc[w * C_N + j] : 0u;
+    }
     DD* dst[3] = {&a.partR[blockIdx.x], &a.partE[blockIdx.x], &a.partD[blockIdx.x]};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       double s = 0.0, cc = 0.0;
-      for (int w = 0; w < n_w; ++w) {
+#pragma unroll
+      for (int w = 0; w < kMaxW; ++w) {
         double x, e;
-        two_sum(s, sd[w * 6 + 2 * k], x, e);
+        two_sum(s, v[w][2 * k], x, e);
         s = x;
-        cc += sd[w * 6 + 2 * k + 1] + e;
+        cc += v[w][2 * k + 1] + e;
       }
       if (s != 0.0 || cc != 0.0) {
         if (pre) { /* dd_merge_slot() on the prefetched value */
@@ -317,9 +330,11 @@ __device__ inline void block_flush(const LevelArgs& a, Acc2& accR, Acc2& accE, A
     }
     u32 tot[C_N];
     bool any = false;
+#pragma unroll
     for (int j = 0; j < C_N; ++j) {
       tot[j] = 0;
-      for (int w = 0; w < n_w; ++w) tot[j] += sc[w * C_N + j];
+#pragma unroll
+      for (int w = 0; w < kMaxW; ++w) tot[j] += cv[w][j];
       any |= tot[j] != 0;
     }
     if (any) {
