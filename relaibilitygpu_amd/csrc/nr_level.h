@@ -117,8 +117,6 @@ __device__ inline int segment_of(const u32* sp, int n_seg, u32 base) {
   return lo;
 }
 
-/* Exclusive prefix of the per-segment counts into sp[0..n_seg] (LDS), by
- * the whole workgroup; `part` is blockDim.x words of LDS scratch. */
 /* Exclusive prefix of the segment counts into sp[0..n_seg] (sp[n_seg] is
  * the total).  The counts are loaded first and `between` runs while they are
  * in flight (the level kernels stage their graph tables there); then a
@@ -281,68 +279,60 @@ __device__ inline void block_flush(const LevelArgs& a, Acc2& accR, Acc2& accE, A
 #pragma unroll
     for (int j = 0; j < C_N; ++j) sc[wave * C_N + j] = t[j];
   }
-  /* prefetched slot values of wave 0, as wave-uniform scalars */
-  u32 pv[4 * 3 + 2 * C_N];
-  if (pre && wave == 0) {
-#pragma unroll
-    for (int i = 0; i < 4 * 3 + 2 * C_N; ++i) pv[i] = __builtin_amdgcn_readlane(*pre, i);
-  }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    /* every wave's totals read at once (blocks have at most 4 waves); the
-     * missing waves of a smaller block count as exact zeros */
-    constexpr int kMaxW = 4;
-    double v[kMaxW][6];
-    u32 cv[kMaxW][C_N];
+  if (wave != 0) return;
+  /* wave 0 merges the waves' totals in wave order: lane k < 3 the double-
+   * double k, lane j < C_N counter j (a few registers per lane); the slot's
+   * prefetched dwords (lane i of wave 0 holds dword i) arrive by shuffles */
+  const u32 pw = pre ? *pre : 0u;
+  const int kd = lane < 3 ? lane : 0, kc = lane < C_N ? lane : 0;
+  const u32 d0 = __shfl(pw, 4 * kd, 64), d1 = __shfl(pw, 4 * kd + 1, 64);
+  const u32 d2 = __shfl(pw, 4 * kd + 2, 64), d3 = __shfl(pw, 4 * kd + 3, 64);
+  const u32 c0 = __shfl(pw, 12 + 2 * kc, 64), c1 = __shfl(pw, 12 + 2 * kc + 1, 64);
+  constexpr int kMaxW = 4; /* launch bounds: at most 256 threads */
+  if (lane < 3) {
+    double v[kMaxW][2];
 #pragma unroll
     for (int w = 0; w < kMaxW; ++w) {
-#pragma unroll
-      for (int k = 0; k < 6; ++k) v[w][k] = w < n_w ? sd[w * 6 + k] : 0.0;
-#pragma unroll
-      for (int j = 0; j < C_N; ++j) cv[w][j] = w < n_w ? sc[w * C_N + j] : 0u;
+      v[w][0] = w < n_w ? sd[w * 6 + 2 * kd] : 0.0; /* a missing wave adds exact zeros */
+      v[w][1] = w < n_w ? sd[w * 6 + 2 * kd + 1] : 0.0;
     }
-    DD* dst[3] = {&a.partR[blockIdx.x], &a.partE[blockIdx.x], &a.partD[blockIdx.x]};
+    double s = 0.0, cc = 0.0;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      double s = 0.0, cc = 0.0;
-#pragma unroll
-      for (int w = 0; w < kMaxW; ++w) {
-        double x, e;
-        two_sum(s, v[w][2 * k], x, e);
-        s = x;
-        cc += v[w][2 * k + 1] + e;
-      }
-      if (s != 0.0 || cc != 0.0) {
-        if (pre) { /* dd_merge_slot() on the prefetched value */
-          const double ohi = dword_pair(pv[4 * k], pv[4 * k + 1]);
-          const double olo = dword_pair(pv[4 * k + 2], pv[4 * k + 3]);
-          double t, e;
-          two_sum(ohi, s, t, e);
-          const double lo = olo + cc + e;
-          double hi2, e2;
-          two_sum(t, lo, hi2, e2);
-          dst[k]->hi = hi2;
-          dst[k]->lo = e2;
-        } else {
-          dd_merge_slot(dst[k], s, cc);
-        }
+    for (int w = 0; w < kMaxW; ++w) {
+      double x, e;
+      two_sum(s, v[w][0], x, e);
+      s = x;
+      cc += v[w][1] + e;
+    }
+    if (s != 0.0 || cc != 0.0) {
+      DD* dst = kd == 0 ? &a.partR[blockIdx.x] : kd == 1 ? &a.partE[blockIdx.x] : &a.partD[blockIdx.x];
+      if (pre) { /* dd_merge_slot() on the prefetched value */
+        const double ohi = dword_pair(d0, d1);
+        const double olo = dword_pair(d2, d3);
+        double t, e;
+        two_sum(ohi, s, t, e);
+        const double lo = olo + cc + e;
+        double hi2, e2;
+        two_sum(t, lo, hi2, e2);
+        dst->hi = hi2;
+        dst->lo = e2;
+      } else {
+        dd_merge_slot(dst, s, cc);
       }
     }
-    u32 tot[C_N];
-    bool any = false;
+  }
+  u32 tot = 0;
+  if (lane < C_N) {
+    u32 cv[kMaxW];
 #pragma unroll
-    for (int j = 0; j < C_N; ++j) {
-      tot[j] = 0;
+    for (int w = 0; w < kMaxW; ++w) cv[w] = w < n_w ? sc[w * C_N + kc] : 0u;
 #pragma unroll
-      for (int w = 0; w < kMaxW; ++w) tot[j] += cv[w][j];
-      any |= tot[j] != 0;
-    }
-    if (any) {
-      u64* slot = &a.cnt[(u64)blockIdx.x * C_N];
-#pragma unroll
-      for (int j = 0; j < C_N; ++j)
-        slot[j] = (pre ? ((u64)pv[12 + 2 * j + 1] << 32 | pv[12 + 2 * j]) : slot[j]) + tot[j];
-    }
+    for (int w = 0; w < kMaxW; ++w) tot += cv[w];
+  }
+  if (__ballot(tot != 0) && lane < C_N) {
+    u64* slot = &a.cnt[(u64)blockIdx.x * C_N + kc];
+    *slot = (pre ? ((u64)c1 << 32 | c0) : *slot) + tot;
   }
 }
 
