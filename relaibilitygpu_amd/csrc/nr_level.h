@@ -117,6 +117,22 @@ __device__ inline int segment_of(const u32* sp, int n_seg, u32 base) {
   return lo;
 }
 
+/* segment_of() by a whole wave (all 64 lanes active, `base` wave-uniform,
+ * n_seg <= 4096): each lane probes one of 64 evenly spaced boundaries, the
+ * ballot's highest hit picks the stretch, and one more probe per lane
+ * inside it finds the segment — two LDS rounds instead of log2(n_seg).
+ * The lanes that see sp[.] <= base form a prefix (sp is non-decreasing). */
+__device__ inline int wave_segment_of(const u32* sp, int n_seg, u32 base) {
+  const int lane = threadIdx.x & 63;
+  const int step = (n_seg + 63) >> 6;
+  const int p = lane * step;
+  u64 m = __ballot(p < n_seg && sp[p] <= base);
+  const int lo = (63 - __builtin_clzll(m)) * step;
+  const int q = lo + lane;
+  m = __ballot(lane < step && q < n_seg && sp[q] <= base);
+  return lo + 63 - __builtin_clzll(m);
+}
+
 /* Exclusive prefix of the segment counts into sp[0..n_seg] (sp[n_seg] is
  * the total).  The counts are loaded first and `between` runs while they are
  * in flight (the level kernels stage their graph tables there); then a
@@ -490,8 +506,10 @@ __device__ __forceinline__ void expand_level_items(const LevelArgs& a, const EX&
     int fork_e = 0;
     /* past the time budget: the wave's items stay pending (carried) */
     const bool late = t_end && wall_clock64() >= t_end;
+    const int s_base = a.n_seg <= 4096 ? wave_segment_of(sp, a.n_seg, (u32)base)
+                                       : segment_of(sp, a.n_seg, (u32)base);
     if ((u64)lane < chunk && i < n_items) {
-      int s = segment_of(sp, a.n_seg, (u32)base);
+      int s = s_base;
       while (sp[s + 1] <= i) ++s;
       slot_in = (u64)s * a.seg_cap + (i - sp[s]);
       load_item<W>(in, a.stride, slot_in, known, up, mass);
