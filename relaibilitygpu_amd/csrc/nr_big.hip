@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(256) k_expand_big(LevelArgs a, int parity, Big
   u32 c[C_N] = {0, 0, 0, 0, 0, 0, 0, 0};
 
   for (u64 i = wave_id; i < n_items; i += n_waves) { /* one item per wave */
-    int s = segment_of(sp, a.n_seg, (u32)i);
+    int s = a.n_seg <= 4096 ? wave_segment_of(sp, a.n_seg, (u32)i) : segment_of(sp, a.n_seg, (u32)i);
     while (sp[s + 1] <= (u32)i) ++s;
     const u64 slot_in = (u64)s * a.seg_cap + (i - sp[s]);
     const u64* ip = in.known + slot_in * row;
