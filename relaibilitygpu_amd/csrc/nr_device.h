@@ -244,10 +244,19 @@ struct Expander {
     pivot = g.pivot;
     lk = g.lk;
   }
-  /* the item's rows into this lane's slice */
+  /* the item's rows into this lane's slice: 16 loads in flight before the
+   * first LDS store waits (one memory round for n <= 16) */
   __device__ void load(const QBuf& in, u64 qstride, u64 slot) const {
     const NS* rin = (const NS*)in.rows + slot;
-    for (int u = 0; u < n; ++u) rows[u * stride] = rin[(u64)u * qstride];
+    for (int u0 = 0; u0 < n; u0 += 16) {
+      NS v[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j)
+        if (u0 + j < n) v[j] = rin[(u64)(u0 + j) * qstride];
+#pragma unroll
+      for (int j = 0; j < 16; ++j)
+        if (u0 + j < n) rows[(u0 + j) * stride] = v[j];
+    }
   }
   /* rows of an item given by its planes alone */
   __device__ void derive(const u64 (&known)[W], const u64 (&up)[W]) const {
