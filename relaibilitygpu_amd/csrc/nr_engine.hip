@@ -702,6 +702,15 @@ hipError_t& last_hip_error() {
  * partial-sum slots, control block (device + pinned host copy), a stream
  * and an event pool.  Buffers grow on demand and are reused, so repeated
  * solves pay no allocation.  One mutex per device serialises calls. */
+/* Graph tables packed into one buffer: p, q (FP64), static neighbour sets,
+ * incident-edge sets, edge endpoints, incident counts, the (u,v) -> edge id
+ * matrix.  Offsets are 16-byte aligned. */
+struct TableBlob {
+  std::vector<unsigned char> bytes;
+  size_t off_p, off_q, off_sadj, off_inc, off_ep, off_deg, off_eidm;
+  size_t off_rp, off_arc, off_irp, off_iarc, off_terms, off_lk;
+};
+
 struct Workspace {
   bool ready = false;
   int device = 0;
@@ -725,6 +734,12 @@ struct Workspace {
   size_t bo_desc = 0, bo_tabs = 0, bo_roots = 0, bo_perm = 0, bo_perm2 = 0;
   std::vector<hipEvent_t> events;
   std::mutex mu;
+  /* the graph whose tables ws.tab holds (pointer, stamp, plane words) and
+   * their host image, kept so repeated solves of one graph upload nothing */
+  TableBlob tab_blob;
+  const nr_graph* tab_g = nullptr;
+  unsigned long long tab_stamp = 0;
+  int tab_W = 0;
 
   int grow(void** p, size_t* have, size_t need) {
     if (need <= *have) return NR_OK;
@@ -736,6 +751,33 @@ struct Workspace {
       return NR_ERR_NOMEM;
     }
     *have = need;
+    return NR_OK;
+  }
+  /* per-kernel launch facts, looked up once: the dynamic-LDS limit set on a
+   * kernel, and the occupancy query per (kernel, block, LDS bytes) */
+  struct Occ {
+    const void* k;
+    int block, lds, per_cu;
+  };
+  std::vector<std::pair<const void*, int>> lds_attr;
+  std::vector<Occ> occ;
+  int allow_lds(const void* k, int bytes) {
+    for (const auto& x : lds_attr)
+      if (x.first == k && x.second == bytes) return NR_OK;
+    NR_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    lds_attr.push_back({k, bytes});
+    return NR_OK;
+  }
+  int blocks_per_cu(const void* k, int block, int lds, int* out) {
+    for (const Occ& o : occ)
+      if (o.k == k && o.block == block && o.lds == lds) {
+        *out = o.per_cu;
+        return NR_OK;
+      }
+    int n = 0;
+    NR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, block, lds));
+    occ.push_back({k, block, lds, n});
+    *out = n;
     return NR_OK;
   }
   int event(size_t i, hipEvent_t* e) {
@@ -770,6 +812,9 @@ struct Workspace {
     btab = nullptr;
     btab_bytes = 0;
     bkey.clear();
+    tab_g = nullptr; /* the device tables are gone */
+    lds_attr.clear();
+    occ.clear();
     if (meta) (void)hipFree(meta);
     if (ctl) (void)hipFree(ctl);
     if (hctl) (void)hipHostFree(hctl);
@@ -783,15 +828,6 @@ struct Workspace {
 
 constexpr int kMaxDevices = 64;
 Workspace g_ws[kMaxDevices];
-
-/* Graph tables packed into one buffer: p, q (FP64), static neighbour sets,
- * incident-edge sets, edge endpoints, incident counts, the (u,v) -> edge id
- * matrix.  Offsets are 16-byte aligned. */
-struct TableBlob {
-  std::vector<unsigned char> bytes;
-  size_t off_p, off_q, off_sadj, off_inc, off_ep, off_deg, off_eidm;
-  size_t off_rp, off_arc, off_irp, off_iarc, off_terms, off_lk;
-};
 
 void pack_tables(const nr_graph* g, int W, TableBlob& b) {
   const int n = g->n, e = std::max(g->e, 1);
@@ -885,12 +921,20 @@ void pack_tables(const nr_graph* g, int W, TableBlob& b) {
 }
 
 int upload_tables(Workspace& ws, const nr_graph* g, int W, DevGraph& d) {
-  TableBlob b;
-  pack_tables(g, W, b);
-  int rc = ws.grow(&ws.tab, &ws.tab_bytes, b.bytes.size());
-  if (rc) return rc;
-  NR_HIP(hipMemcpyAsync(ws.tab, b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice,
-                        ws.stream));
+  /* the device copy stays valid while the same graph (pointer and stamp:
+   * a new graph or new terminals get a new stamp) comes back with the same W */
+  TableBlob& b = ws.tab_blob;
+  if (!(ws.tab_g == g && ws.tab_stamp == g->stamp && ws.tab_W == W)) {
+    ws.tab_g = nullptr;
+    pack_tables(g, W, b);
+    int rc = ws.grow(&ws.tab, &ws.tab_bytes, b.bytes.size());
+    if (rc) return rc;
+    NR_HIP(hipMemcpyAsync(ws.tab, b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice,
+                          ws.stream));
+    ws.tab_g = g;
+    ws.tab_stamp = g->stamp;
+    ws.tab_W = W;
+  }
   unsigned char* base = (unsigned char*)ws.tab;
   d.p = (const double*)(base + b.off_p);
   d.q = (const double*)(base + b.off_q);
@@ -1315,16 +1359,16 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
       choose_block(var, g, W, cfg->block_size, 4 * (1026 + 256 + kClasses) + tile_bytes);
   if (block <= 0) return NR_ERR_UNSUPPORTED;
   Layout lay0 = layout_for(var, g, W, block, 4 * (block + kClasses) + tile_bytes);
-  NR_HIP(hipFuncSetAttribute(kfun, hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
+  if ((rc = ws.allow_lds(kfun, 64 * 1024))) return rc;
   int per_cu0 = 0;
-  NR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu0, kfun, block, lay0.total_bytes));
+  if ((rc = ws.blocks_per_cu(kfun, block, lay0.total_bytes, &per_cu0))) return rc;
   if (per_cu0 < 1) return NR_ERR_UNSUPPORTED;
   int grid = cfg->grid_blocks > 0 ? cfg->grid_blocks : per_cu0 * cus;
   const int extra = 4 * (grid + 2 + block + kClasses) + tile_bytes;
   Layout lay = layout_for(var, g, W, block, extra);
   if (lay.total_bytes > 64 * 1024) return NR_ERR_UNSUPPORTED;
   int per_cu = 0;
-  NR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfun, block, lay.total_bytes));
+  if ((rc = ws.blocks_per_cu(kfun, block, lay.total_bytes, &per_cu))) return rc;
   if (per_cu < 1) return NR_ERR_UNSUPPORTED;
   if (cfg->grid_blocks <= 0) grid = std::min(grid, per_cu * cus);
   const u64 n_slots = (u64)grid; /* partial sums and counters per workgroup */
@@ -1521,7 +1565,7 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
     const u32 max_in = std::min<u32>(256u, std::min<u32>((u32)block, seg_cap / per));
     if (max_in >= 1) {
       const void* ks = (const void*)var.small();
-      NR_HIP(hipFuncSetAttribute(ks, hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
+      if ((rc = ws.allow_lds(ks, 64 * 1024))) return rc;
       hipEvent_t kb = nullptr, ke = nullptr;
       if ((size_t)timed < max_timed) {
         if ((rc = ws.event(2 + 2 * timed, &kb)) || (rc = ws.event(3 + 2 * timed, &ke))) return rc;

@@ -8,6 +8,8 @@ import random
 
 import pytest
 
+from conftest import graph_path
+
 pytestmark = pytest.mark.gpu
 
 nr = pytest.importorskip("relaibilitygpu_amd")
@@ -128,3 +130,42 @@ def test_directed_graph_boundaries():
         g = nr.Graph.from_csr(rp, ci, ei, probs, 0, nn - 1)
         og = OracleGraph.from_csr(rp, ci, ei, probs)
         check(g, og, 0, nn - 1, 1e-6)
+
+
+def test_device_tables_follow_the_graph():
+    """Repeated solves reuse the uploaded graph tables only while the graph
+    (and its terminals) is the same: new terminals, a terminal set, another
+    graph in between and a graph made at a freed graph's address all give the
+    fresh graph's result."""
+    print("case table reuse", flush=True)
+    k8 = graph_path("COMP8.txt")
+    k9 = graph_path("COMP9.txt")
+
+    def fresh(path, s, t, terms=None):
+        g = nr.Graph.load(path, s, t)
+        if terms:
+            g.set_terminal_set(terms)
+        r = nr.solve(g, truncation_eps=0.0, queue_capacity=1 << 22)
+        g.close()
+        return r
+
+    def same(r, o):
+        assert (r["paths"], r["nodes"], r["iterations"]) == (o["paths"], o["nodes"], o["iterations"])
+        assert r["R"] == o["R"]
+
+    g = nr.Graph.load(k8, 0, 7)
+    same(nr.solve(g, truncation_eps=0.0, queue_capacity=1 << 22), fresh(k8, 0, 7))
+    same(nr.solve(g, truncation_eps=0.0, queue_capacity=1 << 22), fresh(k8, 0, 7))
+    g.set_terminals(2, 5)
+    same(nr.solve(g, truncation_eps=0.0, queue_capacity=1 << 22), fresh(k8, 2, 5))
+    h = nr.Graph.load(k9, 0, 8)
+    same(nr.solve(h, truncation_eps=0.0, queue_capacity=1 << 22), fresh(k9, 0, 8))
+    same(nr.solve(g, truncation_eps=0.0, queue_capacity=1 << 22), fresh(k8, 2, 5))
+    g.set_terminal_set([0, 3, 7])
+    same(nr.solve(g, truncation_eps=0.0, queue_capacity=1 << 22), fresh(k8, 0, 7, [0, 3, 7]))
+    g.close()
+    h.close()
+    for _ in range(3):  # new graphs, possibly at the freed addresses
+        g2 = nr.Graph.load(k9, 1, 6)
+        same(nr.solve(g2, truncation_eps=0.0, queue_capacity=1 << 22), fresh(k9, 1, 6))
+        g2.close()
