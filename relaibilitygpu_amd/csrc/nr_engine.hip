@@ -724,6 +724,8 @@ struct Workspace {
   size_t meta_bytes = 0;
   LevelCtl* ctl = nullptr;
   LevelCtl* hctl = nullptr; /* pinned */
+  u64* hmirror = nullptr;   /* pinned [3], written by the level kernel */
+  u64* dmirror = nullptr;   /* its device address */
   /* device copy of the last batch's instance tables, roots and admission
    * order, reused while the same graphs (pointer and stamp) come back */
   void* btab = nullptr;
@@ -797,6 +799,8 @@ struct Workspace {
     NR_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     NR_HIP(hipMalloc((void**)&ctl, sizeof(LevelCtl)));
     NR_HIP(hipHostMalloc((void**)&hctl, sizeof(LevelCtl), hipHostMallocDefault));
+    NR_HIP(hipHostMalloc((void**)&hmirror, 4 * sizeof(u64), hipHostMallocDefault));
+    NR_HIP(hipHostGetDevicePointer((void**)&dmirror, hmirror, 0));
     ready = true;
     return NR_OK;
   }
@@ -818,6 +822,8 @@ struct Workspace {
     if (meta) (void)hipFree(meta);
     if (ctl) (void)hipFree(ctl);
     if (hctl) (void)hipHostFree(hctl);
+    if (hmirror) (void)hipHostFree(hmirror);
+    hmirror = dmirror = nullptr;
     (void)hipStreamDestroy(stream);
     q = tab = meta = nullptr;
     ctl = hctl = nullptr;
@@ -1444,6 +1450,7 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
     a.hist = q + 2 * grid + 8;
   }
   a.ctl = ws.ctl;
+  a.hmirror = ws.dmirror;
   a.mass_mode = mass ? 1 : 0;
   a.thr_class = kClasses;
   a.drop_class = kClasses;
@@ -1590,12 +1597,11 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
       const int launched = chunk;
       for (int c = 0; c < chunk; ++c, ++level)
         if ((rc = launch_level(level & 1, level))) return rc;
-      NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
-      NR_HIP(hipStreamSynchronize(st));
-      if (ws.hctl->last_items == 0) finished = true;
+      NR_HIP(hipStreamSynchronize(st)); /* the last launch wrote ws.hmirror */
+      const u64 w = ws.hmirror[0], max_w = ws.hmirror[2];
+      if (w == 0) finished = true;
       /* past the widest level: shorter chunks as the levels thin out, so
        * few launches run after the last level */
-      const u64 w = ws.hctl->last_items, max_w = ws.hctl->max_width;
       const bool shrinking = w < max_w;
       if (!finished && expired()) {
         if (save)

@@ -53,6 +53,10 @@ struct LevelArgs {
   u32* seg_count[2]; /* [n_seg] items written by each workgroup */
   u32* hist;         /* [n_seg][kClasses] class histogram of each segment */
   LevelCtl* ctl;
+  /* pinned host words [last_items, iterations, max_width] that workgroup 0
+   * of each level launch also writes (read by the host after a stream
+   * synchronisation, so no copy is queued for them), or null */
+  u64* hmirror;
   DD* partR;
   DD* partE;
   DD* partD;
@@ -802,9 +806,17 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
   const u64 t_end = a.budget_ticks ? a.ctl->start_tick + a.budget_ticks : 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) { /* count the level (main.cu:970-973) */
     a.ctl->last_items = n_items;
+    u64 it = a.ctl->iterations, mw = a.ctl->max_width;
     if (n_items && !(t_end && wall_clock64() >= t_end)) {
-      a.ctl->iterations += 1;
-      if (n_items > a.ctl->max_width) a.ctl->max_width = n_items;
+      it += 1;
+      if (n_items > mw) mw = n_items;
+      a.ctl->iterations = it;
+      a.ctl->max_width = mw;
+    }
+    if (a.hmirror) {
+      a.hmirror[0] = n_items;
+      a.hmirror[1] = it;
+      a.hmirror[2] = mw;
     }
   }
   if (n_items == 0) {
