@@ -177,11 +177,22 @@ __global__ void __launch_bounds__(256) k_finalize(LevelArgs a, u64 n_slots) {
   __shared__ u64 pn[C_N][256];
   const int t = threadIdx.x;
   const DD* src[3] = {a.partR, a.partE, a.partD};
+  /* per thread: slots t, t + 256, ... (Neumaier), four slots' loads issued
+   * before the first sum waits */
   for (int k = 0; k < 3; ++k) {
     double s = 0.0, c = 0.0;
-    for (u64 i = t; i < n_slots; i += blockDim.x) {
-      neu_add(s, c, src[k][i].hi);
-      neu_add(s, c, src[k][i].lo);
+    for (u64 i0 = t; i0 < n_slots; i0 += 4 * (u64)blockDim.x) {
+      DD v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const u64 i = i0 + (u64)j * blockDim.x;
+        v[j] = i < n_slots ? src[k][i] : DD{0.0, 0.0};
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        neu_add(s, c, v[j].hi);
+        neu_add(s, c, v[j].lo);
+      }
     }
     ps[k][t] = s;
     pc[k][t] = c;
@@ -206,13 +217,25 @@ __global__ void __launch_bounds__(256) k_finalize(LevelArgs a, u64 n_slots) {
     __syncthreads();
   }
   if (t == 0) {
+    u64* m = a.hmirror; /* the host reads the result here, no copy queued */
     for (int k = 0; k < 3; ++k) {
       double hi, lo;
       two_sum(ps[k][0], pc[k][0], hi, lo);
       a.ctl->sum[k] = hi;
       a.ctl->sum_lo[k] = lo;
+      if (m) {
+        m[kMirSum + k] = (u64)__double_as_longlong(hi);
+        m[kMirSumLo + k] = (u64)__double_as_longlong(lo);
+      }
     }
-    for (int j = 0; j < C_N; ++j) a.ctl->counts[j] = pn[j][0];
+    for (int j = 0; j < C_N; ++j) {
+      a.ctl->counts[j] = pn[j][0];
+      if (m) m[kMirCounts + j] = pn[j][0];
+    }
+    if (m) {
+      m[1] = a.ctl->iterations;
+      m[2] = a.ctl->max_width;
+    }
   }
 }
 
@@ -724,7 +747,7 @@ struct Workspace {
   size_t meta_bytes = 0;
   LevelCtl* ctl = nullptr;
   LevelCtl* hctl = nullptr; /* pinned */
-  u64* hmirror = nullptr;   /* pinned [3], written by the level kernel */
+  u64* hmirror = nullptr;   /* pinned [kMirWords], written by the level kernel and k_finalize */
   u64* dmirror = nullptr;   /* its device address */
   /* device copy of the last batch's instance tables, roots and admission
    * order, reused while the same graphs (pointer and stamp) come back */
@@ -799,7 +822,7 @@ struct Workspace {
     NR_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     NR_HIP(hipMalloc((void**)&ctl, sizeof(LevelCtl)));
     NR_HIP(hipHostMalloc((void**)&hctl, sizeof(LevelCtl), hipHostMallocDefault));
-    NR_HIP(hipHostMalloc((void**)&hmirror, 4 * sizeof(u64), hipHostMallocDefault));
+    NR_HIP(hipHostMalloc((void**)&hmirror, kMirWords * sizeof(u64), hipHostMallocDefault));
     NR_HIP(hipHostGetDevicePointer((void**)&dmirror, hmirror, 0));
     ready = true;
     return NR_OK;
@@ -1737,13 +1760,21 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
     NR_HIP(hipLaunchKernel((const void*)&k_finalize, dim3(1), dim3(256), args, 0, st));
   }
   NR_HIP(hipEventRecord(ev_stop, st));
-  NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
   NR_HIP(hipStreamSynchronize(st));
   float ms = 0.f;
   NR_HIP(hipEventElapsedTime(&ms, ev_start, ev_stop));
+  /* the result as k_finalize mirrored it into pinned host memory */
+  LevelCtl& h = *ws.hctl;
+  {
+    const u64* m = ws.hmirror;
+    h.iterations = m[1];
+    h.max_width = m[2];
+    std::memcpy(h.sum, m + kMirSum, sizeof h.sum);
+    std::memcpy(h.sum_lo, m + kMirSumLo, sizeof h.sum_lo);
+    for (int j = 0; j < C_N; ++j) h.counts[j] = m[kMirCounts + j];
+  }
   /* summed time of every expand launch, including the few empty launches
    * enqueued past the last level of the final chunk */
-  const LevelCtl& h = *ws.hctl;
   double kms = 0.0;
   for (long long i = 0; i < timed; ++i) {
     float x = 0.f;

@@ -46,6 +46,11 @@ struct LevelCtl {
 };
 
 
+/* Words of the pinned host mirror: [0] last_items, [1] iterations,
+ * [2] max_width (written by each level launch), then the final sums, their
+ * low parts and the counters (written by k_finalize). */
+constexpr int kMirSum = 3, kMirSumLo = 6, kMirCounts = 9, kMirWords = 9 + C_N;
+
 struct LevelArgs {
   DevGraph g;
   Layout lay;
@@ -53,9 +58,10 @@ struct LevelArgs {
   u32* seg_count[2]; /* [n_seg] items written by each workgroup */
   u32* hist;         /* [n_seg][kClasses] class histogram of each segment */
   LevelCtl* ctl;
-  /* pinned host words [last_items, iterations, max_width] that workgroup 0
-   * of each level launch also writes (read by the host after a stream
-   * synchronisation, so no copy is queued for them), or null */
+  /* pinned host mirror (kMir* words) of the counters the host reads at a
+   * chunk boundary and at the end, written by workgroup 0 of each level
+   * launch and by k_finalize; read after a stream synchronisation, so no
+   * copy is queued for them; or null */
   u64* hmirror;
   DD* partR;
   DD* partE;
