@@ -1615,12 +1615,24 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
     /* level order: chunks of back-to-back level launches; under a time
      * budget a chunk is sized from the measured time per launch so the
      * deadline is checked at least every quarter of the remaining budget */
+    /* without a budget or a checkpoint to honour, the next chunk's first two
+     * launches are queued before the host waits for this chunk, so the GPU
+     * keeps working while the host decides (at most two launches run past
+     * the last level); an empty level in the mirror, whichever launch wrote
+     * it, means the queue is empty for good */
+    const int ahead = (!has_budget && !save) ? 2 : 0;
+    hipEvent_t ev_chunk = nullptr;
+    if ((rc = ws.event(2 + 2 * max_timed, &ev_chunk))) return rc;
+    int queued = 0; /* launches of the coming chunk already queued */
     while (!finished) {
       const auto c0 = std::chrono::steady_clock::now();
       const int launched = chunk;
-      for (int c = 0; c < chunk; ++c, ++level)
+      for (int c = queued; c < chunk; ++c, ++level)
         if ((rc = launch_level(level & 1, level))) return rc;
-      NR_HIP(hipStreamSynchronize(st)); /* the last launch wrote ws.hmirror */
+      NR_HIP(hipEventRecord(ev_chunk, st));
+      for (queued = 0; queued < ahead; ++queued, ++level)
+        if ((rc = launch_level(level & 1, level))) return rc;
+      NR_HIP(hipEventSynchronize(ev_chunk)); /* its last launch wrote ws.hmirror */
       const u64 w = ws.hmirror[0], max_w = ws.hmirror[2];
       if (w == 0) finished = true;
       /* past the widest level: shorter chunks as the levels thin out, so
