@@ -901,12 +901,13 @@ __global__ void __launch_bounds__(256) k_small_levels(LevelArgs a, u32 max_in, i
   const int lane = threadIdx.x & 63;
   int p = 0;
   u32 n = a.seg_count[0][0];
+  /* levels counted in registers (main.cu:970-973) and written once at the
+   * end: a global read-modify-write per level would stall every barrier */
+  u64 levels = 0, widest = 0;
   for (int lv = 0; n > 0 && n <= max_in && lv < max_levels; ++lv) {
-    if (threadIdx.x == 0) {
-      *s_cnt = 0;
-      a.ctl->iterations += 1; /* main.cu:970-973 */
-      if (n > a.ctl->max_width) a.ctl->max_width = n;
-    }
+    if (threadIdx.x == 0) *s_cnt = 0;
+    levels += 1;
+    widest = n > widest ? n : widest;
     __syncthreads();
     const QBuf in = a.buf[p];
     const QBuf out = a.buf[p ^ 1];
@@ -1005,6 +1006,8 @@ __global__ void __launch_bounds__(256) k_small_levels(LevelArgs a, u32 max_in, i
   if (threadIdx.x == 0) {
     a.seg_count[0][0] = n;
     a.ctl->last_items = n;
+    a.ctl->iterations += levels;
+    if (widest > a.ctl->max_width) a.ctl->max_width = widest;
   }
   block_flush(a, accR, accE, accD, c, (unsigned char*)part);
 }
