@@ -372,6 +372,15 @@ __device__ inline void block_flush(const LevelArgs& a, Acc2& accR, Acc2& accE, A
  * slots, every store instruction covers consecutive slots, and no lane idles
  * while another works through a long path.  Every lane of the wave calls
  * it. */
+/* Orders one wave's LDS accesses across its lanes (a wave barrier between
+ * wavefront-scope fences): lanes that read what other lanes of the wave
+ * stored in LDS call it in between. */
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int W, class EX>
 __device__ inline void emit_children_wave(const EX& ex, const QBuf& out, u64 qstride,
                                           u64 out_base, u32 seg_cap, u32 wbase, int incl, int k,
@@ -379,6 +388,7 @@ __device__ inline void emit_children_wave(const EX& ex, const QBuf& out, u64 qst
                                           Acc2& accE, u32 (&c)[C_N]) {
   const int lane = threadIdx.x & 63;
   const int total = __shfl(incl, 63, 64);
+  wave_lds_sync(); /* the owners' hop stacks and rows, stored by their lanes */
   for (int c0 = 0; c0 < total; c0 += 64) {
     const int cc = c0 + lane;
     const int target = cc < total ? cc : total - 1;
