@@ -684,11 +684,9 @@ __device__ __forceinline__ void expand_level_items(const LevelArgs& a, const EX&
 #pragma unroll
         for (int x = 0; x < kFeat; x += 2)
           trow[x / 2] = (u32)bf16_bits(f[x]) | ((u32)bf16_bits(f[x + 1]) << 16);
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        wave_lds_sync(); /* every lane's row before the tile is read */
         const float y = mlp_score_tile(afrag, sign, b2, tile);
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
+        wave_lds_sync(); /* the tile read before the next round rewrites it */
         if (has) {
           const int wi = e >> 6;
           const u64 bit = 1ull << (e & 63);
