@@ -340,8 +340,9 @@ def main() -> int:
         return 0
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
-        print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this "
-              "build (the path would shard by subtree, see DESIGN.md)", file=sys.stderr)
+        print("bench.py: this engine runs on one GPU; multi-GPU runs are not supported "
+              "(the solve would shard by subtree: frontier items dealt to the GPUs, five "
+              "scalars all-reduced at the end)", file=sys.stderr)
         return 2
 
     import relaibilitygpu_amd as nr
