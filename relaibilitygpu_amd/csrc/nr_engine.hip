@@ -235,6 +235,8 @@ __global__ void __launch_bounds__(256) k_finalize(LevelArgs a, u64 n_slots) {
     if (m) {
       m[1] = a.ctl->iterations;
       m[2] = a.ctl->max_width;
+      __threadfence_system(); /* the result words before the stamp */
+      m[kMirSeq] = a.seq;
     }
   }
 }
@@ -748,6 +750,7 @@ struct Workspace {
   LevelCtl* ctl = nullptr;
   LevelCtl* hctl = nullptr; /* pinned */
   u64* hmirror = nullptr;   /* pinned [kMirWords], written by the level kernel and k_finalize */
+  u64 solves = 0;           /* sequence number of the last solve */
   u64* dmirror = nullptr;   /* its device address */
   /* device copy of the last batch's instance tables, roots and admission
    * order, reused while the same graphs (pointer and stamp) come back */
@@ -1474,6 +1477,11 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   }
   a.ctl = ws.ctl;
   a.hmirror = ws.dmirror;
+  a.seq = ++ws.solves;
+  /* until this solve's first launch writes it, the level word reads as
+   * unfinished: a chunk boundary never sees the last solve's empty level */
+  ws.hmirror[0] = ~0ull;
+  ws.hmirror[kMirSeq] = 0;
   a.mass_mode = mass ? 1 : 0;
   a.thr_class = kClasses;
   a.drop_class = kClasses;
@@ -1775,9 +1783,12 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   NR_HIP(hipStreamSynchronize(st));
   float ms = 0.f;
   NR_HIP(hipEventElapsedTime(&ms, ev_start, ev_stop));
-  /* the result as k_finalize mirrored it into pinned host memory */
+  /* the result as k_finalize mirrored it into pinned host memory; should
+   * its stamp not be there yet, the device copy is read instead */
   LevelCtl& h = *ws.hctl;
-  {
+  if (__atomic_load_n(&ws.hmirror[kMirSeq], __ATOMIC_ACQUIRE) != a.seq) {
+    NR_HIP(hipMemcpy(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost));
+  } else {
     const u64* m = ws.hmirror;
     h.iterations = m[1];
     h.max_width = m[2];

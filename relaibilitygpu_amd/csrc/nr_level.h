@@ -48,8 +48,9 @@ struct LevelCtl {
 
 /* Words of the pinned host mirror: [0] last_items, [1] iterations,
  * [2] max_width (written by each level launch), then the final sums, their
- * low parts and the counters (written by k_finalize). */
-constexpr int kMirSum = 3, kMirSumLo = 6, kMirCounts = 9, kMirWords = 9 + C_N;
+ * low parts, the counters and the solve's sequence number (written by
+ * k_finalize, the sequence number last). */
+constexpr int kMirSum = 3, kMirSumLo = 6, kMirCounts = 9, kMirSeq = 9 + C_N, kMirWords = 10 + C_N;
 
 struct LevelArgs {
   DevGraph g;
@@ -63,6 +64,7 @@ struct LevelArgs {
    * launch and by k_finalize; read after a stream synchronisation, so no
    * copy is queued for them; or null */
   u64* hmirror;
+  u64 seq; /* this solve's number, stamped into the mirror by k_finalize */
   DD* partR;
   DD* partE;
   DD* partD;
