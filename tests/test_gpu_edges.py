@@ -169,3 +169,21 @@ def test_device_tables_follow_the_graph():
         g2 = nr.Graph.load(k9, 1, 6)
         same(nr.solve(g2, truncation_eps=0.0, queue_capacity=1 << 22), fresh(k9, 1, 6))
         g2.close()
+
+
+@pytest.mark.parametrize("block,grid", [(64, 37), (128, 100), (256, 1), (256, 5000), (64, 4097)])
+def test_launch_shapes_same_tree(expected, block, grid):
+    """Level launches at other workgroup sizes and grids (one workgroup, a
+    grid past 4096 segments, more than eight segment counts per thread in
+    the prefix, blocks of one and two waves) build the same tree: the
+    reference log's K9 counts, and R equal to the default launch's."""
+    print(f"case launch shape block={block} grid={grid}", flush=True)
+    g = nr.Graph.load(graph_path("COMP9.txt"), 0, 8)
+    base = nr.solve(g, truncation_eps=0.0, queue_capacity=1 << 22)
+    r = nr.solve(g, truncation_eps=0.0, queue_capacity=1 << 22, block_size=block, grid_blocks=grid)
+    assert (r["block_size"], r["grid_blocks"]) == (block, grid)
+    lg = [x for x in expected["logged_counts"] if x["graph"] == "COMP9.txt"][0]
+    assert (r["paths"], r["nodes"], r["iterations"], r["max_queue_items"]) == (
+        lg["paths"], lg["nodes"], lg["iterations"], lg["max_width"])
+    assert r["overflowed"] == 0
+    assert close(r["R"], base["R"])
