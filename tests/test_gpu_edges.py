@@ -187,3 +187,28 @@ def test_launch_shapes_same_tree(expected, block, grid):
         lg["paths"], lg["nodes"], lg["iterations"], lg["max_width"])
     assert r["overflowed"] == 0
     assert close(r["R"], base["R"])
+
+
+@pytest.mark.parametrize("form", ["csr", "alternate", "forkbest"])
+@pytest.mark.parametrize("block,grid", [(64, 4097), (128, 77)])
+def test_launch_shapes_other_forms(form, block, grid):
+    """The CSR form (a directed 60-node graph), the alternating rule and
+    forkBest (K9) at other launch shapes: counts equal the default launch's,
+    R to 1e-12."""
+    print(f"case launch shape {form} block={block} grid={grid}", flush=True)
+    if form == "csr":
+        n, edges, probs = wl.random_digraph(60, 2.5, 0.6, 0.95, 0xD1003)
+        rp, ci, ei = wl.to_csr_directed(n, edges)
+        g = nr.Graph.from_csr(rp, ci, ei, probs, 0, n - 1)
+        kw = dict(truncation_eps=1e-7, queue_capacity=1 << 22)
+    else:
+        g = nr.Graph.load(graph_path("COMP9.txt"), 0, 8)
+        kw = dict(truncation_eps=0.0, queue_capacity=1 << 22, rule=form)
+    base = nr.solve(g, **kw)
+    r = nr.solve(g, block_size=block, grid_blocks=grid, **kw)
+    assert (r["block_size"], r["grid_blocks"]) == (block, grid)
+    assert (r["paths"], r["nodes"], r["dead"], r["truncated"], r["iterations"],
+            r["max_queue_items"]) == (base["paths"], base["nodes"], base["dead"],
+                                      base["truncated"], base["iterations"], base["max_queue_items"])
+    assert r["overflowed"] == 0 == base["overflowed"]
+    assert close(r["R"], base["R"]) and close(r["ESP"], base["ESP"])
