@@ -4,12 +4,15 @@
  *
  *   network_reliability <graph_file> <src> <dst> [--truncation_eps f]
  *       [--queue_capacity n] [--thresh_high f] [--thresh_low f]
- *       [--order level|mass|learned] [--rule path|forkbest] [--model file]
+ *       [--order level|mass|learned] [--rule path|forkbest|alternate|pivot]
+ *       [--sweep_period n] [--model file]
  *       [--alpha a] [--terminals a,b,c,...] [--time_budget s] [--device d] [--json]
  *
  * Argument handling follows workingPad/main.cu:753-790: fewer than three
  * positionals print the usage on stderr and exit 1; unknown flags and flags
- * missing their value are ignored; --thresh_high/--thresh_low are accepted
+ * missing their value are ignored (an unknown --order or --rule value prints
+ * the usage and exits 1 instead of running another method); --thresh_high/
+ * --thresh_low are accepted
  * and, as in the reference (main.cu:783-786), unused.  The report reproduces
  * the printf layout of main.cu:1037-1056 and the pre-run lines of
  * graph.h:142-143 and main.cu:743-745, 838-848, 944-948.
@@ -67,15 +70,32 @@ int main(int argc, char** argv) {
       thresh_low = std::atof(argv[++i]);
     } else if (!std::strcmp(argv[i], "--order") && has) {
       const char* o = argv[++i];
-      cfg.order = !std::strcmp(o, "mass")      ? NR_ORDER_MASS
-                  : !std::strcmp(o, "learned") ? NR_ORDER_LEARNED
-                                               : NR_ORDER_LEVEL;
+      if (!std::strcmp(o, "level")) {
+        cfg.order = NR_ORDER_LEVEL;
+      } else if (!std::strcmp(o, "mass")) {
+        cfg.order = NR_ORDER_MASS;
+      } else if (!std::strcmp(o, "learned")) {
+        cfg.order = NR_ORDER_LEARNED;
+      } else { /* never run another method than the one asked for */
+        std::fprintf(stderr, "unknown --order %s\n", o);
+        usage(argv[0], cfg.queue_capacity);
+        return 1;
+      }
     } else if (!std::strcmp(argv[i], "--rule") && has) {
       const char* r = argv[++i];
-      cfg.rule = !std::strcmp(r, "forkbest")    ? NR_RULE_FORKBEST
-                 : !std::strcmp(r, "alternate") ? NR_RULE_ALTERNATE
-                 : !std::strcmp(r, "pivot")     ? NR_RULE_PIVOT
-                                                : NR_RULE_PATH;
+      if (!std::strcmp(r, "path")) {
+        cfg.rule = NR_RULE_PATH;
+      } else if (!std::strcmp(r, "forkbest")) {
+        cfg.rule = NR_RULE_FORKBEST;
+      } else if (!std::strcmp(r, "alternate")) {
+        cfg.rule = NR_RULE_ALTERNATE;
+      } else if (!std::strcmp(r, "pivot")) {
+        cfg.rule = NR_RULE_PIVOT;
+      } else {
+        std::fprintf(stderr, "unknown --rule %s\n", r);
+        usage(argv[0], cfg.queue_capacity);
+        return 1;
+      }
     } else if (!std::strcmp(argv[i], "--sweep_period") && has) {
       cfg.sweep_period = std::atoi(argv[++i]);
     } else if (!std::strcmp(argv[i], "--model") && has) {

@@ -2436,127 +2436,138 @@ struct WsLock {
 extern "C" {
 
 int nr_solve(const nr_graph* g, const nr_config* cfg, nr_result* out) {
-  if (!g || !out) return NR_ERR_ARG;
-  nr_config def;
-  if (!cfg) {
-    nr_config_init(&def);
-    cfg = &def;
-  }
-  if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0 || cfg->time_budget_s < 0)
-    return NR_ERR_ARG;
-  if (cfg->order == NR_ORDER_LEARNED) return NR_ERR_ARG; /* needs a model: nr_solve_learned */
-  if (cfg->order != NR_ORDER_LEVEL && cfg->order != NR_ORDER_MASS) return NR_ERR_ARG;
-  if (cfg->rule != NR_RULE_PATH && cfg->rule != NR_RULE_FORKBEST &&
-      cfg->rule != NR_RULE_ALTERNATE && cfg->rule != NR_RULE_PIVOT)
-    return NR_ERR_ARG;
-  if (cfg->sweep_period < 0) return NR_ERR_ARG;
-  nr::Variant var;
-  int rc = nr::pick_variant(g, var);
-  if (rc == NR_ERR_UNSUPPORTED) { /* past the per-lane forms: one wave per item */
+  return nr::guarded([&]() -> int {
+    if (!g || !out) return NR_ERR_ARG;
+    nr_config def;
+    if (!cfg) {
+      nr_config_init(&def);
+      cfg = &def;
+    }
+    if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0 || cfg->time_budget_s < 0)
+      return NR_ERR_ARG;
+    if (cfg->order == NR_ORDER_LEARNED) return NR_ERR_ARG; /* needs a model: nr_solve_learned */
+    if (cfg->order != NR_ORDER_LEVEL && cfg->order != NR_ORDER_MASS) return NR_ERR_ARG;
+    if (cfg->rule != NR_RULE_PATH && cfg->rule != NR_RULE_FORKBEST &&
+        cfg->rule != NR_RULE_ALTERNATE && cfg->rule != NR_RULE_PIVOT)
+      return NR_ERR_ARG;
+    if (cfg->sweep_period < 0) return NR_ERR_ARG;
+    nr::Variant var;
+    int rc = nr::pick_variant(g, var);
+    if (rc == NR_ERR_UNSUPPORTED) { /* past the per-lane forms: one wave per item */
+      nr::WsLock L(cfg->device);
+      if (L.rc) return L.rc;
+      return nr::solve_big(*L.ws, g, cfg, out);
+    }
+    if (rc) return rc;
     nr::WsLock L(cfg->device);
     if (L.rc) return L.rc;
-    return nr::solve_big(*L.ws, g, cfg, out);
-  }
-  if (rc) return rc;
-  nr::WsLock L(cfg->device);
-  if (L.rc) return L.rc;
-  return nr::solve_level(*L.ws, g, cfg, out);
+    return nr::solve_level(*L.ws, g, cfg, out);
+  });
 }
 
 int nr_tune_sweep_period(const nr_graph* g, const nr_config* cfg, double pilot_eps,
                          int max_period, int* best, double* pilot_esp) {
-  if (!g || !best || max_period < 1 || max_period > 64 || !(pilot_eps > 0.0)) return NR_ERR_ARG;
-  nr_config c;
-  if (cfg)
-    c = *cfg;
-  else
-    nr_config_init(&c);
-  c.rule = NR_RULE_ALTERNATE;
-  c.order = NR_ORDER_LEVEL;
-  c.truncation_eps = pilot_eps;
-  c.time_budget_s = 0.0;
-  *best = 0;
-  double best_esp = 0.0;
-  unsigned long long best_nodes = 0;
-  for (int per = 1; per <= max_period; ++per) {
-    c.sweep_period = per;
-    nr_result r;
-    const int rc = nr_solve(g, &c, &r);
-    if (rc) return rc;
-    if (pilot_esp) pilot_esp[per - 1] = r.ESP;
-    if (*best == 0 || r.ESP < best_esp || (r.ESP == best_esp && r.nodes < best_nodes)) {
-      *best = per;
-      best_esp = r.ESP;
-      best_nodes = r.nodes;
+  return nr::guarded([&]() -> int {
+    if (!g || !best || max_period < 1 || max_period > 64 || !(pilot_eps > 0.0)) return NR_ERR_ARG;
+    nr_config c;
+    if (cfg)
+      c = *cfg;
+    else
+      nr_config_init(&c);
+    c.rule = NR_RULE_ALTERNATE;
+    c.order = NR_ORDER_LEVEL;
+    c.truncation_eps = pilot_eps;
+    c.time_budget_s = 0.0;
+    *best = 0;
+    double best_esp = 0.0;
+    unsigned long long best_nodes = 0;
+    for (int per = 1; per <= max_period; ++per) {
+      c.sweep_period = per;
+      nr_result r;
+      const int rc = nr_solve(g, &c, &r);
+      if (rc) return rc;
+      if (pilot_esp) pilot_esp[per - 1] = r.ESP;
+      if (*best == 0 || r.ESP < best_esp || (r.ESP == best_esp && r.nodes < best_nodes)) {
+        *best = per;
+        best_esp = r.ESP;
+        best_nodes = r.nodes;
+      }
     }
-  }
-  return NR_OK;
+    return NR_OK;
+  });
 }
 
 int nr_solve_learned(const nr_graph* g, const nr_config* cfg, const nr_model* model,
                      nr_result* out) {
-  if (!g || !out || !model) return NR_ERR_ARG;
-  nr_config def;
-  if (!cfg) {
-    nr_config_init(&def);
-    cfg = &def;
-  }
-  if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0 || cfg->time_budget_s < 0)
-    return NR_ERR_ARG;
-  if (cfg->rule != NR_RULE_PATH && cfg->rule != NR_RULE_ALTERNATE)
-    return NR_ERR_UNSUPPORTED; /* scores path children */
-  if (cfg->sweep_period < 0) return NR_ERR_ARG;
-  nr::Variant var;
-  int rc = nr::pick_variant(g, var);
-  if (rc) return rc;
-  nr::WsLock L(cfg->device);
-  if (L.rc) return L.rc;
-  return nr::solve_learned(*L.ws, g, cfg, model, out);
+  return nr::guarded([&]() -> int {
+    if (!g || !out || !model) return NR_ERR_ARG;
+    nr_config def;
+    if (!cfg) {
+      nr_config_init(&def);
+      cfg = &def;
+    }
+    if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0 || cfg->time_budget_s < 0)
+      return NR_ERR_ARG;
+    if (cfg->rule != NR_RULE_PATH && cfg->rule != NR_RULE_ALTERNATE)
+      return NR_ERR_UNSUPPORTED; /* scores path children */
+    if (cfg->sweep_period < 0) return NR_ERR_ARG;
+    nr::Variant var;
+    int rc = nr::pick_variant(g, var);
+    if (rc) return rc;
+    nr::WsLock L(cfg->device);
+    if (L.rc) return L.rc;
+    return nr::solve_learned(*L.ws, g, cfg, model, out);
+  });
 }
 
 int nr_solve_checkpointed(const nr_graph* g, const nr_config* cfg, const nr_checkpoint* resume,
                           nr_checkpoint** kept, nr_result* out) {
-  if (!g || !out || !kept) return NR_ERR_ARG;
-  *kept = nullptr;
-  nr_config def;
-  if (!cfg) {
-    nr_config_init(&def);
-    cfg = &def;
-  }
-  if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0 || cfg->time_budget_s < 0)
-    return NR_ERR_ARG;
-  if (cfg->order != NR_ORDER_LEVEL && cfg->order != NR_ORDER_MASS) return NR_ERR_ARG;
-  if (cfg->rule != NR_RULE_PATH && cfg->rule != NR_RULE_FORKBEST &&
-      cfg->rule != NR_RULE_ALTERNATE && cfg->rule != NR_RULE_PIVOT)
-    return NR_ERR_ARG;
-  if (cfg->sweep_period < 0) return NR_ERR_ARG;
-  if (resume && (resume->graph_hash != nr::graph_hash(g) || resume->w != nr::mask_words(g) ||
-                 resume->rule != cfg->rule || resume->order != cfg->order ||
-                 (cfg->rule == NR_RULE_ALTERNATE &&
-                  resume->period != (cfg->sweep_period > 0 ? cfg->sweep_period : 2))))
-    return NR_ERR_ARG; /* another graph, rule, sweep period or order */
-  nr::Variant var;
-  int rc = nr::pick_variant(g, var);
-  if (rc) return rc;
-  nr::WsLock L(cfg->device);
-  if (L.rc) return L.rc;
-  nr::LevelExtra x;
-  x.resume = resume;
-  x.save = kept;
-  return nr::solve_level(*L.ws, g, cfg, out, &x);
+  return nr::guarded([&]() -> int {
+    if (!g || !out || !kept) return NR_ERR_ARG;
+    *kept = nullptr;
+    nr_config def;
+    if (!cfg) {
+      nr_config_init(&def);
+      cfg = &def;
+    }
+    if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0 || cfg->time_budget_s < 0)
+      return NR_ERR_ARG;
+    if (cfg->order != NR_ORDER_LEVEL && cfg->order != NR_ORDER_MASS) return NR_ERR_ARG;
+    if (cfg->rule != NR_RULE_PATH && cfg->rule != NR_RULE_FORKBEST &&
+        cfg->rule != NR_RULE_ALTERNATE && cfg->rule != NR_RULE_PIVOT)
+      return NR_ERR_ARG;
+    if (cfg->sweep_period < 0) return NR_ERR_ARG;
+    if (resume && (resume->graph_hash != nr::graph_hash(g) || resume->w != nr::mask_words(g) ||
+                   resume->rule != cfg->rule || resume->order != cfg->order ||
+                 resume->eps != (cfg->truncation_eps > 0 ? cfg->truncation_eps : 0.0) ||
+                   (cfg->rule == NR_RULE_ALTERNATE &&
+                    resume->period != (cfg->sweep_period > 0 ? cfg->sweep_period : 2))))
+      return NR_ERR_ARG; /* another graph, rule, sweep period, order or eps */
+    nr::Variant var;
+    int rc = nr::pick_variant(g, var);
+    if (rc) return rc;
+    nr::WsLock L(cfg->device);
+    if (L.rc) return L.rc;
+    nr::LevelExtra x;
+    x.resume = resume;
+    x.save = kept;
+    return nr::solve_level(*L.ws, g, cfg, out, &x);
+  });
 }
 
 int nr_checkpoint_info(const nr_checkpoint* c, long long* pending, double* pending_mass,
                        long long* nodes_done) {
-  if (!c) return NR_ERR_ARG;
-  if (pending) *pending = (long long)c->count;
-  if (pending_mass) {
-    double s = 0.0;
-    for (double m : c->mass) s += m;
-    *pending_mass = s;
-  }
-  if (nodes_done) *nodes_done = (long long)c->counts[nr::C_NODES];
-  return NR_OK;
+  return nr::guarded([&]() -> int {
+    if (!c) return NR_ERR_ARG;
+    if (pending) *pending = (long long)c->count;
+    if (pending_mass) {
+      double s = 0.0;
+      for (double m : c->mass) s += m;
+      *pending_mass = s;
+    }
+    if (nodes_done) *nodes_done = (long long)c->counts[nr::C_NODES];
+    return NR_OK;
+  });
 }
 
 void nr_checkpoint_destroy(nr_checkpoint* c) { delete c; }
@@ -2564,142 +2575,159 @@ void nr_checkpoint_destroy(nr_checkpoint* c) { delete c; }
 static const char kCkptMagic[8] = {'N', 'R', 'C', 'K', 'P', 'T', '0', '1'};
 
 int nr_checkpoint_save(const nr_checkpoint* c, const char* path) {
-  if (!c || !path) return NR_ERR_ARG;
-  FILE* f = std::fopen(path, "wb");
-  if (!f) return NR_ERR_IO;
-  const int hdr[8] = {c->n, c->e, c->w, c->src, c->dst, c->rule, c->order, c->period};
-  const unsigned long long tail[3] = {c->iterations, c->max_width, c->count};
-  bool ok = std::fwrite(kCkptMagic, 1, 8, f) == 8 && std::fwrite(hdr, sizeof hdr, 1, f) == 1 &&
-            std::fwrite(&c->eps, 8, 1, f) == 1 && std::fwrite(&c->graph_hash, 8, 1, f) == 1 &&
-            std::fwrite(c->sum_hi, 8, 3, f) == 3 && std::fwrite(c->sum_lo, 8, 3, f) == 3 &&
-            std::fwrite(c->counts, 8, 8, f) == 8 && std::fwrite(tail, 8, 3, f) == 3;
-  const size_t nw = (size_t)c->count * c->w;
-  ok = ok && std::fwrite(c->known.data(), 8, nw, f) == nw &&
-       std::fwrite(c->up.data(), 8, nw, f) == nw &&
-       std::fwrite(c->mass.data(), 8, c->count, f) == c->count;
-  ok = (std::fclose(f) == 0) && ok;
-  return ok ? NR_OK : NR_ERR_IO;
+  return nr::guarded([&]() -> int {
+    if (!c || !path) return NR_ERR_ARG;
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return NR_ERR_IO;
+    const int hdr[8] = {c->n, c->e, c->w, c->src, c->dst, c->rule, c->order, c->period};
+    const unsigned long long tail[3] = {c->iterations, c->max_width, c->count};
+    bool ok = std::fwrite(kCkptMagic, 1, 8, f) == 8 && std::fwrite(hdr, sizeof hdr, 1, f) == 1 &&
+              std::fwrite(&c->eps, 8, 1, f) == 1 && std::fwrite(&c->graph_hash, 8, 1, f) == 1 &&
+              std::fwrite(c->sum_hi, 8, 3, f) == 3 && std::fwrite(c->sum_lo, 8, 3, f) == 3 &&
+              std::fwrite(c->counts, 8, 8, f) == 8 && std::fwrite(tail, 8, 3, f) == 3;
+    const size_t nw = (size_t)c->count * c->w;
+    ok = ok && std::fwrite(c->known.data(), 8, nw, f) == nw &&
+         std::fwrite(c->up.data(), 8, nw, f) == nw &&
+         std::fwrite(c->mass.data(), 8, c->count, f) == c->count;
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? NR_OK : NR_ERR_IO;
+  });
 }
 
 int nr_checkpoint_load(const char* path, nr_checkpoint** out) {
-  if (!path || !out) return NR_ERR_ARG;
-  *out = nullptr;
-  FILE* f = std::fopen(path, "rb");
-  if (!f) return NR_ERR_IO;
-  std::unique_ptr<nr_checkpoint> c(new (std::nothrow) nr_checkpoint);
-  char magic[8];
-  int hdr[8];
-  unsigned long long tail[3];
-  bool ok = c && std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kCkptMagic, 8) == 0 &&
-            std::fread(hdr, sizeof hdr, 1, f) == 1 && std::fread(&c->eps, 8, 1, f) == 1 &&
-            std::fread(&c->graph_hash, 8, 1, f) == 1 && std::fread(c->sum_hi, 8, 3, f) == 3 &&
-            std::fread(c->sum_lo, 8, 3, f) == 3 && std::fread(c->counts, 8, 8, f) == 8 &&
-            std::fread(tail, 8, 3, f) == 3;
-  if (ok) {
-    c->n = hdr[0];
-    c->e = hdr[1];
-    c->w = hdr[2];
-    c->src = hdr[3];
-    c->dst = hdr[4];
-    c->rule = hdr[5];
-    c->order = hdr[6];
-    c->period = hdr[7];
-    c->iterations = tail[0];
-    c->max_width = tail[1];
-    c->count = tail[2];
-    ok = c->w >= 1 && c->w <= 16 && c->count < (1ull << 34);
-  }
-  if (ok) {
-    const size_t nw = (size_t)c->count * c->w;
-    c->known.resize(nw);
-    c->up.resize(nw);
-    c->mass.resize(c->count);
-    ok = std::fread(c->known.data(), 8, nw, f) == nw && std::fread(c->up.data(), 8, nw, f) == nw &&
-         std::fread(c->mass.data(), 8, c->count, f) == c->count;
-  }
-  std::fclose(f);
-  if (!ok) return c ? NR_ERR_PARSE : NR_ERR_NOMEM;
-  *out = c.release();
-  return NR_OK;
+  return nr::guarded([&]() -> int {
+    if (!path || !out) return NR_ERR_ARG;
+    *out = nullptr;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return NR_ERR_IO;
+    std::unique_ptr<nr_checkpoint> c(new (std::nothrow) nr_checkpoint);
+    char magic[8];
+    int hdr[8];
+    unsigned long long tail[3];
+    bool ok = c && std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kCkptMagic, 8) == 0 &&
+              std::fread(hdr, sizeof hdr, 1, f) == 1 && std::fread(&c->eps, 8, 1, f) == 1 &&
+              std::fread(&c->graph_hash, 8, 1, f) == 1 && std::fread(c->sum_hi, 8, 3, f) == 3 &&
+              std::fread(c->sum_lo, 8, 3, f) == 3 && std::fread(c->counts, 8, 8, f) == 8 &&
+              std::fread(tail, 8, 3, f) == 3;
+    if (ok) {
+      c->n = hdr[0];
+      c->e = hdr[1];
+      c->w = hdr[2];
+      c->src = hdr[3];
+      c->dst = hdr[4];
+      c->rule = hdr[5];
+      c->order = hdr[6];
+      c->period = hdr[7];
+      c->iterations = tail[0];
+      c->max_width = tail[1];
+      c->count = tail[2];
+      ok = c->w >= 1 && c->w <= 16 && c->count < (1ull << 34);
+    }
+    if (ok) { /* the items must be in the file before they size any buffer */
+      const long here = std::ftell(f);
+      ok = here >= 0 && std::fseek(f, 0, SEEK_END) == 0;
+      const long end = ok ? std::ftell(f) : -1;
+      ok = ok && end >= here && std::fseek(f, here, SEEK_SET) == 0 &&
+           (unsigned long long)(end - here) == c->count * (16ull * (unsigned)c->w + 8ull);
+    }
+    if (ok) {
+      const size_t nw = (size_t)c->count * c->w;
+      c->known.resize(nw);
+      c->up.resize(nw);
+      c->mass.resize(c->count);
+      ok = std::fread(c->known.data(), 8, nw, f) == nw && std::fread(c->up.data(), 8, nw, f) == nw &&
+           std::fread(c->mass.data(), 8, c->count, f) == c->count;
+    }
+    std::fclose(f);
+    if (!ok) return c ? NR_ERR_PARSE : NR_ERR_NOMEM;
+    *out = c.release();
+    return NR_OK;
+  });
 }
 
 int nr_log_subproblems(const nr_graph* g, const nr_config* cfg, long long max_records,
                        unsigned long long seed, float* features, float* labels,
                        long long* n_out, nr_result* result) {
-  if (!g || !cfg || max_records < 0 || !n_out || !result ||
-      (max_records > 0 && (!features || !labels)))
-    return NR_ERR_ARG;
-  if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0 || cfg->sweep_period < 0)
-    return NR_ERR_ARG;
-  if (cfg->rule != NR_RULE_PATH && cfg->rule != NR_RULE_ALTERNATE) return NR_ERR_UNSUPPORTED;
-  nr::Variant var;
-  int rc = nr::pick_variant(g, var);
-  if (rc) return rc;
-  if (!var.level_log) return NR_ERR_UNSUPPORTED;
-  nr::WsLock L(cfg->device);
-  if (L.rc) return L.rc;
-  return nr::log_subproblems(*L.ws, g, cfg, max_records, seed, features, labels, n_out, result);
+  return nr::guarded([&]() -> int {
+    if (!g || !cfg || max_records < 0 || !n_out || !result ||
+        (max_records > 0 && (!features || !labels)))
+      return NR_ERR_ARG;
+    if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0 || cfg->sweep_period < 0)
+      return NR_ERR_ARG;
+    if (cfg->rule != NR_RULE_PATH && cfg->rule != NR_RULE_ALTERNATE) return NR_ERR_UNSUPPORTED;
+    nr::Variant var;
+    int rc = nr::pick_variant(g, var);
+    if (rc) return rc;
+    if (!var.level_log) return NR_ERR_UNSUPPORTED;
+    nr::WsLock L(cfg->device);
+    if (L.rc) return L.rc;
+    return nr::log_subproblems(*L.ws, g, cfg, max_records, seed, features, labels, n_out, result);
+  });
 }
 
 int nr_tune_batch(const nr_graph* const* graphs, int count, const nr_config* cfg, int sample,
                   nr_config* tuned, double* best_rate) {
-  if (!graphs || count <= 0 || !tuned || sample < 0) return NR_ERR_ARG;
-  nr_config c;
-  if (cfg)
-    c = *cfg;
-  else
-    nr_config_init(&c);
-  const int n = sample > 0 && sample < count ? sample : count;
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, c.device) != hipSuccess) return NR_ERR_DEVICE;
-  const int cus = prop.multiProcessorCount;
-  std::vector<nr_result> res((size_t)n);
-  *tuned = c;
-  double best = -1.0;
-  const int blocks[3] = {256, 128, 64};
-  const int per_cu[4] = {0, 1, 2, 4}; /* 0: the occupancy limit */
-  for (int b : blocks)
-    for (int m : per_cu) {
-      nr_config t = c;
-      t.block_size = b;
-      t.grid_blocks = m * cus;
-      double rate = 0.0;
-      int rc = NR_OK;
-      for (int rep = 0; rep < 2 && rc == NR_OK; ++rep) { /* the first run warms up */
-        rc = nr_solve_batch(graphs, n, &t, res.data());
-        if (rc == NR_OK && rep == 1) {
-          double nodes = 0.0;
-          for (const nr_result& r : res) nodes += (double)r.nodes;
-          rate = res[0].elapsed_ms > 0 ? nodes / (res[0].elapsed_ms * 1e-3) : 0.0;
+  return nr::guarded([&]() -> int {
+    if (!graphs || count <= 0 || !tuned || sample < 0) return NR_ERR_ARG;
+    nr_config c;
+    if (cfg)
+      c = *cfg;
+    else
+      nr_config_init(&c);
+    const int n = sample > 0 && sample < count ? sample : count;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c.device) != hipSuccess) return NR_ERR_DEVICE;
+    const int cus = prop.multiProcessorCount;
+    std::vector<nr_result> res((size_t)n);
+    *tuned = c;
+    double best = -1.0;
+    const int blocks[3] = {256, 128, 64};
+    const int per_cu[4] = {0, 1, 2, 4}; /* 0: the occupancy limit */
+    for (int b : blocks)
+      for (int m : per_cu) {
+        nr_config t = c;
+        t.block_size = b;
+        t.grid_blocks = m * cus;
+        double rate = 0.0;
+        int rc = NR_OK;
+        for (int rep = 0; rep < 2 && rc == NR_OK; ++rep) { /* the first run warms up */
+          rc = nr_solve_batch(graphs, n, &t, res.data());
+          if (rc == NR_OK && rep == 1) {
+            double nodes = 0.0;
+            for (const nr_result& r : res) nodes += (double)r.nodes;
+            rate = res[0].elapsed_ms > 0 ? nodes / (res[0].elapsed_ms * 1e-3) : 0.0;
+          }
+        }
+        if (rc == NR_ERR_UNSUPPORTED || rc == NR_ERR_NOMEM) continue; /* does not fit */
+        if (rc) return rc;
+        if (rate > best) {
+          best = rate;
+          *tuned = t;
         }
       }
-      if (rc == NR_ERR_UNSUPPORTED || rc == NR_ERR_NOMEM) continue; /* does not fit */
-      if (rc) return rc;
-      if (rate > best) {
-        best = rate;
-        *tuned = t;
-      }
-    }
-  if (best < 0) return NR_ERR_UNSUPPORTED;
-  if (best_rate) *best_rate = best;
-  return NR_OK;
+    if (best < 0) return NR_ERR_UNSUPPORTED;
+    if (best_rate) *best_rate = best;
+    return NR_OK;
+  });
 }
 
 int nr_solve_batch(const nr_graph* const* graphs, int count, const nr_config* cfg,
                    nr_result* results) {
-  if (!graphs || count < 0 || !results) return NR_ERR_ARG;
-  if (count == 0) return NR_OK;
-  nr_config def;
-  if (!cfg) {
-    nr_config_init(&def);
-    cfg = &def;
-  }
-  if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0 || cfg->time_budget_s < 0)
-    return NR_ERR_ARG;
-  if (cfg->order != NR_ORDER_LEVEL || cfg->rule != NR_RULE_PATH) return NR_ERR_UNSUPPORTED;
-  nr::WsLock L(cfg->device);
-  if (L.rc) return L.rc;
-  return nr::solve_batch(*L.ws, graphs, count, cfg, results);
+  return nr::guarded([&]() -> int {
+    if (!graphs || count < 0 || !results) return NR_ERR_ARG;
+    if (count == 0) return NR_OK;
+    nr_config def;
+    if (!cfg) {
+      nr_config_init(&def);
+      cfg = &def;
+    }
+    if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0 || cfg->time_budget_s < 0)
+      return NR_ERR_ARG;
+    if (cfg->order != NR_ORDER_LEVEL || cfg->rule != NR_RULE_PATH) return NR_ERR_UNSUPPORTED;
+    nr::WsLock L(cfg->device);
+    if (L.rc) return L.rc;
+    return nr::solve_batch(*L.ws, graphs, count, cfg, results);
+  });
 }
 
 int nr_expand_items(const nr_graph* g, int device, int count,
@@ -2707,15 +2735,17 @@ int nr_expand_items(const nr_graph* g, int device, int count,
                     const double* mass, int max_children, int* nchild, double* succ,
                     unsigned long long* child_known, unsigned long long* child_up,
                     double* child_mass) {
-  if (!g || count < 0 || (count > 0 && (!known || !up || !mass || !nchild || !succ)))
-    return NR_ERR_ARG;
-  nr::Variant var;
-  int rc = nr::pick_variant(g, var);
-  if (rc) return rc;
-  nr::WsLock L(device);
-  if (L.rc) return L.rc;
-  return nr::expand_items(*L.ws, g, NR_RULE_PATH, 0, count, known, up, mass, max_children,
-                          nchild, succ, child_known, child_up, child_mass);
+  return nr::guarded([&]() -> int {
+    if (!g || count < 0 || (count > 0 && (!known || !up || !mass || !nchild || !succ)))
+      return NR_ERR_ARG;
+    nr::Variant var;
+    int rc = nr::pick_variant(g, var);
+    if (rc) return rc;
+    nr::WsLock L(device);
+    if (L.rc) return L.rc;
+    return nr::expand_items(*L.ws, g, NR_RULE_PATH, 0, count, known, up, mass, max_children,
+                            nchild, succ, child_known, child_up, child_mass);
+  });
 }
 
 int nr_expand_items_rule(const nr_graph* g, const nr_config* cfg, int count,
@@ -2723,37 +2753,43 @@ int nr_expand_items_rule(const nr_graph* g, const nr_config* cfg, int count,
                          const double* mass, int max_children, int* nchild, double* succ,
                          unsigned long long* child_known, unsigned long long* child_up,
                          double* child_mass) {
-  if (!g || !cfg || count < 0 || (count > 0 && (!known || !up || !mass || !nchild || !succ)))
-    return NR_ERR_ARG;
-  if (cfg->rule != NR_RULE_PATH && cfg->rule != NR_RULE_ALTERNATE) return NR_ERR_UNSUPPORTED;
-  nr::Variant var;
-  int rc = nr::pick_variant(g, var);
-  if (rc) return rc;
-  nr::WsLock L(cfg->device);
-  if (L.rc) return L.rc;
-  return nr::expand_items(*L.ws, g, cfg->rule, cfg->sweep_period, count, known, up, mass,
-                          max_children, nchild, succ, child_known, child_up, child_mass);
+  return nr::guarded([&]() -> int {
+    if (!g || !cfg || count < 0 || (count > 0 && (!known || !up || !mass || !nchild || !succ)))
+      return NR_ERR_ARG;
+    if (cfg->rule != NR_RULE_PATH && cfg->rule != NR_RULE_ALTERNATE) return NR_ERR_UNSUPPORTED;
+    nr::Variant var;
+    int rc = nr::pick_variant(g, var);
+    if (rc) return rc;
+    nr::WsLock L(cfg->device);
+    if (L.rc) return L.rc;
+    return nr::expand_items(*L.ws, g, cfg->rule, cfg->sweep_period, count, known, up, mass,
+                            max_children, nchild, succ, child_known, child_up, child_mass);
+  });
 }
 
 int nr_device_release(int device) {
-  if (device < 0 || device >= nr::kMaxDevices) return NR_ERR_ARG;
-  nr::Workspace& ws = nr::g_ws[device];
-  std::lock_guard<std::mutex> lk(ws.mu);
-  ws.release();
-  return NR_OK;
+  return nr::guarded([&]() -> int {
+    if (device < 0 || device >= nr::kMaxDevices) return NR_ERR_ARG;
+    nr::Workspace& ws = nr::g_ws[device];
+    std::lock_guard<std::mutex> lk(ws.mu);
+    ws.release();
+    return NR_OK;
+  });
 }
 
 int nr_device_query(int device, char* name, int name_len, int* compute_units,
                     int* wave_size, long long* total_mem) {
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return NR_ERR_DEVICE;
-  if (name && name_len > 0) {
-    std::snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
-  }
-  if (compute_units) *compute_units = prop.multiProcessorCount;
-  if (wave_size) *wave_size = prop.warpSize;
-  if (total_mem) *total_mem = (long long)prop.totalGlobalMem;
-  return NR_OK;
+  return nr::guarded([&]() -> int {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return NR_ERR_DEVICE;
+    if (name && name_len > 0) {
+      std::snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (compute_units) *compute_units = prop.multiProcessorCount;
+    if (wave_size) *wave_size = prop.warpSize;
+    if (total_mem) *total_mem = (long long)prop.totalGlobalMem;
+    return NR_OK;
+  });
 }
 
 }  // extern "C"

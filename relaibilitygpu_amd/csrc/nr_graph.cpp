@@ -105,7 +105,11 @@ int load_graph_text(const char* text, size_t len, nr_graph* g) {
   std::vector<std::vector<Entry>> adj((size_t)n);
   for (long long u = 0; u < n; ++u) {
     long long deg = 0;
-    if (!sc.integer(&deg) || deg < 0) return NR_ERR_PARSE;
+    /* each (nbr, p) pair takes at least two tokens of one byte and a
+     * separator each: a degree the rest of the text cannot hold is
+     * malformed (and must not size an allocation) */
+    if (!sc.integer(&deg) || deg < 0 || deg > (long long)(sc.end - sc.s) / 2)
+      return NR_ERR_PARSE;
     adj[u].reserve((size_t)deg);
     for (long long j = 0; j < deg; ++j) {
       long long v = 0;
@@ -189,121 +193,137 @@ int finish_graph(nr_graph* g) {
 extern "C" {
 
 int nr_graph_load(const char* path, int src, int dst, nr_graph** out) {
-  if (!path || !out) return NR_ERR_ARG;
-  *out = nullptr;
-  FILE* f = std::fopen(path, "rb");
-  if (!f) return NR_ERR_IO;
-  std::string text;
-  char buf[1 << 16];
-  size_t got;
-  while ((got = std::fread(buf, 1, sizeof buf, f)) > 0) text.append(buf, got);
-  std::fclose(f);
-  nr_graph* g = new nr_graph();
-  int rc = nr::load_graph_text(text.data(), text.size(), g);
-  if (rc == NR_OK) rc = nr_graph_set_terminals(g, src, dst);
-  if (rc != NR_OK) {
-    delete g;
-    return rc;
-  }
-  *out = g;
-  return NR_OK;
+  return nr::guarded([&]() -> int {
+    if (!path || !out) return NR_ERR_ARG;
+    *out = nullptr;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return NR_ERR_IO;
+    std::string text;
+    char buf[1 << 16];
+    size_t got;
+    while ((got = std::fread(buf, 1, sizeof buf, f)) > 0) text.append(buf, got);
+    std::fclose(f);
+    nr_graph* g = new nr_graph();
+    int rc = nr::load_graph_text(text.data(), text.size(), g);
+    if (rc == NR_OK) rc = nr_graph_set_terminals(g, src, dst);
+    if (rc != NR_OK) {
+      delete g;
+      return rc;
+    }
+    *out = g;
+    return NR_OK;
+  });
 }
 
 int nr_graph_create(int n, int e, const int* row_ptr, const int* col_idx,
                     const int* edge_id, const double* p, int src, int dst,
                     nr_graph** out) {
-  if (!out || n <= 0 || e < 0 || !row_ptr || (e > 0 && !p)) return NR_ERR_ARG;
-  *out = nullptr;
-  int nnz = row_ptr[n];
-  if (nnz < 0 || row_ptr[0] != 0) return NR_ERR_ARG;
-  for (int u = 0; u < n; ++u)
-    if (row_ptr[u + 1] < row_ptr[u]) return NR_ERR_ARG;
-  if (nnz > 0 && (!col_idx || !edge_id)) return NR_ERR_ARG;
-  nr_graph* g = new nr_graph();
-  g->n = n;
-  g->e = e;
-  g->nnz = nnz;
-  g->header_e = e;
-  g->row_ptr.assign(row_ptr, row_ptr + n + 1);
-  g->col_idx.assign(col_idx, col_idx + nnz);
-  g->edge_id.assign(edge_id, edge_id + nnz);
-  g->p.assign(p, p + e);
-  int rc = nr::finish_graph(g);
-  if (rc == NR_OK) rc = nr_graph_set_terminals(g, src, dst);
-  if (rc != NR_OK) {
-    delete g;
-    return rc;
-  }
-  *out = g;
-  return NR_OK;
+  return nr::guarded([&]() -> int {
+    if (!out || n <= 0 || e < 0 || !row_ptr || (e > 0 && !p)) return NR_ERR_ARG;
+    *out = nullptr;
+    int nnz = row_ptr[n];
+    if (nnz < 0 || row_ptr[0] != 0) return NR_ERR_ARG;
+    for (int u = 0; u < n; ++u)
+      if (row_ptr[u + 1] < row_ptr[u]) return NR_ERR_ARG;
+    if (nnz > 0 && (!col_idx || !edge_id)) return NR_ERR_ARG;
+    nr_graph* g = new nr_graph();
+    g->n = n;
+    g->e = e;
+    g->nnz = nnz;
+    g->header_e = e;
+    g->row_ptr.assign(row_ptr, row_ptr + n + 1);
+    g->col_idx.assign(col_idx, col_idx + nnz);
+    g->edge_id.assign(edge_id, edge_id + nnz);
+    g->p.assign(p, p + e);
+    int rc = nr::finish_graph(g);
+    if (rc == NR_OK) rc = nr_graph_set_terminals(g, src, dst);
+    if (rc != NR_OK) {
+      delete g;
+      return rc;
+    }
+    *out = g;
+    return NR_OK;
+  });
 }
 
 int nr_graph_set_terminals(nr_graph* g, int src, int dst) {
-  if (!g) return NR_ERR_ARG;
-  if (src < 0 || src >= g->n || dst < 0 || dst >= g->n) return NR_ERR_RANGE;
-  g->src = src;
-  g->dst = dst;
-  g->terms.clear(); /* back to the two-terminal problem */
-  g->stamp = nr::next_stamp();
-  return NR_OK;
+  return nr::guarded([&]() -> int {
+    if (!g) return NR_ERR_ARG;
+    if (src < 0 || src >= g->n || dst < 0 || dst >= g->n) return NR_ERR_RANGE;
+    g->src = src;
+    g->dst = dst;
+    g->terms.clear(); /* back to the two-terminal problem */
+    g->stamp = nr::next_stamp();
+    return NR_OK;
+  });
 }
 
 int nr_graph_get_info(const nr_graph* g, nr_graph_info* info) {
-  if (!g || !info) return NR_ERR_ARG;
-  info->n = g->n;
-  info->e = g->e;
-  info->nnz = g->nnz;
-  info->header_e = g->header_e;
-  info->type = g->type;
-  info->src = g->src;
-  info->dst = g->dst;
-  info->mask_words = nr::mask_words(g);
-  info->symmetric = g->symmetric ? 1 : 0;
-  info->precontracted = g->precontracted;
-  info->predeleted = g->predeleted;
-  return NR_OK;
+  return nr::guarded([&]() -> int {
+    if (!g || !info) return NR_ERR_ARG;
+    info->n = g->n;
+    info->e = g->e;
+    info->nnz = g->nnz;
+    info->header_e = g->header_e;
+    info->type = g->type;
+    info->src = g->src;
+    info->dst = g->dst;
+    info->mask_words = nr::mask_words(g);
+    info->symmetric = g->symmetric ? 1 : 0;
+    info->precontracted = g->precontracted;
+    info->predeleted = g->predeleted;
+    return NR_OK;
+  });
 }
 
 int nr_graph_get_csr(const nr_graph* g, int* row_ptr, int* col_idx, int* edge_id,
                      double* p) {
-  if (!g) return NR_ERR_ARG;
-  if (row_ptr) std::copy(g->row_ptr.begin(), g->row_ptr.end(), row_ptr);
-  if (col_idx) std::copy(g->col_idx.begin(), g->col_idx.end(), col_idx);
-  if (edge_id) std::copy(g->edge_id.begin(), g->edge_id.end(), edge_id);
-  if (p) std::copy(g->p.begin(), g->p.end(), p);
-  return NR_OK;
+  return nr::guarded([&]() -> int {
+    if (!g) return NR_ERR_ARG;
+    if (row_ptr) std::copy(g->row_ptr.begin(), g->row_ptr.end(), row_ptr);
+    if (col_idx) std::copy(g->col_idx.begin(), g->col_idx.end(), col_idx);
+    if (edge_id) std::copy(g->edge_id.begin(), g->edge_id.end(), edge_id);
+    if (p) std::copy(g->p.begin(), g->p.end(), p);
+    return NR_OK;
+  });
 }
 
 void nr_graph_destroy(nr_graph* g) { delete g; }
 
 int nr_graph_set_terminal_set(nr_graph* g, const int* terminals, int k) {
-  if (!g || k < 0 || k == 1 || (k > 0 && !terminals)) return NR_ERR_ARG;
-  if (k > 256) return NR_ERR_UNSUPPORTED;
-  for (int i = 0; i < k; ++i)
-    if (terminals[i] < 0 || terminals[i] >= g->n) return NR_ERR_RANGE;
-  g->terms.assign(terminals, terminals + k);
-  if (k >= 2) {
-    g->src = terminals[0];
-    g->dst = terminals[1];
-  }
-  g->stamp = nr::next_stamp();
-  return NR_OK;
+  return nr::guarded([&]() -> int {
+    if (!g || k < 0 || k == 1 || (k > 0 && !terminals)) return NR_ERR_ARG;
+    if (k > 256) return NR_ERR_UNSUPPORTED;
+    for (int i = 0; i < k; ++i)
+      if (terminals[i] < 0 || terminals[i] >= g->n) return NR_ERR_RANGE;
+    g->terms.assign(terminals, terminals + k);
+    if (k >= 2) {
+      g->src = terminals[0];
+      g->dst = terminals[1];
+    }
+    g->stamp = nr::next_stamp();
+    return NR_OK;
+  });
 }
 
 int nr_graph_get_terminal_set(const nr_graph* g, int* terminals, int max_k, int* k) {
-  if (!g || !k) return NR_ERR_ARG;
-  *k = (int)g->terms.size();
-  for (int i = 0; i < *k && i < max_k && terminals; ++i) terminals[i] = g->terms[i];
-  return NR_OK;
+  return nr::guarded([&]() -> int {
+    if (!g || !k) return NR_ERR_ARG;
+    *k = (int)g->terms.size();
+    for (int i = 0; i < *k && i < max_k && terminals; ++i) terminals[i] = g->terms[i];
+    return NR_OK;
+  });
 }
 
 int nr_config_init(nr_config* cfg) {
-  if (!cfg) return NR_ERR_ARG;
-  std::memset(cfg, 0, sizeof(*cfg));
-  cfg->queue_capacity = 1 << 20; /* DEFAULT_QUEUE_CAPACITY, common.h:33 */
-  cfg->truncation_eps = 1e-15;   /* DEFAULT_TRUNCATION_EPS, common.h:34 */
-  cfg->order = NR_ORDER_LEVEL;
-  return NR_OK;
+  return nr::guarded([&]() -> int {
+    if (!cfg) return NR_ERR_ARG;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->queue_capacity = 1 << 20; /* DEFAULT_QUEUE_CAPACITY, common.h:33 */
+    cfg->truncation_eps = 1e-15;   /* DEFAULT_TRUNCATION_EPS, common.h:34 */
+    cfg->order = NR_ORDER_LEVEL;
+    return NR_OK;
+  });
 }
 
 const char* nr_status_str(int s) {

@@ -6,6 +6,8 @@
 #define NR_INTERNAL_H
 
 #include <cstdint>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -37,6 +39,22 @@ int load_graph_text(const char* text, size_t len, nr_graph* g);
 int finish_graph(nr_graph* g);
 inline int mask_words(const nr_graph* g) { return g->e > 0 ? (g->e + 63) / 64 : 1; }
 unsigned long long next_stamp();
+
+/* Runs the body of a C-ABI entry point so that no C++ exception crosses the
+ * boundary: a failed host allocation (std::bad_alloc, or std::length_error
+ * for an impossible size) returns NR_ERR_NOMEM, anything else NR_ERR_ARG. */
+template <class F>
+int guarded(F&& body) noexcept {
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+    return NR_ERR_NOMEM;
+  } catch (const std::length_error&) {
+    return NR_ERR_NOMEM;
+  } catch (...) {
+    return NR_ERR_ARG;
+  }
+}
 
 }  // namespace nr
 

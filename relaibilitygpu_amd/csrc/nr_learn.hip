@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/nr_api.h"
+#include "nr_internal.h"
 #include "nr_learn.h"
 #include "nr_level.h"
 
@@ -251,166 +252,178 @@ using namespace nr;
 extern "C" {
 
 int nr_mlp_init(unsigned long long seed, float* params) {
-  if (!params) return NR_ERR_ARG;
-  std::mt19937_64 rng(seed);
-  std::normal_distribution<float> nd(0.0f, 1.0f);
-  const float s1 = std::sqrt(2.0f / kFeat), s2 = std::sqrt(1.0f / kHidden);
-  for (int i = 0; i < kFeat * kHidden; ++i) params[i] = nd(rng) * s1;
-  for (int h = 0; h < kHidden; ++h) params[kFeat * kHidden + h] = nd(rng) * s2;
-  params[kMlpParams - 1] = 0.0f;
-  return NR_OK;
+  return nr::guarded([&]() -> int {
+    if (!params) return NR_ERR_ARG;
+    std::mt19937_64 rng(seed);
+    std::normal_distribution<float> nd(0.0f, 1.0f);
+    const float s1 = std::sqrt(2.0f / kFeat), s2 = std::sqrt(1.0f / kHidden);
+    for (int i = 0; i < kFeat * kHidden; ++i) params[i] = nd(rng) * s1;
+    for (int h = 0; h < kHidden; ++h) params[kFeat * kHidden + h] = nd(rng) * s2;
+    params[kMlpParams - 1] = 0.0f;
+    return NR_OK;
+  });
 }
 
 int nr_mlp_grad(const float* X, const float* Y, long long n, const float* params, float* grad,
                 double* loss, int device) {
-  if (!X || !Y || n <= 0 || !params || !grad) return NR_ERR_ARG;
-  NR_CHK(hipSetDevice(device));
-  TrainBufs tb;
-  float *dX, *dY, *dP, *dG, *part = nullptr;
-  double *dl, *lpart = nullptr;
-  long long cap = 0;
-  int rc;
-  if ((rc = tb.alloc(&dX, (size_t)n * kFeat)) || (rc = tb.alloc(&dY, n)) ||
-      (rc = tb.alloc(&dP, kMlpParams)) || (rc = tb.alloc(&dG, kMlpParams)) ||
-      (rc = tb.alloc(&dl, 1)))
-    return rc;
-  hipStream_t st;
-  NR_CHK(hipStreamCreate(&st));
-  struct S {
-    hipStream_t s;
-    ~S() { (void)hipStreamDestroy(s); }
-  } guard{st};
-  NR_CHK(hipMemcpyAsync(dX, X, sizeof(float) * n * kFeat, hipMemcpyHostToDevice, st));
-  NR_CHK(hipMemcpyAsync(dY, Y, sizeof(float) * n, hipMemcpyHostToDevice, st));
-  NR_CHK(hipMemcpyAsync(dP, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, st));
-  if ((rc = mlp_grad_dev(st, dX, dY, nullptr, n, dP, dG, dl, tb, part, lpart, cap))) return rc;
-  double hl = 0.0;
-  NR_CHK(hipMemcpyAsync(grad, dG, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, st));
-  NR_CHK(hipMemcpyAsync(&hl, dl, sizeof(double), hipMemcpyDeviceToHost, st));
-  NR_CHK(hipStreamSynchronize(st));
-  if (loss) *loss = hl / (double)n;
-  return NR_OK;
+  return nr::guarded([&]() -> int {
+    if (!X || !Y || n <= 0 || !params || !grad) return NR_ERR_ARG;
+    NR_CHK(hipSetDevice(device));
+    TrainBufs tb;
+    float *dX, *dY, *dP, *dG, *part = nullptr;
+    double *dl, *lpart = nullptr;
+    long long cap = 0;
+    int rc;
+    if ((rc = tb.alloc(&dX, (size_t)n * kFeat)) || (rc = tb.alloc(&dY, n)) ||
+        (rc = tb.alloc(&dP, kMlpParams)) || (rc = tb.alloc(&dG, kMlpParams)) ||
+        (rc = tb.alloc(&dl, 1)))
+      return rc;
+    hipStream_t st;
+    NR_CHK(hipStreamCreate(&st));
+    struct S {
+      hipStream_t s;
+      ~S() { (void)hipStreamDestroy(s); }
+    } guard{st};
+    NR_CHK(hipMemcpyAsync(dX, X, sizeof(float) * n * kFeat, hipMemcpyHostToDevice, st));
+    NR_CHK(hipMemcpyAsync(dY, Y, sizeof(float) * n, hipMemcpyHostToDevice, st));
+    NR_CHK(hipMemcpyAsync(dP, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, st));
+    if ((rc = mlp_grad_dev(st, dX, dY, nullptr, n, dP, dG, dl, tb, part, lpart, cap))) return rc;
+    double hl = 0.0;
+    NR_CHK(hipMemcpyAsync(grad, dG, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, st));
+    NR_CHK(hipMemcpyAsync(&hl, dl, sizeof(double), hipMemcpyDeviceToHost, st));
+    NR_CHK(hipStreamSynchronize(st));
+    if (loss) *loss = hl / (double)n;
+    return NR_OK;
+  });
 }
 
 int nr_train_config_init(nr_train_config* c) {
-  if (!c) return NR_ERR_ARG;
-  std::memset(c, 0, sizeof *c);
-  c->epochs = 8;
-  c->batch = 4096;
-  c->lr = 3e-3f;
-  c->weight_decay = 1e-5f;
-  c->seed = 1;
-  c->device = 0;
-  c->holdout = 0.1f;
-  return NR_OK;
+  return nr::guarded([&]() -> int {
+    if (!c) return NR_ERR_ARG;
+    std::memset(c, 0, sizeof *c);
+    c->epochs = 8;
+    c->batch = 4096;
+    c->lr = 3e-3f;
+    c->weight_decay = 1e-5f;
+    c->seed = 1;
+    c->device = 0;
+    c->holdout = 0.1f;
+    return NR_OK;
+  });
 }
 
 int nr_mlp_train(const float* X, const float* Y, long long n, const nr_train_config* cfg,
                  float* params, float* train_loss, float* holdout_loss) {
-  if (!X || !Y || n < 2 || !cfg || !params || cfg->epochs < 1 || cfg->batch < 1 ||
-      cfg->holdout < 0.0f || cfg->holdout >= 1.0f)
-    return NR_ERR_ARG;
-  NR_CHK(hipSetDevice(cfg->device));
-  const long long n_hold = (long long)((double)n * cfg->holdout);
-  const long long n_train = n - n_hold;
-  std::mt19937_64 rng(cfg->seed);
-  std::vector<int> order((size_t)n);
-  for (long long i = 0; i < n; ++i) order[(size_t)i] = (int)i;
-  std::shuffle(order.begin(), order.end(), rng);
-  std::vector<int> hold(order.begin() + n_train, order.end());
-  std::vector<int> train(order.begin(), order.begin() + n_train);
-  TrainBufs tb;
-  float *dX, *dY, *dP, *dG, *dM, *dV, *part = nullptr;
-  double *dl, *lpart = nullptr;
-  int *didx, *dhold;
-  long long cap = 0;
-  int rc;
-  if ((rc = tb.alloc(&dX, (size_t)n * kFeat)) || (rc = tb.alloc(&dY, n)) ||
-      (rc = tb.alloc(&dP, kMlpParams)) || (rc = tb.alloc(&dG, kMlpParams)) ||
-      (rc = tb.alloc(&dM, kMlpParams)) || (rc = tb.alloc(&dV, kMlpParams)) ||
-      (rc = tb.alloc(&dl, 1)) || (rc = tb.alloc(&didx, n_train)) ||
-      (rc = tb.alloc(&dhold, std::max<long long>(n_hold, 1))))
-    return rc;
-  hipStream_t st;
-  NR_CHK(hipStreamCreate(&st));
-  struct S {
-    hipStream_t s;
-    ~S() { (void)hipStreamDestroy(s); }
-  } guard{st};
-  NR_CHK(hipMemcpyAsync(dX, X, sizeof(float) * n * kFeat, hipMemcpyHostToDevice, st));
-  NR_CHK(hipMemcpyAsync(dY, Y, sizeof(float) * n, hipMemcpyHostToDevice, st));
-  NR_CHK(hipMemcpyAsync(dP, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, st));
-  if (n_hold)
-    NR_CHK(hipMemcpyAsync(dhold, hold.data(), sizeof(int) * n_hold, hipMemcpyHostToDevice, st));
-  std::vector<float> zeros(kMlpParams, 0.0f);
-  NR_CHK(hipMemcpyAsync(dM, zeros.data(), sizeof(float) * kMlpParams, hipMemcpyHostToDevice, st));
-  NR_CHK(hipMemcpyAsync(dV, zeros.data(), sizeof(float) * kMlpParams, hipMemcpyHostToDevice, st));
-  int step = 0;
-  for (int ep = 0; ep < cfg->epochs; ++ep) {
-    std::shuffle(train.begin(), train.end(), rng);
-    NR_CHK(hipMemcpyAsync(didx, train.data(), sizeof(int) * n_train, hipMemcpyHostToDevice, st));
-    double sum = 0.0;
-    for (long long i0 = 0; i0 < n_train; i0 += cfg->batch) {
-      const long long nb = std::min<long long>(cfg->batch, n_train - i0);
-      if ((rc = mlp_grad_dev(st, dX, dY, didx + i0, nb, dP, dG, dl, tb, part, lpart, cap)))
-        return rc;
-      hipLaunchKernelGGL(k_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, st, dP,
-                         (const float*)dG, dM, dV, cfg->lr, 0.9f, 0.999f, 1e-8f,
-                         cfg->weight_decay, ++step);
-      double hl = 0.0;
-      NR_CHK(hipMemcpyAsync(&hl, dl, sizeof(double), hipMemcpyDeviceToHost, st));
-      NR_CHK(hipStreamSynchronize(st));
-      sum += hl;
-    }
-    if (train_loss) train_loss[ep] = (float)(sum / (double)n_train);
-    if (holdout_loss) {
-      double hl = 0.0;
-      if (n_hold) {
-        if ((rc = mlp_grad_dev(st, dX, dY, dhold, n_hold, dP, dG, dl, tb, part, lpart, cap)))
+  return nr::guarded([&]() -> int {
+    if (!X || !Y || n < 2 || !cfg || !params || cfg->epochs < 1 || cfg->batch < 1 ||
+        cfg->holdout < 0.0f || cfg->holdout >= 1.0f)
+      return NR_ERR_ARG;
+    NR_CHK(hipSetDevice(cfg->device));
+    const long long n_hold = (long long)((double)n * cfg->holdout);
+    const long long n_train = n - n_hold;
+    std::mt19937_64 rng(cfg->seed);
+    std::vector<int> order((size_t)n);
+    for (long long i = 0; i < n; ++i) order[(size_t)i] = (int)i;
+    std::shuffle(order.begin(), order.end(), rng);
+    std::vector<int> hold(order.begin() + n_train, order.end());
+    std::vector<int> train(order.begin(), order.begin() + n_train);
+    TrainBufs tb;
+    float *dX, *dY, *dP, *dG, *dM, *dV, *part = nullptr;
+    double *dl, *lpart = nullptr;
+    int *didx, *dhold;
+    long long cap = 0;
+    int rc;
+    if ((rc = tb.alloc(&dX, (size_t)n * kFeat)) || (rc = tb.alloc(&dY, n)) ||
+        (rc = tb.alloc(&dP, kMlpParams)) || (rc = tb.alloc(&dG, kMlpParams)) ||
+        (rc = tb.alloc(&dM, kMlpParams)) || (rc = tb.alloc(&dV, kMlpParams)) ||
+        (rc = tb.alloc(&dl, 1)) || (rc = tb.alloc(&didx, n_train)) ||
+        (rc = tb.alloc(&dhold, std::max<long long>(n_hold, 1))))
+      return rc;
+    hipStream_t st;
+    NR_CHK(hipStreamCreate(&st));
+    struct S {
+      hipStream_t s;
+      ~S() { (void)hipStreamDestroy(s); }
+    } guard{st};
+    NR_CHK(hipMemcpyAsync(dX, X, sizeof(float) * n * kFeat, hipMemcpyHostToDevice, st));
+    NR_CHK(hipMemcpyAsync(dY, Y, sizeof(float) * n, hipMemcpyHostToDevice, st));
+    NR_CHK(hipMemcpyAsync(dP, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, st));
+    if (n_hold)
+      NR_CHK(hipMemcpyAsync(dhold, hold.data(), sizeof(int) * n_hold, hipMemcpyHostToDevice, st));
+    std::vector<float> zeros(kMlpParams, 0.0f);
+    NR_CHK(hipMemcpyAsync(dM, zeros.data(), sizeof(float) * kMlpParams, hipMemcpyHostToDevice, st));
+    NR_CHK(hipMemcpyAsync(dV, zeros.data(), sizeof(float) * kMlpParams, hipMemcpyHostToDevice, st));
+    int step = 0;
+    for (int ep = 0; ep < cfg->epochs; ++ep) {
+      std::shuffle(train.begin(), train.end(), rng);
+      NR_CHK(hipMemcpyAsync(didx, train.data(), sizeof(int) * n_train, hipMemcpyHostToDevice, st));
+      double sum = 0.0;
+      for (long long i0 = 0; i0 < n_train; i0 += cfg->batch) {
+        const long long nb = std::min<long long>(cfg->batch, n_train - i0);
+        if ((rc = mlp_grad_dev(st, dX, dY, didx + i0, nb, dP, dG, dl, tb, part, lpart, cap)))
           return rc;
+        hipLaunchKernelGGL(k_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, st, dP,
+                           (const float*)dG, dM, dV, cfg->lr, 0.9f, 0.999f, 1e-8f,
+                           cfg->weight_decay, ++step);
+        double hl = 0.0;
         NR_CHK(hipMemcpyAsync(&hl, dl, sizeof(double), hipMemcpyDeviceToHost, st));
         NR_CHK(hipStreamSynchronize(st));
-        hl /= (double)n_hold;
+        sum += hl;
       }
-      holdout_loss[ep] = (float)hl;
+      if (train_loss) train_loss[ep] = (float)(sum / (double)n_train);
+      if (holdout_loss) {
+        double hl = 0.0;
+        if (n_hold) {
+          if ((rc = mlp_grad_dev(st, dX, dY, dhold, n_hold, dP, dG, dl, tb, part, lpart, cap)))
+            return rc;
+          NR_CHK(hipMemcpyAsync(&hl, dl, sizeof(double), hipMemcpyDeviceToHost, st));
+          NR_CHK(hipStreamSynchronize(st));
+          hl /= (double)n_hold;
+        }
+        holdout_loss[ep] = (float)hl;
+      }
     }
-  }
-  NR_CHK(hipMemcpyAsync(params, dP, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, st));
-  NR_CHK(hipStreamSynchronize(st));
-  return NR_OK;
+    NR_CHK(hipMemcpyAsync(params, dP, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, st));
+    NR_CHK(hipStreamSynchronize(st));
+    return NR_OK;
+  });
 }
 
 int nr_model_create(const float* params, float alpha, nr_model** out) {
-  if (!params || !out) return NR_ERR_ARG;
-  nr_model* m = new (std::nothrow) nr_model;
-  if (!m) return NR_ERR_NOMEM;
-  std::memcpy(m->params, params, sizeof(float) * kMlpParams);
-  m->alpha = alpha;
-  pack_model(params, alpha, m->dev);
-  *out = m;
-  return NR_OK;
+  return nr::guarded([&]() -> int {
+    if (!params || !out) return NR_ERR_ARG;
+    nr_model* m = new (std::nothrow) nr_model;
+    if (!m) return NR_ERR_NOMEM;
+    std::memcpy(m->params, params, sizeof(float) * kMlpParams);
+    m->alpha = alpha;
+    pack_model(params, alpha, m->dev);
+    *out = m;
+    return NR_OK;
+  });
 }
 
 void nr_model_destroy(nr_model* m) { delete m; }
 
 int nr_model_score(const nr_model* m, const float* X, long long n, float* Y, int device) {
-  if (!m || !X || !Y || n < 0) return NR_ERR_ARG;
-  if (n == 0) return NR_OK;
-  NR_CHK(hipSetDevice(device));
-  TrainBufs tb;
-  float *dX, *dY;
-  MlpDev* dm;
-  int rc;
-  if ((rc = tb.alloc(&dX, (size_t)n * kFeat)) || (rc = tb.alloc(&dY, n)) ||
-      (rc = tb.alloc(&dm, 1)))
-    return rc;
-  NR_CHK(hipMemcpy(dX, X, sizeof(float) * n * kFeat, hipMemcpyHostToDevice));
-  NR_CHK(hipMemcpy(dm, &m->dev, sizeof(MlpDev), hipMemcpyHostToDevice));
-  hipLaunchKernelGGL(k_mlp_score, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, 0,
-                     (const float*)dX, n, (const MlpDev*)dm, dY);
-  NR_CHK(hipGetLastError());
-  NR_CHK(hipMemcpy(Y, dY, sizeof(float) * n, hipMemcpyDeviceToHost));
-  return NR_OK;
+  return nr::guarded([&]() -> int {
+    if (!m || !X || !Y || n < 0) return NR_ERR_ARG;
+    if (n == 0) return NR_OK;
+    NR_CHK(hipSetDevice(device));
+    TrainBufs tb;
+    float *dX, *dY;
+    MlpDev* dm;
+    int rc;
+    if ((rc = tb.alloc(&dX, (size_t)n * kFeat)) || (rc = tb.alloc(&dY, n)) ||
+        (rc = tb.alloc(&dm, 1)))
+      return rc;
+    NR_CHK(hipMemcpy(dX, X, sizeof(float) * n * kFeat, hipMemcpyHostToDevice));
+    NR_CHK(hipMemcpy(dm, &m->dev, sizeof(MlpDev), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_mlp_score, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, 0,
+                       (const float*)dX, n, (const MlpDev*)dm, dY);
+    NR_CHK(hipGetLastError());
+    NR_CHK(hipMemcpy(Y, dY, sizeof(float) * n, hipMemcpyDeviceToHost));
+    return NR_OK;
+  });
 }
 
 }  // extern "C"

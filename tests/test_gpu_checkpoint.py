@@ -109,6 +109,9 @@ def test_checkpoint_info_and_rejects(tmp_path):
     with pytest.raises(nr.NRError):                              # other order
         nr.solve_checkpointed(g, resume=ck, truncation_eps=0.0, queue_capacity=1 << 22,
                               order="mass")
+    with pytest.raises(nr.NRError) as ei:                        # other truncation eps
+        nr.solve_checkpointed(g, resume=ck, truncation_eps=1e-9, queue_capacity=1 << 22)
+    assert ei.value.status == 4
     # without a budget nothing is kept
     r2, ck2 = nr.solve_checkpointed(g, resume=ck, truncation_eps=0.0, queue_capacity=1 << 22)
     assert ck2 is None and r2["nodes"] == 14866064

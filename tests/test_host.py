@@ -151,3 +151,44 @@ def test_checkpoint_files_fail_cleanly(tmp_path):
     with pytest.raises(nr.NRError) as ei:
         nr.Checkpoint.load(str(bad))
     assert ei.value.status == 2
+
+
+def _ckpt_header(w: int, count: int) -> bytes:
+    """A checkpoint file header (layout of nr_checkpoint_save): magic, eight
+    ints, eps, graph hash, three sums and their low parts, eight counters,
+    iterations, widest level and the item count."""
+    import struct
+
+    return (b"NRCKPT01" + struct.pack("<8i", 10, 45, w, 0, 9, 0, 0, 0)
+            + struct.pack("<2d", 0.0, 0.0) + b"\x00" * 8 * 3 * 2 + b"\x00" * 8 * 8
+            + struct.pack("<3Q", 0, 0, count))
+
+
+def test_checkpoint_oversized_headers_fail_cleanly(tmp_path):
+    """A header whose item count the file cannot hold is a parse error, never
+    an allocation of that size (nor an abort of the caller)."""
+    f = tmp_path / "big.bin"
+    for w, count, items in ((16, (1 << 34) - 1, b""), (1, 1000, b"\x00" * 24 * 999),
+                            (1, 2, b"\x00" * 24 * 3)):
+        f.write_bytes(_ckpt_header(w, count) + items)
+        with pytest.raises(nr.NRError) as ei:
+            nr.Checkpoint.load(str(f))
+        assert ei.value.status == 2, (w, count)
+    f.write_bytes(_ckpt_header(1, 2) + b"\x00" * 24 * 2)
+    ck = nr.Checkpoint.load(str(f))
+    assert ck.info()["pending"] == 2
+
+
+def test_graph_degree_beyond_the_text_is_a_parse_error(tmp_path):
+    f = tmp_path / "g.txt"
+    f.write_text("2 2 U\n100000000000000 1 0.5\n1 0 0.5\n")
+    with pytest.raises(nr.NRError) as ei:
+        nr.Graph.load(str(f), 0, 1)
+    assert ei.value.status == 2
+
+
+def test_cli_rejects_unknown_rule_or_order():
+    for flag, value in (("--rule", "forkBest"), ("--rule", "heap"), ("--order", "heap")):
+        r = subprocess.run([nr.CLI_PATH, graph_path("test_simple.txt"), "0", "3", flag, value],
+                           capture_output=True, text=True)
+        assert r.returncode == 1 and "Usage:" in r.stderr and value in r.stderr, (flag, value)
