@@ -9,12 +9,19 @@ library caches its per-device workspace), so a timed step is the solve itself.
 Prints one JSON line with the metric, a roofline object for the expand
 kernel (HBM bound: algorithmic bytes per subproblem = 2 * (16*W + 8), one
 write when the item is created and one read when it is expanded), the CPU
-baseline (the oracle on the host cores, same workload) and the metric's
-second part: the certified gap ESP at a time budget (60 s by default) on
-K15 and K25 at delta = 1e-12, the plain method (path rule, level and
-max-mass order) against the alternating-sweep rule in the learned order
-(esp_at_budget), and the same comparison on a 10x10 grid and a
-Barabasi-Albert graph at a 1 s budget (esp_sparse_graphs).
+baselines (the oracle on the host cores, and the reference's own CPU heap
+program seq/seq.cpp built from source into oracle/_ref, same workload), the
+batched workload (4096 Erdos-Renyi instances) with its own roofline, and the
+metric's second part, the certified gap ESP at a 60 s budget:
+  * K15 and K25 at delta = 1e-12 (the configured workload): the plain path
+    rule against the alternating-sweep rule (the rule's gain) and against
+    the learned path choice on that rule (the trained model's gain);
+  * K15 untruncated (delta = 0) in max-mass order, where the queue capacity
+    binds: plain, alternating and learned;
+  * a 10x10 grid and a Barabasi-Albert graph (graphs the models were not
+    trained on) at a 1 s budget.
+Both learned networks are trained here on the GPU by the engine's own
+kernels from logs the engine writes (relaibilitygpu_amd/learn.py).
 """
 from __future__ import annotations
 
@@ -61,7 +68,7 @@ def cpu_baseline(wl: dict) -> dict:
     threads = min(os.cpu_count() or 1, 16)
     runs = []
     nodes = 0
-    for _ in range(3):
+    for _ in range(5):
         r = g.solve(wl["src"], wl["dst"], order=ORDER_DFS, threads=threads,
                     trunc_eps=wl["eps"])
         runs.append(r["elapsed_ms"])
@@ -80,11 +87,45 @@ def cpu_baseline(wl: dict) -> dict:
         "cores": threads,
         "kind": "port",
         "sample": (f"whole workload ({nodes} subproblems) by the oracle (oracle/"
-                   f"cpu_reference.cpp, depth-first order, {threads} threads, median of 3 "
+                   f"cpu_reference.cpp, depth-first order, {threads} threads, median of 5 "
                    f"= {med:.1f} ms); 1 thread: {r1['elapsed_ms']:.1f} ms = "
                    f"{nodes / (r1['elapsed_ms'] / 1e3):.4g} subproblems/s; CPU {model}"),
         "single_thread_value": nodes / (r1["elapsed_ms"] / 1e3),
     }
+
+
+def reference_cpu(wl: dict, runs: int = 5) -> dict:
+    """The reference's own CPU program on the workload: seq/seq.cpp (max-mass
+    heap, FP64, one thread), compiled from the reference sources into
+    oracle/_ref by `make -C oracle ref`.  `runs` copies run at once, one core
+    each; the median of their own timers is reported."""
+    import re
+
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_seq_heap")
+    if not os.path.exists(exe):
+        return {"absent": "oracle/_ref/ref_seq_heap was not built (reference sources absent "
+                          "where the build ran)"}
+    path = os.path.join(ROOT, "tests", "golden", "graphs", wl["graph"])
+    procs = [subprocess.Popen([exe, path, str(wl["src"]), str(wl["dst"])],
+                              stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+             for _ in range(runs)]
+    secs, rel, paths = [], None, None
+    for p in procs:
+        out, _ = p.communicate(timeout=600)
+        m = re.search(r"^Time:\s+([0-9.eE+-]+) seconds", out, flags=re.M)
+        if p.returncode != 0 or not m:
+            return {"failed": out[-400:]}
+        secs.append(float(m.group(1)))
+        rel = float(re.search(r"^Reliability:\s+([0-9.]+)", out, flags=re.M).group(1))
+        paths = int(re.search(r"^Paths enumerated:\s+(\d+)", out, flags=re.M).group(1))
+    med = statistics.median(secs)
+    return {"value": wl["subproblems"] / med, "unit": "subproblems/s", "cores": 1,
+            "kind": "reference",
+            "sample": (f"whole workload by the reference's seq/seq.cpp (max-mass heap, FP64, one "
+                       f"thread) built from source, {runs} concurrent runs one core each, median "
+                       f"{med:.2f} s (runs {', '.join(f'{x:.2f}' for x in secs)}); value = the "
+                       f"workload's {wl['subproblems']} subproblems / median time"),
+            "R": rel, "paths": paths, "median_s": med}
 
 
 def er_graphs(count: int):
@@ -127,7 +168,7 @@ def cpu_baseline_batch(insts, eps: float, sample: int) -> dict:
     }
 
 
-def run_batch(args, wl: dict) -> dict:
+def run_batch(args, wl: dict, steps: int | None = None, warmup: int | None = None) -> dict:
     import relaibilitygpu_amd as nr
     from relaibilitygpu_amd import workloads
 
@@ -141,13 +182,15 @@ def run_batch(args, wl: dict) -> dict:
         graphs.append(g)
     cfg = nr.make_config(queue_capacity=wl["cap"], truncation_eps=wl["eps"], order="level")
     batch = nr.Batch(graphs, cfg)  # tables go to the device with the first (warmup) call
-    for w in range(max(1, args.warmup)):
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
+    for w in range(max(1, warmup)):
         r = batch.run()
         print(f"warmup {w}: {r['elapsed_ms'][0]:.1f} ms, reruns {int(r['rerun'].sum())}",
               file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     arrays = []
-    for k in range(args.steps):
+    for k in range(steps):
         arrays.append(batch.run())
         print(f"step {k}: {arrays[-1]['elapsed_ms'][0]:.1f} ms", file=sys.stderr, flush=True)
     t1 = time.perf_counter()
@@ -164,8 +207,8 @@ def run_batch(args, wl: dict) -> dict:
     achieved = nodes * bytes_per_sub / (timed["kernel_ms"] / 1e3) / 1e9
     r0 = runs[0]
     return {
-        "value": nodes * args.steps / wall_s,
-        "ms_per_step": wall_s * 1e3 / args.steps,
+        "value": nodes * steps / wall_s,
+        "ms_per_step": wall_s * 1e3 / steps,
         "config": {
             "workload": wl["desc"],
             "instances": len(graphs),
@@ -183,7 +226,7 @@ def run_batch(args, wl: dict) -> dict:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles("k_expand_batch", args.workload),
+            "traffic": traffic_from_profiles("k_expand_batch", "er4096"),
             "kernel": "k_expand_batch",
             "bytes_per_subproblem": bytes_per_sub,
             "avg_launch_ms": timed["kernel_ms"] / max(1, timed["kernel_launches"]),
@@ -197,45 +240,44 @@ def run_batch(args, wl: dict) -> dict:
     }
 
 
-def train_scorer():
-    """The learned scorer, trained here on the GPU on 2e6 children logged from
-    K10 solves under the alternating rule (delta 0, 1e-6, 1e-9); labels are
-    log2 subtree sizes.  Returns (model, training summary)."""
-    import numpy as np
-
+def train_models():
+    """Both learned networks, trained here on the GPU (relaibilitygpu_amd/
+    learn.py): the order scorer on children logged from K10, the path
+    chooser on choice logs of K10 (full) and K11-K13 (truncated), its margin
+    picked on validation graphs (K14, a 7x7 grid, BA(100)) that no leg below
+    evaluates.  Returns (Model, training summary)."""
     import relaibilitygpu_amd as nr
+    from relaibilitygpu_amd import learn
 
-    G = os.path.join(ROOT, "tests", "golden", "graphs")
-    t0 = time.perf_counter()
-    k10 = nr.Graph.load(os.path.join(G, "COMP10-05.txt"), 0, 9)
-    Xs, Ys = [], []
-    for eps, m in ((0.0, 1600000), (1e-6, 200000), (1e-9, 200000)):
-        X, Y, _ = nr.log_subproblems(k10, m, seed=7, truncation_eps=eps, queue_capacity=1 << 22,
-                                     rule="alternate")
-        Xs.append(X)
-        Ys.append(Y)
-    X, Y = np.concatenate(Xs), np.concatenate(Ys)
-    t_log = time.perf_counter() - t0
-    params, tl, hl = nr.mlp_train(X, Y, None, epochs=10, batch=4096, lr=3e-3, seed=3)
-    t_train = time.perf_counter() - t0 - t_log
-    info = {"records": int(len(Y)), "label_var": float(Y.var()), "holdout_mse": float(hl[-1]),
-            "log_s": t_log, "train_s": t_train, "alpha": 0.5, "rule": "alternate"}
-    return nr.Model(params, alpha=0.5), info
+    order_params, order_info = learn.train_order_scorer(alpha=0.5)
+    ch_params, margin, ch_info = learn.train_chooser()
+    ch_info = {k: v for k, v in ch_info.items() if k != "trials"}
+    model = nr.Model(order_params, alpha=0.5, chooser=ch_params, margin=margin)
+    return model, {"order_scorer": order_info, "path_chooser": ch_info}
 
 
 def _row(r):
     return {"ESP": r["ESP"], "R": r["R"], "nodes": r["nodes"], "elapsed_ms": r["elapsed_ms"],
-            "deadline_hit": r["deadline_hit"]}
+            "deadline_hit": r["deadline_hit"], "folded": r["folded"],
+            "overflowed": r["overflowed"]}
+
+
+def _ratio(a, b):
+    return a / b if b > 0 else None
 
 
 def esp_leg(model, budget_s: float = 60.0) -> dict:
     """Certified gap ESP at a time budget on K15 and K25 (p = 0.5, capacity
-    96 777 216 items).  At delta = 1e-12 (the configured workload): the
-    plain GPU method (path rule) in level and max-mass order against the
-    alternating-sweep rule (sweep period picked by pilot solves at delta
-    1e-9) in level order and in the learned order.  At delta = 1e-15, where
-    the order could decide what the capacity trims: the alternating rule in
-    level, max-mass and learned order."""
+    96 777 216 items), every row with the same budget:
+      delta_1e-12 (the configured workload): plain path rule (level, max-mass
+        order), alternating-sweep rule (period from pilot solves at 1e-9),
+        learned path choice on that rule (level order; and with the learned
+        order);
+      delta_0 (K15; nothing truncated, the capacity binds in max-mass order):
+        plain, alternating and learned choice in max-mass order, and the
+        learned choice in the learned order.
+    Ratios: plain over alternating is the rule's gain, alternating over
+    learned the trained model's own gain, plain over learned both."""
     import relaibilitygpu_amd as nr
 
     G = os.path.join(ROOT, "tests", "golden", "graphs")
@@ -245,68 +287,88 @@ def esp_leg(model, budget_s: float = 60.0) -> dict:
         g = nr.Graph.load(os.path.join(G, f"COMP{n}-05.txt"), 0, n - 1)
         period, pilot = nr.tune_sweep_period(g, pilot_eps=1e-9, max_period=6)
         res = {"sweep_period": period, "pilot_esp": pilot}
-        for delta in (1e-12, 1e-15):
-            kw = dict(truncation_eps=delta, queue_capacity=cap, time_budget_s=budget_s)
-            alt = dict(kw, rule="alternate", sweep_period=period)
-            rows = {}
-            if delta == 1e-12:
-                rows["plain_level"] = nr.solve(g, **kw)
-                rows["plain_mass"] = nr.solve(g, order="mass", **kw)
-            rows["alternate_level"] = nr.solve(g, **alt)
-            if delta == 1e-15:
-                rows["alternate_mass"] = nr.solve(g, order="mass", **alt)
-            rows["alternate_learned"] = nr.solve_learned(g, model, **alt)
-            res[f"delta_{delta:g}"] = {k: _row(v) for k, v in rows.items()}
-            print(f"esp {name} delta {delta:g} (period {period}): "
-                  + ", ".join(f"{k} {v['ESP']:.3e} ({v['elapsed_ms']:.0f} ms)"
-                              for k, v in rows.items()), file=sys.stderr, flush=True)
-        d12 = res["delta_1e-12"]
-        res["ESP_plain_over_learned"] = d12["plain_level"]["ESP"] / d12["alternate_learned"]["ESP"]
-        res["ESP_plain_heap_over_learned"] = (d12["plain_mass"]["ESP"]
-                                              / d12["alternate_learned"]["ESP"])
+        kw = dict(truncation_eps=1e-12, queue_capacity=cap, time_budget_s=budget_s)
+        alt = dict(kw, rule="alternate", sweep_period=period)
+        lrn = dict(kw, rule="learned", sweep_period=period)
+        rows = {"plain_level": nr.solve(g, **kw), "plain_mass": nr.solve(g, order="mass", **kw),
+                "alternate_level": nr.solve(g, **alt),
+                "alternate_mass": nr.solve(g, order="mass", **alt),
+                "learned_choice_level": nr.solve_learned(g, model, order="level", **lrn),
+                "learned_choice_mass": nr.solve_learned(g, model, order="mass", **lrn),
+                "learned_choice_learned_order": nr.solve_learned(g, model, **lrn)}
+        d = {k: _row(v) for k, v in rows.items()}
+        res["delta_1e-12"] = d
+        res["ESP_plain_over_alternate"] = _ratio(d["plain_level"]["ESP"],
+                                                 d["alternate_level"]["ESP"])
+        res["ESP_alternate_over_learned_choice"] = _ratio(d["alternate_level"]["ESP"],
+                                                          d["learned_choice_level"]["ESP"])
+        res["ESP_plain_over_learned_choice"] = _ratio(d["plain_level"]["ESP"],
+                                                      d["learned_choice_level"]["ESP"])
+        print(f"esp {name} delta 1e-12 (period {period}): "
+              + ", ".join(f"{k} {v['ESP']:.3e} ({v['elapsed_ms']:.0f} ms)" for k, v in d.items()),
+              file=sys.stderr, flush=True)
+        if n == 15:
+            kw0 = dict(truncation_eps=0.0, queue_capacity=cap, time_budget_s=budget_s)
+            rows0 = {"plain_mass": nr.solve(g, order="mass", **kw0),
+                     "alternate_mass": nr.solve(g, order="mass", rule="alternate",
+                                                sweep_period=period, **kw0),
+                     "learned_choice_mass": nr.solve_learned(g, model, order="mass",
+                                                             rule="learned", sweep_period=period,
+                                                             **kw0),
+                     "learned_choice_learned_order": nr.solve_learned(
+                         g, model, rule="learned", sweep_period=period, **kw0)}
+            d0 = {k: _row(v) for k, v in rows0.items()}
+            res["delta_0"] = d0
+            res["delta_0_ESP_plain_over_alternate"] = _ratio(d0["plain_mass"]["ESP"],
+                                                             d0["alternate_mass"]["ESP"])
+            res["delta_0_ESP_alternate_over_learned_choice"] = _ratio(
+                d0["alternate_mass"]["ESP"], d0["learned_choice_mass"]["ESP"])
+            print(f"esp {name} delta 0 (period {period}): "
+                  + ", ".join(f"{k} {v['ESP']:.3e} ({v['elapsed_ms']:.0f} ms, folded "
+                              f"{v['folded']})" for k, v in d0.items()),
+                  file=sys.stderr, flush=True)
         out[name] = res
     return out
 
 
 def sparse_leg(model, budget_s: float = 1.0) -> dict:
-    """The scorer evaluated off its training graph: a 10x10 grid (180 edges,
+    """Graphs the models were not trained on: a 10x10 grid (180 edges,
     p ~ U[0.85, 0.95] seeded, corner to corner) and Barabasi-Albert n = 150,
     m = 2, p = 0.95 (0 -> 149), at delta = 1e-12 and an equal time budget:
-    plain path rule (level, max-mass) against the alternating rule (period
-    from pilot solves) in level, max-mass and learned order.  No closed form
+    plain path rule, alternating rule (period from pilot solves) and the
+    learned path choice on it, in level and max-mass order.  No closed form
     exists; every bracket must contain R, so all brackets must intersect."""
     import relaibilitygpu_amd as nr
-    from relaibilitygpu_amd import workloads as wl
+    from relaibilitygpu_amd import learn
 
-    rng = wl.Xoshiro256ss(0x6121D)
-    n, edges, _ = wl.grid_graph(10, 10, [0.0] * 1000)
-    probs = [0.85 + 0.10 * rng.uniform() for _ in edges]
-    graphs = [("grid10x10", n, edges, probs, 0, 99, 1e-6)]
-    n, edges, probs = wl.barabasi_albert(150, 2, 0.95, 0xBA150)
-    graphs.append(("ba150", n, edges, probs, 0, 149, 1e-9))
+    graphs = [("grid10x10", learn.grid(10, 10, 0x6121D), 1e-6),
+              ("ba150", learn.barabasi_albert(150, 0xBA150), 1e-9)]
     out = {"budget_s": budget_s, "delta": 1e-12, "queue_capacity": 96777216}
-    for name, n, edges, probs, s, t, pilot_eps in graphs:
-        rp, ci, ei = wl.to_csr(n, edges)
-        g = nr.Graph.from_csr(rp, ci, ei, probs, s, t)
+    for name, g, pilot_eps in graphs:
         period, _ = nr.tune_sweep_period(g, pilot_eps=pilot_eps, max_period=6)
         kw = dict(truncation_eps=1e-12, queue_capacity=96777216, time_budget_s=budget_s)
         alt = dict(kw, rule="alternate", sweep_period=period)
+        lrn = dict(kw, rule="learned", sweep_period=period)
         rows = {"plain_level": nr.solve(g, **kw), "plain_mass": nr.solve(g, order="mass", **kw),
                 "alternate_level": nr.solve(g, **alt),
                 "alternate_mass": nr.solve(g, order="mass", **alt),
-                "alternate_learned": nr.solve_learned(g, model, **alt)}
+                "learned_choice_level": nr.solve_learned(g, model, order="level", **lrn),
+                "learned_choice_mass": nr.solve_learned(g, model, order="mass", **lrn)}
         lo = max(r["R"] for r in rows.values())
         hi = min(r["R"] + r["ESP"] for r in rows.values())
-        res = {"sweep_period": period, "rows": {k: _row(v) for k, v in rows.items()},
+        d = {k: _row(v) for k, v in rows.items()}
+        res = {"sweep_period": period, "rows": d,
                "brackets_intersect": bool(lo <= hi * (1 + 1e-12)),
-               "ESP_plain_over_learned": rows["plain_level"]["ESP"]
-               / rows["alternate_learned"]["ESP"],
-               "ESP_plain_heap_over_learned": rows["plain_mass"]["ESP"]
-               / rows["alternate_learned"]["ESP"]}
+               "ESP_plain_over_alternate": _ratio(d["plain_level"]["ESP"],
+                                                  d["alternate_level"]["ESP"]),
+               "ESP_alternate_over_learned_choice": _ratio(d["alternate_level"]["ESP"],
+                                                           d["learned_choice_level"]["ESP"]),
+               "ESP_alternate_mass_over_learned_choice_mass": _ratio(
+                   d["alternate_mass"]["ESP"], d["learned_choice_mass"]["ESP"])}
         out[name] = res
         print(f"sparse {name} (period {period}): "
-              + ", ".join(f"{k} {v['ESP']:.3e} ({v['elapsed_ms']:.0f} ms)"
-                          for k, v in rows.items()), file=sys.stderr, flush=True)
+              + ", ".join(f"{k} {v['ESP']:.3e} ({v['elapsed_ms']:.0f} ms)" for k, v in d.items()),
+              file=sys.stderr, flush=True)
     return out
 
 
@@ -321,6 +383,8 @@ def main() -> int:
                     help="batched workloads: instances timed on the CPU")
     ap.add_argument("--no-esp", action="store_true",
                     help="skip the learned-vs-plain certified-gap leg")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched (4096 Erdos-Renyi instances) object")
     ap.add_argument("--esp-budget", type=float, default=60.0)
     ap.add_argument("--sparse-budget", type=float, default=1.0,
                     help="time budget of the grid / Barabasi-Albert certified-gap leg")
@@ -430,8 +494,19 @@ def main() -> int:
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(wl)
+        line["cpu_baseline_reference"] = reference_cpu(dict(wl, subproblems=nodes))
+    if not args.no_batch:
+        bw = WORKLOADS["er4096"]
+        b = run_batch(args, bw, steps=3, warmup=1)
+        insts = b.pop("_insts")
+        if not args.no_cpu:
+            b["cpu_baseline"] = cpu_baseline_batch(insts, bw["eps"], args.cpu_sample)
+        b["unit"] = "subproblems/s"
+        line["batch_er4096"] = b
+        print(f"batch er4096: {b['value']:.4g} subproblems/s, {b['ms_per_step']:.1f} ms per "
+              f"batch", file=sys.stderr, flush=True)
     if not args.no_esp:
-        model, info = train_scorer()
+        model, info = train_models()
         line["esp_at_budget"] = esp_leg(model, args.esp_budget)
         line["esp_at_budget"]["training"] = info
         line["esp_sparse_graphs"] = sparse_leg(model, args.sparse_budget)

@@ -93,6 +93,12 @@ typedef enum nr_rule {
                           dst end otherwise.  Undirected graphs of up to 256
                           nodes, two terminals; NR_ERR_UNSUPPORTED
                           otherwise */
+  NR_RULE_LEARNED = 4,  /* no counterpart in the reference: the path of
+                          NR_RULE_ALTERNATE, swept from the end the model's
+                          path chooser predicts the smaller subtree for, by
+                          more than its margin (nr_model_set_chooser; else
+                          the alternating direction).  nr_solve_learned only;
+                          same graphs as NR_RULE_ALTERNATE */
   NR_RULE_PIVOT = 3     /* edge-pivot factoring (workingPad/cpu_reference.cpp
                           :107-145): success on a confirmed WORKING path,
                           dead when dst is unreachable, else a two-way branch
@@ -298,6 +304,65 @@ void nr_checkpoint_destroy(nr_checkpoint* ckpt);
 
 typedef struct nr_model nr_model;
 
+/* ------------------------------------------------------------------------
+ * Learned path choice (NR_RULE_LEARNED).  No counterpart in the reference:
+ * its one path rule (device_code.cuh:80-194, main.cu:114-283) is joined by a
+ * per-subproblem choice between two exact decompositions of the same
+ * fewest-UNKNOWN path: children swept from the src end or from the dst end.
+ * A second network of the same shape (16 -> 64 ReLU -> 1) predicts log2 of
+ * the subproblem's subtree size for each direction (trained on choice logs:
+ * a direction, then the alternating rule below); the sweep leaves the
+ * alternating rule's direction when the other is predicted smaller by more
+ * than a margin.
+ *
+ * Choice features (NR_CHOICE_FEATURES integers, |x| <= 63), for a sweep that
+ * starts at end A (src or dst) and ends at B; "component" is the end's node
+ * set joined by WORKING edges:
+ *   0 UNKNOWN hops of the path        1 nodes of A's component
+ *   2 UNKNOWN edges leaving A's comp. 3 FAILED edges leaving A's component
+ *   4-6 the same for B                7 32*FAILED/E  8 32*UNKNOWN/E
+ *   9 0 (unused)                      10 floor((floor(log2 mass) - le) / 2)
+ *                                        in 0..32, le = floor(log2 eps), or
+ *                                        -64 when eps < 2^-64 or 0
+ *  11 -sum over the hops of lq(p)    12 -lq(q) of A's hop  13 -lq(q) of B's
+ *  14 (2) - (5)                       15 constant 1
+ * where lq(x) = round(-4 log2 x) clamped to 0..63, and features 0-6, 11, 14
+ * are clamped to +-63.  Only mass relative to the truncation enters (10), so
+ * a solve truncated deeper than the training logs, or not at all, shows the
+ * chooser the same kind of states.  The inference form is exact: weights a/256 with
+ * integer |a| <= 255 (W1 times |w2|, one scale), so BF16 operands and FP32
+ * sums are exact and the choice is an integer comparison.
+ * ------------------------------------------------------------------------ */
+#define NR_CHOICE_FEATURES 16
+
+/* Attach a path chooser (NR_MLP_PARAMS floats in the layout of the order
+ * scorer, trained on choice features) to a model.  A subproblem leaves the
+ * alternating direction only when the other direction's predicted log2
+ * subtree size is smaller by more than `margin` (>= 0; a huge margin keeps
+ * the alternating rule). */
+int nr_model_set_chooser(nr_model* model, const float* params, float margin);
+
+/* The chooser's exact integer form: weights[16*64] (row k = feature, column
+ * h = hidden unit, whose value is relu(sum_k x_k a[k][h])), signs[64] (1:
+ * the unit counts negatively; a direction's score is the signed sum of the
+ * units) and the margin in score units: the sweep leaves the alternating
+ * direction d when score(d) - score(other) > margin.  NR_ERR_ARG without a
+ * chooser. */
+int nr_model_get_chooser(const nr_model* model, int* weights, int* signs, int* margin);
+
+/* Solve in level order with the alternating rule (cfg->sweep_period) while
+ * the sweep direction explores: it is flipped for a reproducible
+ * pseudo-random quarter of the subproblems (the top two bits of a
+ * splitmix64-finaliser chain over the planes are zero), so the labels
+ * measure each direction followed by the rule the chooser improves on.
+ * Logs for every expanded
+ * subproblem with at least two UNKNOWN hops its choice features of the
+ * direction taken; labels = log2 of its subtree nodes under this policy.
+ * Sampling and the other arguments as nr_log_subproblems. */
+int nr_log_choices(const nr_graph* g, const nr_config* cfg, long long max_records,
+                   unsigned long long seed, float* features, float* labels,
+                   long long* n_out, nr_result* result);
+
 /* Solve in level order while logging every child (features at creation,
  * parent, depth), then derive each child's subtree size bottom-up on the
  * device and return up to max_records of them: the records with the
@@ -338,7 +403,8 @@ int nr_mlp_grad(const float* X, const float* y, long long n, const float* params
 int nr_mlp_train(const float* X, const float* y, long long n, const nr_train_config* cfg,
                  float* params, float* train_loss, float* holdout_loss);
 
-/* Inference form of trained parameters (BF16 weights, FP32 accumulation). */
+/* Inference form of trained parameters (BF16 weights, FP32 accumulation);
+ * params may be NULL for a model that only carries a path chooser. */
 int nr_model_create(const float* params, float alpha, nr_model** out);
 void nr_model_destroy(nr_model* model);
 
@@ -346,7 +412,11 @@ void nr_model_destroy(nr_model* model);
 int nr_model_score(const nr_model* model, const float* features, long long n,
                    float* scores, int device);
 
-/* nr_solve with the learned order (cfg->order is ignored). */
+/* nr_solve with a model: cfg->order NR_ORDER_LEARNED orders the queue by
+ * the model's order scorer (NR_ORDER_LEVEL and NR_ORDER_MASS as nr_solve);
+ * cfg->rule NR_RULE_LEARNED picks sweeps by its path chooser (NR_RULE_PATH
+ * and NR_RULE_ALTERNATE as nr_solve).  NR_ERR_ARG when the model lacks the
+ * part asked for. */
 int nr_solve_learned(const nr_graph* g, const nr_config* cfg, const nr_model* model,
                      nr_result* out);
 

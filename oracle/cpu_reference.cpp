@@ -217,6 +217,11 @@ struct Engine {
   const int* terms = nullptr; /* K-terminal mode when nterms >= 2 */
   int nterms = 0;
   int period = 2; /* ORC_MODE_ALTERNATE: sweep flips every `period` FAILED edges */
+  /* ORC_MODE_LEARNED: integer chooser weights and signs; per-edge steps
+   * round(-4 log2 p), round(-4 log2 q) clamped to 0..63 */
+  const int* chooser = nullptr;
+  const unsigned char* lqp = nullptr;
+  const unsigned char* lqq = nullptr;
 
   bool allowed(const u64* known, const u64* up, int e) const {
     u64 bit = 1ull << (e & 63);
@@ -614,6 +619,77 @@ int expand_minunk_impl(const Engine& en, const u64* known, const u64* up, double
   return k;
 }
 
+/* Whether the exploring policy of ORC_MODE_EXPLORE flips the alternating
+ * direction of this item: the top two bits of a splitmix64-finaliser chain
+ * over its planes are zero (a quarter of the items). */
+static bool explore_flip(const u64* known, const u64* up, int W) {
+  auto fin = [](u64 z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  u64 h = 0x9E3779B97F4A7C15ull;
+  for (int i = 0; i < W; ++i) {
+    h = fin(h ^ known[i]);
+    h = fin(h ^ up[i]);
+  }
+  return (h >> 62) == 0;
+}
+
+/* Learned sweep choice (ORC_MODE_LEARNED): the chooser's score of sweeping
+ * the path hops[0..L) (src end first) from the dst end (from_dst) or the
+ * src end, in units of 2^-8, computed with integers: hidden unit h is
+ * relu(sum_k x_k a[k][h]), the score the signed sum of the units.  The
+ * features x restate the engine's choice features from the item itself. */
+static int chooser_score(const Engine& en, const std::vector<char>& S,
+                         const std::vector<char>& T, const int* hops, int L, int f, int unk,
+                         double mass, const u64* known, const u64* up, bool from_dst) {
+  const orc_graph* g = en.g;
+  auto leaving = [&](const std::vector<char>& C, int* nodes, int* unknown, int* failed) {
+    *nodes = *unknown = *failed = 0;
+    for (int u = 0; u < g->n; ++u) {
+      if (!C[u]) continue;
+      ++*nodes;
+      for (int j = g->row_ptr[u]; j < g->row_ptr[u + 1]; ++j) {
+        if (C[g->col_idx[j]]) continue;
+        if (en.allowed(known, up, g->edge_id[j])) ++*unknown; else ++*failed;
+      }
+    }
+  };
+  int nS, uS, fS, nT, uT, fT;
+  leaving(S, &nS, &uS, &fS);
+  leaving(T, &nT, &uT, &fT);
+  auto lim = [](int v) { return std::max(-63, std::min(63, v)); };
+  auto half_down = [](int v) { return (int)std::floor(v / 2.0); };
+  auto exponent = [](double m) {
+    u64 b;
+    std::memcpy(&b, &m, 8);
+    const int e = (int)((b >> 52) & 0x7ff);
+    return e == 0 ? -1075 : e - 1023;
+  };
+  const int lm = exponent(mass);
+  /* headroom against eps, or 2^-64 below that, saturating at 32 half-binades */
+  const int le = en.eps > std::ldexp(1.0, -64) ? exponent(en.eps) : -64;
+  const int head = std::max(0, std::min(32, half_down(lm - le)));
+  int lp = 0;
+  for (int i = 0; i < L; ++i) lp += en.lqp[hops[i]];
+  const int first = from_dst ? hops[L - 1] : hops[0];
+  const int last = from_dst ? hops[0] : hops[L - 1];
+  const int nA = from_dst ? nT : nS, uA = from_dst ? uT : uS, fA = from_dst ? fT : fS;
+  const int nB = from_dst ? nS : nT, uB = from_dst ? uS : uT, fB = from_dst ? fS : fT;
+  const int E = std::max(g->e, 1);
+  const int x[16] = {lim(L), lim(nA), lim(uA), lim(fA), lim(nB), lim(uB), lim(fB),
+                     (32 * f) / E, (32 * unk) / E, 0, head, -lim(lp),
+                     -(int)en.lqq[first], -(int)en.lqq[last], lim(uA - uB), 1};
+  long long score = 0;
+  for (int h = 0; h < 64; ++h) {
+    long long z = 0;
+    for (int k = 0; k < 16; ++k) z += (long long)x[k] * en.chooser[k * 64 + h];
+    if (z > 0) score += en.chooser[16 * 64 + h] ? -z : z;
+  }
+  return (int)score;
+}
+
 /* Alternating-sweep rule (ORC_MODE_ALTERNATE; no counterpart in the
  * reference).  Any src->dst path, and any order of its UNKNOWN hops, splits
  * an item into disjoint children plus the all-WORKING success term, so this
@@ -715,7 +791,24 @@ int expand_alt_impl(const Engine& en, const u64* known, const u64* up, double ma
   int f = 0;
   for (int i = 0; i < W; ++i) f += __builtin_popcountll(known[i] & ~up[i]);
   const int per = en.period > 0 ? en.period : 2;
-  if ((f / per) & 1) std::reverse(hops, hops + L);
+  bool rev = ((f / per) & 1) != 0;
+  if (en.mode == ORC_MODE_EXPLORE && L >= 2) {
+    rev = rev != explore_flip(known, up, W);
+  } else if (en.mode == ORC_MODE_LEARNED && L >= 2) {
+    /* src's component is the level-0 node set; dst's is closed anew */
+    std::vector<char> S(n, 0);
+    for (int v = 0; v < n; ++v) S[v] = lev[v] == 0;
+    const std::vector<char> T = comp(en.dst);
+    int unk = 0;
+    for (int e = 0; e < g->e; ++e)
+      if (!((known[e >> 6] >> (e & 63)) & 1)) ++unk;
+    const int score_fwd = chooser_score(en, S, T, hops, L, f, unk, mass, known, up, false);
+    const int score_rev = chooser_score(en, S, T, hops, L, f, unk, mass, known, up, true);
+    /* leave the alternating direction only for an advantage above the margin */
+    const int s_alt = rev ? score_rev : score_fwd, s_other = rev ? score_fwd : score_rev;
+    if (s_alt - s_other > en.chooser[16 * 64 + 64]) rev = !rev;
+  }
+  if (rev) std::reverse(hops, hops + L);
   u64 pre[kMaxW] = {0};
   double run = mass;
   for (int h = 0; h < L; ++h) {
@@ -813,7 +906,8 @@ struct Worker {
     else if (en.mode == ORC_MODE_MINUNK)
       k = expand_minunk_impl(en, known, up, mass, ck.data(), cu.data(), cm.data(), &succ,
                              hops.data());
-    else if (en.mode == ORC_MODE_ALTERNATE)
+    else if (en.mode == ORC_MODE_ALTERNATE || en.mode == ORC_MODE_LEARNED ||
+             en.mode == ORC_MODE_EXPLORE)
       k = expand_alt_impl(en, known, up, mass, ck.data(), cu.data(), cm.data(), &succ,
                           hops.data());
     else
@@ -825,7 +919,8 @@ struct Worker {
       return 0;
     }
     if (en.mode == ORC_MODE_PATH || en.mode == ORC_MODE_MINUNK ||
-        en.mode == ORC_MODE_ALTERNATE || k == 0) {
+        en.mode == ORC_MODE_ALTERNATE || en.mode == ORC_MODE_LEARNED ||
+        en.mode == ORC_MODE_EXPLORE || k == 0) {
       acc.R.add(succ);
       acc.paths++;
     }
@@ -1295,6 +1390,22 @@ static int check_common(const orc_graph* g, int src, int dst) {
   return ORC_OK;
 }
 
+/* Per-edge probability steps of the learned choice: round(-4 log2 p) and
+ * round(-4 log2 q), clamped to 0..63 (63 for a zero probability). */
+static void choice_steps(const orc_graph* g, std::vector<unsigned char>& lqp,
+                         std::vector<unsigned char>& lqq) {
+  auto step = [](double x) -> unsigned char {
+    if (!(x > 0.0)) return 63;
+    return (unsigned char)std::max(0.0, std::min(63.0, std::nearbyint(-4.0 * std::log2(x))));
+  };
+  lqp.resize(g->e);
+  lqq.resize(g->e);
+  for (int e = 0; e < g->e; ++e) {
+    lqp[e] = step(g->p[e]);
+    lqq[e] = step(g->q[e]);
+  }
+}
+
 int orc_solve(const orc_graph* g, int src, int dst, const orc_config* cfg,
               orc_result* out) {
   int rc = check_common(g, src, dst);
@@ -1324,6 +1435,12 @@ int orc_solve(const orc_graph* g, int src, int dst, const orc_config* cfg,
   std::memset(out, 0, sizeof(*out));
   Engine en{g, src, dst, g->w, g->nw, cfg->mode, cfg->trunc_eps};
   en.period = cfg->sweep_period;
+  if (cfg->mode == ORC_MODE_LEARNED && !cfg->chooser) return ORC_ERR_ARG;
+  en.chooser = cfg->chooser;
+  std::vector<unsigned char> lqp, lqq;
+  choice_steps(g, lqp, lqq);
+  en.lqp = lqp.data();
+  en.lqq = lqq.data();
   Acc acc;
   auto t0 = Clock::now();
   switch (cfg->order) {
@@ -1429,6 +1546,38 @@ int orc_expand_rule(const orc_graph* g, int src, int dst, int mode, int period,
   if (succ) *succ = (k < 0) ? 0.0 : s;
   if (k < 0) return -1;
   if (path_eids && mode == ORC_MODE_ALTERNATE)
+    for (int i = 0; i < k; ++i) path_eids[i] = hops[i];
+  const int m = std::min(k, max_children);
+  if (child_known) std::copy(ck.begin(), ck.begin() + (size_t)m * g->w, child_known);
+  if (child_up) std::copy(cu.begin(), cu.begin() + (size_t)m * g->w, child_up);
+  if (child_mass) std::copy(cm.begin(), cm.begin() + m, child_mass);
+  return k;
+}
+
+int orc_expand_choice(const orc_graph* g, int src, int dst, int mode, int period,
+                      const int* chooser, double eps, const uint64_t* known,
+                      const uint64_t* up, double mass, int max_children, uint64_t* child_known,
+                      uint64_t* child_up, double* child_mass, double* succ, int* path_eids) {
+  if (check_common(g, src, dst)) return -2;
+  if (mode != ORC_MODE_ALTERNATE && mode != ORC_MODE_LEARNED && mode != ORC_MODE_EXPLORE)
+    return -2;
+  if (mode == ORC_MODE_LEARNED && !chooser) return -2;
+  Engine en{g, src, dst, g->w, g->nw, mode, eps};
+  en.period = period;
+  en.chooser = chooser;
+  std::vector<unsigned char> lqp, lqq;
+  choice_steps(g, lqp, lqq);
+  en.lqp = lqp.data();
+  en.lqq = lqq.data();
+  std::vector<int> hops(g->n + 1);
+  std::vector<u64> ck((size_t)(g->n + 1) * g->w), cu((size_t)(g->n + 1) * g->w);
+  std::vector<double> cm(g->n + 1);
+  double s = 0.0;
+  const int k =
+      expand_alt_impl(en, known, up, mass, ck.data(), cu.data(), cm.data(), &s, hops.data());
+  if (succ) *succ = (k < 0) ? 0.0 : s;
+  if (k < 0) return -1;
+  if (path_eids)
     for (int i = 0; i < k; ++i) path_eids[i] = hops[i];
   const int m = std::min(k, max_children);
   if (child_known) std::copy(ck.begin(), ck.begin() + (size_t)m * g->w, child_known);

@@ -47,6 +47,15 @@ extern "C" {
 #define ORC_MODE_ALTERNATE 4 /* fewest-UNKNOWN path (lowest-index walk back over
                                 WORKING components), children swept from the src
                                 end or the dst end by the FAILED edge count */
+#define ORC_MODE_LEARNED 5   /* the same path, swept as ORC_MODE_ALTERNATE unless
+                                the integer chooser (orc_config.chooser) scores
+                                the other direction lower by more than its
+                                margin */
+#define ORC_MODE_EXPLORE 6   /* the same path, swept as ORC_MODE_ALTERNATE but
+                                in the other direction where the top two bits
+                                of a splitmix64 hash of the planes are zero
+                                (the exploring policy of the engine's choice
+                                log) */
 
 /* status codes */
 #define ORC_OK 0
@@ -65,6 +74,10 @@ typedef struct orc_config {
   double time_budget_s;  /* 0 = none; on expiry the queue folds into ESP */
   int sweep_period;      /* ORC_MODE_ALTERNATE: flip every this many FAILED
                             edges (<= 0: 2) */
+  const int* chooser;    /* ORC_MODE_LEARNED: a[16*64] integer weights (row =
+                            feature), s[64] signs (1 = negative unit), then
+                            the margin (score units) needed to leave the
+                            alternating direction */
 } orc_config;
 
 typedef struct orc_result {
@@ -119,6 +132,14 @@ int orc_expand_rule(const orc_graph* g, int src, int dst, int mode, int period,
                     const uint64_t* known, const uint64_t* up, double mass, int max_children,
                     uint64_t* child_known, uint64_t* child_up, double* child_mass, double* succ,
                     int* path_eids);
+
+/* orc_expand_rule for ORC_MODE_ALTERNATE, ORC_MODE_LEARNED (chooser as in
+ * orc_config, eps its truncation for the headroom feature) and
+ * ORC_MODE_EXPLORE; path_eids receives the hops in sweep order. */
+int orc_expand_choice(const orc_graph* g, int src, int dst, int mode, int period,
+                      const int* chooser, double eps, const uint64_t* known,
+                      const uint64_t* up, double mass, int max_children, uint64_t* child_known,
+                      uint64_t* child_up, double* child_mass, double* succ, int* path_eids);
 
 /* Per-instance exact counts for a list of items processed to completion in
  * DFS order (full subtree under each item, with truncation eps). */

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "liboracle_nr.so")
 CLI_PATH = os.path.join(HERE, "bin", "cpu_reference")
 
 MODE_PATH, MODE_PIVOT, MODE_FORKBEST, MODE_MINUNK, MODE_ALTERNATE = 0, 1, 2, 3, 4
+MODE_LEARNED, MODE_EXPLORE = 5, 6
 ORDER_LEVEL, ORDER_HEAP, ORDER_DFS = 0, 1, 2
 
 
@@ -27,6 +28,7 @@ class OrcConfig(ctypes.Structure):
         ("capacity", ctypes.c_longlong),
         ("time_budget_s", ctypes.c_double),
         ("sweep_period", ctypes.c_int),
+        ("chooser", ctypes.POINTER(ctypes.c_int)),
     ]
 
 
@@ -85,6 +87,11 @@ def lib() -> ctypes.CDLL:
         L.orc_expand_rule.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int, U, U, ctypes.c_double, ctypes.c_int, U, U,
                                       P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_int)]
+        L.orc_expand_choice.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, P(ctypes.c_int), ctypes.c_double, U, U,
+                                        ctypes.c_double, ctypes.c_int, U, U,
+                                        P(ctypes.c_double), P(ctypes.c_double),
+                                        P(ctypes.c_int)]
         L.orc_subtree_stats.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, U, U,
                                         P(ctypes.c_double), ctypes.c_double, U,
                                         P(ctypes.c_double)]
@@ -143,8 +150,19 @@ class OracleGraph:
 
     def solve(self, src: int, dst: int, mode: int = MODE_PATH, order: int = ORDER_LEVEL,
               threads: int = 1, trunc_eps: float = 0.0, capacity: int = 0,
-              time_budget_s: float = 0.0, sweep_period: int = 0) -> dict:
+              time_budget_s: float = 0.0, sweep_period: int = 0, chooser=None) -> dict:
+        """chooser (MODE_LEARNED): (weights [16, 64], signs [64], margin)
+        integers, the exact form the engine reports for a model's path
+        chooser (margin 0 when omitted)."""
         cfg = OrcConfig(mode, order, threads, trunc_eps, capacity, time_budget_s, sweep_period)
+        keep = None
+        if chooser is not None:
+            w, sg, margin = chooser if len(chooser) == 3 else (*chooser, 0)
+            keep = np.ascontiguousarray(np.concatenate([np.asarray(w, np.int32).reshape(-1),
+                                                        np.asarray(sg, np.int32).reshape(-1),
+                                                        np.asarray([margin], np.int32)]))
+            assert keep.size == 16 * 64 + 64 + 1
+            cfg.chooser = _p(keep, ctypes.c_int)
         r = OrcResult()
         rc = lib().orc_solve(self._h, src, dst, ctypes.byref(cfg), ctypes.byref(r))
         if rc:
@@ -184,6 +202,31 @@ class OracleGraph:
                              ctypes.byref(succ), ctypes.byref(nh), _p(path, ctypes.c_int))
         kk = max(k, 0)
         return k, succ.value, ck[:kk], cu[:kk], cm[:kk], path[: max(nh.value, 0)]
+
+    def expand_choice(self, src: int, dst: int, known, up, mass: float, mode: int,
+                      period: int = 0, chooser=None, eps: float = 0.0):
+        """One expansion under MODE_ALTERNATE, MODE_LEARNED (chooser =
+        (weights [16, 64], signs [64])) or MODE_EXPLORE: (k, hops in sweep
+        order); k = -1 when dead."""
+        W = self.mask_words
+        n = self.dims()["n"]
+        known = np.ascontiguousarray(known, np.uint64).reshape(W)
+        up = np.ascontiguousarray(up, np.uint64).reshape(W)
+        path = np.zeros(n + 1, np.int32)
+        keep = None
+        if chooser is not None:
+            w, sg, margin = chooser if len(chooser) == 3 else (*chooser, 0)
+            keep = np.ascontiguousarray(np.concatenate([np.asarray(w, np.int32).reshape(-1),
+                                                        np.asarray(sg, np.int32).reshape(-1),
+                                                        np.asarray([margin], np.int32)]))
+        U = ctypes.c_ulonglong
+        k = lib().orc_expand_choice(self._h, src, dst, mode, period,
+                                    None if keep is None else _p(keep, ctypes.c_int), float(eps),
+                                    _p(known, U), _p(up, U), float(mass), 0, None, None, None,
+                                    None, _p(path, ctypes.c_int))
+        if k < -1:
+            raise RuntimeError("orc_expand_choice: bad arguments")
+        return k, [int(h) for h in path[: max(k, 0)]]
 
     def expand_rule(self, src: int, dst: int, known, up, mass: float, mode: int = MODE_PATH,
                     period: int = 0):

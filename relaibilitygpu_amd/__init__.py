@@ -34,8 +34,9 @@ NR_RULE_PATH = 0
 NR_RULE_FORKBEST = 1
 NR_RULE_ALTERNATE = 2
 NR_RULE_PIVOT = 3
+NR_RULE_LEARNED = 4
 RULES = {"path": NR_RULE_PATH, "forkbest": NR_RULE_FORKBEST, "alternate": NR_RULE_ALTERNATE,
-         "pivot": NR_RULE_PIVOT}
+         "pivot": NR_RULE_PIVOT, "learned": NR_RULE_LEARNED}
 
 STATUS = {
     0: "NR_OK",
@@ -137,6 +138,7 @@ class TrainConfig(ctypes.Structure):
 MLP_FEATURES = 16
 MLP_HIDDEN = 64
 MLP_PARAMS = 16 * 64 + 64 + 1
+CHOICE_FEATURES = 16
 
 _lib = None
 
@@ -200,6 +202,10 @@ def lib() -> ctypes.CDLL:
     L.nr_model_destroy.restype = None
     L.nr_model_score.argtypes = [vp, fp, ctypes.c_longlong, fp, ctypes.c_int]
     L.nr_solve_learned.argtypes = [vp, P(Config), vp, P(Result)]
+    L.nr_model_set_chooser.argtypes = [vp, fp, ctypes.c_float]
+    L.nr_model_get_chooser.argtypes = [vp, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int)]
+    L.nr_log_choices.argtypes = [vp, P(Config), ctypes.c_longlong, ctypes.c_ulonglong, fp, fp,
+                                 P(ctypes.c_longlong), P(Result)]
     L.nr_solve_checkpointed.argtypes = [vp, P(Config), vp, P(vp), P(Result)]
     L.nr_checkpoint_info.argtypes = [vp, P(ctypes.c_longlong), P(ctypes.c_double),
                                      P(ctypes.c_longlong)]
@@ -463,6 +469,22 @@ def log_subproblems(graph: Graph, max_records: int, seed: int = 1,
     return X[: n.value], Y[: n.value], res.as_dict()
 
 
+def log_choices(graph: Graph, max_records: int, seed: int = 1,
+                config: Config | None = None, **kw):
+    """Choice log (nr_log_choices): (features [m,16] of the sweep direction
+    taken, labels [m] = log2 subtree nodes, result dict) for m <= max_records
+    subproblems of an exploring solve of the alternating rule's path."""
+    cfg = config if config is not None else make_config(**kw)
+    X = np.zeros((max(max_records, 1), CHOICE_FEATURES), np.float32)
+    Y = np.zeros(max(max_records, 1), np.float32)
+    n = ctypes.c_longlong()
+    res = Result()
+    _check(lib().nr_log_choices(graph.handle, ctypes.byref(cfg), int(max_records), int(seed),
+                                _ptr(X, ctypes.c_float), _ptr(Y, ctypes.c_float),
+                                ctypes.byref(n), ctypes.byref(res)), "nr_log_choices")
+    return X[: n.value], Y[: n.value], res.as_dict()
+
+
 def mlp_init(seed: int = 1) -> np.ndarray:
     p = np.zeros(MLP_PARAMS, np.float32)
     _check(lib().nr_mlp_init(int(seed), _ptr(p, ctypes.c_float)), "nr_mlp_init")
@@ -500,15 +522,36 @@ def mlp_train(X, y, params=None, epochs: int = 8, batch: int = 4096, lr: float =
 
 
 class Model:
-    """Inference form of trained parameters (nr_model)."""
+    """Inference form of trained parameters (nr_model): the order scorer
+    (params, may be None) and optionally a path chooser (chooser params)."""
 
-    def __init__(self, params, alpha: float = 1.0):
-        self.params = _f32(params).reshape(MLP_PARAMS).copy()
+    def __init__(self, params=None, alpha: float = 1.0, chooser=None, margin: float = 0.0):
+        self.params = None if params is None else _f32(params).reshape(MLP_PARAMS).copy()
         self.alpha = float(alpha)
         h = ctypes.c_void_p()
-        _check(lib().nr_model_create(_ptr(self.params, ctypes.c_float), self.alpha,
-                                     ctypes.byref(h)), "nr_model_create")
+        pp = None if self.params is None else _ptr(self.params, ctypes.c_float)
+        _check(lib().nr_model_create(pp, self.alpha, ctypes.byref(h)), "nr_model_create")
         self._h = h
+        self.chooser = None
+        self.margin = float(margin)
+        if chooser is not None:
+            self.set_chooser(chooser, margin)
+
+    def set_chooser(self, params, margin: float = 0.0) -> None:
+        self.chooser = _f32(params).reshape(MLP_PARAMS).copy()
+        self.margin = float(margin)
+        _check(lib().nr_model_set_chooser(self._h, _ptr(self.chooser, ctypes.c_float),
+                                          self.margin), "nr_model_set_chooser")
+
+    def chooser_weights(self):
+        """(weights int32 [16, 64], signs int32 [64], margin int): the
+        chooser's exact form."""
+        w = np.zeros(16 * 64, np.int32)
+        sg = np.zeros(64, np.int32)
+        mi = ctypes.c_int()
+        _check(lib().nr_model_get_chooser(self._h, _ptr(w, ctypes.c_int), _ptr(sg, ctypes.c_int),
+                                          ctypes.byref(mi)), "nr_model_get_chooser")
+        return w.reshape(16, 64), sg, mi.value
 
     @property
     def handle(self):
@@ -529,6 +572,10 @@ class Model:
 
 
 def solve_learned(graph: Graph, model: Model, config: Config | None = None, **kw) -> dict:
+    """nr_solve_learned: order "learned" (the default here) uses the model's
+    order scorer, rule "learned" its path chooser."""
+    if config is None:
+        kw.setdefault("order", "learned")
     cfg = config if config is not None else make_config(**kw)
     r = Result()
     _check(lib().nr_solve_learned(graph.handle, ctypes.byref(cfg), model.handle, ctypes.byref(r)),

@@ -4,8 +4,8 @@
  *
  *   network_reliability <graph_file> <src> <dst> [--truncation_eps f]
  *       [--queue_capacity n] [--thresh_high f] [--thresh_low f]
- *       [--order level|mass|learned] [--rule path|forkbest|alternate|pivot]
- *       [--sweep_period n] [--model file]
+ *       [--order level|mass|learned] [--rule path|forkbest|alternate|pivot|learned]
+ *       [--sweep_period n] [--model file] [--chooser file] [--margin m]
  *       [--alpha a] [--terminals a,b,c,...] [--time_budget s] [--device d] [--json]
  *
  * Argument handling follows workingPad/main.cu:753-790: fewer than three
@@ -32,10 +32,13 @@ static void usage(const char* prog, long long cap) {
                "  --thresh_high <float>      High-priority log-prob threshold\n"
                "  --thresh_low  <float>      Low-priority  log-prob threshold\n"
                "  --order <level|mass|learned> Work order (default level)\n"
-               "  --rule <path|forkbest|alternate|pivot> Decomposition rule (default path)\n"
+               "  --rule <path|forkbest|alternate|pivot|learned> Decomposition rule (default path)\n"
                "  --sweep_period <int>       Alternate rule: flip every n levels (default 2)\n"
                "  --model <file>             Learned order: NR_MLP_PARAMS raw float32\n"
                "  --alpha <float>            Learned order: score weight (default 0.5)\n"
+               "  --chooser <file>           Learned rule: NR_MLP_PARAMS raw float32\n"
+               "  --margin <float>           Learned rule: log2 advantage to leave the\n"
+               "                             alternating direction (default 0)\n"
                "  --terminals <a,b,...>      K-terminal reliability (first is the root)\n"
                "  --time_budget <seconds>    Fold the queue into ESP after this\n"
                "  --device <int>             HIP device ordinal\n"
@@ -55,8 +58,10 @@ int main(int argc, char** argv) {
   int dst = std::atoi(argv[3]);
   bool json = false;
   const char* model_file = nullptr;
+  const char* chooser_file = nullptr;
   const char* terminals = nullptr;
   float alpha = 0.5f;
+  float margin = 0.0f; /* learned rule: prediction advantage to leave the alternation */
   double thresh_high = -4.60517, thresh_low = -9.21034; /* common.h:35-36 */
   for (int i = 4; i < argc; ++i) {
     const bool has = i + 1 < argc;
@@ -91,6 +96,8 @@ int main(int argc, char** argv) {
         cfg.rule = NR_RULE_ALTERNATE;
       } else if (!std::strcmp(r, "pivot")) {
         cfg.rule = NR_RULE_PIVOT;
+      } else if (!std::strcmp(r, "learned")) {
+        cfg.rule = NR_RULE_LEARNED;
       } else {
         std::fprintf(stderr, "unknown --rule %s\n", r);
         usage(argv[0], cfg.queue_capacity);
@@ -100,6 +107,10 @@ int main(int argc, char** argv) {
       cfg.sweep_period = std::atoi(argv[++i]);
     } else if (!std::strcmp(argv[i], "--model") && has) {
       model_file = argv[++i];
+    } else if (!std::strcmp(argv[i], "--chooser") && has) {
+      chooser_file = argv[++i];
+    } else if (!std::strcmp(argv[i], "--margin") && has) {
+      margin = (float)std::atof(argv[++i]);
     } else if (!std::strcmp(argv[i], "--terminals") && has) {
       terminals = argv[++i];
     } else if (!std::strcmp(argv[i], "--alpha") && has) {
@@ -158,19 +169,29 @@ int main(int argc, char** argv) {
     std::fflush(stdout);
   }
   nr_result r;
-  if (cfg.order == NR_ORDER_LEARNED) {
-    float params[NR_MLP_PARAMS];
-    FILE* f = model_file ? std::fopen(model_file, "rb") : nullptr;
-    const size_t got = f ? std::fread(params, sizeof(float), NR_MLP_PARAMS, f) : 0;
-    if (f) std::fclose(f);
-    if (got != NR_MLP_PARAMS) {
-      std::fprintf(stderr, "--order learned needs --model with %d float32 parameters\n",
+  if (cfg.order == NR_ORDER_LEARNED || cfg.rule == NR_RULE_LEARNED) {
+    /* raw float32 parameter files: the order scorer, the path chooser */
+    auto read_params = [](const char* path, float* params) {
+      FILE* f = path ? std::fopen(path, "rb") : nullptr;
+      const size_t got = f ? std::fread(params, sizeof(float), NR_MLP_PARAMS, f) : 0;
+      if (f) std::fclose(f);
+      return got == NR_MLP_PARAMS;
+    };
+    static float params[NR_MLP_PARAMS], cparams[NR_MLP_PARAMS];
+    const bool need_order = cfg.order == NR_ORDER_LEARNED;
+    const bool need_chooser = cfg.rule == NR_RULE_LEARNED;
+    if ((need_order && !read_params(model_file, params)) ||
+        (need_chooser && !read_params(chooser_file, cparams))) {
+      std::fprintf(stderr,
+                   "--order learned needs --model and --rule learned needs --chooser, each "
+                   "a file of %d float32 parameters\n",
                    NR_MLP_PARAMS);
       nr_graph_destroy(g);
       return 1;
     }
     nr_model* m = nullptr;
-    rc = nr_model_create(params, alpha, &m);
+    rc = nr_model_create(need_order ? params : nullptr, alpha, &m);
+    if (rc == NR_OK && need_chooser) rc = nr_model_set_chooser(m, cparams, margin);
     if (rc == NR_OK) rc = nr_solve_learned(g, &cfg, m, &r);
     nr_model_destroy(m);
   } else {

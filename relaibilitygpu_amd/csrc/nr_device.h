@@ -116,7 +116,44 @@ struct DevGraph {
   int n, e, w, src, dst;
   int period; /* alternating sweep: flip every `period` FAILED edges */
   int pivot;  /* two-way rule: branch on the best UNKNOWN edge of the whole graph */
+  /* learned path choice: [e] round(-4 log2 p_e) and [e] round(-4 log2 q_e),
+   * each clamped to 0..63 (host-computed, so device and oracle share them) */
+  const unsigned char* lqp;
+  const unsigned char* lqq;
 };
+
+/* What the fewest-UNKNOWN path search of one item records for the learned
+ * choice of its sweep direction (both directions' features derive from it). */
+struct ChoiceRaw {
+  int k;                /* UNKNOWN hops of the path */
+  int nS, cutS, outS;   /* src's WORKING component: nodes, allowed edges leaving
+                           it (all UNKNOWN), all edges leaving it */
+  int nT, cutT, outT;   /* the same for dst's component */
+  int f, u;             /* FAILED and UNKNOWN edges of the item */
+  int lp;               /* sum of lqp over the path's hops */
+  int e_src, e_dst;     /* the hop at the src end and the hop at the dst end */
+};
+
+/* Reproducible pseudo-random draw of an item (splitmix64 finaliser over its
+ * planes): 1 for a quarter of the items (top two bits zero).  A choice log
+ * sweeps in the alternating rule's direction and flips it where this is 1,
+ * so both directions occur in every kind of state while the subtrees below
+ * follow the rule the learned choice is to improve on. */
+__host__ __device__ inline unsigned long long explore_mix(unsigned long long z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+template <int W>
+__host__ __device__ inline int explore_bit(const unsigned long long (&known)[W],
+                                           const unsigned long long (&up)[W]) {
+  unsigned long long h = 0x9E3779B97F4A7C15ull;
+  for (int i = 0; i < W; ++i) {
+    h = explore_mix(h ^ known[i]);
+    h = explore_mix(h ^ up[i]);
+  }
+  return (h >> 62) == 0 ? 1 : 0;
+}
 
 /* The workgroup size as the stride of a lane's LDS slices ([slot][lane]):
  * always a power of two, so i * stride is a shift (a 32-bit multiply is a
@@ -235,6 +272,7 @@ struct Expander {
   /* ---- expander interface used by the level and items kernels ---- */
   static constexpr bool kStage = true; /* graph tables staged in LDS */
   static constexpr int kRowBytes = (int)sizeof(NS);
+  static constexpr bool kChoose = false; /* sweep direction picked by the learned chooser */
   __device__ void bind_graph(unsigned char* smem, const Layout& L, const DevGraph& g, int slot) {
     bind(smem, L, slot, g.src, g.dst);
     terms = g.terms;
@@ -731,6 +769,7 @@ template <int W, int NW>
 struct CsrExpander {
   static constexpr bool kStage = false;
   static constexpr int kRowBytes = 0;
+  static constexpr bool kChoose = false;
   const double* p;
   const double* q;
   const int* rp;
@@ -1102,9 +1141,11 @@ struct CsrExpander {
 template <int W, int NW>
 struct CsrAltExpander : CsrExpander<W, NW> {
   int period;
+  const unsigned char* lqp; /* choice tables (global memory) */
   __device__ void bind_graph(unsigned char* smem, const Layout& L, const DevGraph& g, int slot) {
     CsrExpander<W, NW>::bind_graph(smem, L, g, slot);
     period = g.period > 0 ? g.period : 2;
+    lqp = g.lqp;
   }
   __device__ static bool working(const u64 (&known)[W], const u64 (&up)[W], int e) {
     const int wi = e >> 6;
@@ -1138,6 +1179,11 @@ struct CsrAltExpander : CsrExpander<W, NW> {
     }
   }
   __device__ int find_item(const u64 (&known)[W], const u64 (&up)[W]) const {
+    return find_alt<false>(known, up, nullptr);
+  }
+  /* CH: as AltExpander::find_alt */
+  template <bool CH>
+  __device__ int find_alt(const u64 (&known)[W], const u64 (&up)[W], ChoiceRaw* raw) const {
     const int n = this->n, src = this->src, dst = this->dst;
     if (src == dst) {
       this->plen = 0;
@@ -1225,13 +1271,57 @@ struct CsrAltExpander : CsrExpander<W, NW> {
     int f = 0;
 #pragma unroll
     for (int i = 0; i < W; ++i) f += __popcll(known[i] & ~up[i]);
-    if ((f / period) & 1)
-      for (int i = 0, j = L - 1; i < j; ++i, --j) {
-        const u16 t = this->uh[i * this->stride];
-        this->uh[i * this->stride] = this->uh[j * this->stride];
-        this->uh[j * this->stride] = t;
-      }
+    if (CH) {
+      /* K is src's WORKING component; dst's is closed anew.  Arcs leaving a
+       * component: allowed ones (UNKNOWN) and all of them. */
+      auto cuts = [&](const u64 (&C)[NW], int& nodes, int& cut, int& out) {
+        nodes = cut = out = 0;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+          nodes += __popcll(C[k]);
+          u64 w = C[k];
+          while (w) {
+            const int x = k * 64 + __builtin_ctzll(w);
+            w &= w - 1;
+            const int j1 = this->rp[x + 1];
+            for (int j = this->rp[x]; j < j1; ++j) {
+              const u32 a = this->arc[j];
+              if (bs_test<NW>(C, (int)(a & 0xffffu))) continue;
+              ++out;
+              if (this->allowed(known, up, (int)(a >> 16))) ++cut;
+            }
+          }
+        }
+      };
+      cuts(K, raw->nS, raw->cutS, raw->outS);
+      u64 T[NW];
+#pragma unroll
+      for (int k = 0; k < NW; ++k) T[k] = none[k] = 0;
+      bs_set<NW>(T, dst);
+      bs_set<NW>(none, dst);
+      close(T, none, known, up);
+      cuts(T, raw->nT, raw->cutT, raw->outT);
+      int kn = 0, lp = 0;
+#pragma unroll
+      for (int i = 0; i < W; ++i) kn += __popcll(known[i]);
+      for (int i = 0; i < L; ++i) lp += lqp[this->uh[i * this->stride]];
+      raw->k = L;
+      raw->f = f;
+      raw->u = this->ne - kn;
+      raw->lp = lp;
+      raw->e_dst = L > 0 ? this->uh[0] : 0;
+      raw->e_src = L > 0 ? this->uh[(L - 1) * this->stride] : 0;
+      return L;
+    }
+    if ((f / period) & 1) reverse_hops(L);
     return L;
+  }
+  __device__ void reverse_hops(int L) const {
+    for (int i = 0, j = L - 1; i < j; ++i, --j) {
+      const u16 t = this->uh[i * this->stride];
+      this->uh[i * this->stride] = this->uh[j * this->stride];
+      this->uh[j * this->stride] = t;
+    }
   }
 };
 
@@ -1293,9 +1383,11 @@ struct LeanExpander : Expander<W, NS> {
 template <int W, typename NS>
 struct AltExpander : LeanExpander<W, NS> {
   int period;
+  const unsigned char* lqp; /* choice tables (global memory) */
   __device__ void bind_graph(unsigned char* smem, const Layout& L, const DevGraph& g, int slot) {
     LeanExpander<W, NS>::bind_graph(smem, L, g, slot);
     period = g.period > 0 ? g.period : 2;
+    lqp = g.lqp;
   }
   __device__ int find_item(const u64 (&known)[W], const u64 (&up)[W]) const {
     return find_alt(known, up);
@@ -1312,7 +1404,11 @@ struct AltExpander : LeanExpander<W, NS> {
     return out_degree(u, k, p);
   }
 
-  __device__ int find_alt(const u64 (&known)[W], const u64 (&up)[W]) const {
+  /* CH: the learned chooser picks the sweep direction; the search records
+   * its ingredients in *raw and leaves the hops in the forward sweep. */
+  template <bool CH = false>
+  __device__ int find_alt(const u64 (&known)[W], const u64 (&up)[W],
+                          ChoiceRaw* raw = nullptr) const {
     const int n = this->n;
     const Pow2 stride = this->stride;
     if (this->src == this->dst) {
@@ -1403,13 +1499,47 @@ struct AltExpander : LeanExpander<W, NS> {
       this->uh[(L - l) * stride] = this->eidm[ub * n + v];
       K = close((NS)1 << ub);
     }
-    if ((f / period) & 1)
-      for (int i = 0, j = L - 1; i < j; ++i, --j) {
-        const u16 t = this->uh[i * stride];
-        this->uh[i * stride] = this->uh[j * stride];
-        this->uh[j * stride] = t;
-      }
+    if (CH) {
+      /* K is src's WORKING component again; dst's is closed anew.  Edges
+       * leaving a component: allowed ones (UNKNOWN, the component being
+       * closed under WORKING edges) and all of them (simple graph). */
+      const NS T = close(dbit);
+      auto cuts = [&](NS C, int& nodes, int& cut, int& out) {
+        nodes = sizeof(NS) == 4 ? __popc((u32)C) : __popcll((u64)C);
+        cut = out = 0;
+        NS c = C;
+        while (c) {
+          const int x = ctzs<NS>(c);
+          c &= c - 1;
+          const NS a = this->adj_of(x, Dn) & ~C, st = (NS)this->sadj[x] & ~C;
+          cut += sizeof(NS) == 4 ? __popc((u32)a) : __popcll((u64)a);
+          out += sizeof(NS) == 4 ? __popc((u32)st) : __popcll((u64)st);
+        }
+      };
+      cuts(K, raw->nS, raw->cutS, raw->outS);
+      cuts(T, raw->nT, raw->cutT, raw->outT);
+      int kn = 0, lp = 0;
+#pragma unroll
+      for (int i = 0; i < W; ++i) kn += __popcll(known[i]);
+      for (int i = 0; i < L; ++i) lp += lqp[this->uh[i * stride]];
+      raw->k = L;
+      raw->f = f;
+      raw->u = this->ne - kn;
+      raw->lp = lp;
+      raw->e_dst = L > 0 ? this->uh[0] : 0;
+      raw->e_src = L > 0 ? this->uh[(L - 1) * stride] : 0;
+      return L;
+    }
+    if ((f / period) & 1) reverse_hops(L);
     return L;
+  }
+  __device__ void reverse_hops(int L) const {
+    const Pow2 stride = this->stride;
+    for (int i = 0, j = L - 1; i < j; ++i, --j) {
+      const u16 t = this->uh[i * stride];
+      this->uh[i * stride] = this->uh[j * stride];
+      this->uh[j * stride] = t;
+    }
   }
 };
 

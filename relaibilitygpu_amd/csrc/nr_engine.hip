@@ -733,7 +733,7 @@ hipError_t& last_hip_error() {
 struct TableBlob {
   std::vector<unsigned char> bytes;
   size_t off_p, off_q, off_sadj, off_inc, off_ep, off_deg, off_eidm;
-  size_t off_rp, off_arc, off_irp, off_iarc, off_terms, off_lk;
+  size_t off_rp, off_arc, off_irp, off_iarc, off_terms, off_lk, off_lqp, off_lqq;
 };
 
 struct Workspace {
@@ -888,6 +888,8 @@ void pack_tables(const nr_graph* g, int W, TableBlob& b) {
   b.off_irp = take(4 * (size_t)(n + 1));
   b.off_iarc = take(4 * (size_t)std::max(g->nnz, 1));
   b.off_lk = take(4 * (size_t)e);
+  b.off_lqp = take((size_t)e);
+  b.off_lqq = take((size_t)e);
   b.bytes.assign((o + 15) / 16 * 16, 0);
   double* p = (double*)&b.bytes[b.off_p];
   double* q = (double*)&b.bytes[b.off_q];
@@ -905,6 +907,14 @@ void pack_tables(const nr_graph* g, int W, TableBlob& b) {
     /* forkBest's key, computed on the host as graph.h:138 does */
     const float pf = (float)g->p[id];
     lk[id] = pf > 0.0f ? logf(pf) : -1e30f;
+    /* the learned chooser's probability steps: round(-4 log2 x) in 0..63 */
+    auto lq = [](double x) -> unsigned char {
+      if (!(x > 0.0)) return 63;
+      const double v = std::nearbyint(-4.0 * std::log2(x));
+      return (unsigned char)(v < 0.0 ? 0.0 : (v > 63.0 ? 63.0 : v));
+    };
+    b.bytes[b.off_lqp + id] = lq(g->p[id]);
+    b.bytes[b.off_lqq + id] = lq(g->q[id]);
     ep[id] = (u32)a | ((u32)c << 16);
     if (!ns) continue;
     inc[(size_t)a * W + (id >> 6)] |= 1ull << (id & 63);
@@ -981,6 +991,8 @@ int upload_tables(Workspace& ws, const nr_graph* g, int W, DevGraph& d) {
   d.iarc = (const u32*)(base + b.off_iarc);
   d.terms = (const u16*)(base + b.off_terms);
   d.lk = (const float*)(base + b.off_lk);
+  d.lqp = base + b.off_lqp;
+  d.lqq = base + b.off_lqq;
   d.nterms = (int)g->terms.size();
   for (int k = 0; k < 4; ++k) d.tmask[k] = 0;
   for (int t : g->terms) d.tmask[t >> 6] |= 1ull << (t & 63);
@@ -1144,6 +1156,8 @@ struct LevelExtra {
   u64* log_count = nullptr;
   u64 log_cap = 0;
   const MlpDev* mlp = nullptr;
+  const MlpDev* chooser = nullptr; /* NR_RULE_LEARNED: the path chooser */
+  float* log_choice = nullptr;     /* tree log of the choices (mode 1) */
   const nr_checkpoint* resume = nullptr; /* start from these pending items */
   nr_checkpoint** save = nullptr;        /* on deadline keep them here */
 };
@@ -1365,13 +1379,21 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
     var = variant_lean(mask_words(g), var.ns_bytes);
     add_mass_kernels(var, mask_words(g));
   }
-  if (cfg->rule == NR_RULE_ALTERNATE) {
+  if (cfg->rule == NR_RULE_ALTERNATE || cfg->rule == NR_RULE_LEARNED) {
     if (!g->symmetric || !g->terms.empty()) return NR_ERR_UNSUPPORTED;
-    var = g->n <= 64 ? variant_alt(mask_words(g), var.ns_bytes)
-                     : variant_alt_csr(mask_words(g), (g->n + 63) / 64);
+    if (cfg->rule == NR_RULE_LEARNED) {
+      /* the chooser scores directions, except in a tree log, which explores */
+      if (mode != 1 && !(x && x->chooser)) return NR_ERR_ARG;
+      var = g->n <= 64 ? variant_choose(mask_words(g), var.ns_bytes)
+                       : variant_choose_csr(mask_words(g), (g->n + 63) / 64);
+    } else {
+      var = g->n <= 64 ? variant_alt(mask_words(g), var.ns_bytes)
+                       : variant_alt_csr(mask_words(g), (g->n + 63) / 64);
+    }
     if (!var.level) return NR_ERR_UNSUPPORTED;
   }
-  const int tile_bytes = mode == 2 ? 32 * 256 + 16 : 0; /* per-wave feature tiles */
+  /* per-wave feature tiles (learned order, learned choice) */
+  const int tile_bytes = (mode == 2 || var.choose) ? 32 * 256 + 16 : 0;
   const int W = mask_words(g);
   const int cus = ws.prop.multiProcessorCount;
   /* The prefix table of the segment counts (one per workgroup) lives in
@@ -1459,6 +1481,10 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
     a.log_cap = x->log_cap;
   }
   if (mode == 2) a.mlp = x->mlp;
+  if (x) {
+    a.chooser = x->chooser;
+    a.log_choice = x->log_choice;
+  }
   /* per-workgroup slots: 3 DD + C_N counters; segment counts */
   const size_t meta_bytes = n_slots * (3 * sizeof(DD) + C_N * sizeof(u64)) +
                             2 * sizeof(u32) * (2 * (size_t)grid + 4) +
@@ -2276,9 +2302,12 @@ __device__ inline u64 mix64(u64 z) {
 
 __global__ void __launch_bounds__(256) k_log_select(const float* feat, const u64* size, u64 n,
                                                     u64 seed, u64 thr, u64* keys, u64* ids,
-                                                    u64* found, u64 cap) {
+                                                    u64* found, u64 cap, int need_const) {
   for (u64 r = (u64)blockIdx.x * blockDim.x + threadIdx.x; r < n;
        r += (u64)gridDim.x * blockDim.x) {
+    /* choice logs: only records that were expanded carry features (their
+     * constant feature is 1; the rest of the log is zero) */
+    if (need_const && feat[r * kFeat + kFeat - 1] != 1.0f) continue;
     u64 h = mix64(seed ^ mix64(size[r]));
     const u32* f = (const u32*)(feat + r * kFeat);
     for (int k = 0; k < kFeat; ++k) h = mix64(h ^ (u64)f[k] ^ ((u64)k << 32));
@@ -2292,12 +2321,18 @@ __global__ void __launch_bounds__(256) k_log_select(const float* feat, const u64
   }
 }
 
+/* Tree log of a level-order solve.  choices = 0: the children's features
+ * at creation (the learned order's samples); choices = 1: the learned
+ * choice's samples, from a solve of the alternating rule's path whose sweep
+ * direction explores (a reproducible pseudo-random bit per item), each
+ * expanded item's features of the direction taken.  Labels: log2 subtree
+ * nodes of the record. */
 int log_subproblems(Workspace& ws, const nr_graph* g, const nr_config* cfg, long long max_out,
                     unsigned long long seed, float* X, float* Y, long long* n_out,
-                    nr_result* res) {
+                    nr_result* res, int choices = 0) {
   size_t free_b = 0, total_b = 0;
   NR_HIP(hipMemGetInfo(&free_b, &total_b));
-  const size_t rec_bytes = kFeat * 4 + 4 + 4 + 8;
+  const size_t rec_bytes = kFeat * 4 * (choices ? 2 : 1) + 4 + 4 + 8;
   const u64 log_cap = std::min<u64>((u64)(free_b / 4 / rec_bytes), 1ull << 31);
   DevMem mem;
   LevelExtra x;
@@ -2310,10 +2345,15 @@ int log_subproblems(Workspace& ws, const nr_graph* g, const nr_config* cfg, long
       (rc = mem.alloc(&x.log_count, 1)) || (rc = mem.alloc(&dsize, log_cap)))
     return rc;
   hipStream_t st = ws.stream;
+  if (choices) {
+    if ((rc = mem.alloc(&x.log_choice, (size_t)log_cap * kChoiceFeat))) return rc;
+    NR_HIP(hipMemsetAsync(x.log_choice, 0, sizeof(float) * kChoiceFeat * log_cap, st));
+  }
   NR_HIP(hipMemsetAsync(x.log_count, 0, sizeof(u64), st));
   nr_config c2 = *cfg;
   c2.order = NR_ORDER_LEVEL;
   c2.time_budget_s = 0.0; /* a log needs whole subtrees */
+  if (choices) c2.rule = NR_RULE_LEARNED;
   if ((rc = solve_level(ws, g, &c2, res, &x))) return rc;
   u64 count = 0;
   NR_HIP(hipMemcpy(&count, x.log_count, sizeof(u64), hipMemcpyDeviceToHost));
@@ -2360,8 +2400,9 @@ int log_subproblems(Workspace& ws, const nr_graph* g, const nr_config* cfg, long
     for (;;) {
       const u64 thr = frac >= 1.0 ? ~0ull : (u64)(frac * 18446744073709551615.0);
       NR_HIP(hipMemsetAsync(dfound, 0, sizeof(u64), st));
-      hipLaunchKernelGGL(k_log_select, dim3(1024), dim3(256), 0, st, (const float*)x.log_feat,
-                         (const u64*)dsize, count, (u64)seed, thr, dkeys, dids, dfound, count);
+      hipLaunchKernelGGL(k_log_select, dim3(1024), dim3(256), 0, st,
+                         (const float*)(choices ? x.log_choice : x.log_feat), (const u64*)dsize,
+                         count, (u64)seed, thr, dkeys, dids, dfound, count, choices);
       NR_HIP(hipMemcpyAsync(&found, dfound, sizeof(u64), hipMemcpyDeviceToHost, st));
       NR_HIP(hipStreamSynchronize(st));
       if (found >= want || frac >= 1.0) break;
@@ -2389,8 +2430,9 @@ int log_subproblems(Workspace& ws, const nr_graph* g, const nr_config* cfg, long
         (rc = mem.alloc(&dY, m)))
       return rc;
     NR_HIP(hipMemcpyAsync(didx, idx.data(), 8 * m, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_log_gather, dim3(1024), dim3(256), 0, st, (const float*)x.log_feat,
-                       (const u64*)dsize, (const u64*)didx, m, dX, dY);
+    hipLaunchKernelGGL(k_log_gather, dim3(1024), dim3(256), 0, st,
+                       (const float*)(choices ? x.log_choice : x.log_feat), (const u64*)dsize,
+                       (const u64*)didx, m, dX, dY);
     NR_HIP(hipMemcpyAsync(X, dX, sizeof(float) * m * kFeat, hipMemcpyDeviceToHost, st));
     NR_HIP(hipMemcpyAsync(Y, dY, sizeof(float) * m, hipMemcpyDeviceToHost, st));
     NR_HIP(hipStreamSynchronize(st));
@@ -2399,18 +2441,33 @@ int log_subproblems(Workspace& ws, const nr_graph* g, const nr_config* cfg, long
   return NR_OK;
 }
 
-const MlpDev* model_dev(const nr_model* m); /* nr_learn.hip */
+const MlpDev* model_dev(const nr_model* m);     /* nr_learn.hip: the order scorer or null */
+const MlpDev* model_chooser(const nr_model* m); /* the path chooser or null */
 
+/* A solve that uses a model: the learned order (cfg->order ==
+ * NR_ORDER_LEARNED, the model's order scorer) and/or the learned path
+ * choice (cfg->rule == NR_RULE_LEARNED, its chooser). */
 int solve_learned(Workspace& ws, const nr_graph* g, const nr_config* cfg, const nr_model* model,
                   nr_result* out) {
+  const MlpDev* order = model_dev(model);
+  const MlpDev* chooser = model_chooser(model);
+  const bool learned_order = cfg->order == NR_ORDER_LEARNED;
+  const bool learned_rule = cfg->rule == NR_RULE_LEARNED;
+  if ((learned_order && !order) || (learned_rule && !chooser)) return NR_ERR_ARG;
   DevMem mem;
   MlpDev* dm;
   int rc;
-  if ((rc = mem.alloc(&dm, 1))) return rc;
-  NR_HIP(hipMemcpyAsync(dm, model_dev(model), sizeof(MlpDev), hipMemcpyHostToDevice, ws.stream));
+  if ((rc = mem.alloc(&dm, 2))) return rc;
   LevelExtra x;
-  x.mode = 2;
-  x.mlp = dm;
+  if (learned_order) {
+    NR_HIP(hipMemcpyAsync(dm, order, sizeof(MlpDev), hipMemcpyHostToDevice, ws.stream));
+    x.mode = 2;
+    x.mlp = dm;
+  }
+  if (learned_rule) {
+    NR_HIP(hipMemcpyAsync(dm + 1, chooser, sizeof(MlpDev), hipMemcpyHostToDevice, ws.stream));
+    x.chooser = dm + 1;
+  }
   return solve_level(ws, g, cfg, out, &x);
 }
 
@@ -2508,8 +2565,12 @@ int nr_solve_learned(const nr_graph* g, const nr_config* cfg, const nr_model* mo
     }
     if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0 || cfg->time_budget_s < 0)
       return NR_ERR_ARG;
-    if (cfg->rule != NR_RULE_PATH && cfg->rule != NR_RULE_ALTERNATE)
-      return NR_ERR_UNSUPPORTED; /* scores path children */
+    if (cfg->order != NR_ORDER_LEVEL && cfg->order != NR_ORDER_MASS &&
+        cfg->order != NR_ORDER_LEARNED)
+      return NR_ERR_ARG;
+    if (cfg->rule != NR_RULE_PATH && cfg->rule != NR_RULE_ALTERNATE &&
+        cfg->rule != NR_RULE_LEARNED)
+      return NR_ERR_UNSUPPORTED; /* scores path children / path sweeps */
     if (cfg->sweep_period < 0) return NR_ERR_ARG;
     nr::Variant var;
     int rc = nr::pick_variant(g, var);
@@ -2662,6 +2723,26 @@ int nr_log_subproblems(const nr_graph* g, const nr_config* cfg, long long max_re
     nr::WsLock L(cfg->device);
     if (L.rc) return L.rc;
     return nr::log_subproblems(*L.ws, g, cfg, max_records, seed, features, labels, n_out, result);
+  });
+}
+
+int nr_log_choices(const nr_graph* g, const nr_config* cfg, long long max_records,
+                   unsigned long long seed, float* features, float* labels, long long* n_out,
+                   nr_result* result) {
+  return nr::guarded([&]() -> int {
+    if (!g || !cfg || max_records < 0 || !n_out || !result ||
+        (max_records > 0 && (!features || !labels)))
+      return NR_ERR_ARG;
+    if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0 || cfg->sweep_period < 0)
+      return NR_ERR_ARG;
+    if (!g->symmetric || !g->terms.empty()) return NR_ERR_UNSUPPORTED;
+    nr::Variant var;
+    int rc = nr::pick_variant(g, var);
+    if (rc) return rc;
+    nr::WsLock L(cfg->device);
+    if (L.rc) return L.rc;
+    return nr::log_subproblems(*L.ws, g, cfg, max_records, seed, features, labels, n_out, result,
+                               1);
   });
 }
 

@@ -48,7 +48,9 @@ struct MlpDev {
   u32 sign[64];        /* bit 16*mt + r of lane l: s_h < 0 for h = 32*mt + row(r, l) */
   float b2;
   float alpha; /* score = log2(mass) - alpha * y */
-  float pad[2];
+  int margin;  /* path chooser: advantage (2^-8 units) needed to leave the
+                  alternating direction */
+  float pad;
 };
 
 /* Round-to-nearest-even fp32 -> bf16 bits (finite inputs). */
@@ -152,6 +154,98 @@ __device__ inline float mlp_score_tile(const bf16x8_t (&a)[2], u32 sign, float b
   for (int nt = 0; nt < 2; ++nt) part[nt] += __shfl_xor(part[nt], 32, 64);
   return (lane < 32 ? part[0] : part[1]) + b2;
 }
+
+/* ------------------------------------------------------------------ */
+/* Learned path choice: the sweep direction of the fewest-UNKNOWN path  */
+/* ------------------------------------------------------------------ */
+/*
+ * The chooser is a second network of the same shape whose input is an
+ * item's path and the two terminals' components, seen from the end a sweep
+ * would start at, and whose output predicts log2 of the item's subtree size
+ * when swept from that end.  Both directions are scored; the sweep leaves
+ * the alternating rule's direction only when the other one's prediction is
+ * smaller by more than the model's margin (a conservative step of policy
+ * improvement over that rule: a large margin is the rule itself).  Either
+ * direction is an exact decomposition, so exactness is untouched.
+ *
+ * Exact arithmetic: every feature is an integer with |x| <= 63 and every
+ * weight of the inference form is a/256 with |a| <= 255 (one scale for the
+ * whole layer, |w2| folded in).  Then every BF16 operand is exact, every
+ * product is a multiple of 2^-8, the hidden sums stay below 16*63 and the
+ * output below 64*16*63 < 2^16: all partial sums are exact in FP32 whatever
+ * order the matrix cores add them in, and the choice is the same integer
+ * comparison a CPU makes.
+ */
+constexpr int kChoiceFeat = 16;
+
+__host__ __device__ inline int floor_log2_of(double m) {
+  unsigned long long bits;
+  memcpy(&bits, &m, 8);
+  const int e = (int)((bits >> 52) & 0x7ff);
+  return e == 0 ? -1075 : e - 1023; /* zero and subnormals: below every use */
+}
+__host__ __device__ inline int floor_half(int v) { return v >= 0 ? v / 2 : -((1 - v) / 2); }
+__host__ __device__ inline int clamp63(int v) { return v < -63 ? -63 : (v > 63 ? 63 : v); }
+
+/* The headroom feature measures an item's mass against the truncation eps,
+ * or against 2^-64 when eps is smaller or zero, in half-binades, saturating
+ * at 32 (64 binades): only mass relative to the truncation enters the
+ * choice, so a solve truncated deeper than the training logs, or not at all,
+ * presents the same kind of states to the chooser. */
+constexpr int kChoiceDepth = 32;
+
+__host__ __device__ inline int choice_headroom(int lm, double eps) {
+  const int le = eps > 0x1p-64 ? floor_log2_of(eps) : -64;
+  const int h = floor_half(lm - le);
+  return h < 0 ? 0 : (h > kChoiceDepth ? kChoiceDepth : h);
+}
+
+/* The 16 features of sweeping an item's path from end A (src, or dst when
+ * from_dst) towards end B: lm = floor(log2 mass), ne = edges. */
+__host__ __device__ inline void choice_features(const ChoiceRaw& r, bool from_dst, int lm,
+                                                int head, int ne, const unsigned char* lqq,
+                                                int* x) {
+  const int nA = from_dst ? r.nT : r.nS, cA = from_dst ? r.cutT : r.cutS;
+  const int oA = from_dst ? r.outT : r.outS;
+  const int nB = from_dst ? r.nS : r.nT, cB = from_dst ? r.cutS : r.cutT;
+  const int oB = from_dst ? r.outS : r.outT;
+  const int e = ne > 0 ? ne : 1;
+  x[0] = clamp63(r.k);          /* UNKNOWN hops = children */
+  x[1] = clamp63(nA);           /* start end's component: nodes, */
+  x[2] = clamp63(cA);           /* UNKNOWN edges leaving it, */
+  x[3] = clamp63(oA - cA);      /* FAILED edges leaving it */
+  x[4] = clamp63(nB);           /* the same for the far end */
+  x[5] = clamp63(cB);
+  x[6] = clamp63(oB - cB);
+  x[7] = (32 * r.f) / e;        /* FAILED and UNKNOWN fractions, 1/32 steps */
+  x[8] = (32 * r.u) / e;
+  x[9] = 0;                     /* (unused) */
+  x[10] = head;                 /* headroom over eps */
+  x[11] = -clamp63(r.lp);       /* path success probability, -4 log2 steps */
+  x[12] = -(int)lqq[from_dst ? r.e_dst : r.e_src]; /* q of the first failed hop */
+  x[13] = -(int)lqq[from_dst ? r.e_src : r.e_dst]; /* q of the last */
+  x[14] = clamp63(cA - cB);     /* cut asymmetry */
+  x[15] = 1;                    /* carries the hidden bias */
+}
+
+/* Alternating-rule expanders whose sweep direction the chooser picks (the
+ * search records a ChoiceRaw and leaves the hops in the forward sweep). */
+template <int W, typename NS>
+struct ChooseAltExpander : AltExpander<W, NS> {
+  static constexpr bool kChoose = true;
+  mutable ChoiceRaw raw;
+  __device__ int find_item(const u64 (&known)[W], const u64 (&up)[W]) const {
+    return this->template find_alt<true>(known, up, &raw);
+  }
+};
+template <int W, int NW>
+struct CsrChooseAltExpander : CsrAltExpander<W, NW> {
+  static constexpr bool kChoose = true;
+  mutable ChoiceRaw raw;
+  __device__ int find_item(const u64 (&known)[W], const u64 (&up)[W]) const {
+    return this->template find_alt<true>(known, up, &raw);
+  }
+};
 
 /* Class of a scored child: two classes per binade of mass / predicted
  * work, class 0 first. */

@@ -22,7 +22,16 @@
 struct nr_model {
   float params[nr::kMlpParams];
   float alpha;
-  nr::MlpDev dev; /* packed inference form */
+  bool has_order = false;
+  nr::MlpDev dev; /* packed inference form of the order scorer */
+  /* learned path choice: integer weights a[k][h] (the layer-1 weights times
+   * |w2|, one scale for the layer, rounded to |a| <= 255), signs of w2, and
+   * the device form with weights a / 256 */
+  bool has_chooser = false;
+  int cw[nr::kFeat * nr::kHidden];
+  int cs[nr::kHidden];
+  int cmargin = 0;
+  nr::MlpDev cdev;
 };
 
 namespace nr {
@@ -180,6 +189,45 @@ __global__ void __launch_bounds__(64) k_mlp_score(const float* X, long long n,
   __syncthreads();
   const float y = mlp_score_tile(a, m->sign[lane], m->b2, tile);
   if (row < n) Y[row] = y;
+}
+
+/* Exact inference form of a chooser (see nr_learn.h): W1'[k][h] =
+ * W1[k][h] |w2[h]| scaled so that the largest magnitude is 255/256 and
+ * rounded to the 1/256 grid; the output bias plays no part in a choice. */
+void pack_chooser(const float* P, float margin, int* a, int* sg, int* mi, MlpDev& d) {
+  std::memset(&d, 0, sizeof d);
+  const float* W1 = P;
+  const float* w2 = P + kFeat * kHidden;
+  double mx = 0.0;
+  for (int k = 0; k < kFeat; ++k)
+    for (int h = 0; h < kHidden; ++h)
+      mx = std::max(mx, std::fabs((double)W1[k * kHidden + h] * std::fabs((double)w2[h])));
+  const double c = mx > 0.0 ? 255.0 / mx : 0.0;
+  for (int k = 0; k < kFeat; ++k)
+    for (int h = 0; h < kHidden; ++h) {
+      const double v = (double)W1[k * kHidden + h] * std::fabs((double)w2[h]) * c;
+      long r = std::lrint(v);
+      a[k * kHidden + h] = (int)std::max(-255L, std::min(255L, r));
+    }
+  for (int h = 0; h < kHidden; ++h) sg[h] = w2[h] < 0.0f ? 1 : 0;
+  /* a prediction difference dy (log2 subtree nodes) is about c * dy in the
+   * integer units of the scores (2^-8 of the device form's output) */
+  const double m = (double)margin * c;
+  *mi = m >= 2e9 ? 2000000000 : (int)std::lrint(std::max(0.0, m));
+  for (int mt = 0; mt < 2; ++mt)
+    for (int l = 0; l < 64; ++l)
+      for (int j = 0; j < 8; ++j) {
+        const int k = 8 * (l / 32) + j, h = 32 * mt + (l % 32);
+        d.afrag[mt][l][j] = bf16_bits((float)a[k * kHidden + h] / 256.0f); /* exact */
+      }
+  for (int l = 0; l < 64; ++l) {
+    u32 m = 0;
+    for (int mt = 0; mt < 2; ++mt)
+      for (int r = 0; r < 16; ++r)
+        if (sg[32 * mt + mfma32_row(r, l)]) m |= 1u << (16 * mt + r);
+    d.sign[l] = m;
+  }
+  d.margin = *mi;
 }
 
 void pack_model(const float* P, float alpha, MlpDev& d) {
@@ -391,13 +439,37 @@ int nr_mlp_train(const float* X, const float* Y, long long n, const nr_train_con
 
 int nr_model_create(const float* params, float alpha, nr_model** out) {
   return nr::guarded([&]() -> int {
-    if (!params || !out) return NR_ERR_ARG;
+    if (!out) return NR_ERR_ARG;
     nr_model* m = new (std::nothrow) nr_model;
     if (!m) return NR_ERR_NOMEM;
-    std::memcpy(m->params, params, sizeof(float) * kMlpParams);
     m->alpha = alpha;
-    pack_model(params, alpha, m->dev);
+    if (params) {
+      std::memcpy(m->params, params, sizeof(float) * kMlpParams);
+      pack_model(params, alpha, m->dev);
+      m->has_order = true;
+    }
     *out = m;
+    return NR_OK;
+  });
+}
+
+int nr_model_set_chooser(nr_model* m, const float* params, float margin) {
+  return nr::guarded([&]() -> int {
+    if (!m || !params || !(margin >= 0.0f)) return NR_ERR_ARG;
+    for (int i = 0; i < kMlpParams; ++i)
+      if (!std::isfinite(params[i])) return NR_ERR_ARG;
+    pack_chooser(params, margin, m->cw, m->cs, &m->cmargin, m->cdev);
+    m->has_chooser = true;
+    return NR_OK;
+  });
+}
+
+int nr_model_get_chooser(const nr_model* m, int* weights, int* signs, int* margin) {
+  return nr::guarded([&]() -> int {
+    if (!m || !m->has_chooser) return NR_ERR_ARG;
+    if (weights) std::memcpy(weights, m->cw, sizeof m->cw);
+    if (signs) std::memcpy(signs, m->cs, sizeof m->cs);
+    if (margin) *margin = m->cmargin;
     return NR_OK;
   });
 }
@@ -406,7 +478,7 @@ void nr_model_destroy(nr_model* m) { delete m; }
 
 int nr_model_score(const nr_model* m, const float* X, long long n, float* Y, int device) {
   return nr::guarded([&]() -> int {
-    if (!m || !X || !Y || n < 0) return NR_ERR_ARG;
+    if (!m || !m->has_order || !X || !Y || n < 0) return NR_ERR_ARG;
     if (n == 0) return NR_OK;
     NR_CHK(hipSetDevice(device));
     TrainBufs tb;
@@ -429,5 +501,8 @@ int nr_model_score(const nr_model* m, const float* X, long long n, float* Y, int
 }  // extern "C"
 
 namespace nr {
-const MlpDev* model_dev(const nr_model* m) { return m ? &m->dev : nullptr; }
+const MlpDev* model_dev(const nr_model* m) { return m && m->has_order ? &m->dev : nullptr; }
+const MlpDev* model_chooser(const nr_model* m) {
+  return m && m->has_chooser ? &m->cdev : nullptr;
+}
 }  // namespace nr

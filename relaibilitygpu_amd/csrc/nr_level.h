@@ -88,6 +88,11 @@ struct LevelArgs {
   u64 log_cap;
   /* learned order */
   const MlpDev* mlp;
+  /* learned path choice: the chooser's exact inference form; in a tree log
+   * (MODE 1) the direction explores instead and each expanded record's
+   * features of the direction taken go to log_choice[record][16] */
+  const MlpDev* chooser;
+  float* log_choice;
   /* time budget in wall-clock ticks from ctl->start_tick (0: none): items a
    * wave reaches after it are carried to the output unexpanded */
   u64 budget_ticks;
@@ -490,6 +495,67 @@ __device__ inline void child_ctx(const LevelArgs& a, const EX& ex, const u64 (&k
   c.eps = a.eps;
 }
 
+/* The learned choice of the sweep direction for the wave's items (every
+ * lane of the wave calls it; `pick` lanes hold a live item with at least two
+ * UNKNOWN hops, whose hops the search left in the forward sweep).  MODE 1
+ * (tree log) explores: the alternating rule's direction, flipped for a
+ * reproducible pseudo-random quarter of the items, and the record of the
+ * item gets the features of the direction taken.  Other
+ * modes score both directions on the matrix cores (two 64-row tiles) and
+ * sweep from the end with the smaller predicted subtree. */
+template <int W, class EX, int MODE>
+__device__ inline void choose_sweep(const LevelArgs& a, const EX& ex, bool live, int k,
+                                    double mass, const u64 (&known)[W], const u64 (&up)[W],
+                                    u32 tag_in, u16* tile) {
+  const bool pick = live && k >= 2;
+  const int lm = floor_log2_of(mass);
+  const int head = choice_headroom(lm, a.eps);
+  if (MODE == 1) {
+    if (!pick) return;
+    const int rev = ((ex.raw.f / ex.period) & 1) ^ explore_bit<W>(known, up);
+    if (tag_in < a.log_cap) {
+      int x[kChoiceFeat];
+      choice_features(ex.raw, rev != 0, lm, head, a.g.e, a.g.lqq, x);
+      float* dst = a.log_choice + (u64)tag_in * kChoiceFeat;
+#pragma unroll
+      for (int i = 0; i < kChoiceFeat; ++i) dst[i] = (float)x[i];
+    }
+    if (rev) ex.reverse_hops(k);
+    return;
+  }
+  if (__ballot(pick) == 0) return; /* wave-uniform */
+  const int lane = threadIdx.x & 63;
+  bf16x8_t af[2];
+#pragma unroll
+  for (int mt = 0; mt < 2; ++mt) af[mt] = *(const bf16x8_t*)a.chooser->afrag[mt][lane];
+  const u32 sign = a.chooser->sign[lane];
+  float y[2];
+#pragma unroll
+  for (int d = 0; d < 2; ++d) {
+    int x[kChoiceFeat];
+    if (pick) {
+      choice_features(ex.raw, d == 1, lm, head, a.g.e, a.g.lqq, x);
+    } else {
+#pragma unroll
+      for (int i = 0; i < kChoiceFeat; ++i) x[i] = 0;
+    }
+    u32* trow = (u32*)(tile + lane * kFeat);
+#pragma unroll
+    for (int i = 0; i < kChoiceFeat; i += 2) /* small integers: exact in BF16 */
+      trow[i / 2] = (u32)bf16_bits((float)x[i]) | ((u32)bf16_bits((float)x[i + 1]) << 16);
+    wave_lds_sync(); /* every lane's row before the tile is read */
+    y[d] = mlp_score_tile(af, sign, 0.0f, tile);
+    wave_lds_sync(); /* the tile read before it is rewritten */
+  }
+  if (pick) {
+    /* the scores are exact multiples of 2^-8 below 2^16: exact as integers */
+    const int alt = (ex.raw.f / ex.period) & 1;
+    const int s_alt = (int)(y[alt] * 256.0f), s_other = (int)(y[alt ^ 1] * 256.0f);
+    const int rev = s_alt - s_other > a.chooser->margin ? alt ^ 1 : alt;
+    if (rev) ex.reverse_hops(k);
+  }
+}
+
 /* One tree level: consume buf[parity], append children to the workgroup's
  * segment of buf[parity^1].  MODE 0: level order; MODE 4: max-mass class
  * rounds; MODE 1: level order that also writes the tree log; MODE 2: class
@@ -577,7 +643,7 @@ __device__ __forceinline__ void expand_level_items(const LevelArgs& a, const EX&
           if (k >= 0) {
             live = true;
             stored = k;
-            if (mass_on) { /* children below eps or over capacity are not queued */
+            if (mass_on && !EX::kChoose) { /* children below eps are not queued */
               stored = 0;
               double run = mass;
               for (int j = k - 1; j >= 0; --j) {
@@ -593,6 +659,18 @@ __device__ __forceinline__ void expand_level_items(const LevelArgs& a, const EX&
             ++c[C_DEAD];
             k = 0;
           }
+        }
+      }
+    }
+    if constexpr (EX::kChoose) {
+      choose_sweep<W, EX, MODE>(a, ex, live, k, mass, known, up, tag_in, tile);
+      if (mass_on && live) { /* children below eps are not queued */
+        stored = 0;
+        double run = mass;
+        for (int j = k - 1; j >= 0; --j) {
+          const int e = ex.uh[j * ex.stride];
+          if (run * ex.q[e] >= a.eps) ++stored;
+          run = run * ex.p[e];
         }
       }
     }
@@ -1102,7 +1180,14 @@ struct Variant {
   int lean;                            /* items carry no rows (LeanExpander) */
   KernelGetter level_mass;             /* MODE 4 (max-mass class rounds) */
   KernelGetter small;                  /* k_small_levels of this form */
+  int choose;                          /* the learned chooser picks the sweep */
 };
+
+/* The alternating rule with the learned sweep choice (ChooseAltExpander, or
+ * CsrChooseAltExpander when nw > 0): MODE 0, 1, 2 and 4 (nr_choose.hip,
+ * nr_choose_csr.hip). */
+Variant variant_choose(int W, int ns_bytes);
+Variant variant_choose_csr(int W, int NW);
 
 /* MODE 4 kernel of a variant (nr_mass.hip), filled into v. */
 void add_mass_kernels(Variant& v, int W);

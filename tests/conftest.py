@@ -34,3 +34,17 @@ def closed_form(expected, n, p):
         if x["n"] == n and x["p"] == p:
             return x["R"]
     raise KeyError((n, p))
+
+
+@pytest.fixture(scope="session")
+def trained_model():
+    """The learned parts as the benchmark trains them, on the GPU through the
+    C-ABI (relaibilitygpu_amd/learn.py): (Model with order scorer and path
+    chooser, chooser summary)."""
+    import relaibilitygpu_amd as nr
+    from relaibilitygpu_amd import learn
+
+    print("case training the order scorer and the path chooser", flush=True)
+    order_params, _ = learn.train_order_scorer(alpha=0.5)
+    ch_params, margin, info = learn.train_chooser()
+    return nr.Model(order_params, alpha=0.5, chooser=ch_params, margin=margin), info

@@ -612,3 +612,165 @@ def test_pivot_mode_matches_the_reference_program():
         assert (r["paths"], r["nodes"]) == (v["paths"], v["nodes"]), v
         tol = 1e-12 if v["graph"] in ("COMP8.txt", "cycle4.txt", "temp4x4.txt") else 1e-7
         assert abs(r["R"] - v["R_float_p"]) <= tol * max(r["R"], 1e-3) + 5e-13, v
+
+
+# ------------------------------------------------- learned sweep choice
+from oracle_ffi import MODE_EXPLORE, MODE_LEARNED  # noqa: E402
+
+
+def _random_chooser(seed):
+    rng = np.random.default_rng(seed)
+    return (rng.integers(-255, 256, size=(16, 64)).astype(np.int32),
+            rng.integers(0, 2, size=64).astype(np.int32))
+
+
+@pytest.mark.parametrize("mode", [MODE_LEARNED, MODE_EXPLORE])
+def test_learned_and_exploring_sweeps_exact(expected, mode):
+    """Either sweep direction splits an item exactly, so whatever the
+    chooser (random integer weights here) or the exploring hash picks, R
+    equals brute force and the closed forms and R + D = 1."""
+    ch = _random_chooser(5)
+    for bf in expected["brute_force"]:
+        r = load(bf["graph"]).solve(bf["src"], bf["dst"], mode=mode, chooser=ch)
+        assert abs(r["R"] - bf["R"]) <= 1e-15, bf
+        assert abs(r["R"] + r["D"] - 1.0) <= 1e-15
+    for name, n, p in [("COMP8.txt", 8, "0.5"), ("COMP8-09.txt", 8, "0.9"),
+                       ("COMP8-03.txt", 8, "0.3")]:
+        r = load(name).solve(0, n - 1, mode=mode, chooser=ch)
+        assert abs(r["R"] - closed_form(expected, n, p)) <= 1e-14
+
+
+def test_zero_chooser_is_the_alternating_rule():
+    """Equal scores fall back to the alternating direction: an all-zero
+    chooser builds the alternating rule's tree exactly."""
+    zero = (np.zeros((16, 64), np.int32), np.zeros(64, np.int32))
+    for name, n, period, eps in [("COMP8.txt", 8, 2, 0.0), ("COMP9.txt", 9, 3, 0.0),
+                                 ("COMP15-05.txt", 15, 4, 1e-7)]:
+        g = load(name)
+        a = g.solve(0, n - 1, mode=MODE_ALTERNATE, sweep_period=period, trunc_eps=eps)
+        b = g.solve(0, n - 1, mode=MODE_LEARNED, sweep_period=period, trunc_eps=eps,
+                    chooser=zero)
+        assert (a["paths"], a["nodes"], a["dead"], a["truncated"], a["iterations"],
+                a["max_width"]) == (b["paths"], b["nodes"], b["dead"], b["truncated"],
+                                    b["iterations"], b["max_width"])
+
+
+def test_learned_counts_independent_of_order_and_threads():
+    g = load("COMP9.txt")
+    ch = _random_chooser(9)
+    ref = g.solve(0, 8, mode=MODE_LEARNED, chooser=ch)
+    for order, threads in [(ORDER_HEAP, 1), (ORDER_DFS, 1), (ORDER_DFS, 4)]:
+        r = g.solve(0, 8, mode=MODE_LEARNED, order=order, threads=threads, chooser=ch)
+        assert (r["paths"], r["nodes"], r["dead"]) == (ref["paths"], ref["nodes"], ref["dead"])
+        assert abs(r["R"] - ref["R"]) <= 1e-15
+
+
+def _lq(x):
+    """round(-4 log2 x) in 0..63 (63 for x <= 0): the choice's probability step."""
+    if not x > 0:
+        return 63
+    return int(min(63.0, max(0.0, np.rint(-4.0 * math.log2(x)))))
+
+
+def _floor_log2(m):
+    return math.frexp(m)[1] - 1
+
+
+def _fdiv2(v):
+    return v // 2  # Python floors
+
+
+def _choice_x(n, ends, p, src, dst, known, up, mass, eps, hops_fwd, from_dst):
+    """The 16 choice features, restated from their definition."""
+    E = len(ends)
+    lab = _components(n, ends, up & known)
+    S = {v for v in range(n) if lab[v] == lab[src]}
+    T = {v for v in range(n) if lab[v] == lab[dst]}
+
+    def leaving(C):
+        cut = failed = 0
+        for i, (a, b) in enumerate(ends):
+            if (a in C) == (b in C):
+                continue
+            if (known >> i) & 1 and not (up >> i) & 1:
+                failed += 1
+            else:
+                cut += 1
+        return len(C), cut, failed
+    A, B = (T, S) if from_dst else (S, T)
+    nA, cA, fA = leaving(A)
+    nB, cB, fB = leaving(B)
+    f = bin(known & ~up).count("1")
+    u = E - bin(known).count("1")
+    lim = lambda v: max(-63, min(63, v))  # noqa: E731
+    lm = _floor_log2(mass)
+    le = _floor_log2(eps) if eps > 2.0 ** -64 else -64
+    head = max(0, min(32, _fdiv2(lm - le)))
+    first, last = (hops_fwd[-1], hops_fwd[0]) if from_dst else (hops_fwd[0], hops_fwd[-1])
+    lp = sum(_lq(p[e]) for e in hops_fwd)
+    return [lim(len(hops_fwd)), lim(nA), lim(cA), lim(fA), lim(nB), lim(cB), lim(fB),
+            (32 * f) // E, (32 * u) // E, 0, head, -lim(lp), -_lq(1.0 - p[first]),
+            -_lq(1.0 - p[last]), lim(cA - cB), 1]
+
+
+def _explore_bit(words_known, words_up):
+    M = (1 << 64) - 1
+
+    def fin(z):
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        return z ^ (z >> 31)
+    h = 0x9E3779B97F4A7C15
+    for k, u in zip(words_known, words_up):
+        h = fin(h ^ k)
+        h = fin(h ^ u)
+    return (h >> 62) == 0
+
+
+@pytest.mark.parametrize("name,src,dst,eps,pf,pw", [("COMP10-05.txt", 0, 9, 0.0, 0.25, 0.15),
+                                                    ("COMP10-05.txt", 2, 7, 1e-9, 0.4, 0.05),
+                                                    ("test_kite.txt", 0, 4, 0.0, 0.25, 0.15),
+                                                    ("COMP25-05.txt", 0, 24, 1e-12, 0.6, 0.003)])
+def test_learned_single_expansion_choice(name, src, dst, eps, pf, pw):
+    """One expansion on random edge states under a random integer chooser:
+    the path is the alternating rule's, swept from the end whose features
+    (restated here from their definition) score lower, the alternating
+    direction on ties; the exploring mode flips the alternating direction
+    where the hash's top two bits are zero."""
+    g = load(name)
+    rp, ci, ei, p = g.arrays()
+    n, E = len(rp) - 1, len(p)
+    ends = [None] * E
+    for u in range(n):
+        for j in range(rp[u], rp[u + 1]):
+            ends[ei[j]] = (min(u, ci[j]), max(u, ci[j]))
+    w, sg = _random_chooser(17)
+    margin = 5000 if name == "test_kite.txt" else 0
+    rnd = random.Random(23)
+    dirs = set()
+    for _ in range(300):
+        known, up = _states(rnd, E, pf, pw)
+        m = rnd.choice([1.0, 0.5, 3e-7, 2.5e-11, 7e-13, 1e-19, 3e-22])
+        kw = [(known >> (64 * i)) & ((1 << 64) - 1) for i in range(g.mask_words)]
+        uw = [(up >> (64 * i)) & ((1 << 64) - 1) for i in range(g.mask_words)]
+        k0, fwd = g.expand_choice(src, dst, kw, uw, m, MODE_ALTERNATE, period=1000)
+        k1, got = g.expand_choice(src, dst, kw, uw, m, MODE_LEARNED, period=3,
+                                  chooser=(w, sg, margin), eps=eps)
+        k2, exp_hops = g.expand_choice(src, dst, kw, uw, m, MODE_EXPLORE, period=3)
+        assert k0 == k1 == k2
+        if k0 < 2:
+            continue
+        score = []
+        for from_dst in (False, True):
+            x = np.array(_choice_x(n, ends, p, src, dst, known, up, m, eps, fwd, from_dst),
+                         np.int64)
+            h = x @ w.astype(np.int64)
+            score.append(int(np.where(sg == 1, -h, h)[h > 0].sum()))
+        f = bin(known & ~up).count("1")
+        alt = (f // 3) % 2
+        rev = bool(alt ^ 1) if score[alt] - score[alt ^ 1] > margin else bool(alt)
+        assert got == (fwd[::-1] if rev else fwd)
+        dirs.add(rev)
+        alt_rev = (f // 3) % 2 == 1
+        assert exp_hops == (fwd[::-1] if alt_rev != _explore_bit(kw, uw) else fwd)
+    assert dirs == {True, False}
