@@ -39,7 +39,9 @@ typedef enum nr_status {
   NR_ERR_ARG = 4,         /* null pointer or invalid config value */
   NR_ERR_UNSUPPORTED = 5, /* graph shape beyond the engine's width classes */
   NR_ERR_DEVICE = 6,      /* HIP runtime error */
-  NR_ERR_NOMEM = 7        /* device allocation failed */
+  NR_ERR_NOMEM = 7,       /* device allocation failed */
+  NR_ERR_SCRATCH = 8      /* nr_solve_static: the solve needs more device
+                             memory than the fixed scratch area holds */
 } nr_status;
 
 /* Work order.  Any order gives an exact decomposition; order only changes
@@ -196,6 +198,22 @@ void nr_graph_destroy(nr_graph* g);
 
 /* Solve one instance on the GPU. */
 int nr_solve(const nr_graph* g, const nr_config* cfg, nr_result* out);
+
+/* nr_solve that allocates nothing: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Its queue
+ * buffers, graph tables, per-workgroup partial sums and control block are
+ * carved from a fixed 64 MiB device array (NR_STATIC_SCRATCH_BYTES, one per
+ * GPU, kept until the process ends) on the null stream, with no events and
+ * no pinned host memory; results equal nr_solve's.  Sizes beyond the array
+ * (queue_capacity * item bytes * 2 plus tables and slots) return
+ * NR_ERR_SCRATCH.  Graphs past the per-lane forms (more than 256 nodes or
+ * 512 edges) return NR_ERR_UNSUPPORTED; elapsed_ms is host time.  Calls on
+ * one GPU are serialised by the library within a process; the array keeps
+ * the last call's contents (each call clears what it needs).  kernel_ms and
+ * kernel_launches stay 0. */
+#define NR_STATIC_SCRATCH_BYTES (64ll << 20)
+int nr_solve_static(const nr_graph* g, const nr_config* cfg, nr_result* out);
 
 /* Solve `count` independent instances (graphs and/or terminal pairs) in one
  * persistent launch sequence; results[i] belongs to graphs[i].  Counts are

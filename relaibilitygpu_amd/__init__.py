@@ -47,7 +47,9 @@ STATUS = {
     5: "NR_ERR_UNSUPPORTED",
     6: "NR_ERR_DEVICE",
     7: "NR_ERR_NOMEM",
+    8: "NR_ERR_SCRATCH",
 }
+STATIC_SCRATCH_BYTES = 64 << 20
 
 
 class NRError(RuntimeError):
@@ -171,6 +173,7 @@ def lib() -> ctypes.CDLL:
     L.nr_graph_destroy.argtypes = [vp]
     L.nr_graph_destroy.restype = None
     L.nr_solve.argtypes = [vp, P(Config), P(Result)]
+    L.nr_solve_static.argtypes = [vp, P(Config), P(Result)]
     L.nr_solve_batch.argtypes = [P(vp), ctypes.c_int, P(Config), P(Result)]
     L.nr_expand_items.argtypes = [
         vp, ctypes.c_int, ctypes.c_int, P(ctypes.c_ulonglong), P(ctypes.c_ulonglong),
@@ -343,6 +346,16 @@ def solve(graph: Graph, config: Config | None = None, **kw) -> dict:
     cfg = config if config is not None else make_config(**kw)
     r = Result()
     _check(lib().nr_solve(graph.handle, ctypes.byref(cfg), ctypes.byref(r)), "nr_solve")
+    return r.as_dict()
+
+
+def solve_static(graph: Graph, config: Config | None = None, **kw) -> dict:
+    """nr_solve_static: nr_solve with every device buffer in the library's
+    fixed scratch array (no device allocation)."""
+    cfg = config if config is not None else make_config(**kw)
+    r = Result()
+    _check(lib().nr_solve_static(graph.handle, ctypes.byref(cfg), ctypes.byref(r)),
+           "nr_solve_static")
     return r.as_dict()
 
 

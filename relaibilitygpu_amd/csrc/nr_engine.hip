@@ -768,8 +768,24 @@ struct Workspace {
   const nr_graph* tab_g = nullptr;
   unsigned long long tab_stamp = 0;
   int tab_W = 0;
+  /* no-allocation mode (nr_solve_static): every device buffer is carved
+   * from the file-scope scratch array (offsets restart with each solve), the
+   * null stream is used, and no events or pinned host memory exist: the
+   * counters are read back with plain copies into host_ctl */
+  bool fixed = false;
+  unsigned char* arena = nullptr;
+  size_t arena_size = 0, arena_used = 0;
+  LevelCtl host_ctl;
 
   int grow(void** p, size_t* have, size_t need) {
+    if (fixed) {
+      const size_t at = (arena_used + 255) / 256 * 256;
+      if (at > arena_size || need > arena_size - at) return NR_ERR_SCRATCH;
+      *p = arena + at;
+      *have = need;
+      arena_used = at + need;
+      return NR_OK;
+    }
     if (need <= *have) return NR_OK;
     if (*p) (void)hipFree(*p);
     *p = nullptr;
@@ -860,6 +876,14 @@ struct Workspace {
 
 constexpr int kMaxDevices = 64;
 Workspace g_ws[kMaxDevices];
+
+/* The no-allocation entry point's device memory: one fixed array per GPU
+ * (each device holds its own copy at its own address until the process
+ * ends).  Host code never passes the array's name to a kernel or a copy: it
+ * uses the address hipGetSymbolAddress returns for the current device. */
+constexpr size_t kScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+Workspace g_ws_fixed[kMaxDevices];
 
 void pack_tables(const nr_graph* g, int W, TableBlob& b) {
   const int n = g->n, e = std::max(g->e, 1);
@@ -1502,12 +1526,14 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
     a.hist = q + 2 * grid + 8;
   }
   a.ctl = ws.ctl;
-  a.hmirror = ws.dmirror;
+  a.hmirror = ws.dmirror; /* null in no-allocation mode */
   a.seq = ++ws.solves;
   /* until this solve's first launch writes it, the level word reads as
    * unfinished: a chunk boundary never sees the last solve's empty level */
-  ws.hmirror[0] = ~0ull;
-  ws.hmirror[kMirSeq] = 0;
+  if (ws.hmirror) {
+    ws.hmirror[0] = ~0ull;
+    ws.hmirror[kMirSeq] = 0;
+  }
   a.mass_mode = mass ? 1 : 0;
   a.thr_class = kClasses;
   a.drop_class = kClasses;
@@ -1574,13 +1600,13 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   }
   nr_checkpoint** save = x ? x->save : nullptr;
   int pend_parity = -1; /* buffer holding the pending items to keep */
-  hipEvent_t ev_start, ev_stop;
-  if ((rc = ws.event(0, &ev_start)) || (rc = ws.event(1, &ev_stop))) return rc;
-  const size_t max_timed = (cfg->flags & NR_FLAG_TIME_LAUNCHES) ? 2048 : 0;
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  if (!ws.fixed && ((rc = ws.event(0, &ev_start)) || (rc = ws.event(1, &ev_stop)))) return rc;
+  const size_t max_timed = (cfg->flags & NR_FLAG_TIME_LAUNCHES) && !ws.fixed ? 2048 : 0;
 
   auto t_begin = std::chrono::steady_clock::now();
   const bool has_budget = cfg->time_budget_s > 0;
-  NR_HIP(hipEventRecord(ev_start, st));
+  if (ev_start) NR_HIP(hipEventRecord(ev_start, st));
   int level = 0, chunk = has_budget ? 1 : 8;
   bool finished = false, deadline = false;
   long long timed = 0, rounds = 0;
@@ -1654,20 +1680,29 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
      * keeps working while the host decides (at most two launches run past
      * the last level); an empty level in the mirror, whichever launch wrote
      * it, means the queue is empty for good */
-    const int ahead = (!has_budget && !save) ? 2 : 0;
+    const int ahead = (!has_budget && !save && !ws.fixed) ? 2 : 0;
     hipEvent_t ev_chunk = nullptr;
-    if ((rc = ws.event(2 + 2 * max_timed, &ev_chunk))) return rc;
+    if (!ws.fixed && (rc = ws.event(2 + 2 * max_timed, &ev_chunk))) return rc;
     int queued = 0; /* launches of the coming chunk already queued */
     while (!finished) {
       const auto c0 = std::chrono::steady_clock::now();
       const int launched = chunk;
       for (int c = queued; c < chunk; ++c, ++level)
         if ((rc = launch_level(level & 1, level))) return rc;
-      NR_HIP(hipEventRecord(ev_chunk, st));
-      for (queued = 0; queued < ahead; ++queued, ++level)
-        if ((rc = launch_level(level & 1, level))) return rc;
-      NR_HIP(hipEventSynchronize(ev_chunk)); /* its last launch wrote ws.hmirror */
-      const u64 w = ws.hmirror[0], max_w = ws.hmirror[2];
+      u64 w = 0, max_w = 0;
+      if (ev_chunk) {
+        NR_HIP(hipEventRecord(ev_chunk, st));
+        for (queued = 0; queued < ahead; ++queued, ++level)
+          if ((rc = launch_level(level & 1, level))) return rc;
+        NR_HIP(hipEventSynchronize(ev_chunk)); /* its last launch wrote ws.hmirror */
+        w = ws.hmirror[0];
+        max_w = ws.hmirror[2];
+      } else { /* no-allocation mode: the counters by a plain copy */
+        NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
+        NR_HIP(hipStreamSynchronize(st));
+        w = ws.hctl->last_items;
+        max_w = ws.hctl->max_width;
+      }
       if (w == 0) finished = true;
       /* past the widest level: shorter chunks as the levels thin out, so
        * few launches run after the last level */
@@ -1805,14 +1840,19 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
     void* args[] = {&a, &nw};
     NR_HIP(hipLaunchKernel((const void*)&k_finalize, dim3(1), dim3(256), args, 0, st));
   }
-  NR_HIP(hipEventRecord(ev_stop, st));
+  if (ev_stop) NR_HIP(hipEventRecord(ev_stop, st));
   NR_HIP(hipStreamSynchronize(st));
   float ms = 0.f;
-  NR_HIP(hipEventElapsedTime(&ms, ev_start, ev_stop));
+  if (ev_stop)
+    NR_HIP(hipEventElapsedTime(&ms, ev_start, ev_stop));
+  else /* no-allocation mode: host clock around the synchronised solve */
+    ms = (float)(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
+                                                           t_begin)
+                     .count());
   /* the result as k_finalize mirrored it into pinned host memory; should
    * its stamp not be there yet, the device copy is read instead */
   LevelCtl& h = *ws.hctl;
-  if (__atomic_load_n(&ws.hmirror[kMirSeq], __ATOMIC_ACQUIRE) != a.seq) {
+  if (!ws.hmirror || __atomic_load_n(&ws.hmirror[kMirSeq], __ATOMIC_ACQUIRE) != a.seq) {
     NR_HIP(hipMemcpy(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost));
   } else {
     const u64* m = ws.hmirror;
@@ -2519,6 +2559,56 @@ int nr_solve(const nr_graph* g, const nr_config* cfg, nr_result* out) {
     nr::WsLock L(cfg->device);
     if (L.rc) return L.rc;
     return nr::solve_level(*L.ws, g, cfg, out);
+  });
+}
+
+int nr_solve_static(const nr_graph* g, const nr_config* cfg, nr_result* out) {
+  return nr::guarded([&]() -> int {
+    if (!g || !out) return NR_ERR_ARG;
+    nr_config def;
+    if (!cfg) {
+      nr_config_init(&def);
+      cfg = &def;
+    }
+    if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0 || cfg->time_budget_s < 0)
+      return NR_ERR_ARG;
+    if (cfg->order != NR_ORDER_LEVEL && cfg->order != NR_ORDER_MASS) return NR_ERR_ARG;
+    if (cfg->rule != NR_RULE_PATH && cfg->rule != NR_RULE_FORKBEST &&
+        cfg->rule != NR_RULE_ALTERNATE && cfg->rule != NR_RULE_PIVOT)
+      return NR_ERR_ARG;
+    if (cfg->sweep_period < 0) return NR_ERR_ARG;
+    if (cfg->device < 0 || cfg->device >= nr::kMaxDevices) return NR_ERR_ARG;
+    nr::Variant var;
+    int rc = nr::pick_variant(g, var);
+    if (rc) return rc; /* the one-wave-per-item form keeps its own buffers */
+    nr::Workspace& ws = nr::g_ws_fixed[cfg->device];
+    std::lock_guard<std::mutex> lk(ws.mu);
+    if (hipSetDevice(cfg->device) != hipSuccess) return NR_ERR_DEVICE;
+    if (!ws.ready) { /* once per device, with the device current */
+      void* p = nullptr;
+      if (hipGetSymbolAddress(&p, HIP_SYMBOL(nr::scratch)) != hipSuccess || !p)
+        return NR_ERR_DEVICE;
+      if (hipGetDeviceProperties(&ws.prop, cfg->device) != hipSuccess) return NR_ERR_DEVICE;
+      ws.device = cfg->device;
+      ws.fixed = true;
+      ws.arena = (unsigned char*)p;
+      ws.arena_size = nr::kScratchBytes;
+      ws.stream = nullptr; /* the null stream */
+      ws.hctl = &ws.host_ctl;
+      ws.hmirror = ws.dmirror = nullptr;
+      ws.ready = true;
+    }
+    /* every buffer is carved afresh: offsets restart, tables are uploaded
+     * again, and k_init clears what must start at zero */
+    ws.arena_used = 0;
+    ws.q = ws.tab = ws.meta = nullptr;
+    ws.q_bytes = ws.tab_bytes = ws.meta_bytes = 0;
+    ws.tab_g = nullptr;
+    void* c = nullptr;
+    size_t have = 0;
+    if ((rc = ws.grow(&c, &have, sizeof(nr::LevelCtl)))) return rc;
+    ws.ctl = (nr::LevelCtl*)c;
+    return nr::solve_level(ws, g, cfg, out);
   });
 }
 

@@ -336,6 +336,7 @@ const char* nr_status_str(int s) {
     case NR_ERR_UNSUPPORTED: return "graph beyond supported width classes";
     case NR_ERR_DEVICE: return "HIP runtime error";
     case NR_ERR_NOMEM: return "device allocation failed";
+    case NR_ERR_SCRATCH: return "solve needs more than the fixed scratch area";
     default: return "unknown status";
   }
 }
