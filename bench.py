@@ -42,6 +42,10 @@ WORKLOADS = {
             "desc": "K10 (N=10, E=45, p=0.5) src 0 -> dst 9, full enumeration, level order"},
     "k8p09": {"graph": "COMP8-09.txt", "src": 0, "dst": 7, "eps": 0.0, "cap": 1 << 16,
               "desc": "K8 (N=8, E=28, p=0.9) src 0 -> dst 7, full enumeration, level order"},
+    "p2p": {"graph": "p2p-Gn08C.txt", "src": 0, "dst": 17, "eps": 1e-6, "cap": 1 << 22,
+            "kernel": "k_expand_big",
+            "desc": ("p2p-Gn08C (SNAP, N=6301, 20777 arcs, p=0.5) src 0 -> dst 17, truncation "
+                     "1e-6, level order, one wave per subproblem")},
     "er4096": {"batch": 4096, "eps": 1e-12, "cap": 1 << 27,
                "desc": ("4096 seeded Erdos-Renyi instances (n 12..40, mean degree 4..8, "
                         "p 0.80..0.99, terminals at distance >= 3), truncation 1e-12, "
@@ -450,7 +454,8 @@ def main() -> int:
     launches = sum(r["kernel_launches"] for r in timed)
     bytes_per_sub = 2 * (16 * info.mask_words + 8)
     achieved = nodes * len(timed) * bytes_per_sub / (kernel_ms / 1e3) / 1e9
-    traffic = traffic_from_profiles("k_expand_level", args.workload)
+    kernel = wl.get("kernel", "k_expand_level")
+    traffic = traffic_from_profiles(kernel, args.workload)
     line = {
         "metric": METRIC,
         "value": value,
@@ -482,7 +487,7 @@ def main() -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "k_expand_level",
+            "kernel": kernel,
             "bytes_per_subproblem": bytes_per_sub,
             "avg_launch_ms": kernel_ms / launches,
             "launches_per_step": launches / len(timed),

@@ -118,6 +118,8 @@ typedef enum nr_rule {
                                    do only while the rows take <= 64 bytes, and
                                    are derived from the planes otherwise; same
                                    results) */
+#define NR_FLAG_LEAN 8          /* items never carry rows (NR_FLAG_ROWS wins when
+                                   both are set); same results */
 
 typedef struct nr_result {
   double R;                     /* reliability lower bound (exact when ESP=0) */

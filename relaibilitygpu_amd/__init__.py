@@ -25,6 +25,7 @@ CLI_PATH = os.path.join(_HERE, "bin", "network_reliability")
 
 NR_FLAG_TIME_LAUNCHES = 1
 NR_FLAG_ROWS = 4
+NR_FLAG_LEAN = 8
 
 NR_ORDER_LEVEL = 0
 NR_ORDER_MASS = 1
@@ -233,7 +234,7 @@ def make_config(queue_capacity: int = 1 << 20, truncation_eps: float = 1e-15,
                 time_budget_s: float = 0.0, order: str | int = "level", device: int = 0,
                 block_size: int = 0, grid_blocks: int = 0, chunk_items: int = 0,
                 time_launches: bool = False, rule: str | int = "path",
-                rows: bool = False, sweep_period: int = 0) -> Config:
+                rows: bool = False, sweep_period: int = 0, lean: bool = False) -> Config:
     c = Config()
     lib().nr_config_init(ctypes.byref(c))
     c.queue_capacity = int(queue_capacity)
@@ -244,7 +245,8 @@ def make_config(queue_capacity: int = 1 << 20, truncation_eps: float = 1e-15,
     c.block_size = int(block_size)
     c.grid_blocks = int(grid_blocks)
     c.chunk_items = int(chunk_items)
-    c.flags = (NR_FLAG_TIME_LAUNCHES if time_launches else 0) | (NR_FLAG_ROWS if rows else 0)
+    c.flags = ((NR_FLAG_TIME_LAUNCHES if time_launches else 0) | (NR_FLAG_ROWS if rows else 0)
+               | (NR_FLAG_LEAN if lean else 0))
     c.rule = RULES[rule] if isinstance(rule, str) else int(rule)
     c.sweep_period = int(sweep_period)
     return c

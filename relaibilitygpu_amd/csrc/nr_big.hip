@@ -145,23 +145,61 @@ __global__ void __launch_bounds__(256) k_expand_big(LevelArgs a, int parity, Big
     bool dead = false, err = false;
     if (src != dst) {
       for (;;) {
-        for (int w = 0; w < NW; ++w) {
-          const u64 word = cur[w];
-          if (!word) continue;
-          if ((word >> lane) & 1ull) {
-            const int u = w * 64 + lane;
-            const int j1 = a.g.rp[u + 1];
-            for (int j = a.g.rp[u]; j < j1; j += 8) { /* eight arc loads in flight */
-              u32 xs[8];
+        /* The frontier's nodes dealt to all 64 lanes: lane l holds word
+         * w0 + l of a block of 64 frontier words, an exclusive prefix of
+         * their popcounts numbers the block's nodes, and in round r lane i
+         * expands node r*64 + i (owner word by a search over the prefix,
+         * then that word's j-th set bit).  Levels are sets, so the order in
+         * which nodes are expanded changes nothing. */
+        for (int w0 = 0; w0 < NW; w0 += 64) {
+          const int wl = w0 + lane;
+          const u64 word = wl < NW ? cur[wl] : 0ull;
+          const int cnt = __popcll(word);
+          const int incl = wave_incl_scan(cnt);
+          const int total = __shfl(incl, 63, 64);
+          for (int r0 = 0; r0 < total; r0 += 64) {
+            const int g = r0 + lane;
+            int lo = 0, hi = 63; /* first lane whose inclusive prefix exceeds g */
 #pragma unroll
-              for (int t = 0; t < 8; ++t) xs[t] = j + t < j1 ? a.g.arc[j + t] : 0u;
+            for (int step = 0; step < 6; ++step) {
+              const int mid = (lo + hi) >> 1;
+              if (__shfl(incl, mid, 64) > g)
+                hi = mid;
+              else
+                lo = mid + 1;
+            }
+            const u64 ow = __shfl(word, lo, 64);
+            const int oi = __shfl(incl, lo, 64) - __shfl(cnt, lo, 64);
+            if (g < total) {
+              /* the (g - oi)-th set bit of ow, by halving on popcounts */
+              int j = g - oi, base = 0;
+              u64 x = ow;
 #pragma unroll
-              for (int t = 0; t < 8; ++t) {
-                if (j + t >= j1) break;
-                const int v = (int)(xs[t] & 0xffffu), e = (int)(xs[t] >> 16);
-                if (lev[v] != 0xff || !allowed(e)) continue;
-                lev[v] = (unsigned char)(L + 1);
-                atomicOr((unsigned long long*)&nxt[v >> 6], 1ull << (v & 63));
+              for (int half = 32; half >= 1; half >>= 1) {
+                const u64 low = x & ((1ull << half) - 1ull);
+                const int c = __popcll(low);
+                if (j >= c) {
+                  j -= c;
+                  x >>= half;
+                  base += half;
+                } else {
+                  x = low;
+                }
+              }
+              const int u = (w0 + lo) * 64 + base;
+              const int j1 = a.g.rp[u + 1];
+              for (int jj = a.g.rp[u]; jj < j1; jj += 8) { /* eight arc loads in flight */
+                u32 xs[8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) xs[t] = jj + t < j1 ? a.g.arc[jj + t] : 0u;
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                  if (jj + t >= j1) break;
+                  const int v = (int)(xs[t] & 0xffffu), e = (int)(xs[t] >> 16);
+                  if (lev[v] != 0xff || !allowed(e)) continue;
+                  lev[v] = (unsigned char)(L + 1);
+                  atomicOr((unsigned long long*)&nxt[v >> 6], 1ull << (v & 63));
+                }
               }
             }
           }

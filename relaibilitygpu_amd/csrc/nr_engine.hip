@@ -1399,7 +1399,7 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
    * 60 B, 19.8 vs 20.4 ms; K25, 100 B, 54.9 vs 30.6 ms), unless the caller
    * asks for the row-carrying kernel */
   if (mode == 0 && cfg->rule == NR_RULE_PATH && g->terms.empty() && !var.nw &&
-      g->n * var.ns_bytes > 64 && !(cfg->flags & NR_FLAG_ROWS)) {
+      (g->n * var.ns_bytes > 64 || (cfg->flags & NR_FLAG_LEAN)) && !(cfg->flags & NR_FLAG_ROWS)) {
     var = variant_lean(mask_words(g), var.ns_bytes);
     add_mass_kernels(var, mask_words(g));
   }
@@ -1452,9 +1452,28 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   const u64 n_slots = (u64)grid; /* partial sums and counters per workgroup */
   const u64 cap = cfg->queue_capacity > 0 ? (u64)cfg->queue_capacity : (1ull << 20);
   const bool mass = cfg->order == NR_ORDER_MASS || mode == 2;
-  /* class rounds trim the pool to `cap` between rounds; the buffers get
-   * twice that so a round's children never overflow before the trim */
-  const u64 phys = mass ? 2 * cap : cap;
+  /* class rounds trim the pool to `cap` between rounds, and a round's
+   * output (the children of the classes it expands plus the items it
+   * carries) lands in buffers of four times that when the device has the
+   * room (twice otherwise): near capacity a round can then expand more of
+   * the pool and carry less of it (grid 10x10, plain rule: rounds expanding
+   * 10-20 M items carried 60-78 M with twice the capacity) */
+  u64 phys = cap;
+  if (mass) {
+    const int ns_est = var.lean || var.nw ? 0 : var.ns_bytes;
+    const size_t item = (size_t)(2 * W + 1) * 8 + (size_t)g->n * ns_est + 4;
+    size_t avail = 0;
+    if (ws.fixed) {
+      avail = ws.arena_size > ws.arena_used ? ws.arena_size - ws.arena_used : 0;
+    } else {
+      size_t free_b = 0, total_b = 0;
+      NR_HIP(hipMemGetInfo(&free_b, &total_b));
+      avail = free_b + ws.q_bytes; /* the queue buffers are given back first */
+    }
+    const size_t need4 = 2 * 4 * (size_t)(cap + (u64)grid) * item;
+    const size_t slack = ws.fixed ? (1ull << 20) : (64ull << 20); /* tables, slots */
+    phys = need4 + slack <= avail ? 4 * cap : 2 * cap;
+  }
   /* capacity is split evenly over the workgroup segments */
   const u64 seg_cap64 = std::max<u64>(1, (phys + grid - 1) / grid);
   if (seg_cap64 * (u64)grid >= (1ull << 32)) return NR_ERR_ARG;
