@@ -74,7 +74,20 @@ typedef struct nr_config {
   int sweep_period;         /* NR_RULE_ALTERNATE: the sweep direction flips
                                every this many FAILED edges (tree levels);
                                0 = 2 */
-  int reserved[4];
+  int prefix_edges;         /* static prefix partition (the reference's
+                               TID_TO_MASK, original/Rel_16M.cu:97-101): the
+                               root is split over its first prefix_edges
+                               UNKNOWN edges (edge id order) into 2^t seed
+                               subproblems, seed i with edge j WORKING when
+                               bit j of i is set and FAILED otherwise, mass
+                               the product of their p and q; the solve starts
+                               from the seeds (0 = from the root; at most 24).
+                               Exact: the seeds partition the states */
+  int shard, shards;        /* with prefix_edges: only the seeds i with
+                               i % shards == shard (0 <= shard < shards;
+                               shards 0 or 1 = all); the shards' R, D, ESP
+                               and counts add up to the whole solve's */
+  int reserved[1];
 } nr_config;
 
 /* Decomposition rule (nr_config.rule).  Both are exact; they build

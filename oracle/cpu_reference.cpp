@@ -954,6 +954,38 @@ struct Deadline {
   bool hit() const { return on && Clock::now() >= t; }
 };
 
+/* The items a solve starts from: the root, or the seeds of the static
+ * prefix partition (original/Rel_16M.cu:97-101, thread-id bits fixing the
+ * first edges) in seed order, masses multiplied in edge-id order. */
+template <class Push>
+void for_each_start(const Engine& en, const orc_config& cfg, Push push) {
+  std::vector<u64> k0, u0;
+  initial_item(en.g, k0, u0);
+  if (cfg.prefix_edges <= 0) {
+    push(k0.data(), u0.data(), 1.0);
+    return;
+  }
+  std::vector<int> pick;
+  for (int e = 0; e < en.g->e && (int)pick.size() < cfg.prefix_edges; ++e)
+    if (!((k0[e >> 6] >> (e & 63)) & 1)) pick.push_back(e);
+  const unsigned long long n_seeds = 1ull << pick.size();
+  const unsigned long long step = cfg.shards > 1 ? (unsigned long long)cfg.shards : 1;
+  std::vector<u64> k(en.W), u(en.W);
+  for (unsigned long long i = (unsigned long long)cfg.shard; i < n_seeds; i += step) {
+    k = k0;
+    u = u0;
+    double m = 1.0;
+    for (size_t j = 0; j < pick.size(); ++j) {
+      const int e = pick[j];
+      k[e >> 6] |= 1ull << (e & 63);
+      const bool working = (i >> j) & 1;
+      if (working) u[e >> 6] |= 1ull << (e & 63);
+      m *= working ? en.g->p[e] : en.g->q[e];
+    }
+    push(k.data(), u.data(), m);
+  }
+}
+
 /* Level order (main.cu:963-1008): the whole current level is processed
  * before the next; children beyond `capacity` go to ESP and count as paths
  * (main.cu:245-252). iterations = number of levels; max_width = widest
@@ -962,9 +994,7 @@ void run_level(const Engine& en, const orc_config& cfg, orc_result* r, Acc& acc)
   Worker wk(en);
   Items cur, nxt;
   cur.W = nxt.W = en.W;
-  std::vector<u64> k0, u0;
-  initial_item(en.g, k0, u0);
-  cur.push(k0.data(), u0.data(), 1.0);
+  for_each_start(en, cfg, [&](const u64* k, const u64* u, double m) { cur.push(k, u, m); });
   Deadline dl(cfg.time_budget_s);
   long long iters = 0, maxw = 0;
   size_t cap = cfg.capacity > 0 ? (size_t)cfg.capacity : (size_t)-1;
@@ -1033,9 +1063,8 @@ void run_heap(const Engine& en, const orc_config& cfg, orc_result* r, Acc& acc) 
       i = par;
     }
   };
-  std::vector<u64> k0, u0, tk(W), tu(W);
-  initial_item(en.g, k0, u0);
-  push(k0.data(), u0.data(), 1.0);
+  std::vector<u64> tk(W), tu(W);
+  for_each_start(en, cfg, [&](const u64* k, const u64* u, double m) { push(k, u, m); });
   Deadline dl(cfg.time_budget_s);
   size_t cap = cfg.capacity > 0 ? (size_t)cfg.capacity : (size_t)-1;
   long long maxw = 1;
@@ -1093,9 +1122,7 @@ void run_dfs(const Engine& en, const orc_config& cfg, orc_result* r, Acc& acc) {
   Worker seed(en);
   Items cur, nxt;
   cur.W = nxt.W = en.W;
-  std::vector<u64> k0, u0;
-  initial_item(en.g, k0, u0);
-  cur.push(k0.data(), u0.data(), 1.0);
+  for_each_start(en, cfg, [&](const u64* k, const u64* u, double m) { cur.push(k, u, m); });
   size_t want = (size_t)64 * T;
   long long iters = 0;
   Deadline dl(cfg.time_budget_s);

@@ -78,6 +78,11 @@ typedef struct orc_config {
                             feature), s[64] signs (1 = negative unit), then
                             the margin (score units) needed to leave the
                             alternating direction */
+  int prefix_edges;      /* > 0: start from the 2^t seeds of the static
+                            prefix partition (original/Rel_16M.cu:97-101)
+                            over the root's first t UNKNOWN edges: seed i
+                            has edge j WORKING iff bit j of i is set */
+  int shard, shards;     /* with prefix_edges: only seeds i % shards == shard */
 } orc_config;
 
 typedef struct orc_result {

@@ -29,6 +29,9 @@ class OrcConfig(ctypes.Structure):
         ("time_budget_s", ctypes.c_double),
         ("sweep_period", ctypes.c_int),
         ("chooser", ctypes.POINTER(ctypes.c_int)),
+        ("prefix_edges", ctypes.c_int),
+        ("shard", ctypes.c_int),
+        ("shards", ctypes.c_int),
     ]
 
 
@@ -150,11 +153,13 @@ class OracleGraph:
 
     def solve(self, src: int, dst: int, mode: int = MODE_PATH, order: int = ORDER_LEVEL,
               threads: int = 1, trunc_eps: float = 0.0, capacity: int = 0,
-              time_budget_s: float = 0.0, sweep_period: int = 0, chooser=None) -> dict:
+              time_budget_s: float = 0.0, sweep_period: int = 0, chooser=None,
+              prefix_edges: int = 0, shard: int = 0, shards: int = 0) -> dict:
         """chooser (MODE_LEARNED): (weights [16, 64], signs [64], margin)
         integers, the exact form the engine reports for a model's path
         chooser (margin 0 when omitted)."""
         cfg = OrcConfig(mode, order, threads, trunc_eps, capacity, time_budget_s, sweep_period)
+        cfg.prefix_edges, cfg.shard, cfg.shards = prefix_edges, shard, shards
         keep = None
         if chooser is not None:
             w, sg, margin = chooser if len(chooser) == 3 else (*chooser, 0)

@@ -74,7 +74,10 @@ class Config(ctypes.Structure):
         ("flags", ctypes.c_int),
         ("rule", ctypes.c_int),
         ("sweep_period", ctypes.c_int),
-        ("reserved", ctypes.c_int * 4),
+        ("prefix_edges", ctypes.c_int),
+        ("shard", ctypes.c_int),
+        ("shards", ctypes.c_int),
+        ("reserved", ctypes.c_int * 1),
     ]
 
 
@@ -234,7 +237,8 @@ def make_config(queue_capacity: int = 1 << 20, truncation_eps: float = 1e-15,
                 time_budget_s: float = 0.0, order: str | int = "level", device: int = 0,
                 block_size: int = 0, grid_blocks: int = 0, chunk_items: int = 0,
                 time_launches: bool = False, rule: str | int = "path",
-                rows: bool = False, sweep_period: int = 0, lean: bool = False) -> Config:
+                rows: bool = False, sweep_period: int = 0, lean: bool = False,
+                prefix_edges: int = 0, shard: int = 0, shards: int = 0) -> Config:
     c = Config()
     lib().nr_config_init(ctypes.byref(c))
     c.queue_capacity = int(queue_capacity)
@@ -249,6 +253,9 @@ def make_config(queue_capacity: int = 1 << 20, truncation_eps: float = 1e-15,
                | (NR_FLAG_LEAN if lean else 0))
     c.rule = RULES[rule] if isinstance(rule, str) else int(rule)
     c.sweep_period = int(sweep_period)
+    c.prefix_edges = int(prefix_edges)
+    c.shard = int(shard)
+    c.shards = int(shards)
     return c
 
 

@@ -1387,6 +1387,48 @@ int solve_big(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_result*
   return NR_OK;
 }
 
+/* The seeds of the static prefix partition (original/Rel_16M.cu:97-101,
+ * where thread-id bits fix the first edges): the root's first t UNKNOWN
+ * edges in id order, seed i with edge j WORKING when bit j of i is set and
+ * FAILED otherwise, mass the product of the p and q so chosen (FP64, edges
+ * in id order); only seeds i % shards == shard when shards > 1. */
+int prefix_seeds(const nr_graph* g, int W, const nr_config* cfg,
+                 std::unique_ptr<nr_checkpoint>& out) {
+  const int t = cfg->prefix_edges;
+  const int shards = cfg->shards > 1 ? cfg->shards : 1;
+  if (t < 0 || t > 24 || cfg->shard < 0 || cfg->shard >= shards) return NR_ERR_ARG;
+  const Root root = root_item(g, W);
+  std::vector<int> free_e;
+  for (int e = 0; e < g->e && (int)free_e.size() < t; ++e)
+    if (!((root.known[e >> 6] >> (e & 63)) & 1ull)) free_e.push_back(e);
+  const int tt = (int)free_e.size(); /* fewer when the root has fewer UNKNOWN edges */
+  out.reset(new (std::nothrow) nr_checkpoint);
+  if (!out) return NR_ERR_NOMEM;
+  nr_checkpoint& c = *out;
+  const u64 all = 1ull << tt;
+  for (u64 i = (u64)cfg->shard; i < all; i += (u64)shards) {
+    double m = 1.0;
+    std::vector<u64> k(root.known, root.known + W), u(root.up, root.up + W);
+    for (int j = 0; j < tt; ++j) {
+      const int e = free_e[j];
+      const u64 b = 1ull << (e & 63);
+      k[e >> 6] |= b;
+      if ((i >> j) & 1ull) {
+        u[e >> 6] |= b;
+        m *= g->p[e];
+      } else {
+        m *= g->q[e];
+      }
+    }
+    c.known.insert(c.known.end(), k.begin(), k.end());
+    c.up.insert(c.up.end(), u.begin(), u.end());
+    c.mass.push_back(m);
+  }
+  c.count = c.mass.size();
+  c.w = W;
+  return NR_OK;
+}
+
 int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_result* out,
                 const LevelExtra* x = nullptr) {
   Variant var;
@@ -1559,6 +1601,13 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
 
   hipStream_t st = ws.stream;
   const nr_checkpoint* ck = x ? x->resume : nullptr;
+  /* static prefix partition: the seeds enter as a resumed solve's pending
+   * items, with no sums or counts yet */
+  std::unique_ptr<nr_checkpoint> seeds;
+  if (!ck && cfg->prefix_edges > 0) {
+    if ((rc = prefix_seeds(g, W, cfg, seeds))) return rc;
+    ck = seeds.get();
+  }
   DevMem seedmem;
   if (ck) { /* resume: pending items spread over the segments of buffer 0 */
     if (ck->count > seg_cap64 * (u64)grid) return NR_ERR_ARG; /* capacity below the checkpoint */
@@ -2570,6 +2619,7 @@ int nr_solve(const nr_graph* g, const nr_config* cfg, nr_result* out) {
     nr::Variant var;
     int rc = nr::pick_variant(g, var);
     if (rc == NR_ERR_UNSUPPORTED) { /* past the per-lane forms: one wave per item */
+      if (cfg->prefix_edges > 0) return NR_ERR_UNSUPPORTED;
       nr::WsLock L(cfg->device);
       if (L.rc) return L.rc;
       return nr::solve_big(*L.ws, g, cfg, out);

@@ -171,3 +171,41 @@ def test_batch_reuses_device_tables_until_a_graph_changes():
     assert (changed["nodes"][1:] == first["nodes"][1:]).all()
     other = nr.solve_batch(graphs[5:25], truncation_eps=1e-9, queue_capacity=1 << 22)
     assert [r["nodes"] for r in other] == [int(x) for x in first["nodes"][5:25]]
+
+
+def test_er4096_full_batch():
+    """The batched workload at full size (4096 seeded ER instances, delta
+    1e-12, the benchmark's capacity 2^27): every instance closes R + D + ESP
+    = 1 with nothing overflowed or re-run, and every 16th instance matches
+    the oracle exactly; the batch's bracket for those meets the oracle's from
+    a tighter truncation (1e-14)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    print("case ER batch 4096 eps=1e-12", flush=True)
+    insts = [wl.er_instance(i) for i in range(4096)]
+    graphs = []
+    for n, edges, probs, s, t in insts:
+        rp, ci, ei = wl.to_csr(n, edges)
+        graphs.append(nr.Graph.from_csr(rp, ci, ei, probs, s, t))
+    res = nr.solve_batch(graphs, truncation_eps=1e-12, queue_capacity=1 << 27)
+    assert len(res) == 4096
+    for r in res:
+        assert r["overflowed"] == 0 and r["rerun"] == 0 and r["deadline_hit"] == 0
+        assert abs(r["R"] + r["ESP"] + r["D"] - 1.0) <= 1e-12
+    assert sum(r["nodes"] for r in res) > 1e9
+
+    def oracle(i):
+        n, edges, probs, s, t = insts[i]
+        rp, ci, ei = wl.to_csr(n, edges)
+        og = OracleGraph.from_csr(rp, ci, ei, probs)
+        return (og.solve(s, t, order=ORDER_DFS, trunc_eps=1e-12),
+                og.solve(s, t, order=ORDER_DFS, trunc_eps=1e-14))
+
+    sample = list(range(0, 4096, 16))
+    with ThreadPoolExecutor(16) as ex:
+        outs = list(ex.map(oracle, sample))
+    for i, (o, tight) in zip(sample, outs):
+        check_inst(res[i], o, levels=False)
+        lo = max(res[i]["R"], tight["R"])
+        hi = min(res[i]["R"] + res[i]["ESP"], tight["R"] + tight["ESP"])
+        assert lo <= hi * (1 + 1e-12), i

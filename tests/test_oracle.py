@@ -774,3 +774,40 @@ def test_learned_single_expansion_choice(name, src, dst, eps, pf, pw):
         alt_rev = (f // 3) % 2 == 1
         assert exp_hops == (fwd[::-1] if alt_rev != _explore_bit(kw, uw) else fwd)
     assert dirs == {True, False}
+
+
+# ------------------------------------------------- static prefix partition
+@pytest.mark.parametrize("t", [1, 3, 6])
+@pytest.mark.parametrize("order", [ORDER_LEVEL, ORDER_HEAP, ORDER_DFS])
+def test_prefix_partition_exact(expected, t, order):
+    """The 2^t seeds over the root's first t UNKNOWN edges partition the
+    states (original/Rel_16M.cu:97-101), so R is the closed form and
+    R + D = 1 whatever t and order."""
+    r = load("COMP8-09.txt").solve(0, 7, order=order, prefix_edges=t, threads=2)
+    assert abs(r["R"] - closed_form(expected, 8, "0.9")) <= 1e-14
+    assert abs(r["R"] + r["D"] - 1.0) <= 1e-14
+    assert r["nodes"] >= 2 ** t
+
+
+def test_prefix_over_every_edge_is_brute_force(expected):
+    """Fixing every edge makes each seed a full state: it is a success (a
+    WORKING path) or dead, so the solve is the brute-force enumeration."""
+    for bf in expected["brute_force"]:
+        g = load(bf["graph"])
+        E = g.dims()["e"]
+        if E > 16:
+            continue
+        r = g.solve(bf["src"], bf["dst"], prefix_edges=E)
+        assert r["nodes"] == 2 ** E and r["paths"] + r["dead"] == 2 ** E
+        assert r["iterations"] == 1
+        assert abs(r["R"] - bf["R"]) <= 1e-15, bf
+
+
+def test_prefix_shards_add_up():
+    g = load("COMP9.txt")
+    whole = g.solve(0, 8, prefix_edges=5)
+    parts = [g.solve(0, 8, prefix_edges=5, shard=s, shards=3) for s in range(3)]
+    for f in ("paths", "nodes", "dead", "truncated"):
+        assert sum(p[f] for p in parts) == whole[f], f
+    assert abs(sum(p["R"] for p in parts) - whole["R"]) <= 1e-14
+    assert abs(sum(p["D"] for p in parts) - whole["D"]) <= 1e-14
