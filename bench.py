@@ -64,6 +64,25 @@ def traffic_from_profiles(kernel: str, workload: str):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def valu_roofline(kernel: str, workload: str, subproblems_per_s: float):
+    """Instruction roofline of the expand kernel: VALU wave instructions per
+    subproblem (rocprofv3 SQ_INSTS_VALU over whole solves, committed under
+    profiles/) times the measured subproblems/s, against the chip's VALU
+    issue peak (256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64
+    instruction), or None without a counters summary."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_counters.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        c = json.load(f)
+    achieved = c["valu_per_subproblem"] * subproblems_per_s
+    return {"bound": "valu", "achieved": achieved, "peak": c["valu_peak_per_s"],
+            "unit": "VALU wave instructions/s", "frac": achieved / c["valu_peak_per_s"],
+            "valu_per_subproblem": c["valu_per_subproblem"],
+            "salu_per_subproblem": c["salu_per_subproblem"], "kernel": kernel,
+            "basis": c["valu_peak_basis"]}
+
+
 def cpu_baseline(wl: dict) -> dict:
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle_ffi import ORDER_DFS, OracleGraph
@@ -492,6 +511,7 @@ def main() -> int:
             "avg_launch_ms": kernel_ms / launches,
             "launches_per_step": launches / len(timed),
         },
+        "roofline_valu": valu_roofline(kernel, args.workload, value),
         "gpu_elapsed_ms_per_step": sum(r["elapsed_ms"] for r in results) / args.steps,
         "kernel_ms_per_step": kernel_ms / len(timed),
         "R": results[-1]["R"],
