@@ -18,7 +18,10 @@ disagree.  So several members are trained, the median of their predictions
 is distilled into one network of the same shape (the form the expand kernel
 evaluates), and its margin (the predicted advantage needed to leave the
 alternating direction) is picked on validation solves of graphs that the
-evaluations do not use.  No counterpart in the reference.
+evaluations do not use (complete graphs truncated and capacity-bound, a grid
+and a Barabasi-Albert graph of other sizes): the geometric mean of the gap
+ratios over those regimes decides, since a chooser that helps in one regime
+can hurt in another.  No counterpart in the reference.
 """
 from __future__ import annotations
 
@@ -62,7 +65,7 @@ def _mlp_forward(params: np.ndarray, X: np.ndarray) -> np.ndarray:
 # ------------------------------------------------------------ order scorer
 def train_order_scorer(alpha: float = 0.5, device: int = 0):
     """The order scorer on children logged from K10 solves of the
-    alternating rule (delta 0, 1e-6, 1e-9): (Model, summary)."""
+    alternating rule (delta 0, 1e-6, 1e-9): (parameters, summary)."""
     t0 = time.perf_counter()
     k10 = complete(10)
     Xs, Ys = [], []
@@ -101,14 +104,18 @@ def log_choice_data(logs=CHOICE_LOGS, period: int = 6, seed: int = 7, device: in
 
 
 def validation_tasks(cap: int = 96777216):
-    """Validation solves (no evaluation graph among them): K14 truncated at
-    1e-12 in level order, K14 untruncated in max-mass order under a 2^22-item
-    capacity, a 7x7 grid and a Barabasi-Albert graph of 100 nodes."""
+    """Validation solves (no evaluation graph among them): K14 and K18
+    truncated at 1e-12 in level order, K14 untruncated in max-mass order
+    under capacities of 2^22 and 2^24 items (the capacity binds), a 7x7 grid
+    and a Barabasi-Albert graph of 100 nodes."""
     k14 = complete(14)
     return {
         "K14": (k14, 6, dict(truncation_eps=1e-12, order="level", queue_capacity=cap)),
+        "K18": (complete(18), 6, dict(truncation_eps=1e-12, order="level", queue_capacity=cap)),
         "K14_capacity": (k14, 6, dict(truncation_eps=0.0, order="mass", queue_capacity=1 << 22,
                                       time_budget_s=5.0)),
+        "K14_capacity_2^24": (k14, 6, dict(truncation_eps=0.0, order="mass",
+                                           queue_capacity=1 << 24, time_budget_s=5.0)),
         "grid7x7": (grid(7, 7, 0x7A11), 1, dict(truncation_eps=1e-10, order="level",
                                                  queue_capacity=cap)),
         "ba100": (barabasi_albert(100, 0xBA100), 1, dict(truncation_eps=1e-13, order="level",
