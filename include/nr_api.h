@@ -87,7 +87,17 @@ typedef struct nr_config {
                                i % shards == shard (0 <= shard < shards;
                                shards 0 or 1 = all); the shards' R, D, ESP
                                and counts add up to the whole solve's */
-  int reserved[1];
+  int ramp_items;           /* fresh level-order solves of the path and
+                               alternating rules: after the levels one
+                               workgroup runs alone, every workgroup expands
+                               its own slice's subtrees level after level
+                               (no launch per level) until the levels hold
+                               about this many items; 0 = 1024 per compute
+                               unit, < 0 = one launch per level from
+                               there.  Same results either way: a ramp that
+                               overflows a workgroup's region is discarded and
+                               the solve runs again without it
+                               (nr_result.rerun) */
 } nr_config;
 
 /* Decomposition rule (nr_config.rule).  Both are exact; they build
@@ -159,7 +169,10 @@ typedef struct nr_result {
   int blocks_per_cu;            /* occupancy reported by the runtime */
   int compute_units;
   int rerun;                    /* nr_solve_batch: 1 if the instance overflowed
-                                   the shared queue and was solved again alone */
+                                   the shared queue and was solved again alone;
+                                   nr_solve: 1 if the ramp (ramp_items)
+                                   overflowed and the solve ran again without
+                                   it */
 } nr_result;
 
 typedef struct nr_graph_info {

@@ -77,7 +77,7 @@ class Config(ctypes.Structure):
         ("prefix_edges", ctypes.c_int),
         ("shard", ctypes.c_int),
         ("shards", ctypes.c_int),
-        ("reserved", ctypes.c_int * 1),
+        ("ramp_items", ctypes.c_int),
     ]
 
 
@@ -238,7 +238,8 @@ def make_config(queue_capacity: int = 1 << 20, truncation_eps: float = 1e-15,
                 block_size: int = 0, grid_blocks: int = 0, chunk_items: int = 0,
                 time_launches: bool = False, rule: str | int = "path",
                 rows: bool = False, sweep_period: int = 0, lean: bool = False,
-                prefix_edges: int = 0, shard: int = 0, shards: int = 0) -> Config:
+                prefix_edges: int = 0, shard: int = 0, shards: int = 0,
+                ramp_items: int = 0) -> Config:
     c = Config()
     lib().nr_config_init(ctypes.byref(c))
     c.queue_capacity = int(queue_capacity)
@@ -256,6 +257,7 @@ def make_config(queue_capacity: int = 1 << 20, truncation_eps: float = 1e-15,
     c.prefix_edges = int(prefix_edges)
     c.shard = int(shard)
     c.shards = int(shards)
+    c.ramp_items = int(ramp_items)
     return c
 
 

@@ -16,6 +16,7 @@ struct KernAlt {
   static void* learn() { return (void*)&k_expand_level<W, EX, 2>; }
   static void* mass() { return (void*)&k_expand_level<W, EX, 4>; }
   static void* small() { return (void*)&k_small_levels<W, EX>; }
+  static void* ramp() { return (void*)&k_ramp_levels<W, EX>; }
   static void* items() { return (void*)&k_expand_items<W, EX>; }
 };
 
@@ -31,6 +32,7 @@ static Variant alt_for_w(int W) {
     v.level_learn = &KernAlt<X, NS>::learn;  \
     v.level_mass = &KernAlt<X, NS>::mass;    \
     v.small = &KernAlt<X, NS>::small;        \
+    v.ramp = &KernAlt<X, NS>::ramp;          \
     break;
     NR_W(1) NR_W(2) NR_W(3) NR_W(4) NR_W(5) NR_W(6) NR_W(7) NR_W(8)
 #undef NR_W

@@ -14,6 +14,7 @@ struct KernAltCsr {
   static void* learn() { return (void*)&k_expand_level<W, EX, 2>; }
   static void* mass() { return (void*)&k_expand_level<W, EX, 4>; }
   static void* small() { return (void*)&k_small_levels<W, EX>; }
+  static void* ramp() { return (void*)&k_ramp_levels<W, EX>; }
   static void* items() { return (void*)&k_expand_items<W, EX>; }
 };
 
@@ -29,6 +30,7 @@ static Variant alt_csr_nw(int W) {
     v.level_learn = &KernAltCsr<X, NW>::learn;  \
     v.level_mass = &KernAltCsr<X, NW>::mass;    \
     v.small = &KernAltCsr<X, NW>::small;        \
+    v.ramp = &KernAltCsr<X, NW>::ramp;          \
     break;
     NR_W(1) NR_W(2) NR_W(3) NR_W(4) NR_W(5) NR_W(6) NR_W(7) NR_W(8)
 #undef NR_W

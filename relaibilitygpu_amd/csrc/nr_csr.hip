@@ -13,6 +13,7 @@ struct KernCsr {
   static void* level() { return (void*)&k_expand_level<W, CsrExpander<W, NW>, 0>; }
   static void* items() { return (void*)&k_expand_items<W, CsrExpander<W, NW>>; }
   static void* small() { return (void*)&k_small_levels<W, CsrExpander<W, NW>>; }
+  static void* ramp() { return (void*)&k_ramp_levels<W, CsrExpander<W, NW>>; }
 };
 
 template <int NW>
@@ -24,6 +25,7 @@ static Variant variant_csr_nw(int W) {
       Variant v{&KernCsr<X, NW>::level, &KernCsr<X, NW>::items, nullptr, nullptr, 0, NW,  \
                 nullptr, nullptr, nullptr, nullptr, 0, nullptr};                          \
       v.small = &KernCsr<X, NW>::small;                                                   \
+      v.ramp = &KernCsr<X, NW>::ramp;                                                     \
       return v;                                                                           \
     }
     NR_W(1) NR_W(2) NR_W(3) NR_W(4) NR_W(5) NR_W(6) NR_W(7) NR_W(8)

@@ -549,30 +549,37 @@ struct Expander {
       plen = 0;
       return 0;
     }
-    const NS dbit = (NS)1 << dst;
+    /* the rows are symmetric (undirected graphs): dst enters the level
+     * after frontier F exactly when its own row meets F, a test of one row,
+     * so the levels before are expanded whole, four rows a step with their
+     * loads in flight together and no test inside the step */
+    const NS drow = rows[dst * stride];
     NS vis = (NS)1 << src;
     NS F = vis;
     lvl[0] = F;
     int L = 0;
-    for (;;) {
+    while (!(drow & F)) {
       NS nx = 0;
       NS f = F;
-      while (f) { /* two rows per step: two LDS loads in flight */
+      while (f) {
         const int u0 = ctzs<NS>(f);
         f &= f - 1;
         const int u1 = f ? ctzs<NS>(f) : u0;
         f &= f - 1;
-        nx |= rows[u0 * stride] | rows[u1 * stride];
-        if (nx & dbit) break;
+        const int u2 = f ? ctzs<NS>(f) : u0;
+        f &= f - 1;
+        const int u3 = f ? ctzs<NS>(f) : u0;
+        f &= f - 1;
+        nx |= (rows[u0 * stride] | rows[u1 * stride]) | (rows[u2 * stride] | rows[u3 * stride]);
       }
       nx &= ~vis;
       if (!nx) return -1;
       ++L;
-      if (nx & dbit) break;
       lvl[L * stride] = nx;
       vis |= nx;
       F = nx;
     }
+    ++L; /* dst's level */
     plen = L;
     int v = dst, k = 0;
     for (int l = L; l >= 1; --l) {

@@ -160,85 +160,15 @@ __global__ void __launch_bounds__(256) k_init(LevelArgs a, Root root, int W, int
   }
 }
 
-/* Reduce the per-wave slots in a fixed order (one block): thread t folds
- * slots t, t+256, ...; thread 0 then folds the 256 partials in index order,
- * so the result does not depend on timing. */
-__device__ inline void neu_add(double& s, double& c, double x) {
-  double u = s + x;
-  if (fabs(s) >= fabs(x))
-    c += (s - u) + x;
-  else
-    c += (x - u) + s;
-  s = u;
+/* The solve's totals (finalize_slots, min(kFinMax, ceil(n_slots / 64))
+ * workgroups), when no empty level launch has computed them. */
+__global__ void __launch_bounds__(256) k_finalize(LevelArgs a, u64 n_slots) {
+  __shared__ u32 s_last;
+  finalize_slots(a, n_slots, &s_last);
 }
 
-__global__ void __launch_bounds__(256) k_finalize(LevelArgs a, u64 n_slots) {
-  __shared__ double ps[3][256], pc[3][256];
-  __shared__ u64 pn[C_N][256];
-  const int t = threadIdx.x;
-  const DD* src[3] = {a.partR, a.partE, a.partD};
-  /* per thread: slots t, t + 256, ... (Neumaier), four slots' loads issued
-   * before the first sum waits */
-  for (int k = 0; k < 3; ++k) {
-    double s = 0.0, c = 0.0;
-    for (u64 i0 = t; i0 < n_slots; i0 += 4 * (u64)blockDim.x) {
-      DD v[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const u64 i = i0 + (u64)j * blockDim.x;
-        v[j] = i < n_slots ? src[k][i] : DD{0.0, 0.0};
-      }
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        neu_add(s, c, v[j].hi);
-        neu_add(s, c, v[j].lo);
-      }
-    }
-    ps[k][t] = s;
-    pc[k][t] = c;
-  }
-  for (int j = 0; j < C_N; ++j) {
-    u64 x = 0;
-    for (u64 i = t; i < n_slots; i += blockDim.x) x += a.cnt[i * C_N + j];
-    pn[j][t] = x;
-  }
-  __syncthreads();
-  /* pairwise tree: slot t absorbs slot t + h with an error-free sum */
-  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
-    if (t < h) {
-      for (int k = 0; k < 3; ++k) {
-        double s, e;
-        two_sum(ps[k][t], ps[k][t + h], s, e);
-        ps[k][t] = s;
-        pc[k][t] = pc[k][t] + pc[k][t + h] + e;
-      }
-      for (int j = 0; j < C_N; ++j) pn[j][t] += pn[j][t + h];
-    }
-    __syncthreads();
-  }
-  if (t == 0) {
-    u64* m = a.hmirror; /* the host reads the result here, no copy queued */
-    for (int k = 0; k < 3; ++k) {
-      double hi, lo;
-      two_sum(ps[k][0], pc[k][0], hi, lo);
-      a.ctl->sum[k] = hi;
-      a.ctl->sum_lo[k] = lo;
-      if (m) {
-        m[kMirSum + k] = (u64)__double_as_longlong(hi);
-        m[kMirSumLo + k] = (u64)__double_as_longlong(lo);
-      }
-    }
-    for (int j = 0; j < C_N; ++j) {
-      a.ctl->counts[j] = pn[j][0];
-      if (m) m[kMirCounts + j] = pn[j][0];
-    }
-    if (m) {
-      m[1] = a.ctl->iterations;
-      m[2] = a.ctl->max_width;
-      __threadfence_system(); /* the result words before the stamp */
-      m[kMirSeq] = a.seq;
-    }
-  }
+inline unsigned finalize_groups(u64 n_slots) {
+  return (unsigned)std::min<u64>(kFinMax, (n_slots + 63) / 64);
 }
 
 /* ---- checkpoints: pending items out of, and back into, the queue ---- */
@@ -1040,6 +970,7 @@ struct Kern {
   static void* refill() { return (void*)&k_refill_batch<W>; }
   static void* seed_rows() { return (void*)&k_seed_rows<W, NS>; }
   static void* small() { return (void*)&k_small_levels<W, Expander<W, NS>>; }
+  static void* ramp() { return (void*)&k_ramp_levels<W, Expander<W, NS>>; }
 };
 
 template <typename NS>
@@ -1051,6 +982,7 @@ Variant variant_for_w(int W) {
               &Kern<X, NS>::refill, (int)sizeof(NS), 0, nullptr, nullptr, nullptr,        \
               &Kern<X, NS>::seed_rows, 0, nullptr};                                       \
     v.small = &Kern<X, NS>::small;                                                        \
+    v.ramp = &Kern<X, NS>::ramp;                                                          \
     return v;                                                                             \
   }
     NR_W(1) NR_W(2) NR_W(3) NR_W(4) NR_W(5) NR_W(6) NR_W(7) NR_W(8)
@@ -1328,7 +1260,7 @@ int solve_big(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_result*
         ++timed;
       }
     }
-    NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
+    NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, kCtlCopyBytes, hipMemcpyDeviceToHost, st));
     NR_HIP(hipStreamSynchronize(st));
     if (ws.hctl->counts[C_ERR] || ws.hctl->last_items == 0) finished = true;
     if (!finished && expired()) {
@@ -1346,10 +1278,11 @@ int solve_big(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_result*
   {
     u64 nw = n_slots;
     void* args[] = {&a, &nw};
-    NR_HIP(hipLaunchKernel((const void*)&k_finalize, dim3(1), dim3(256), args, 0, st));
+    NR_HIP(hipLaunchKernel((const void*)&k_finalize, dim3(finalize_groups(n_slots)), dim3(256),
+                           args, 0, st));
   }
   NR_HIP(hipEventRecord(ev_stop, st));
-  NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
+  NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, kCtlCopyBytes, hipMemcpyDeviceToHost, st));
   NR_HIP(hipStreamSynchronize(st));
   float ms = 0.f;
   NR_HIP(hipEventElapsedTime(&ms, ev_start, ev_stop));
@@ -1598,6 +1531,7 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   a.mass_mode = mass ? 1 : 0;
   a.thr_class = kClasses;
   a.drop_class = kClasses;
+  a.fin_on_empty = (!mass && ws.hmirror) ? 1 : 0;
 
   hipStream_t st = ws.stream;
   const nr_checkpoint* ck = x ? x->resume : nullptr;
@@ -1658,14 +1592,18 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
                                rargs, ls.total_bytes, st));
       }
     }
-  } else {
+  }
+  /* a fresh solve: the root in buffer 0, segment 0 */
+  auto init_root = [&]() -> int {
     Root root = root_item(g, W);
     int Wi = W, ni = g->n, nsb = item_ns;
     u64 nw = n_slots;
     const SeedMeta* none = nullptr;
     void* args[] = {&a, &root, &Wi, &ni, &nsb, &nw, &none};
     NR_HIP(hipLaunchKernel((const void*)&k_init, dim3(64), dim3(256), args, 0, st));
-  }
+    return NR_OK;
+  };
+  if (!ck && (rc = init_root())) return rc;
   nr_checkpoint** save = x ? x->save : nullptr;
   int pend_parity = -1; /* buffer holding the pending items to keep */
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
@@ -1716,29 +1654,53 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   };
   /* the first, small levels in one workgroup (fresh level-order solves of
    * the path rule): a level of at most max_in items, whose children fit
-   * segment 0 (k <= n - 1 hops each), never needs a second workgroup */
-  if (var.small && !ck && mode == 0 && !mass &&
-      (cfg->rule == NR_RULE_PATH || cfg->rule == NR_RULE_ALTERNATE)) {
+   * segment 0 (k <= n - 1 hops each), never needs a second workgroup; then
+   * the ramp's levels, each workgroup on its own slice's subtrees, up to
+   * about one item per lane of the grid.  When a ramp region overflows (the
+   * level launches report it at the first chunk boundary) the solve starts
+   * again without the ramp, so counts never depend on it. */
+  const bool small_ok = var.small && !ck && mode == 0 && !mass &&
+                        (cfg->rule == NR_RULE_PATH || cfg->rule == NR_RULE_ALTERNATE);
+  bool ramp_used = false;
+  int ramp_restarts = 0;
+  auto launch_small = [&](bool ramp) -> int {
     const u32 per = (u32)std::max(1, g->n - 1);
     const u32 max_in = std::min<u32>(256u, std::min<u32>((u32)block, seg_cap / per));
-    if (max_in >= 1) {
-      const void* ks = (const void*)var.small();
-      if ((rc = ws.allow_lds(ks, 64 * 1024))) return rc;
+    if (max_in < 1) return NR_OK;
+    auto timed_launch = [&](const void* k, unsigned blocks, void** args) -> int {
+      int r = ws.allow_lds(k, 64 * 1024);
+      if (r) return r;
       hipEvent_t kb = nullptr, ke = nullptr;
       if ((size_t)timed < max_timed) {
-        if ((rc = ws.event(2 + 2 * timed, &kb)) || (rc = ws.event(3 + 2 * timed, &ke))) return rc;
+        if ((r = ws.event(2 + 2 * timed, &kb)) || (r = ws.event(3 + 2 * timed, &ke))) return r;
         NR_HIP(hipEventRecord(kb, st));
       }
-      u32 mi = max_in;
-      int ml = 64;
-      void* sargs[] = {&a, &mi, &ml};
-      NR_HIP(hipLaunchKernel(ks, dim3(1), dim3(block), sargs, lay.total_bytes, st));
+      NR_HIP(hipLaunchKernel(k, dim3(blocks), dim3(block), args, lay.total_bytes, st));
       if (kb) {
         NR_HIP(hipEventRecord(ke, st));
         ++timed;
       }
-    }
-  }
+      return NR_OK;
+    };
+    u32 mi = max_in;
+    int ml = 64;
+    void* sargs[] = {&a, &mi, &ml};
+    int r = timed_launch((const void*)var.small(), 1, sargs);
+    if (r || !ramp || !var.ramp || grid < 2 || cfg->ramp_items < 0) return r;
+    /* one workgroup per CU (each runs its levels alone on its CU: one wave
+     * step of a level then takes about as long as in k_small_levels), each
+     * with the segments 1.. of the grid split evenly; by default the ramp
+     * ends near 1024 items per CU (four waves per SIMD) */
+    const int groups = std::min(grid - 1, cus);
+    int span = (grid - 1) / groups;
+    u32 target = cfg->ramp_items > 0 ? (u32)cfg->ramp_items : 1024u * (u32)cus;
+    void* rargs[] = {&a, &span, &target};
+    if ((r = timed_launch((const void*)var.ramp(), (unsigned)groups, rargs))) return r;
+    ramp_used = true;
+    return NR_OK;
+  };
+  if (ws.hmirror) ws.hmirror[kMirRamp] = 0;
+  if (small_ok && (rc = launch_small(true))) return rc;
   if (!mass) {
     /* level order: chunks of back-to-back level launches; under a time
      * budget a chunk is sized from the measured time per launch so the
@@ -1758,6 +1720,7 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
       for (int c = queued; c < chunk; ++c, ++level)
         if ((rc = launch_level(level & 1, level))) return rc;
       u64 w = 0, max_w = 0;
+      bool ramp_over = false;
       if (ev_chunk) {
         NR_HIP(hipEventRecord(ev_chunk, st));
         for (queued = 0; queued < ahead; ++queued, ++level)
@@ -1765,11 +1728,28 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
         NR_HIP(hipEventSynchronize(ev_chunk)); /* its last launch wrote ws.hmirror */
         w = ws.hmirror[0];
         max_w = ws.hmirror[2];
+        ramp_over = ws.hmirror[kMirRamp] != 0;
       } else { /* no-allocation mode: the counters by a plain copy */
-        NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
+        NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, kCtlCopyBytes, hipMemcpyDeviceToHost, st));
         NR_HIP(hipStreamSynchronize(st));
         w = ws.hctl->last_items;
         max_w = ws.hctl->max_width;
+        ramp_over = ws.hctl->ramp_over != 0;
+      }
+      if (ramp_used && ramp_over) { /* a ramp region filled: again without the ramp */
+        NR_HIP(hipStreamSynchronize(st));
+        if (ws.hmirror) { /* nothing of the discarded run may look like a result */
+          ws.hmirror[0] = ~0ull;
+          ws.hmirror[kMirSeq] = 0;
+        }
+        if ((rc = init_root()) || (rc = launch_small(false))) return rc;
+        ramp_used = false;
+        ++ramp_restarts;
+        timed = 0; /* kernel_ms and kernel_launches: the solve that counts */
+        level = 0;
+        queued = 0;
+        chunk = has_budget ? 1 : 8;
+        continue;
       }
       if (w == 0) finished = true;
       /* past the widest level: shorter chunks as the levels thin out, so
@@ -1783,10 +1763,11 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
         deadline = true;
         finished = true;
       }
-      chunk = std::min(chunk * 2, 16);
-      if (shrinking) chunk = std::min(chunk, 8);
-      if (shrinking && 4 * w < max_w) chunk = std::min(chunk, 4);
-      if (shrinking && 64 * w < max_w) chunk = std::min(chunk, 2);
+      /* at most 8 launches a chunk: a chunk that runs past the last level
+       * leaves empty launches behind it */
+      chunk = std::min(chunk * 2, 8);
+      if (shrinking) chunk = std::min(chunk, 4);
+      if (shrinking && 16 * w < max_w) chunk = std::min(chunk, 2);
       if (has_budget && !finished) chunk = budget_chunk(c0, launched, chunk);
     }
   } else {
@@ -1843,7 +1824,7 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
       if ((rc = launch_level(parity, (int)rounds))) return rc;
       void* hargs[] = {&a};
       NR_HIP(hipLaunchKernel((const void*)&k_hist_total, dim3(kClasses), dim3(256), hargs, 0, st));
-      NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
+      NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, kCtlCopyBytes, hipMemcpyDeviceToHost, st));
       NR_HIP(hipStreamSynchronize(st));
       ++rounds;
       ++launches_here;
@@ -1903,13 +1884,23 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
       NR_HIP(hipStreamSynchronize(st));
     }
   }
-  {
+  /* a queue that ran empty had its totals computed by the first empty level
+   * launch (fin_on_empty); otherwise (a fold, kept items, no mirror) by
+   * k_finalize */
+  bool have_totals = false;
+  if (a.fin_on_empty && !deadline && pend_parity < 0) {
+    if (ev_stop) NR_HIP(hipEventRecord(ev_stop, st));
+    NR_HIP(hipStreamSynchronize(st));
+    have_totals = __atomic_load_n(&ws.hmirror[kMirSeq], __ATOMIC_ACQUIRE) == a.seq;
+  }
+  if (!have_totals) {
     u64 nw = n_slots;
     void* args[] = {&a, &nw};
-    NR_HIP(hipLaunchKernel((const void*)&k_finalize, dim3(1), dim3(256), args, 0, st));
+    NR_HIP(hipLaunchKernel((const void*)&k_finalize, dim3(finalize_groups(n_slots)), dim3(256),
+                           args, 0, st));
+    if (ev_stop) NR_HIP(hipEventRecord(ev_stop, st));
+    NR_HIP(hipStreamSynchronize(st));
   }
-  if (ev_stop) NR_HIP(hipEventRecord(ev_stop, st));
-  NR_HIP(hipStreamSynchronize(st));
   float ms = 0.f;
   if (ev_stop)
     NR_HIP(hipEventElapsedTime(&ms, ev_start, ev_stop));
@@ -1921,7 +1912,7 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
    * its stamp not be there yet, the device copy is read instead */
   LevelCtl& h = *ws.hctl;
   if (!ws.hmirror || __atomic_load_n(&ws.hmirror[kMirSeq], __ATOMIC_ACQUIRE) != a.seq) {
-    NR_HIP(hipMemcpy(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost));
+    NR_HIP(hipMemcpy(ws.hctl, a.ctl, kCtlCopyBytes, hipMemcpyDeviceToHost));
   } else {
     const u64* m = ws.hmirror;
     h.iterations = m[1];
@@ -1963,6 +1954,7 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   out->lds_bytes = lay.total_bytes;
   out->blocks_per_cu = per_cu;
   out->compute_units = cus;
+  out->rerun = ramp_restarts ? 1 : 0;
   if (save) *save = nullptr;
   if (kept) { /* the bracket stays certified: pending mass counts in ESP */
     kept->n = g->n;
@@ -2222,7 +2214,7 @@ int solve_batch(Workspace& ws, const nr_graph* const* gs, int count, const nr_co
         NR_HIP(hipLaunchKernel(kref, dim3(refill_grid), dim3(256), rargs, 0, st));
       }
       launches += chunk;
-      NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
+      NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, kCtlCopyBytes, hipMemcpyDeviceToHost, st));
       NR_HIP(hipMemcpyAsync(&hsc, a.sc, sizeof hsc, hipMemcpyDeviceToHost, st));
       NR_HIP(hipStreamSynchronize(st));
       for (int c = 0; timing && c < chunk; ++c) {

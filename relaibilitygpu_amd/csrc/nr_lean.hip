@@ -11,6 +11,7 @@ template <int W, typename NS>
 struct KernLean {
   static void* level() { return (void*)&k_expand_level<W, LeanExpander<W, NS>, 0>; }
   static void* small() { return (void*)&k_small_levels<W, LeanExpander<W, NS>>; }
+  static void* ramp() { return (void*)&k_ramp_levels<W, LeanExpander<W, NS>>; }
 };
 
 template <typename NS>
@@ -21,6 +22,7 @@ static Variant lean_for_w(int W) {
   case X:                                  \
     v.level = &KernLean<X, NS>::level;     \
     v.small = &KernLean<X, NS>::small;     \
+    v.ramp = &KernLean<X, NS>::ramp;       \
     break;
     NR_W(1) NR_W(2) NR_W(3) NR_W(4) NR_W(5) NR_W(6) NR_W(7) NR_W(8)
 #undef NR_W

@@ -10,6 +10,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstring>
 
 #include "nr_device.h"
@@ -33,6 +34,17 @@ __host__ __device__ inline int mass_class(double m) {
 /* Per-wave counter slots. */
 enum { C_NODES = 0, C_PATHS, C_DEAD, C_TRUNC, C_OVER, C_FOLD, C_ERR, C_N = 8 };
 
+/* Most levels one k_ramp_levels launch runs. */
+constexpr int kRampMax = 24;
+
+/* Partial totals of one finalize_slots() workgroup, and the most such
+ * workgroups. */
+struct FinPart {
+  DD s[3]; /* R, ESP, D */
+  u64 n[C_N];
+};
+constexpr int kFinMax = 64;
+
 /* Control block of a level-order solve (device memory). */
 struct LevelCtl {
   u64 hist[kClasses]; /* class histogram of the pool (class rounds) */
@@ -43,14 +55,28 @@ struct LevelCtl {
   double sum[3];  /* R, ESP, D after k_finalize */
   double sum_lo[3]; /* their compensation terms (double-double low parts) */
   u64 start_tick;   /* wall clock (s_memrealtime) when the solve was seeded */
+  /* k_ramp_levels: items entering each of its levels over all workgroups
+   * (the last entry: the level it hands over), finished workgroups, and
+   * whether any workgroup's region overflowed */
+  u64 ramp_width[kRampMax + 1];
+  u32 ramp_done, ramp_over;
+  /* finalize_slots(): partials of its workgroups, finished workgroups, and
+   * whether this solve's totals are final */
+  FinPart fin[kFinMax];
+  u32 fin_ticket, fin_done;
 };
 
+
+/* Bytes of the control block the host reads back (not the partials). */
+constexpr size_t kCtlCopyBytes = offsetof(LevelCtl, fin);
 
 /* Words of the pinned host mirror: [0] last_items, [1] iterations,
  * [2] max_width (written by each level launch), then the final sums, their
  * low parts, the counters and the solve's sequence number (written by
- * k_finalize, the sequence number last). */
-constexpr int kMirSum = 3, kMirSumLo = 6, kMirCounts = 9, kMirSeq = 9 + C_N, kMirWords = 10 + C_N;
+ * k_finalize, the sequence number last), and the ramp's overflow flag
+ * (written by k_ramp_levels). */
+constexpr int kMirSum = 3, kMirSumLo = 6, kMirCounts = 9, kMirSeq = 9 + C_N,
+              kMirRamp = 10 + C_N, kMirWords = 11 + C_N;
 
 struct LevelArgs {
   DevGraph g;
@@ -96,6 +122,10 @@ struct LevelArgs {
   /* time budget in wall-clock ticks from ctl->start_tick (0: none): items a
    * wave reaches after it are carried to the output unexpanded */
   u64 budget_ticks;
+  /* level order: the first launch that finds the queue empty computes the
+   * totals (finalize_slots over n_seg slots), so no launch follows the last
+   * level but the empty ones already queued */
+  int fin_on_empty;
 };
 
 template <int W>
@@ -366,6 +396,116 @@ __device__ inline void block_flush(const LevelArgs& a, Acc2& accR, Acc2& accE, A
   if (__ballot(tot != 0) && lane < C_N) {
     u64* slot = &a.cnt[(u64)blockIdx.x * C_N + kc];
     *slot = (pre ? ((u64)c1 << 32 | c0) : *slot) + tot;
+  }
+}
+
+__device__ inline u64 wave_total_u64(u64 x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+/* The solve's totals from the per-workgroup slots, in an order fixed by the
+ * slot count alone (replaces the single-address FP64 CAS of main.cu:68-79 and
+ * the global counters of main.cu:58-63): workgroup w < n_fin = min(kFinMax,
+ * ceil(n_slots / 64)) takes slots w * 64 + lane (+ 64 n_fin, ...); part
+ * k < 3 (R, ESP, D) is added up lane by lane with error-free additions and
+ * then over the wave, part 3 (the counters) likewise, wave k of the
+ * workgroup taking parts k, k + waves, ...; the partials go to ctl->fin[w].
+ * The last of these workgroups to finish (ctl->fin_ticket) adds the partials
+ * up the same way into ctl->sum, sum_lo and counts and the host mirror, each
+ * wave fencing its mirror words before thread 0 stamps the sequence number,
+ * then sets ctl->fin_done and clears the ticket.  Whole workgroups of any
+ * multiple of 64 threads; s_last is one word of LDS. */
+__device__ inline void finalize_slots(const LevelArgs& a, u64 n_slots, u32* s_last) {
+  const int t = threadIdx.x, lane = t & 63, n_w = (int)(blockDim.x >> 6);
+  const u64 n_fin = min((u64)kFinMax, (n_slots + 63) / 64);
+  if (blockIdx.x >= n_fin) return;
+  LevelCtl* ctl = a.ctl;
+  FinPart& mine = ctl->fin[blockIdx.x];
+  for (int k = t >> 6; k < 4; k += n_w) {
+    if (k < 3) {
+      const DD* src = k == 0 ? a.partR : (k == 1 ? a.partE : a.partD);
+      double s = 0.0, c = 0.0;
+      for (u64 i = (u64)blockIdx.x * 64 + lane; i < n_slots; i += 64 * n_fin) {
+        const DD v = src[i];
+        double x, e;
+        two_sum(s, v.hi, x, e);
+        s = x;
+        c += v.lo + e;
+      }
+      wave_total_dd(s, c);
+      if (lane == 0) {
+        mine.s[k].hi = s;
+        mine.s[k].lo = c;
+      }
+    } else {
+      u64 n[C_N];
+#pragma unroll
+      for (int j = 0; j < C_N; ++j) n[j] = 0;
+      for (u64 i = (u64)blockIdx.x * 64 + lane; i < n_slots; i += 64 * n_fin)
+#pragma unroll
+        for (int j = 0; j < C_N; ++j) n[j] += a.cnt[i * C_N + j];
+#pragma unroll
+      for (int j = 0; j < C_N; ++j) n[j] = wave_total_u64(n[j]);
+      if (lane == 0)
+#pragma unroll
+        for (int j = 0; j < C_N; ++j) mine.n[j] = n[j];
+    }
+  }
+  __syncthreads();
+  if (t == 0) {
+    __threadfence();
+    *s_last = atomicAdd(&ctl->fin_ticket, 1u) == (u32)(n_fin - 1) ? 1u : 0u;
+  }
+  __syncthreads();
+  if (!*s_last) return;
+  __threadfence(); /* the other workgroups' partials */
+  u64* m = a.hmirror;
+  for (int k = t >> 6; k < 4; k += n_w) {
+    if (k < 3) {
+      double s = 0.0, c = 0.0;
+      if ((u64)lane < n_fin) {
+        s = ctl->fin[lane].s[k].hi;
+        c = ctl->fin[lane].s[k].lo;
+      }
+      wave_total_dd(s, c);
+      if (lane == 0) {
+        double hi, lo;
+        two_sum(s, c, hi, lo);
+        ctl->sum[k] = hi;
+        ctl->sum_lo[k] = lo;
+        if (m) {
+          m[kMirSum + k] = (u64)__double_as_longlong(hi);
+          m[kMirSumLo + k] = (u64)__double_as_longlong(lo);
+          __threadfence_system();
+        }
+      }
+    } else {
+      u64 n[C_N];
+#pragma unroll
+      for (int j = 0; j < C_N; ++j)
+        n[j] = wave_total_u64((u64)lane < n_fin ? ctl->fin[lane].n[j] : 0);
+      if (lane == 0) {
+#pragma unroll
+        for (int j = 0; j < C_N; ++j) {
+          ctl->counts[j] = n[j];
+          if (m) m[kMirCounts + j] = n[j];
+        }
+        if (m) __threadfence_system();
+      }
+    }
+  }
+  __syncthreads();
+  if (t == 0) {
+    ctl->fin_ticket = 0;
+    ctl->fin_done = 1;
+    if (m) {
+      m[1] = ctl->iterations;
+      m[2] = ctl->max_width;
+      __threadfence_system(); /* the result words before the stamp */
+      m[kMirSeq] = a.seq;
+    }
   }
 }
 
@@ -918,6 +1058,15 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
     if (mass_on)
       for (int i = threadIdx.x; i < kClasses; i += blockDim.x)
         a.hist[(u64)blockIdx.x * kClasses + i] = 0;
+    if (a.fin_on_empty && !mass_on && blockIdx.x < kFinMax) {
+      /* the queue is empty for good: every slot is final */
+      u32* s_flag = s_cnt;
+      if (threadIdx.x == 0) *s_flag = a.ctl->fin_done;
+      __syncthreads();
+      const bool done = *s_flag != 0;
+      __syncthreads();
+      if (!done) finalize_slots(a, (u64)a.n_seg, s_flag);
+    }
     return;
   }
   if (threadIdx.x == 0) *s_cnt = 0;
@@ -965,6 +1114,90 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
   }
 }
 
+/* One level of a workgroup's own items: items in_base .. in_base + n - 1 of
+ * `in` are expanded, one lane each, and their children appended to `out` at
+ * out_base + 0 .. out_cap - 1 in the order the waves reserve room (*s_cnt,
+ * zero on entry, counts every child, those past out_cap included: they go
+ * to ESP as overflowed, main.cu:245-252).  The whole workgroup calls it. */
+template <int W, class EX>
+__device__ inline void group_pass(const LevelArgs& a, const EX& ex, const QBuf& in, u64 in_base,
+                                  u32 n, const QBuf& out, u64 out_base, u32 out_cap, u32* s_cnt,
+                                  Acc2& accR, Acc2& accE, Acc2& accD, u32 (&c)[C_N]) {
+  const int lane = threadIdx.x & 63;
+  for (u32 base = 0; base < n; base += blockDim.x) {
+    const u32 i = base + threadIdx.x;
+    u64 known[W], up[W];
+    double mass = 0.0;
+    int k = 0;
+    bool live = false;
+    if (i < n) {
+      load_item<W>(in, a.stride, in_base + i, known, up, mass);
+      ex.load(in, a.stride, in_base + i); /* with the planes: one memory round */
+      ++c[C_NODES];
+      if (mass < a.eps) { /* main.cu:537-543 */
+        accE.add(mass);
+        ++c[C_TRUNC];
+      } else {
+        k = ex.find_item(known, up);
+        if (k >= 0) {
+          live = true;
+        } else {
+          if (k == -2) ++c[C_ERR];
+          accD.add(mass); /* main.cu:579-582 */
+          ++c[C_DEAD];
+          k = 0;
+        }
+      }
+    }
+    const int incl = wave_incl_scan(k);
+    const int total = __shfl(incl, 63, 64);
+    u32 wbase = 0;
+    if (total) {
+      if (lane == 63) wbase = atomicAdd(s_cnt, (u32)total);
+      wbase = __shfl(wbase, 63, 64);
+    }
+    if (EX::kRowBytes > 0) {
+      emit_children_wave<W, EX>(ex, out, a.stride, out_base, out_cap, wbase, incl, k, known, up,
+                                mass, accE, c);
+      if (live) {
+        accR.add(success_mass(ex, mass, k));
+        ++c[C_PATHS];
+      }
+    } else if (live) { /* main.cu:214-266 */
+      u32 off = wbase + (u32)(incl - k);
+      double run = mass;
+      u64 pre[W];
+#pragma unroll
+      for (int w = 0; w < W; ++w) pre[w] = 0;
+      for (int j = k - 1; j >= 0; --j, ++off) {
+        const int e = ex.uh[j * ex.stride];
+        const int wi = e >> 6;
+        const u64 bit = 1ull << (e & 63);
+        const double cm = run * ex.q[e];
+        if (off < out_cap) {
+          u64 ck[W], cu[W];
+#pragma unroll
+          for (int w = 0; w < W; ++w) {
+            ck[w] = known[w] | pre[w] | (w == wi ? bit : 0ull);
+            cu[w] = up[w] | pre[w];
+          }
+          store_item<W>(out, a.stride, out_base + off, ck, cu, cm);
+        } else { /* region full: main.cu:245-252 */
+          accE.add(cm);
+          ++c[C_PATHS];
+          ++c[C_OVER];
+        }
+        run = run * ex.p[e];
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+          if (w == wi) pre[w] |= bit;
+      }
+      accR.add(run); /* success term, main.cu:254-262 */
+      ++c[C_PATHS];
+    }
+  }
+}
+
 /* The first levels of a level-order solve (path rule) in one workgroup:
  * while a level holds at most max_in items (one lane each), the workgroup
  * expands it from segment 0 of one buffer into segment 0 of the other and
@@ -986,7 +1219,6 @@ __global__ void __launch_bounds__(256) k_small_levels(LevelArgs a, u32 max_in, i
   accE.init();
   accD.init();
   u32 c[C_N] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const int lane = threadIdx.x & 63;
   int p = 0;
   u32 n = a.seg_count[0][0];
   /* levels counted in registers (main.cu:970-973) and written once at the
@@ -997,80 +1229,8 @@ __global__ void __launch_bounds__(256) k_small_levels(LevelArgs a, u32 max_in, i
     levels += 1;
     widest = n > widest ? n : widest;
     __syncthreads();
-    const QBuf in = a.buf[p];
-    const QBuf out = a.buf[p ^ 1];
-    for (u32 base = 0; base < n; base += blockDim.x) {
-      const u32 i = base + threadIdx.x;
-      u64 known[W], up[W];
-      double mass = 0.0;
-      int k = 0;
-      bool live = false;
-      if (i < n) {
-        load_item<W>(in, a.stride, i, known, up, mass);
-        ex.load(in, a.stride, i); /* with the planes: one memory round */
-        ++c[C_NODES];
-        if (mass < a.eps) { /* main.cu:537-543 */
-          accE.add(mass);
-          ++c[C_TRUNC];
-        } else {
-          k = ex.find_item(known, up);
-          if (k >= 0) {
-            live = true;
-          } else {
-            if (k == -2) ++c[C_ERR];
-            accD.add(mass); /* main.cu:579-582 */
-            ++c[C_DEAD];
-            k = 0;
-          }
-        }
-      }
-      const int incl = wave_incl_scan(k);
-      const int total = __shfl(incl, 63, 64);
-      u32 wbase = 0;
-      if (total) {
-        if (lane == 63) wbase = atomicAdd(s_cnt, (u32)total);
-        wbase = __shfl(wbase, 63, 64);
-      }
-      if (EX::kRowBytes > 0) {
-        emit_children_wave<W, EX>(ex, out, a.stride, 0, a.seg_cap, wbase, incl, k, known, up,
-                                  mass, accE, c);
-        if (live) {
-          accR.add(success_mass(ex, mass, k));
-          ++c[C_PATHS];
-        }
-      } else if (live) { /* main.cu:214-266 */
-        u32 off = wbase + (u32)(incl - k);
-        double run = mass;
-        u64 pre[W];
-#pragma unroll
-        for (int w = 0; w < W; ++w) pre[w] = 0;
-        for (int j = k - 1; j >= 0; --j, ++off) {
-          const int e = ex.uh[j * ex.stride];
-          const int wi = e >> 6;
-          const u64 bit = 1ull << (e & 63);
-          const double cm = run * ex.q[e];
-          if (off < a.seg_cap) {
-            u64 ck[W], cu[W];
-#pragma unroll
-            for (int w = 0; w < W; ++w) {
-              ck[w] = known[w] | pre[w] | (w == wi ? bit : 0ull);
-              cu[w] = up[w] | pre[w];
-            }
-            store_item<W>(out, a.stride, off, ck, cu, cm);
-          } else { /* segment full: main.cu:245-252 */
-            accE.add(cm);
-            ++c[C_PATHS];
-            ++c[C_OVER];
-          }
-          run = run * ex.p[e];
-#pragma unroll
-          for (int w = 0; w < W; ++w)
-            if (w == wi) pre[w] |= bit;
-        }
-        accR.add(run); /* success term, main.cu:254-262 */
-        ++c[C_PATHS];
-      }
-    }
+    group_pass<W, EX>(a, ex, a.buf[p], 0, n, a.buf[p ^ 1], 0, a.seg_cap, s_cnt, accR, accE, accD,
+                      c);
     __syncthreads();
     const u32 made = *s_cnt;
     n = made < a.seg_cap ? made : a.seg_cap;
@@ -1098,6 +1258,114 @@ __global__ void __launch_bounds__(256) k_small_levels(LevelArgs a, u32 max_in, i
     if (widest > a.ctl->max_width) a.ctl->max_width = widest;
   }
   block_flush(a, accR, accE, accD, c, (unsigned char*)part);
+}
+
+/* Levels of the ramp after k_small_levels: levels that are still too narrow
+ * to fill the chip, for which a launch per level would cost mostly its fixed
+ * costs.  The level k_small_levels left in buffer 0, segment 0 (n0 items,
+ * after `iterations` levels) is dealt out in contiguous slices, one per
+ * workgroup; each workgroup then expands its slice's subtrees level after
+ * level on its own (group_pass), in a region of `span` segments starting at
+ * segment 1 + span * blockIdx.x, alternating between the buffers, with no
+ * launch or grid-wide step in between.  Every workgroup runs the same number
+ * of levels L: the largest even number for which n0 g^L stays near
+ * `target` items, g the mean growth per level so far (n0^(1/iterations)),
+ * so the last level ends in buffer 0.  The items entering each level are
+ * added up over the workgroups (ctl->ramp_width); the last workgroup to
+ * finish counts those levels and the widest of them into the control block
+ * as the level launches would (main.cu:970-973), and sets the segment counts
+ * that it alone may write.  A workgroup whose region fills sets
+ * ctl->ramp_over (and the mirror's ramp word): its later children went to
+ * ESP, so the caller solves again without the ramp.  Workgroups never wait
+ * for each other. */
+template <int W, class EX>
+__global__ void __launch_bounds__(256) k_ramp_levels(LevelArgs a, int span, u32 target) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const u32 n0 = a.seg_count[0][0];
+  const u64 done_levels = a.ctl->iterations;
+  if (n0 < 2 || done_levels == 0) return;
+  /* the same L in every workgroup: computed from the same words */
+  const double g = exp2(log2((double)n0) / (double)done_levels);
+  const double want = g > 1.0 ? log2((double)target / (double)n0) / log2(g) : (double)kRampMax;
+  int L = want > 0.0 ? 2 * (int)(0.5 * want + 0.5) : 0;
+  if (L > kRampMax) L = kRampMax;
+  if (L < 2) return;
+  u32* s_cnt = (u32*)(smem + a.lay.off_extra);
+  u32* part = s_cnt + 4;          /* block_flush scratch: 80 bytes per wave */
+  u32* s_width = s_cnt + 4 + 84;  /* [kRampMax + 1] items entering each level */
+  const u32 lo = (u32)((u64)blockIdx.x * n0 / gridDim.x);
+  const u32 hi = (u32)((u64)(blockIdx.x + 1) * n0 / gridDim.x);
+  const u64 region = (u64)(1 + (u64)span * blockIdx.x) * a.seg_cap;
+  const u32 region_cap = (u32)span * a.seg_cap;
+  u32 n = hi - lo;
+  bool over = false;
+  if (n > 0) {
+    const u32 pre = prefetch_slot(a);
+    if (EX::kStage) stage_tables(a.g, a.lay, smem);
+    EX ex;
+    ex.bind_graph(smem, a.lay, a.g, threadIdx.x);
+    Acc2 accR, accE, accD;
+    accR.init();
+    accE.init();
+    accD.init();
+    u32 c[C_N] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int lv = 0; lv < L; ++lv) {
+      if (threadIdx.x == 0) {
+        *s_cnt = 0;
+        s_width[lv] = n;
+      }
+      __syncthreads();
+      if (n) {
+        const QBuf& in = a.buf[lv & 1];
+        group_pass<W, EX>(a, ex, in, lv == 0 ? (u64)lo : region, n, a.buf[(lv + 1) & 1], region,
+                          region_cap, s_cnt, accR, accE, accD, c);
+      }
+      __syncthreads();
+      const u32 made = *s_cnt;
+      over = over || made > region_cap;
+      n = made < region_cap ? made : region_cap;
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) s_width[L] = n;
+    block_flush(a, accR, accE, accD, c, (unsigned char*)part, &pre);
+    /* the widths go out once, after the levels (block_flush's barrier
+     * orders thread 0's last LDS word before these reads) */
+    if ((int)threadIdx.x <= L) {
+      const u32 w = s_width[threadIdx.x];
+      if (w) atomicAdd(&a.ctl->ramp_width[threadIdx.x], (u64)w);
+      if (threadIdx.x == 0 && over) atomicOr(&a.ctl->ramp_over, 1u);
+      __threadfence();
+    }
+    __syncthreads();
+  }
+  /* segment counts of the region (its items are contiguous from its start) */
+  for (int i = threadIdx.x; i < span; i += blockDim.x) {
+    const u64 before = (u64)i * a.seg_cap;
+    const u64 left = n > before ? n - before : 0;
+    a.seg_count[0][1 + (u64)span * blockIdx.x + i] = (u32)(left < a.seg_cap ? left : a.seg_cap);
+  }
+  if (threadIdx.x != 0) return;
+  __threadfence();
+  if (atomicAdd(&a.ctl->ramp_done, 1u) != gridDim.x - 1) return;
+  /* the last workgroup: every other one has read n0 and added its widths */
+  __threadfence();
+  u64 it = a.ctl->iterations, mw = a.ctl->max_width;
+  for (int lv = 0; lv < L; ++lv) {
+    const u64 w = __hip_atomic_load(&a.ctl->ramp_width[lv], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+    if (w) {
+      it += 1;
+      if (w > mw) mw = w;
+    }
+  }
+  const u32 ov = __hip_atomic_load(&a.ctl->ramp_over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  a.ctl->iterations = it;
+  a.ctl->max_width = mw;
+  a.ctl->last_items =
+      __hip_atomic_load(&a.ctl->ramp_width[L], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  a.seg_count[0][0] = 0;
+  for (u64 s = 1 + (u64)span * gridDim.x; s < (u64)a.n_seg; ++s) a.seg_count[0][s] = 0;
+  if (a.hmirror) a.hmirror[kMirRamp] = ov;
 }
 
 /* One expansion of each caller-supplied item into fixed output slots (the
@@ -1181,6 +1449,7 @@ struct Variant {
   KernelGetter level_mass;             /* MODE 4 (max-mass class rounds) */
   KernelGetter small;                  /* k_small_levels of this form */
   int choose;                          /* the learned chooser picks the sweep */
+  KernelGetter ramp;                   /* k_ramp_levels of this form */
 };
 
 /* The alternating rule with the learned sweep choice (ChooseAltExpander, or
