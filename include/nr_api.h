@@ -143,6 +143,11 @@ typedef enum nr_rule {
                                    results) */
 #define NR_FLAG_LEAN 8          /* items never carry rows (NR_FLAG_ROWS wins when
                                    both are set); same results */
+#define NR_FLAG_QUEUE_ALL 16    /* level order of the path rule on a row-form
+                                   graph: queue every child, as the reference
+                                   does, instead of counting the dead ones and
+                                   those below the truncation when they are
+                                   created; same results */
 
 typedef struct nr_result {
   double R;                     /* reliability lower bound (exact when ESP=0) */

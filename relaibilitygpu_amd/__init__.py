@@ -26,6 +26,7 @@ CLI_PATH = os.path.join(_HERE, "bin", "network_reliability")
 NR_FLAG_TIME_LAUNCHES = 1
 NR_FLAG_ROWS = 4
 NR_FLAG_LEAN = 8
+NR_FLAG_QUEUE_ALL = 16
 
 NR_ORDER_LEVEL = 0
 NR_ORDER_MASS = 1
@@ -239,7 +240,7 @@ def make_config(queue_capacity: int = 1 << 20, truncation_eps: float = 1e-15,
                 time_launches: bool = False, rule: str | int = "path",
                 rows: bool = False, sweep_period: int = 0, lean: bool = False,
                 prefix_edges: int = 0, shard: int = 0, shards: int = 0,
-                ramp_items: int = 0) -> Config:
+                ramp_items: int = 0, queue_all: bool = False) -> Config:
     c = Config()
     lib().nr_config_init(ctypes.byref(c))
     c.queue_capacity = int(queue_capacity)
@@ -251,7 +252,7 @@ def make_config(queue_capacity: int = 1 << 20, truncation_eps: float = 1e-15,
     c.grid_blocks = int(grid_blocks)
     c.chunk_items = int(chunk_items)
     c.flags = ((NR_FLAG_TIME_LAUNCHES if time_launches else 0) | (NR_FLAG_ROWS if rows else 0)
-               | (NR_FLAG_LEAN if lean else 0))
+               | (NR_FLAG_LEAN if lean else 0) | (NR_FLAG_QUEUE_ALL if queue_all else 0))
     c.rule = RULES[rule] if isinstance(rule, str) else int(rule)
     c.sweep_period = int(sweep_period)
     c.prefix_edges = int(prefix_edges)

@@ -246,6 +246,10 @@ struct Expander {
   Pow2 stride;
   int n, src, dst;
   mutable int plen; /* hops of the last path found */
+  /* find<true>(): bit h set when hop h (BFS level h-1 to h) is UNKNOWN, and
+   * the path's nodes */
+  mutable u64 hopm;
+  mutable NS pathm;
   const u16* terms;
   int nterms;
   NS tmask;
@@ -273,6 +277,8 @@ struct Expander {
   static constexpr bool kStage = true; /* graph tables staged in LDS */
   static constexpr int kRowBytes = (int)sizeof(NS);
   static constexpr bool kChoose = false; /* sweep direction picked by the learned chooser */
+  /* dead children found at creation (find_item_marked, dead_children) */
+  static constexpr bool kSettle = true;
   __device__ void bind_graph(unsigned char* smem, const Layout& L, const DevGraph& g, int slot) {
     bind(smem, L, slot, g.src, g.dst);
     terms = g.terms;
@@ -305,6 +311,66 @@ struct Expander {
   }
   __device__ int find_item(const u64 (&known)[W], const u64 (&)[W]) const {
     return nterms >= 2 ? find_tree(known) : find(known);
+  }
+  /* find() for two terminals that leaves the path in the level sets for
+   * dead_children() */
+  __device__ int find_item_marked(const u64 (&known)[W], const u64 (&)[W]) const {
+    return find<true>(known);
+  }
+
+  /* The children of the item find<true>() just walked whose FAILED hop is a
+   * bridge between src and dst in the item's allowed graph (edges not
+   * FAILED): such a child has no src-dst path left and is dead when it is
+   * dequeued (main.cu:579-582); every other child has one.  Hop h joins
+   * v(h-1) and v(h), v(0) = src, v(plen) = dst, and find<true>() left the
+   * node set {v(h)} in lvl[h].  R is grown as the set reached from
+   * v(0..h-1) without crossing hop h; hop h is a bridge exactly when R meets
+   * no node of v(h..plen).  Only the row of v(h-1) loses hop h (no other row
+   * holds it while R misses the suffix), R only grows with h so each node
+   * is expanded once, a WORKING hop needs no test, and once R holds dst no
+   * later hop is a bridge.  Bit j: the j-th UNKNOWN hop from the src side
+   * (child j). */
+  __device__ u64 dead_children() const {
+    const NS dbit = (NS)1 << dst;
+    NS R = 0, F = 0, pre = 0;
+    u64 dead = 0, todo = hopm;
+    int j = 0;
+    NS vp = lvl[0];
+    for (int h = 1; todo; ++h) {
+      const NS vh = h < plen ? lvl[h * stride] : dbit;
+      pre |= vp;
+      if (!(R & vp)) {
+        R |= vp;
+        F |= vp;
+      }
+      if ((todo >> h) & 1ull) {
+        todo &= todo - 1; /* bits below h are clear */
+        const NS suf = pathm & ~pre;
+        const int up = ctzs<NS>(vp);
+        auto rw = [&](int u) {
+          const NS r = rows[u * stride];
+          return u == up ? r & ~vh : r;
+        };
+        while (F && !(R & suf)) {
+          const int u0 = ctzs<NS>(F);
+          F &= F - 1;
+          const int u1 = F ? ctzs<NS>(F) : u0;
+          F &= F - 1;
+          const int u2 = F ? ctzs<NS>(F) : u0;
+          F &= F - 1;
+          const int u3 = F ? ctzs<NS>(F) : u0;
+          F &= F - 1;
+          const NS x = ((rw(u0) | rw(u1)) | (rw(u2) | rw(u3))) & ~R;
+          R |= x;
+          F |= x;
+        }
+        if (!(R & suf)) dead |= 1ull << j;
+        ++j;
+        if (R & dbit) break;
+      }
+      vp = vh;
+    }
+    return dead;
   }
 
   /* K-terminal form of find(): BFS from terms[0], whole levels, until every
@@ -544,7 +610,12 @@ struct Expander {
    * allowed edge.  Pushes the path's UNKNOWN hops on uh[] in dst->src order.
    * Returns the number of UNKNOWN hops, -1 if dst is unreachable, -2 on an
    * internal inconsistency (never expected). */
+  template <bool kMark = false>
   __device__ int find(const u64 (&known)[W]) const {
+    if (kMark) {
+      hopm = 0;
+      pathm = (NS)1 << dst;
+    }
     if (src == dst) {
       plen = 0;
       return 0;
@@ -587,7 +658,14 @@ struct Expander {
       if (!cand) return -2;
       const int u = ctzs<NS>(cand);
       const int e = eidm[u * n + v];
-      if (!((pick<W>(known, e >> 6) >> (e & 63)) & 1ull)) uh[(k++) * stride] = (u16)e;
+      if (!((pick<W>(known, e >> 6) >> (e & 63)) & 1ull)) {
+        uh[(k++) * stride] = (u16)e;
+        if (kMark) hopm |= 1ull << l;
+      }
+      if (kMark) { /* the level set is spent: it keeps the path's node */
+        lvl[(l - 1) * stride] = (NS)1 << u;
+        pathm |= (NS)1 << u;
+      }
       v = u;
     }
     return k;
@@ -777,6 +855,7 @@ struct CsrExpander {
   static constexpr bool kStage = false;
   static constexpr int kRowBytes = 0;
   static constexpr bool kChoose = false;
+  static constexpr bool kSettle = false;
   const double* p;
   const double* q;
   const int* rp;
@@ -1357,6 +1436,7 @@ template <int W, typename NS>
 struct LeanExpander : Expander<W, NS> {
   static constexpr bool kStage = true;
   static constexpr int kRowBytes = 0;
+  static constexpr bool kSettle = false;
   unsigned char* lvb;
   __device__ void bind_graph(unsigned char* smem, const Layout& L, const DevGraph& g, int slot) {
     Expander<W, NS>::bind_graph(smem, L, g, slot);

@@ -128,6 +128,10 @@ __global__ void __launch_bounds__(256) k_init(LevelArgs a, Root root, int W, int
     a.seg_count[0][s] =
         seed ? (u32)(seed->count / ns + (s < seed->count % ns ? 1 : 0)) : (s == 0 ? 1u : 0u);
     a.seg_count[1][s] = 0u;
+    if (a.seg_settle[0]) {
+      a.seg_settle[0][s] = 0u;
+      a.seg_settle[1][s] = 0u;
+    }
   }
   if (seed) { /* a resumed solve: sums and counters so far in slot 0 */
     if (tid == 0) {
@@ -983,6 +987,7 @@ Variant variant_for_w(int W) {
               &Kern<X, NS>::seed_rows, 0, nullptr};                                       \
     v.small = &Kern<X, NS>::small;                                                        \
     v.ramp = &Kern<X, NS>::ramp;                                                          \
+    v.settle = 1;                                                                         \
     return v;                                                                             \
   }
     NR_W(1) NR_W(2) NR_W(3) NR_W(4) NR_W(5) NR_W(6) NR_W(7) NR_W(8)
@@ -1409,7 +1414,7 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   if (!getter) return NR_ERR_UNSUPPORTED;
   const void* kfun = (const void*)getter();
   int block =
-      choose_block(var, g, W, cfg->block_size, 4 * (1026 + 256 + kClasses) + tile_bytes);
+      choose_block(var, g, W, cfg->block_size, 4 * (1027 + 256 + kClasses) + tile_bytes);
   if (block <= 0) return NR_ERR_UNSUPPORTED;
   Layout lay0 = layout_for(var, g, W, block, 4 * (block + kClasses) + tile_bytes);
   if ((rc = ws.allow_lds(kfun, 64 * 1024))) return rc;
@@ -1417,7 +1422,7 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   if ((rc = ws.blocks_per_cu(kfun, block, lay0.total_bytes, &per_cu0))) return rc;
   if (per_cu0 < 1) return NR_ERR_UNSUPPORTED;
   int grid = cfg->grid_blocks > 0 ? cfg->grid_blocks : per_cu0 * cus;
-  const int extra = 4 * (grid + 2 + block + kClasses) + tile_bytes;
+  const int extra = 4 * (grid + 3 + block + kClasses) + tile_bytes;
   Layout lay = layout_for(var, g, W, block, extra);
   if (lay.total_bytes > 64 * 1024) return NR_ERR_UNSUPPORTED;
   int per_cu = 0;
@@ -1503,9 +1508,10 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
     a.chooser = x->chooser;
     a.log_choice = x->log_choice;
   }
-  /* per-workgroup slots: 3 DD + C_N counters; segment counts */
+  /* per-workgroup slots: 3 DD + C_N counters; segment counts and settled
+   * children per buffer; class histograms */
   const size_t meta_bytes = n_slots * (3 * sizeof(DD) + C_N * sizeof(u64)) +
-                            2 * sizeof(u32) * (2 * (size_t)grid + 4) +
+                            sizeof(u32) * (4 * (size_t)grid + 8) +
                             sizeof(u32) * (size_t)grid * kClasses;
   if ((rc = ws.grow(&ws.meta, &ws.meta_bytes, meta_bytes))) return rc;
   {
@@ -1517,7 +1523,9 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
     u32* q = (u32*)(a.cnt + n_slots * C_N);
     a.seg_count[0] = q;
     a.seg_count[1] = q + grid;
-    a.hist = q + 2 * grid + 8;
+    a.seg_settle[0] = q + 2 * grid;
+    a.seg_settle[1] = q + 3 * grid;
+    a.hist = q + 4 * grid + 8;
   }
   a.ctl = ws.ctl;
   a.hmirror = ws.dmirror; /* null in no-allocation mode */
@@ -1532,6 +1540,15 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   a.thr_class = kClasses;
   a.drop_class = kClasses;
   a.fin_on_empty = (!mass && ws.hmirror) ? 1 : 0;
+  /* level order of the plain path rule in the row form, two terminals, no
+   * budget, checkpoint or seeds: dead children (a bridge hop) and children
+   * below the truncation are counted when created, not queued; the counts,
+   * levels and widest level are those of queueing them */
+  a.settle = (var.settle && mode == 0 && !mass && cfg->rule == NR_RULE_PATH && g->terms.empty() &&
+              g->simple && cfg->time_budget_s <= 0 && !(x && (x->save || x->resume)) &&
+              cfg->prefix_edges <= 0 && !(cfg->flags & NR_FLAG_QUEUE_ALL))
+                 ? 1
+                 : 0;
 
   hipStream_t st = ws.stream;
   const nr_checkpoint* ck = x ? x->resume : nullptr;

@@ -176,6 +176,13 @@ int finish_graph(nr_graph* g) {
     if (g->ea[id] < 0) return NR_ERR_RANGE; /* edge id never used */
     if (g->dirmask[id] != 3) g->symmetric = false;
   }
+  {
+    std::vector<uint64_t> pairs((size_t)g->e);
+    for (int id = 0; id < g->e; ++id)
+      pairs[id] = (uint64_t)(uint32_t)g->ea[id] << 32 | (uint32_t)g->eb[id];
+    std::sort(pairs.begin(), pairs.end());
+    g->simple = std::adjacent_find(pairs.begin(), pairs.end()) == pairs.end();
+  }
   g->q.resize(g->e);
   g->precontracted = g->predeleted = 0;
   for (int id = 0; id < g->e; ++id) {

@@ -26,6 +26,7 @@ struct nr_graph {
   std::vector<int> ea, eb;      /* endpoints of each edge id, ea <= eb */
   std::vector<uint8_t> dirmask; /* bit0: ea->eb in CSR, bit1: eb->ea */
   bool symmetric = true;
+  bool simple = true; /* no two edge ids join the same pair of nodes */
   int precontracted = 0, predeleted = 0;
   std::vector<int> terms; /* K-terminal set (empty: the src/dst pair) */
   /* unique per state: new at creation and at every change of terminals

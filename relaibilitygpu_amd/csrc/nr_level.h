@@ -83,6 +83,13 @@ struct LevelArgs {
   Layout lay;
   QBuf buf[2];
   u32* seg_count[2]; /* [n_seg] items written by each workgroup */
+  /* [n_seg] children each workgroup settled at creation (dead, or below
+   * the truncation) instead of writing them: they belong to the width of
+   * the level the buffer holds (read when `settle` is set) */
+  u32* seg_settle[2];
+  /* level order, row form, two terminals: children that are dead or below
+   * the truncation are counted when they are created and never queued */
+  int settle;
   u32* hist;         /* [n_seg][kClasses] class histogram of each segment */
   LevelCtl* ctl;
   /* pinned host mirror (kMir* words) of the counters the host reads at a
@@ -528,11 +535,17 @@ __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+/* `k` is the lane's number of stored children, `hops` its UNKNOWN hops on
+ * uh[] and bit j of `lmask` marks child j (src side first) as stored; when
+ * `sel` is false every child is stored (k == hops) and lmask is unused.
+ * The r-th stored child of an owner is its child j for the r-th set bit j
+ * of its mask. */
 template <int W, class EX>
 __device__ inline void emit_children_wave(const EX& ex, const QBuf& out, u64 qstride,
                                           u64 out_base, u32 seg_cap, u32 wbase, int incl, int k,
-                                          const u64 (&known)[W], const u64 (&up)[W], double mass,
-                                          Acc2& accE, u32 (&c)[C_N]) {
+                                          int hops, u64 lmask, bool sel, const u64 (&known)[W],
+                                          const u64 (&up)[W], double mass, Acc2& accE,
+                                          u32 (&c)[C_N]) {
   const int lane = threadIdx.x & 63;
   const int total = __shfl(incl, 63, 64);
   wave_lds_sync(); /* the owners' hop stacks and rows, stored by their lanes */
@@ -550,7 +563,9 @@ __device__ inline void emit_children_wave(const EX& ex, const QBuf& out, u64 qst
         lo = mid + 1;
     }
     const int o = lo;
-    const int ko = __shfl(k, o, 64);
+    const int so = __shfl(k, o, 64);
+    const int ko = sel ? __shfl(hops, o, 64) : so;
+    const u64 lmo = sel ? __shfl(lmask, o, 64) : 0ull;
     const int io = __shfl(incl, o, 64);
     u64 ok[W], ou[W];
 #pragma unroll
@@ -560,7 +575,12 @@ __device__ inline void emit_children_wave(const EX& ex, const QBuf& out, u64 qst
     }
     const double om = __shfl(mass, o, 64);
     if (cc < total) {
-      const int j = cc - (io - ko); /* child index from the src side */
+      int j = cc - (io - so); /* child index from the src side */
+      if (sel) {
+        u64 m = lmo;
+        for (int r = 0; r < j; ++r) m &= m - 1;
+        j = __builtin_ctzll(m);
+      }
       const int delta = o - lane;
       const u16* uho = ex.uh + delta;
       double run = om;
@@ -604,6 +624,40 @@ __device__ inline double success_mass(const EX& ex, double mass, int k) {
   double run = mass;
   for (int j = k - 1; j >= 0; --j) run = run * ex.p[ex.uh[j * ex.stride]];
   return run;
+}
+
+/* The children of a live item found by find_item_marked(), src side first:
+ * those below the truncation (main.cu:537-543) and the dead ones
+ * (dead_children(), main.cu:579-582) are counted now, as the processed
+ * nodes they would be when dequeued, instead of being queued; bit j of the
+ * result marks child j as queued.  `succ` gets the success mass, the same
+ * product as success_mass(). */
+template <class EX>
+__device__ inline u64 settle_children(const EX& ex, double mass, int k, double eps, Acc2& accE,
+                                      Acc2& accD, u32 (&c)[C_N], u32& settled, double& succ) {
+  const u64 dead = ex.dead_children();
+  u64 keep = 0;
+  double run = mass;
+  for (int j = 0; j < k; ++j) {
+    const int e = ex.uh[(k - 1 - j) * ex.stride];
+    const double cm = run * ex.q[e];
+    if (cm < eps) {
+      accE.add(cm);
+      ++c[C_NODES];
+      ++c[C_TRUNC];
+      ++settled;
+    } else if ((dead >> j) & 1ull) {
+      accD.add(cm);
+      ++c[C_NODES];
+      ++c[C_DEAD];
+      ++settled;
+    } else {
+      keep |= 1ull << j;
+    }
+    run = run * ex.p[e];
+  }
+  succ = run;
+  return keep;
 }
 
 /* Parent context of an item's children for the feature function. */
@@ -710,8 +764,10 @@ __device__ __forceinline__ void expand_level_items(const LevelArgs& a, const EX&
                                                    u32* s_cnt, u32* s_hist, u16* tile, u64 t_end,
                                                    const bf16x8_t (&afrag)[2], u32 sign, float b2,
                                                    float alpha, Acc2& accR, Acc2& accE,
-                                                   Acc2& accD, u32 (&c)[C_N]) {
+                                                   Acc2& accD, u32 (&c)[C_N], u32& settled) {
   const bool mass_on = MODE == 4 || ((MODE == 2 || MODE == 3) && a.mass_mode);
+  /* dead and truncated children counted at creation (row form only) */
+  const bool settle = MODE == 0 && EX::kSettle && a.settle;
   const QBuf in = a.buf[parity];
   const QBuf out = a.buf[parity ^ 1];
   const int lane = threadIdx.x & 63;
@@ -732,6 +788,8 @@ __device__ __forceinline__ void expand_level_items(const LevelArgs& a, const EX&
     u64 slot_in = 0;
     u32 tag_in = 0xffffffffu;
     int fork_e = 0;
+    u64 lmask = 0;     /* settle: the children that are queued */
+    double succ = 0.0; /* settle: the success mass */
     /* past the time budget: the wave's items stay pending (carried) */
     const bool late = t_end && wall_clock64() >= t_end;
     const int s_base = a.n_seg <= 4096 ? wave_segment_of(sp, a.n_seg, (u32)base)
@@ -779,10 +837,19 @@ __device__ __forceinline__ void expand_level_items(const LevelArgs& a, const EX&
           }
         } else {
           if (MODE != 0) ex.load(in, a.stride, slot_in);
-          k = ex.find_item(known, up);
+          if constexpr (MODE == 0 && EX::kSettle)
+            k = settle ? ex.find_item_marked(known, up) : ex.find_item(known, up);
+          else
+            k = ex.find_item(known, up);
           if (k >= 0) {
             live = true;
             stored = k;
+            if constexpr (MODE == 0 && EX::kSettle) {
+              if (settle) {
+                lmask = settle_children(ex, mass, k, a.eps, accE, accD, c, settled, succ);
+                stored = __popcll(lmask);
+              }
+            }
             if (mass_on && !EX::kChoose) { /* children below eps are not queued */
               stored = 0;
               double run = mass;
@@ -948,10 +1015,10 @@ __device__ __forceinline__ void expand_level_items(const LevelArgs& a, const EX&
     } else if (MODE == 0 && EX::kRowBytes > 0) { /* children with rows: child-major */
       /* `late` is wave-uniform: a late wave only carries, and has no children */
       if (!late)
-        emit_children_wave<W, EX>(ex, out, a.stride, out_base, a.seg_cap, wbase, incl, stored,
-                                  known, up, mass, accE, c);
+        emit_children_wave<W, EX>(ex, out, a.stride, out_base, a.seg_cap, wbase, incl, stored, k,
+                                  lmask, settle, known, up, mass, accE, c);
       if (live) {
-        accR.add(success_mass(ex, mass, k));
+        accR.add(settle ? succ : success_mass(ex, mass, k));
         ++c[C_PATHS];
       }
     } else if (live) {
@@ -1032,18 +1099,40 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
   u32* s_cnt = sp + a.n_seg + 1;
   u32* part = s_cnt + 1;
   const u32 pre = prefetch_slot(a);
+  /* workgroup 0 adds up the children settled at this level's creation
+   * (level width, below): 16 loads per lane in flight together */
+  const bool count_settled = a.settle && blockIdx.x == 0 && threadIdx.x < 64;
+  constexpr int kPend = 16;
+  u32 pv[kPend];
+  if (count_settled) {
+#pragma unroll
+    for (int j = 0; j < kPend; ++j) {
+      const int s = (int)threadIdx.x + 64 * j;
+      pv[j] = s < a.n_seg ? a.seg_settle[parity][s] : 0u;
+    }
+  }
   /* the graph tables are staged while the segment counts are in flight */
   block_prefix(a.seg_count[parity], a.n_seg, sp, part, [&] {
     if (EX::kStage) stage_tables(a.g, a.lay, smem);
   });
   const u32 n_items = sp[a.n_seg];
+  u64 pend = 0;
+  if (count_settled) { /* wave 0 of workgroup 0, every lane */
+    u32 t = 0;
+#pragma unroll
+    for (int j = 0; j < kPend; ++j) t += pv[j];
+    for (int s = (int)threadIdx.x + 64 * kPend; s < a.n_seg; s += 64) t += a.seg_settle[parity][s];
+    pend = wave_total_u32(t);
+  }
   const u64 t_end = a.budget_ticks ? a.ctl->start_tick + a.budget_ticks : 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) { /* count the level (main.cu:970-973) */
     a.ctl->last_items = n_items;
     u64 it = a.ctl->iterations, mw = a.ctl->max_width;
-    if (n_items && !(t_end && wall_clock64() >= t_end)) {
+    /* the level's items: those queued and those settled when created */
+    const u64 width = (u64)n_items + pend;
+    if (width && !(t_end && wall_clock64() >= t_end)) {
       it += 1;
-      if (n_items > mw) mw = n_items;
+      if (width > mw) mw = width;
       a.ctl->iterations = it;
       a.ctl->max_width = mw;
     }
@@ -1054,7 +1143,10 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
     }
   }
   if (n_items == 0) {
-    if (threadIdx.x == 0) a.seg_count[parity ^ 1][blockIdx.x] = 0;
+    if (threadIdx.x == 0) {
+      a.seg_count[parity ^ 1][blockIdx.x] = 0;
+      if (a.settle) a.seg_settle[parity ^ 1][blockIdx.x] = 0;
+    }
     if (mass_on)
       for (int i = threadIdx.x; i < kClasses; i += blockDim.x)
         a.hist[(u64)blockIdx.x * kClasses + i] = 0;
@@ -1069,10 +1161,14 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
     }
     return;
   }
-  if (threadIdx.x == 0) *s_cnt = 0;
   u32* s_hist = part + blockDim.x; /* [kClasses], class rounds only */
+  u32* s_set = s_hist + kClasses;  /* children settled at creation */
+  if (threadIdx.x == 0) {
+    *s_cnt = 0;
+    *s_set = 0;
+  }
   /* per-wave feature tile [64][kFeat] bf16 (learned order), 16-byte aligned */
-  u16* tile = (u16*)(((size_t)(s_hist + kClasses) + 15) & ~(size_t)15) +
+  u16* tile = (u16*)(((size_t)(s_set + 1) + 15) & ~(size_t)15) +
               (threadIdx.x >> 6) * 64 * kFeat;
   if (mass_on)
     for (int i = threadIdx.x; i < kClasses; i += blockDim.x) s_hist[i] = 0;
@@ -1098,9 +1194,14 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
   accE.init();
   accD.init();
   u32 c[C_N] = {0, 0, 0, 0, 0, 0, 0, 0};
+  u32 settled = 0;
 
   expand_level_items<W, EX, MODE>(a, ex, parity, level, sp, n_items, s_cnt, s_hist, tile, t_end,
-                                  afrag, sign, b2, alpha, accR, accE, accD, c);
+                                  afrag, sign, b2, alpha, accR, accE, accD, c, settled);
+  if (MODE == 0 && EX::kSettle && a.settle) {
+    const u32 t = wave_total_u32(settled);
+    if ((threadIdx.x & 63) == 0 && t) atomicAdd(s_set, t);
+  }
 
   /* per-workgroup flush; then publish this workgroup's segment count */
   __syncthreads();
@@ -1111,25 +1212,32 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
   if (threadIdx.x == 0) {
     const u32 n = *s_cnt;
     a.seg_count[parity ^ 1][blockIdx.x] = n < a.seg_cap ? n : a.seg_cap;
+    if (a.settle) a.seg_settle[parity ^ 1][blockIdx.x] = *s_set;
   }
 }
 
 /* One level of a workgroup's own items: items in_base .. in_base + n - 1 of
  * `in` are expanded, one lane each, and their children appended to `out` at
- * out_base + 0 .. out_cap - 1 in the order the waves reserve room (*s_cnt,
- * zero on entry, counts every child, those past out_cap included: they go
- * to ESP as overflowed, main.cu:245-252).  The whole workgroup calls it. */
+ * out_base + 0 .. out_cap - 1 in the order the waves reserve room (s_cnt[0],
+ * zero on entry, counts every queued child, those past out_cap included:
+ * they go to ESP as overflowed, main.cu:245-252); children settled at
+ * creation (a.settle) are added to s_cnt[1].  The whole workgroup calls
+ * it. */
 template <int W, class EX>
 __device__ inline void group_pass(const LevelArgs& a, const EX& ex, const QBuf& in, u64 in_base,
                                   u32 n, const QBuf& out, u64 out_base, u32 out_cap, u32* s_cnt,
                                   Acc2& accR, Acc2& accE, Acc2& accD, u32 (&c)[C_N]) {
   const int lane = threadIdx.x & 63;
+  const bool settle = EX::kSettle && a.settle;
+  u32 settled = 0;
   for (u32 base = 0; base < n; base += blockDim.x) {
     const u32 i = base + threadIdx.x;
     u64 known[W], up[W];
     double mass = 0.0;
-    int k = 0;
+    int k = 0, hops = 0; /* children queued; UNKNOWN hops */
     bool live = false;
+    u64 lmask = 0;
+    double succ = 0.0;
     if (i < n) {
       load_item<W>(in, a.stride, in_base + i, known, up, mass);
       ex.load(in, a.stride, in_base + i); /* with the planes: one memory round */
@@ -1138,9 +1246,19 @@ __device__ inline void group_pass(const LevelArgs& a, const EX& ex, const QBuf& 
         accE.add(mass);
         ++c[C_TRUNC];
       } else {
-        k = ex.find_item(known, up);
+        if constexpr (EX::kSettle)
+          k = settle ? ex.find_item_marked(known, up) : ex.find_item(known, up);
+        else
+          k = ex.find_item(known, up);
         if (k >= 0) {
           live = true;
+          hops = k;
+          if constexpr (EX::kSettle) {
+            if (settle) {
+              lmask = settle_children(ex, mass, hops, a.eps, accE, accD, c, settled, succ);
+              k = __popcll(lmask);
+            }
+          }
         } else {
           if (k == -2) ++c[C_ERR];
           accD.add(mass); /* main.cu:579-582 */
@@ -1157,10 +1275,10 @@ __device__ inline void group_pass(const LevelArgs& a, const EX& ex, const QBuf& 
       wbase = __shfl(wbase, 63, 64);
     }
     if (EX::kRowBytes > 0) {
-      emit_children_wave<W, EX>(ex, out, a.stride, out_base, out_cap, wbase, incl, k, known, up,
-                                mass, accE, c);
+      emit_children_wave<W, EX>(ex, out, a.stride, out_base, out_cap, wbase, incl, k, hops, lmask,
+                                settle, known, up, mass, accE, c);
       if (live) {
-        accR.add(success_mass(ex, mass, k));
+        accR.add(settle ? succ : success_mass(ex, mass, hops));
         ++c[C_PATHS];
       }
     } else if (live) { /* main.cu:214-266 */
@@ -1196,6 +1314,10 @@ __device__ inline void group_pass(const LevelArgs& a, const EX& ex, const QBuf& 
       ++c[C_PATHS];
     }
   }
+  if (settle) {
+    const u32 t = wave_total_u32(settled);
+    if (lane == 0 && t) atomicAdd(s_cnt + 1, t);
+  }
 }
 
 /* The first levels of a level-order solve (path rule) in one workgroup:
@@ -1221,21 +1343,33 @@ __global__ void __launch_bounds__(256) k_small_levels(LevelArgs a, u32 max_in, i
   u32 c[C_N] = {0, 0, 0, 0, 0, 0, 0, 0};
   int p = 0;
   u32 n = a.seg_count[0][0];
+  /* the level's children settled at creation: part of its width */
+  u32 pend = a.settle ? a.seg_settle[0][0] : 0u;
   /* levels counted in registers (main.cu:970-973) and written once at the
    * end: a global read-modify-write per level would stall every barrier */
   u64 levels = 0, widest = 0;
   for (int lv = 0; n > 0 && n <= max_in && lv < max_levels; ++lv) {
-    if (threadIdx.x == 0) *s_cnt = 0;
+    if (threadIdx.x == 0) {
+      s_cnt[0] = 0;
+      s_cnt[1] = 0;
+    }
     levels += 1;
-    widest = n > widest ? n : widest;
+    const u64 width = (u64)n + pend;
+    widest = width > widest ? width : widest;
     __syncthreads();
     group_pass<W, EX>(a, ex, a.buf[p], 0, n, a.buf[p ^ 1], 0, a.seg_cap, s_cnt, accR, accE, accD,
                       c);
     __syncthreads();
-    const u32 made = *s_cnt;
+    const u32 made = s_cnt[0];
+    pend = s_cnt[1];
     n = made < a.seg_cap ? made : a.seg_cap;
     p ^= 1;
     __syncthreads();
+  }
+  if (n == 0 && pend) { /* a last level of settled children only */
+    levels += 1;
+    widest = pend > widest ? pend : widest;
+    pend = 0;
   }
   if (p == 1) { /* hand the level over in buffer 0 */
     const QBuf from = a.buf[1];
@@ -1253,6 +1387,7 @@ __global__ void __launch_bounds__(256) k_small_levels(LevelArgs a, u32 max_in, i
   __syncthreads();
   if (threadIdx.x == 0) {
     a.seg_count[0][0] = n;
+    if (a.settle) a.seg_settle[0][0] = pend;
     a.ctl->last_items = n;
     a.ctl->iterations += levels;
     if (widest > a.ctl->max_width) a.ctl->max_width = widest;
@@ -1298,6 +1433,9 @@ __global__ void __launch_bounds__(256) k_ramp_levels(LevelArgs a, int span, u32 
   const u64 region = (u64)(1 + (u64)span * blockIdx.x) * a.seg_cap;
   const u32 region_cap = (u32)span * a.seg_cap;
   u32 n = hi - lo;
+  /* children settled at creation of the level being expanded (those of the
+   * handed level are added by the last workgroup) */
+  u32 pend = 0;
   bool over = false;
   if (n > 0) {
     const u32 pre = prefetch_slot(a);
@@ -1311,8 +1449,9 @@ __global__ void __launch_bounds__(256) k_ramp_levels(LevelArgs a, int span, u32 
     u32 c[C_N] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int lv = 0; lv < L; ++lv) {
       if (threadIdx.x == 0) {
-        *s_cnt = 0;
-        s_width[lv] = n;
+        s_cnt[0] = 0;
+        s_cnt[1] = 0;
+        s_width[lv] = n + pend;
       }
       __syncthreads();
       if (n) {
@@ -1321,12 +1460,13 @@ __global__ void __launch_bounds__(256) k_ramp_levels(LevelArgs a, int span, u32 
                           region_cap, s_cnt, accR, accE, accD, c);
       }
       __syncthreads();
-      const u32 made = *s_cnt;
+      const u32 made = s_cnt[0];
+      pend = s_cnt[1];
       over = over || made > region_cap;
       n = made < region_cap ? made : region_cap;
       __syncthreads();
     }
-    if (threadIdx.x == 0) s_width[L] = n;
+    if (threadIdx.x == 0) s_width[L] = n + pend;
     block_flush(a, accR, accE, accD, c, (unsigned char*)part, &pre);
     /* the widths go out once, after the levels (block_flush's barrier
      * orders thread 0's last LDS word before these reads) */
@@ -1343,6 +1483,7 @@ __global__ void __launch_bounds__(256) k_ramp_levels(LevelArgs a, int span, u32 
     const u64 before = (u64)i * a.seg_cap;
     const u64 left = n > before ? n - before : 0;
     a.seg_count[0][1 + (u64)span * blockIdx.x + i] = (u32)(left < a.seg_cap ? left : a.seg_cap);
+    if (a.settle) a.seg_settle[0][1 + (u64)span * blockIdx.x + i] = i == 0 ? pend : 0u;
   }
   if (threadIdx.x != 0) return;
   __threadfence();
@@ -1351,8 +1492,9 @@ __global__ void __launch_bounds__(256) k_ramp_levels(LevelArgs a, int span, u32 
   __threadfence();
   u64 it = a.ctl->iterations, mw = a.ctl->max_width;
   for (int lv = 0; lv < L; ++lv) {
-    const u64 w = __hip_atomic_load(&a.ctl->ramp_width[lv], __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT);
+    u64 w = __hip_atomic_load(&a.ctl->ramp_width[lv], __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_AGENT);
+    if (lv == 0 && a.settle) w += a.seg_settle[0][0];
     if (w) {
       it += 1;
       if (w > mw) mw = w;
@@ -1365,6 +1507,10 @@ __global__ void __launch_bounds__(256) k_ramp_levels(LevelArgs a, int span, u32 
       __hip_atomic_load(&a.ctl->ramp_width[L], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   a.seg_count[0][0] = 0;
   for (u64 s = 1 + (u64)span * gridDim.x; s < (u64)a.n_seg; ++s) a.seg_count[0][s] = 0;
+  if (a.settle) {
+    a.seg_settle[0][0] = 0;
+    for (u64 s = 1 + (u64)span * gridDim.x; s < (u64)a.n_seg; ++s) a.seg_settle[0][s] = 0;
+  }
   if (a.hmirror) a.hmirror[kMirRamp] = ov;
 }
 
@@ -1450,6 +1596,7 @@ struct Variant {
   KernelGetter small;                  /* k_small_levels of this form */
   int choose;                          /* the learned chooser picks the sweep */
   KernelGetter ramp;                   /* k_ramp_levels of this form */
+  int settle; /* the level kernel can settle dead children at creation */
 };
 
 /* The alternating rule with the learned sweep choice (ChooseAltExpander, or
