@@ -456,6 +456,7 @@ __global__ void __launch_bounds__(256) k_expand_batch(BatchArgs a, int parity) {
      * counts packed nodes | truncated << 16 | dead << 32 | overflowed << 48 */
     u64 cnt = 0, paths = 0;
     u64 r2 = 0, r1 = 0, r0 = 0, e2 = 0, e1 = 0, e0 = 0, d2 = 0, d1 = 0, d0 = 0;
+    int stored = 0; /* children queued: those below the truncation are settled */
     if ((u64)lane < chunk && i < n_items) {
       int s = segment_of(sp, a.n_seg, (u32)base);
       while (sp[s + 1] <= i) ++s;
@@ -485,6 +486,15 @@ __global__ void __launch_bounds__(256) k_expand_batch(BatchArgs a, int parity) {
         k = ex.find_lean(known, up, lvb);
         if (k >= 0) {
           live = true;
+          /* children below the truncation are counted when created (as
+           * the truncated nodes they would be when dequeued) and never
+           * queued */
+          double run = mass;
+          for (int j = k - 1; j >= 0; --j) {
+            const int e = ex.uh[j * ex.stride];
+            if (run * ex.q[e] >= a.eps) ++stored;
+            run = run * ex.p[e];
+          }
         } else {
           if (k == -2) atomicAdd((unsigned long long*)&a.res[(u64)id * B_N + B_ERR], 1ull);
           fx_limbs_add(d2, d1, d0, mass); /* main.cu:579-582 */
@@ -493,10 +503,11 @@ __global__ void __launch_bounds__(256) k_expand_batch(BatchArgs a, int parity) {
         }
       }
     }
-    const int incl = wave_incl_scan(k);
+    const int incl = wave_incl_scan(stored);
     const int total = __shfl(incl, 63, 64);
     u32 wbase = 0;
     u64 sbase = 0;
+    bool settled_any = false;
     if (total) {
       if (lane == 63) {
         wbase = atomicAdd(s_cnt, (u32)total);
@@ -509,18 +520,28 @@ __global__ void __launch_bounds__(256) k_expand_batch(BatchArgs a, int parity) {
       sbase = __shfl(sbase, 63, 64);
     }
     if (live) {
-      u32 off = wbase + (u32)(incl - k);
+      u32 off = wbase + (u32)(incl - stored);
       const u32 seg_end = wbase > a.seg_cap ? wbase : a.seg_cap;
       const u32 child_tag = id | ((depth + 1u) << kTagIdBits);
       double run = mass;
       u64 pre[W];
 #pragma unroll
       for (int w = 0; w < W; ++w) pre[w] = 0;
-      for (int j = k - 1; j >= 0; --j, ++off) {
+      for (int j = k - 1; j >= 0; --j) {
         const int e = ex.uh[j * ex.stride];
         const int wi = e >> 6;
         const u64 bit = 1ull << (e & 63);
         const double cm = run * ex.q[e];
+        if (cm < a.eps) { /* settled at creation (main.cu:537-543) */
+          fx_limbs_add(e2, e1, e0, cm);
+          cnt += 1ull | (1ull << 16);
+          settled_any = true;
+          run = run * ex.p[e];
+#pragma unroll
+          for (int w = 0; w < W; ++w)
+            if (w == wi) pre[w] |= bit;
+          continue;
+        }
         u64 ck[W], cu[W];
 #pragma unroll
         for (int w = 0; w < W; ++w) {
@@ -541,6 +562,7 @@ __global__ void __launch_bounds__(256) k_expand_batch(BatchArgs a, int parity) {
             cnt += 1ull << 48;
           }
         }
+        ++off;
         run = run * ex.p[e];
 #pragma unroll
         for (int w = 0; w < W; ++w)
@@ -568,7 +590,8 @@ __global__ void __launch_bounds__(256) k_expand_batch(BatchArgs a, int parity) {
       const u64 g_d2 = wave_sum_if(mine ? d2 : 0ull, any_d),
                 g_d1 = wave_sum_if(mine ? d1 : 0ull, any_d),
                 g_d0 = wave_sum_if(mine ? d0 : 0ull, any_d);
-      u32 gdepth = mine ? depth + 1u : 0u;
+      /* levels: this item's, and its settled children's below it */
+      u32 gdepth = mine ? depth + (settled_any ? 2u : 1u) : 0u;
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) gdepth = max(gdepth, (u32)__shfl_xor(gdepth, off, 64));
       if (lane == leader) {
@@ -1540,15 +1563,19 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   a.thr_class = kClasses;
   a.drop_class = kClasses;
   a.fin_on_empty = (!mass && ws.hmirror) ? 1 : 0;
-  /* level order of the plain path rule in the row form, two terminals, no
-   * budget, checkpoint or seeds: dead children (a bridge hop) and children
-   * below the truncation are counted when created, not queued; the counts,
-   * levels and widest level are those of queueing them */
-  a.settle = (var.settle && mode == 0 && !mass && cfg->rule == NR_RULE_PATH && g->terms.empty() &&
-              g->simple && cfg->time_budget_s <= 0 && !(x && (x->save || x->resume)) &&
-              cfg->prefix_edges <= 0 && !(cfg->flags & NR_FLAG_QUEUE_ALL))
-                 ? 1
-                 : 0;
+  /* level order (MODE 0 kernels) with no budget, checkpoint or seeds:
+   * children below the truncation are counted when created, not queued, and
+   * in the row form of the path rule with two terminals on a simple graph
+   * so are dead children (a bridge hop); the counts, levels and widest
+   * level are those of queueing them */
+  a.settle = 0;
+  if (mode == 0 && !mass && cfg->rule != NR_RULE_FORKBEST && cfg->rule != NR_RULE_PIVOT &&
+      cfg->time_budget_s <= 0 && !(x && (x->save || x->resume)) && cfg->prefix_edges <= 0 &&
+      !(cfg->flags & NR_FLAG_QUEUE_ALL)) {
+    a.settle = kSettleTrunc;
+    if (var.settle && cfg->rule == NR_RULE_PATH && g->terms.empty() && g->simple)
+      a.settle |= kSettleDead;
+  }
 
   hipStream_t st = ws.stream;
   const nr_checkpoint* ck = x ? x->resume : nullptr;

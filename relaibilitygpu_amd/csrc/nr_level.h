@@ -31,6 +31,9 @@ __host__ __device__ inline int mass_class(double m) {
   return c < 0 ? 0 : (c >= kClasses ? kClasses - 1 : c);
 }
 
+/* LevelArgs::settle bits. */
+constexpr int kSettleTrunc = 1, kSettleDead = 2;
+
 /* Per-wave counter slots. */
 enum { C_NODES = 0, C_PATHS, C_DEAD, C_TRUNC, C_OVER, C_FOLD, C_ERR, C_N = 8 };
 
@@ -87,8 +90,9 @@ struct LevelArgs {
    * the truncation) instead of writing them: they belong to the width of
    * the level the buffer holds (read when `settle` is set) */
   u32* seg_settle[2];
-  /* level order, row form, two terminals: children that are dead or below
-   * the truncation are counted when they are created and never queued */
+  /* level order: children counted when they are created and never queued
+   * (kSettleTrunc: those below the truncation; kSettleDead: the dead ones,
+   * row form with two terminals) */
   int settle;
   u32* hist;         /* [n_seg][kClasses] class histogram of each segment */
   LevelCtl* ctl;
@@ -626,16 +630,16 @@ __device__ inline double success_mass(const EX& ex, double mass, int k) {
   return run;
 }
 
-/* The children of a live item found by find_item_marked(), src side first:
- * those below the truncation (main.cu:537-543) and the dead ones
- * (dead_children(), main.cu:579-582) are counted now, as the processed
- * nodes they would be when dequeued, instead of being queued; bit j of the
- * result marks child j as queued.  `succ` gets the success mass, the same
- * product as success_mass(). */
+/* The children of a live item, src side first: those below the truncation
+ * (main.cu:537-543) and those marked in `dead` (dead_children(),
+ * main.cu:579-582) are counted now, as the processed nodes they would be
+ * when dequeued, instead of being queued; bit j of the result marks child j
+ * as queued.  `succ` gets the success mass, the same product as
+ * success_mass(). */
 template <class EX>
-__device__ inline u64 settle_children(const EX& ex, double mass, int k, double eps, Acc2& accE,
-                                      Acc2& accD, u32 (&c)[C_N], u32& settled, double& succ) {
-  const u64 dead = ex.dead_children();
+__device__ inline u64 settle_children(const EX& ex, double mass, int k, double eps, u64 dead,
+                                      Acc2& accE, Acc2& accD, u32 (&c)[C_N], u32& settled,
+                                      double& succ) {
   u64 keep = 0;
   double run = mass;
   for (int j = 0; j < k; ++j) {
@@ -766,8 +770,10 @@ __device__ __forceinline__ void expand_level_items(const LevelArgs& a, const EX&
                                                    float alpha, Acc2& accR, Acc2& accE,
                                                    Acc2& accD, u32 (&c)[C_N], u32& settled) {
   const bool mass_on = MODE == 4 || ((MODE == 2 || MODE == 3) && a.mass_mode);
-  /* dead and truncated children counted at creation (row form only) */
-  const bool settle = MODE == 0 && EX::kSettle && a.settle;
+  /* level order: children below the truncation (any form) and dead ones
+   * (row form, kSettleDead) counted at creation */
+  const bool settle = MODE == 0 && a.settle != 0;
+  const bool settle_dead = settle && EX::kSettle && (a.settle & kSettleDead);
   const QBuf in = a.buf[parity];
   const QBuf out = a.buf[parity ^ 1];
   const int lane = threadIdx.x & 63;
@@ -838,19 +844,23 @@ __device__ __forceinline__ void expand_level_items(const LevelArgs& a, const EX&
         } else {
           if (MODE != 0) ex.load(in, a.stride, slot_in);
           if constexpr (MODE == 0 && EX::kSettle)
-            k = settle ? ex.find_item_marked(known, up) : ex.find_item(known, up);
+            k = settle_dead ? ex.find_item_marked(known, up) : ex.find_item(known, up);
           else
             k = ex.find_item(known, up);
           if (k >= 0) {
             live = true;
             stored = k;
-            if constexpr (MODE == 0 && EX::kSettle) {
+            if constexpr (MODE == 0 && EX::kRowBytes > 0) { /* child-major emission */
               if (settle) {
-                lmask = settle_children(ex, mass, k, a.eps, accE, accD, c, settled, succ);
+                u64 dead = 0;
+                if constexpr (EX::kSettle)
+                  if (settle_dead) dead = ex.dead_children();
+                lmask = settle_children(ex, mass, k, a.eps, dead, accE, accD, c, settled, succ);
                 stored = __popcll(lmask);
               }
             }
-            if (mass_on && !EX::kChoose) { /* children below eps are not queued */
+            if ((mass_on || (settle && EX::kRowBytes == 0)) && !EX::kChoose) {
+              /* children below eps are not queued */
               stored = 0;
               double run = mass;
               for (int j = k - 1; j >= 0; --j) {
@@ -871,7 +881,7 @@ __device__ __forceinline__ void expand_level_items(const LevelArgs& a, const EX&
     }
     if constexpr (EX::kChoose) {
       choose_sweep<W, EX, MODE>(a, ex, live, k, mass, known, up, tag_in, tile);
-      if (mass_on && live) { /* children below eps are not queued */
+      if ((mass_on || settle) && live) { /* children below eps are not queued */
         stored = 0;
         double run = mass;
         for (int j = k - 1; j >= 0; --j) {
@@ -1035,11 +1045,12 @@ __device__ __forceinline__ void expand_level_items(const LevelArgs& a, const EX&
         const u64 bit = 1ull << (e & 63);
         const double cm = run * ex.q[e];
         bool queue = true;
-        if (mass_on) {
+        if (mass_on || settle) {
           if (cm < a.eps) { /* settled at creation: counted as a truncated node */
             accE.add(cm);
             ++c[C_NODES];
             ++c[C_TRUNC];
+            ++settled;
             queue = false;
           }
         }
@@ -1198,7 +1209,7 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
 
   expand_level_items<W, EX, MODE>(a, ex, parity, level, sp, n_items, s_cnt, s_hist, tile, t_end,
                                   afrag, sign, b2, alpha, accR, accE, accD, c, settled);
-  if (MODE == 0 && EX::kSettle && a.settle) {
+  if (MODE == 0 && a.settle) {
     const u32 t = wave_total_u32(settled);
     if ((threadIdx.x & 63) == 0 && t) atomicAdd(s_set, t);
   }
@@ -1228,7 +1239,8 @@ __device__ inline void group_pass(const LevelArgs& a, const EX& ex, const QBuf& 
                                   u32 n, const QBuf& out, u64 out_base, u32 out_cap, u32* s_cnt,
                                   Acc2& accR, Acc2& accE, Acc2& accD, u32 (&c)[C_N]) {
   const int lane = threadIdx.x & 63;
-  const bool settle = EX::kSettle && a.settle;
+  const bool settle = a.settle != 0;
+  const bool settle_dead = settle && EX::kSettle && (a.settle & kSettleDead);
   u32 settled = 0;
   for (u32 base = 0; base < n; base += blockDim.x) {
     const u32 i = base + threadIdx.x;
@@ -1247,17 +1259,18 @@ __device__ inline void group_pass(const LevelArgs& a, const EX& ex, const QBuf& 
         ++c[C_TRUNC];
       } else {
         if constexpr (EX::kSettle)
-          k = settle ? ex.find_item_marked(known, up) : ex.find_item(known, up);
+          k = settle_dead ? ex.find_item_marked(known, up) : ex.find_item(known, up);
         else
           k = ex.find_item(known, up);
         if (k >= 0) {
           live = true;
           hops = k;
-          if constexpr (EX::kSettle) {
-            if (settle) {
-              lmask = settle_children(ex, mass, hops, a.eps, accE, accD, c, settled, succ);
-              k = __popcll(lmask);
-            }
+          if (settle) {
+            u64 dead = 0;
+            if constexpr (EX::kSettle)
+              if (settle_dead) dead = ex.dead_children();
+            lmask = settle_children(ex, mass, hops, a.eps, dead, accE, accD, c, settled, succ);
+            k = __popcll(lmask);
           }
         } else {
           if (k == -2) ++c[C_ERR];
@@ -1287,23 +1300,27 @@ __device__ inline void group_pass(const LevelArgs& a, const EX& ex, const QBuf& 
       u64 pre[W];
 #pragma unroll
       for (int w = 0; w < W; ++w) pre[w] = 0;
-      for (int j = k - 1; j >= 0; --j, ++off) {
+      for (int j = hops - 1; j >= 0; --j) {
         const int e = ex.uh[j * ex.stride];
         const int wi = e >> 6;
         const u64 bit = 1ull << (e & 63);
         const double cm = run * ex.q[e];
-        if (off < out_cap) {
-          u64 ck[W], cu[W];
+        /* children settled at creation were counted by settle_children() */
+        if (!settle || ((lmask >> (hops - 1 - j)) & 1ull)) {
+          if (off < out_cap) {
+            u64 ck[W], cu[W];
 #pragma unroll
-          for (int w = 0; w < W; ++w) {
-            ck[w] = known[w] | pre[w] | (w == wi ? bit : 0ull);
-            cu[w] = up[w] | pre[w];
+            for (int w = 0; w < W; ++w) {
+              ck[w] = known[w] | pre[w] | (w == wi ? bit : 0ull);
+              cu[w] = up[w] | pre[w];
+            }
+            store_item<W>(out, a.stride, out_base + off, ck, cu, cm);
+          } else { /* region full: main.cu:245-252 */
+            accE.add(cm);
+            ++c[C_PATHS];
+            ++c[C_OVER];
           }
-          store_item<W>(out, a.stride, out_base + off, ck, cu, cm);
-        } else { /* region full: main.cu:245-252 */
-          accE.add(cm);
-          ++c[C_PATHS];
-          ++c[C_OVER];
+          ++off;
         }
         run = run * ex.p[e];
 #pragma unroll
