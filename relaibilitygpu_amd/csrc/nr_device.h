@@ -247,7 +247,7 @@ struct Expander {
   int n, src, dst;
   mutable int plen; /* hops of the last path found */
   /* find<true>(): bit h set when hop h (BFS level h-1 to h) is UNKNOWN, and
-   * the path's nodes */
+   * the path's nodes (one per BFS level, dst on level plen) */
   mutable u64 hopm;
   mutable NS pathm;
   const u16* terms;
@@ -312,8 +312,8 @@ struct Expander {
   __device__ int find_item(const u64 (&known)[W], const u64 (&)[W]) const {
     return nterms >= 2 ? find_tree(known) : find(known);
   }
-  /* find() for two terminals that leaves the path in the level sets for
-   * dead_children() */
+  /* find() for two terminals that also records the path's nodes and its
+   * UNKNOWN hops for dead_children() */
   __device__ int find_item_marked(const u64 (&known)[W], const u64 (&)[W]) const {
     return find<true>(known);
   }
@@ -322,53 +322,56 @@ struct Expander {
    * bridge between src and dst in the item's allowed graph (edges not
    * FAILED): such a child has no src-dst path left and is dead when it is
    * dequeued (main.cu:579-582); every other child has one.  Hop h joins
-   * v(h-1) and v(h), v(0) = src, v(plen) = dst, and find<true>() left the
-   * node set {v(h)} in lvl[h].  R is grown as the set reached from
-   * v(0..h-1) without crossing hop h; hop h is a bridge exactly when R meets
-   * no node of v(h..plen).  Only the row of v(h-1) loses hop h (no other row
-   * holds it while R misses the suffix), R only grows with h so each node
-   * is expanded once, a WORKING hop needs no test, and once R holds dst no
-   * later hop is a bridge.  Bit j: the j-th UNKNOWN hop from the src side
-   * (child j). */
+   * v(h-1) and v(h), v(0) = src, v(plen) = dst; v(h) is the path node of BFS
+   * level h (pathm & lvl[h]).  Every node of the levels before h is reached
+   * from src without hop h (along its own shortest path), and so is every
+   * node of level h but v(h) (its BFS parent is on level h-1, and the edge
+   * from it is not hop h).  Hop h is a bridge exactly when the set R reached
+   * without it meets none of v(h..plen): it is not when v(h) has a
+   * neighbour other than v(h-1) on the levels up to h (one row test);
+   * otherwise R starts as those levels without v(h) and grows from level h
+   * (whose neighbours may lie deeper) or, for the last hop, from level
+   * plen-1 (level plen was not built), v(h-1)'s row without v(h).  A
+   * WORKING hop has no child.  Bit j: the j-th UNKNOWN hop from the src
+   * side (child j). */
   __device__ u64 dead_children() const {
     const NS dbit = (NS)1 << dst;
-    NS R = 0, F = 0, pre = 0;
+    NS below = 0; /* levels 0 .. h-1 */
     u64 dead = 0, todo = hopm;
     int j = 0;
-    NS vp = lvl[0];
+    NS lprev = lvl[0];
     for (int h = 1; todo; ++h) {
-      const NS vh = h < plen ? lvl[h * stride] : dbit;
-      pre |= vp;
-      if (!(R & vp)) {
-        R |= vp;
-        F |= vp;
-      }
+      below |= lprev;
+      const NS lh = h < plen ? lvl[h * stride] : (NS)0;
       if ((todo >> h) & 1ull) {
         todo &= todo - 1; /* bits below h are clear */
-        const NS suf = pathm & ~pre;
-        const int up = ctzs<NS>(vp);
-        auto rw = [&](int u) {
-          const NS r = rows[u * stride];
-          return u == up ? r & ~vh : r;
-        };
-        while (F && !(R & suf)) {
-          const int u0 = ctzs<NS>(F);
-          F &= F - 1;
-          const int u1 = F ? ctzs<NS>(F) : u0;
-          F &= F - 1;
-          const int u2 = F ? ctzs<NS>(F) : u0;
-          F &= F - 1;
-          const int u3 = F ? ctzs<NS>(F) : u0;
-          F &= F - 1;
-          const NS x = ((rw(u0) | rw(u1)) | (rw(u2) | rw(u3))) & ~R;
-          R |= x;
-          F |= x;
+        const NS vh = h < plen ? (lh & pathm) : dbit;
+        const NS vp = lprev & pathm;
+        const NS rest = below | (lh & ~vh); /* reached without hop h */
+        if (!(rows[ctzs<NS>(vh) * stride] & rest & ~vp)) {
+          const NS suf = pathm & ~below;
+          const int up = ctzs<NS>(vp);
+          NS R = rest;
+          NS F = h < plen ? (lh & ~vh) : lprev;
+          bool hit = false;
+          while (F) {
+            const int u = ctzs<NS>(F);
+            F &= F - 1;
+            NS r = rows[u * stride];
+            if (u == up) r &= ~vh;
+            const NS x = r & ~R;
+            if (x & suf) {
+              hit = true;
+              break;
+            }
+            R |= x;
+            F |= x;
+          }
+          if (!hit) dead |= 1ull << j;
         }
-        if (!(R & suf)) dead |= 1ull << j;
         ++j;
-        if (R & dbit) break;
       }
-      vp = vh;
+      lprev = lh;
     }
     return dead;
   }
@@ -662,10 +665,7 @@ struct Expander {
         uh[(k++) * stride] = (u16)e;
         if (kMark) hopm |= 1ull << l;
       }
-      if (kMark) { /* the level set is spent: it keeps the path's node */
-        lvl[(l - 1) * stride] = (NS)1 << u;
-        pathm |= (NS)1 << u;
-      }
+      if (kMark) pathm |= (NS)1 << u;
       v = u;
     }
     return k;
