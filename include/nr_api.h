@@ -92,7 +92,7 @@ typedef struct nr_config {
                                workgroup runs alone, every workgroup expands
                                its own slice's subtrees level after level
                                (no launch per level) until the levels hold
-                               about this many items; 0 = 1024 per compute
+                               about this many items; 0 = 512 per compute
                                unit, < 0 = one launch per level from
                                there.  Same results either way: a ramp that
                                overflows a workgroup's region is discarded and

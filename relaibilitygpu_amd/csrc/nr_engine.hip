@@ -1734,10 +1734,11 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
     /* one workgroup per CU (each runs its levels alone on its CU: one wave
      * step of a level then takes about as long as in k_small_levels), each
      * with the segments 1.. of the grid split evenly; by default the ramp
-     * ends near 1024 items per CU (four waves per SIMD) */
+     * ends near 512 items per CU (K10 on MI355X: 0.746 ms per solve against
+     * 0.756-0.768 ms with 1024 and 0.80 ms without the ramp) */
     const int groups = std::min(grid - 1, cus);
     int span = (grid - 1) / groups;
-    u32 target = cfg->ramp_items > 0 ? (u32)cfg->ramp_items : 1024u * (u32)cus;
+    u32 target = cfg->ramp_items > 0 ? (u32)cfg->ramp_items : 512u * (u32)cus;
     void* rargs[] = {&a, &span, &target};
     if ((r = timed_launch((const void*)var.ramp(), (unsigned)groups, rargs))) return r;
     ramp_used = true;
