@@ -42,6 +42,8 @@ __global__ void __launch_bounds__(256) k_init_big(LevelArgs a, const u64* root, 
   for (u64 s = tid; s < (u64)a.n_seg; s += stride) {
     a.seg_count[0][s] = s == 0 ? 1u : 0u;
     a.seg_count[1][s] = 0u;
+    a.seg_settle[0][s] = 0u;
+    a.seg_settle[1][s] = 0u;
   }
   for (u64 w = tid; w < 2 * (u64)W; w += stride) a.buf[0].known[w] = root[w];
   if (tid == 0) {
@@ -56,22 +58,46 @@ __global__ void __launch_bounds__(256) k_expand_big(LevelArgs a, int parity, Big
   u32* sp = (u32*)(smem + a.lay.off_extra);
   u32* s_cnt = sp + a.n_seg + 1;
   u32* part = s_cnt + 1;
+  u32* s_set = part + blockDim.x; /* children settled at creation */
   block_prefix(a.seg_count[parity], a.n_seg, sp, part);
   const u32 n_items = sp[a.n_seg];
+  /* the level's children settled when created (below the truncation) */
+  u64 pend = 0;
+  if (a.settle && blockIdx.x == 0 && threadIdx.x < 64) { /* wave 0, 16 loads in flight */
+    u32 t = 0;
+    for (int s0 = 0; s0 < a.n_seg; s0 += 64 * 16) {
+      u32 v[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int s = s0 + (int)threadIdx.x + 64 * j;
+        v[j] = s < a.n_seg ? a.seg_settle[parity][s] : 0u;
+      }
+#pragma unroll
+      for (int j = 0; j < 16; ++j) t += v[j];
+    }
+    pend = wave_total_u32(t);
+  }
   const u64 t_end = a.budget_ticks ? a.ctl->start_tick + a.budget_ticks : 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) { /* count the level (main.cu:970-973) */
     a.ctl->last_items = n_items;
-    if (n_items && !(t_end && wall_clock64() >= t_end)) {
+    const u64 width = (u64)n_items + pend;
+    if (width && !(t_end && wall_clock64() >= t_end)) {
       a.ctl->iterations += 1;
-      if (n_items > a.ctl->max_width) a.ctl->max_width = n_items;
+      if (width > a.ctl->max_width) a.ctl->max_width = width;
     }
   }
   if (n_items == 0) {
-    if (threadIdx.x == 0) a.seg_count[parity ^ 1][blockIdx.x] = 0;
+    if (threadIdx.x == 0) {
+      a.seg_count[parity ^ 1][blockIdx.x] = 0;
+      if (a.settle) a.seg_settle[parity ^ 1][blockIdx.x] = 0;
+    }
     return;
   }
   const u32 pre = prefetch_slot(a);
-  if (threadIdx.x == 0) *s_cnt = 0;
+  if (threadIdx.x == 0) {
+    *s_cnt = 0;
+    *s_set = 0;
+  }
   __syncthreads();
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -95,6 +121,7 @@ __global__ void __launch_bounds__(256) k_expand_big(LevelArgs a, int parity, Big
   accE.init();
   accD.init();
   u32 c[C_N] = {0, 0, 0, 0, 0, 0, 0, 0};
+  u32 settled = 0; /* lane 0: children settled at creation */
 
   for (u64 i = wave_id; i < n_items; i += n_waves) { /* one item per wave */
     int s = a.n_seg <= 4096 ? wave_segment_of(sp, a.n_seg, (u32)i) : segment_of(sp, a.n_seg, (u32)i);
@@ -272,18 +299,37 @@ __global__ void __launch_bounds__(256) k_expand_big(LevelArgs a, int parity, Big
     }
     /* children src side first (hops[] holds them dst side first): child t
      * has the earlier hops WORKING and hop t FAILED; each hop is made
-     * WORKING in the wave's planes once its child is written */
+     * WORKING in the wave's planes once its child is written.  With
+     * a.settle a child below the truncation is counted when created
+     * (main.cu:537-543) and takes no slot. */
+    int kq = k;
+    if (a.settle) {
+      kq = 0;
+      double r0 = mass;
+      for (int t = 0; t < k; ++t) {
+        const int e = hops[k - 1 - t];
+        if (r0 * a.g.q[e] >= a.eps) ++kq;
+        r0 = r0 * a.g.p[e];
+      }
+    }
     u32 obase = 0;
-    if (lane == 0 && k) obase = atomicAdd(s_cnt, (u32)k);
+    if (lane == 0 && kq) obase = atomicAdd(s_cnt, (u32)kq);
     obase = __shfl(obase, 0, 64);
     double run = mass;
+    u32 o = obase;
     for (int t = 0; t < k; ++t) {
       const int e = hops[k - 1 - t];
       const int we = e >> 6;
       const u64 b = 1ull << (e & 63);
       const double cm = run * a.g.q[e];
-      const u32 o = obase + (u32)t;
-      if (o < a.seg_cap) {
+      if (a.settle && cm < a.eps) { /* settled at creation */
+        if (lane == 0) {
+          accE.add(cm);
+          ++c[C_NODES];
+          ++c[C_TRUNC];
+          ++settled;
+        }
+      } else if (o < a.seg_cap) {
         u64* op = out.known + (out_base + o) * row;
         for (u64 w = lane; w < row; w += 64) op[w] = planes[w] | ((int)w == we ? b : 0ull);
         if (lane == 0) out.mass[out_base + o] = cm;
@@ -292,6 +338,7 @@ __global__ void __launch_bounds__(256) k_expand_big(LevelArgs a, int parity, Big
         ++c[C_PATHS];
         ++c[C_OVER];
       }
+      if (!(a.settle && cm < a.eps)) ++o;
       wave_sync();
       if (lane == 0) {
         planes[we] |= b;
@@ -306,11 +353,13 @@ __global__ void __launch_bounds__(256) k_expand_big(LevelArgs a, int parity, Big
     }
   }
 
+  if (lane == 0 && settled) atomicAdd(s_set, settled);
   __syncthreads();
   block_flush(a, accR, accE, accD, c, (unsigned char*)part, &pre);
   if (threadIdx.x == 0) {
     const u32 nn = *s_cnt;
     a.seg_count[parity ^ 1][blockIdx.x] = nn < a.seg_cap ? nn : a.seg_cap;
+    if (a.settle) a.seg_settle[parity ^ 1][blockIdx.x] = *s_set;
   }
 }
 

@@ -1224,7 +1224,7 @@ int solve_big(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_result*
   }
   u64* droot = (u64*)((unsigned char*)ws.q + 2 * per_buf);
   const size_t meta_bytes = n_slots * (3 * sizeof(DD) + C_N * sizeof(u64)) +
-                            2 * sizeof(u32) * (2 * (size_t)grid + 4);
+                            sizeof(u32) * (4 * (size_t)grid + 8);
   if ((rc = ws.grow(&ws.meta, &ws.meta_bytes, meta_bytes))) return rc;
   {
     unsigned char* m = (unsigned char*)ws.meta;
@@ -1235,7 +1235,12 @@ int solve_big(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_result*
     u32* q = (u32*)(a.cnt + n_slots * C_N);
     a.seg_count[0] = q;
     a.seg_count[1] = q + grid;
+    a.seg_settle[0] = q + 2 * grid;
+    a.seg_settle[1] = q + 3 * grid;
   }
+  /* without a budget, children below the truncation are counted when they
+   * are created and never queued (the level widths count them) */
+  a.settle = (cfg->time_budget_s <= 0 && !(cfg->flags & NR_FLAG_QUEUE_ALL)) ? kSettleTrunc : 0;
   a.ctl = ws.ctl;
   hipStream_t st = ws.stream;
   {
