@@ -1664,6 +1664,10 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   if (ev_start) NR_HIP(hipEventRecord(ev_start, st));
   int level = 0, chunk = has_budget ? 1 : 8;
   bool finished = false, deadline = false;
+  /* the queue was seen empty at a chunk boundary (ev_chunk): the launch that
+   * found it empty has computed the totals, so the launches queued after it
+   * (empty, and leaving the mirror alone) need not be waited for */
+  hipEvent_t ev_done = nullptr;
   long long timed = 0, rounds = 0;
   u64 max_pool = 1;
   auto launch_level = [&](int parity, int lv) -> int {
@@ -1779,6 +1783,7 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
         w = ws.hmirror[0];
         max_w = ws.hmirror[2];
         ramp_over = ws.hmirror[kMirRamp] != 0;
+        if (w == 0) ev_done = ev_chunk;
       } else { /* no-allocation mode: the counters by a plain copy */
         NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, kCtlCopyBytes, hipMemcpyDeviceToHost, st));
         NR_HIP(hipStreamSynchronize(st));
@@ -1939,9 +1944,15 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
    * k_finalize */
   bool have_totals = false;
   if (a.fin_on_empty && !deadline && pend_parity < 0) {
-    if (ev_stop) NR_HIP(hipEventRecord(ev_stop, st));
-    NR_HIP(hipStreamSynchronize(st));
-    have_totals = __atomic_load_n(&ws.hmirror[kMirSeq], __ATOMIC_ACQUIRE) == a.seq;
+    if (ev_done && max_timed == 0 && !ramp_restarts &&
+        __atomic_load_n(&ws.hmirror[kMirSeq], __ATOMIC_ACQUIRE) == a.seq) {
+      ev_stop = ev_done; /* the solve's GPU time ends there */
+      have_totals = true;
+    } else {
+      if (ev_stop) NR_HIP(hipEventRecord(ev_stop, st));
+      NR_HIP(hipStreamSynchronize(st));
+      have_totals = __atomic_load_n(&ws.hmirror[kMirSeq], __ATOMIC_ACQUIRE) == a.seq;
+    }
   }
   if (!have_totals) {
     u64 nw = n_slots;

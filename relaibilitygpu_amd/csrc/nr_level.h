@@ -1147,7 +1147,9 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
       a.ctl->iterations = it;
       a.ctl->max_width = mw;
     }
-    if (a.hmirror) {
+    /* a launch queued past the one that computed the totals leaves the
+     * mirror alone: the host may already be on its next solve */
+    if (a.hmirror && !a.ctl->fin_done) {
       a.hmirror[0] = n_items;
       a.hmirror[1] = it;
       a.hmirror[2] = mw;
