@@ -1238,9 +1238,9 @@ int solve_big(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_result*
     a.seg_settle[0] = q + 2 * grid;
     a.seg_settle[1] = q + 3 * grid;
   }
-  /* without a budget, children below the truncation are counted when they
-   * are created and never queued (the level widths count them) */
-  a.settle = (cfg->time_budget_s <= 0 && !(cfg->flags & NR_FLAG_QUEUE_ALL)) ? kSettleTrunc : 0;
+  /* children below the truncation are counted when they are created and
+   * never queued (the level widths count them) */
+  a.settle = (cfg->flags & NR_FLAG_QUEUE_ALL) ? 0 : kSettleTrunc;
   a.ctl = ws.ctl;
   hipStream_t st = ws.stream;
   {
@@ -1568,15 +1568,15 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   a.thr_class = kClasses;
   a.drop_class = kClasses;
   a.fin_on_empty = (!mass && ws.hmirror) ? 1 : 0;
-  /* level order (MODE 0 kernels) with no budget, checkpoint or seeds:
-   * children below the truncation are counted when created, not queued, and
-   * in the row form of the path rule with two terminals on a simple graph
-   * so are dead children (a bridge hop); the counts, levels and widest
-   * level are those of queueing them */
+  /* level order (MODE 0 kernels) outside checkpoint chains: children below
+   * the truncation are counted when created, not queued, and in the row form
+   * of the path rule with two terminals on a simple graph so are dead
+   * children (a bridge hop); the counts, levels and widest level are those
+   * of queueing them.  Under a time budget a settled child is resolved (ESP
+   * or D) where a queued one could be folded into ESP at the deadline. */
   a.settle = 0;
   if (mode == 0 && !mass && cfg->rule != NR_RULE_FORKBEST && cfg->rule != NR_RULE_PIVOT &&
-      cfg->time_budget_s <= 0 && !(x && (x->save || x->resume)) && cfg->prefix_edges <= 0 &&
-      !(cfg->flags & NR_FLAG_QUEUE_ALL)) {
+      !(x && (x->save || x->resume)) && !(cfg->flags & NR_FLAG_QUEUE_ALL)) {
     a.settle = kSettleTrunc;
     if (var.settle && cfg->rule == NR_RULE_PATH && g->terms.empty() && g->simple)
       a.settle |= kSettleDead;
