@@ -67,8 +67,11 @@ typedef struct nr_config {
   int block_size;           /* threads per block: 64, 128 or 256; other
                                values (0) = auto, the largest that fits LDS */
   int grid_blocks;          /* blocks per launch; 0 = auto from occupancy */
-  int chunk_items;          /* persistent modes: items claimed per wave refill;
-                               0 = auto */
+  int chunk_items;          /* nr_solve_batch: the queue chunk, the number of
+                               items the refill between two levels tops the
+                               next level up to (spilled children first, then
+                               new instance roots); 0 = a quarter of the
+                               capacity; nr_tune_batch picks it */
   int flags;                /* NR_FLAG_* */
   int rule;                 /* nr_rule: which decomposition each item gets */
   int sweep_period;         /* NR_RULE_ALTERNATE: the sweep direction flips
@@ -256,11 +259,13 @@ int nr_solve_batch(const nr_graph* const* graphs, int count,
 
 /* Launch schedule of nr_solve_batch for these instances, picked by timing
  * trial batches of the first `sample` instances (all of them when 0):
- * workgroups of 256, 128 or 64 threads, 1, 2 or 4 workgroups per CU or the
- * occupancy limit, the rest of cfg (or the defaults) unchanged.  *tuned
- * receives cfg with block_size and grid_blocks of the fastest trial, and
- * *best_rate (optional) its subproblems per second.  No counterpart in the
- * reference (its launch shape is fixed, main.cu:882-948). */
+ * workgroups of 256, 128 or 64 threads and 1, 2 or 4 workgroups per CU or
+ * the occupancy limit; then, for the fastest of those, a queue chunk
+ * (chunk_items) of an eighth, a quarter or half of the capacity; the rest of
+ * cfg (or the defaults) unchanged.  *tuned receives cfg with block_size,
+ * grid_blocks and chunk_items of the fastest trial, and *best_rate
+ * (optional) its subproblems per second.  No counterpart in the reference
+ * (its launch shape is fixed, main.cu:882-948). */
 int nr_tune_batch(const nr_graph* const* graphs, int count, const nr_config* cfg,
                   int sample, nr_config* tuned, double* best_rate);
 

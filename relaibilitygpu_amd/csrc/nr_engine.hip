@@ -2058,6 +2058,15 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
  * is dropped unless the stack itself is full; an instance that overflowed
  * then is solved again alone, so its counts are those of an unbounded queue
  * unless it overflows on its own. */
+/* The HIP graphs of one batch pass's chunks (8 and 16 level pairs). */
+struct ChunkGraphs {
+  hipGraphExec_t g8 = nullptr, g16 = nullptr;
+  ~ChunkGraphs() {
+    if (g8) (void)hipGraphExecDestroy(g8);
+    if (g16) (void)hipGraphExecDestroy(g16);
+  }
+};
+
 int solve_batch(Workspace& ws, const nr_graph* const* gs, int count, const nr_config* cfg,
                 nr_result* outs) {
   int Wmax = 1, Nmax = 1;
@@ -2251,7 +2260,9 @@ int solve_batch(Workspace& ws, const nr_graph* const* gs, int count, const nr_co
                             st));
     a.count = (u32)todo.size();
     a.fill_step = (u32)fill_cap;
-    a.fill_to = (u32)std::max<u64>(cap / 4, 1);
+    /* the queue chunk: refills top the next level up to this many items */
+    a.fill_to = cfg->chunk_items > 0 ? (u32)std::min<u64>((u64)cfg->chunk_items, cap)
+                                     : (u32)std::max<u64>(cap / 4, 1);
     a.root_step = alone ? 1u : (u32)fill_cap;
     a.roots_when_empty = alone ? 1 : 0;
     {
@@ -2264,9 +2275,12 @@ int solve_batch(Workspace& ws, const nr_graph* const* gs, int count, const nr_co
     int level = 0, chunk = 8;
     bool finished = false;
     StreamCtl hsc;
-    while (!finished) {
-      for (int c = 0; c < chunk; ++c, ++level) {
-        int parity = level & 1, nxt = parity ^ 1;
+    /* A chunk is a fixed sequence of (expand, refill) launches starting on
+     * buffer 0 (chunks have even lengths): without per-launch timing it is
+     * captured once per pass and length into a HIP graph and replayed. */
+    auto enqueue_chunk = [&](int len) -> int {
+      for (int c = 0; c < len; ++c) {
+        int parity = c & 1, nxt = parity ^ 1;
         void* args[] = {&a, &parity};
         if (timing) NR_HIP(hipEventRecord(ws.events[2 + 2 * c], st));
         NR_HIP(hipLaunchKernel(kfun, dim3(grid), dim3(block), args, lay.total_bytes, st));
@@ -2274,6 +2288,34 @@ int solve_batch(Workspace& ws, const nr_graph* const* gs, int count, const nr_co
         void* rargs[] = {&a, &nxt};
         NR_HIP(hipLaunchKernel(kref, dim3(refill_grid), dim3(256), rargs, 0, st));
       }
+      return NR_OK;
+    };
+    ChunkGraphs graphs;
+    while (!finished) {
+      if (timing) {
+        if ((rc = enqueue_chunk(chunk))) return rc;
+      } else {
+        hipGraphExec_t& ge = chunk == 8 ? graphs.g8 : graphs.g16;
+        if (!ge) {
+          hipGraph_t gr = nullptr;
+          NR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+          const int erc = enqueue_chunk(chunk);
+          const hipError_t ec = hipStreamEndCapture(st, &gr);
+          if (erc) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return erc;
+          }
+          if (ec != hipSuccess) return NR_ERR_DEVICE;
+          const hipError_t ei = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(gr);
+          if (ei != hipSuccess) {
+            ge = nullptr;
+            return NR_ERR_DEVICE;
+          }
+        }
+        NR_HIP(hipGraphLaunch(ge, st));
+      }
+      level += chunk;
       launches += chunk;
       NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, kCtlCopyBytes, hipMemcpyDeviceToHost, st));
       NR_HIP(hipMemcpyAsync(&hsc, a.sc, sizeof hsc, hipMemcpyDeviceToHost, st));
@@ -2974,6 +3016,20 @@ int nr_tune_batch(const nr_graph* const* graphs, int count, const nr_config* cfg
     std::vector<nr_result> res((size_t)n);
     *tuned = c;
     double best = -1.0;
+    /* one trial: the subproblems per second of a warm batch, or < 0 when the
+     * schedule does not fit */
+    auto trial = [&](const nr_config& t, double* rate) -> int {
+      *rate = -1.0;
+      for (int rep = 0; rep < 2; ++rep) { /* the first run warms up */
+        const int rc = nr_solve_batch(graphs, n, &t, res.data());
+        if (rc == NR_ERR_UNSUPPORTED || rc == NR_ERR_NOMEM) return NR_OK; /* does not fit */
+        if (rc) return rc;
+      }
+      double nodes = 0.0;
+      for (const nr_result& r : res) nodes += (double)r.nodes;
+      *rate = res[0].elapsed_ms > 0 ? nodes / (res[0].elapsed_ms * 1e-3) : 0.0;
+      return NR_OK;
+    };
     const int blocks[3] = {256, 128, 64};
     const int per_cu[4] = {0, 1, 2, 4}; /* 0: the occupancy limit */
     for (int b : blocks)
@@ -2982,23 +3038,26 @@ int nr_tune_batch(const nr_graph* const* graphs, int count, const nr_config* cfg
         t.block_size = b;
         t.grid_blocks = m * cus;
         double rate = 0.0;
-        int rc = NR_OK;
-        for (int rep = 0; rep < 2 && rc == NR_OK; ++rep) { /* the first run warms up */
-          rc = nr_solve_batch(graphs, n, &t, res.data());
-          if (rc == NR_OK && rep == 1) {
-            double nodes = 0.0;
-            for (const nr_result& r : res) nodes += (double)r.nodes;
-            rate = res[0].elapsed_ms > 0 ? nodes / (res[0].elapsed_ms * 1e-3) : 0.0;
-          }
-        }
-        if (rc == NR_ERR_UNSUPPORTED || rc == NR_ERR_NOMEM) continue; /* does not fit */
-        if (rc) return rc;
+        if (int rc = trial(t, &rate)) return rc;
         if (rate > best) {
           best = rate;
           *tuned = t;
         }
       }
     if (best < 0) return NR_ERR_UNSUPPORTED;
+    /* the queue chunk for the fastest launch shape */
+    const long long cap = c.queue_capacity > 0 ? c.queue_capacity : (1ll << 20);
+    const nr_config shape = *tuned;
+    for (long long div : {8ll, 2ll}) {
+      nr_config t = shape;
+      t.chunk_items = (int)std::min<long long>(std::max<long long>(cap / div, 1), 1ll << 30);
+      double rate = 0.0;
+      if (int rc = trial(t, &rate)) return rc;
+      if (rate > best) {
+        best = rate;
+        *tuned = t;
+      }
+    }
     if (best_rate) *best_rate = best;
     return NR_OK;
   });

@@ -209,3 +209,36 @@ def test_er4096_full_batch():
         lo = max(res[i]["R"], tight["R"])
         hi = min(res[i]["R"] + res[i]["ESP"], tight["R"] + tight["ESP"])
         assert lo <= hi * (1 + 1e-12), i
+
+
+def test_graph_replays_equal_eager_launches():
+    """The chunks of a batch are HIP graphs replayed between host checks;
+    with per-launch timing the same launches run eagerly.  Four batches in a
+    row (graph, eager, graph after an unrelated eager solve, graph) must give
+    identical per-instance results."""
+    print("case batch graph replays vs eager launches", flush=True)
+    graphs = [er_pair(i)[0] for i in range(96)]
+    k8 = nr.Graph.load(graph_path("COMP8-09.txt"), 0, 7)
+    kw = dict(truncation_eps=1e-9, queue_capacity=1 << 20)
+    runs = [nr.solve_batch(graphs, **kw), nr.solve_batch(graphs, time_launches=True, **kw)]
+    nr.solve(k8, truncation_eps=0.0)
+    runs.append(nr.solve_batch(graphs, **kw))
+    runs.append(nr.solve_batch(graphs, **kw))
+    keys = ("paths", "nodes", "dead", "truncated", "iterations", "R", "ESP", "D")
+    for other in runs[1:]:
+        for a, b in zip(runs[0], other):
+            assert tuple(a[k] for k in keys) == tuple(b[k] for k in keys)
+    assert all(r["kernel_launches"] > 0 for r in runs[0])
+
+
+def test_tune_batch_picks_a_queue_chunk():
+    print("case nr_tune_batch", flush=True)
+    graphs = [er_pair(i)[0] for i in range(48)]
+    cfg, rate = nr.tune_batch(graphs, truncation_eps=1e-8, queue_capacity=1 << 20)
+    assert rate > 0
+    assert cfg.block_size in (64, 128, 256)
+    assert cfg.chunk_items in (0, (1 << 20) // 8, (1 << 20) // 2)
+    res = nr.solve_batch(graphs, cfg)
+    ref = nr.solve_batch(graphs, truncation_eps=1e-8, queue_capacity=1 << 20)
+    for a, b in zip(res, ref):
+        assert (a["paths"], a["nodes"], a["R"]) == (b["paths"], b["nodes"], b["R"])
