@@ -1022,7 +1022,10 @@ __device__ __forceinline__ void expand_level_items(const LevelArgs& a, const EX&
         accR.add(run); /* success term, main.cu:254-262 */
         ++c[C_PATHS];
       }
-    } else if (MODE == 0 && EX::kRowBytes > 0) { /* children with rows: child-major */
+    } else if (MODE == 0 && EX::kRowBytes > 0 && !settle) {
+      /* children with rows, every child queued: child-major (with settled
+       * children a live item queues one child on average and each lane
+       * writes its own, below) */
       /* `late` is wave-uniform: a late wave only carries, and has no children */
       if (!late)
         emit_children_wave<W, EX>(ex, out, a.stride, out_base, a.seg_cap, wbase, incl, stored, k,
@@ -1045,7 +1048,9 @@ __device__ __forceinline__ void expand_level_items(const LevelArgs& a, const EX&
         const u64 bit = 1ull << (e & 63);
         const double cm = run * ex.q[e];
         bool queue = true;
-        if (mass_on || settle) {
+        if (MODE == 0 && EX::kRowBytes > 0 && settle) {
+          queue = (lmask >> (k - 1 - j)) & 1ull; /* settle_children() counted the rest */
+        } else if (mass_on || settle) {
           if (cm < a.eps) { /* settled at creation: counted as a truncated node */
             accE.add(cm);
             ++c[C_NODES];
@@ -1106,6 +1111,11 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
   /* class rounds: always in MODE 4 (max-mass order of the path rule), never
    * in MODE 0 (level order) and MODE 1, by the argument in MODE 2 and 3 */
   const bool mass_on = MODE == 4 || ((MODE == 2 || MODE == 3) && a.mass_mode);
+  /* a launch queued past the one that found the queue empty for good and
+   * computed the totals has nothing to do */
+  if (a.fin_on_empty && __hip_atomic_load(&a.ctl->fin_done, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT))
+    return;
   u32* sp = (u32*)(smem + a.lay.off_extra);
   u32* s_cnt = sp + a.n_seg + 1;
   u32* part = s_cnt + 1;
