@@ -164,3 +164,45 @@ def test_learned_order_needs_a_model():
     with pytest.raises(nr.NRError) as ei:
         nr.solve(g, order="learned")
     assert ei.value.status == 4
+
+
+def _feature_multiset_equal(Xg, Yg, Xo, Yo):
+    """Rows of the engine's tree log against the CPU restatement: the exact
+    features (counts, ratios, q, flags) and the label must match exactly,
+    the four log2-based ones (f0, f8, f9, f13: float32 log2 on either side,
+    then bfloat16) within one bfloat16 step."""
+    from features_oracle import bf16
+
+    exact = [1, 2, 3, 4, 5, 6, 7, 10, 11, 12, 14, 15]
+    logs = [0, 8, 9, 13]
+    Xo = bf16(Xo)
+
+    def groups(X, Y):
+        d = {}
+        for x, y in zip(X, Y):
+            key = (tuple(x[exact].view(np.uint32).tolist()), round(float(y), 4))
+            d.setdefault(key, []).append(x[logs].astype(np.float64))
+        return d
+
+    gg, go = groups(Xg, Yg), groups(Xo, Yo)
+    assert gg.keys() == go.keys()
+    for key in gg:
+        a = np.array(sorted(map(tuple, gg[key])))
+        b = np.array(sorted(map(tuple, go[key])))
+        assert a.shape == b.shape, key
+        assert np.all(np.abs(a - b) <= 2.0 ** -7 * np.maximum(np.abs(a), np.abs(b)) + 1e-6), key
+
+
+@pytest.mark.parametrize("name,s,t,eps,cap", [("COMP8-09.txt", 0, 7, 0.0, 1 << 16),
+                                              ("COMP8-03.txt", 0, 7, 1e-7, 1 << 18)])
+def test_tree_log_features_match_restatement(name, s, t, eps, cap):
+    """Every child's 16 logged features (the learned order's inputs) against
+    oracle/features_oracle.py's restatement over the same tree."""
+    from features_oracle import child_feature_rows
+
+    print(f"case tree-log features {name} eps={eps}", flush=True)
+    g = nr.Graph.load(graph_path(name), s, t)
+    Xg, Yg, res = nr.log_subproblems(g, 1 << 20, seed=3, truncation_eps=eps, queue_capacity=cap)
+    Xo, Yo, nodes = child_feature_rows(OracleGraph.load(graph_path(name)), s, t, eps)
+    assert res["nodes"] == nodes and len(Yg) == len(Yo) == nodes - 1
+    _feature_multiset_equal(Xg, Yg, Xo, Yo)

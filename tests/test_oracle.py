@@ -811,3 +811,26 @@ def test_prefix_shards_add_up():
         assert sum(p[f] for p in parts) == whole[f], f
     assert abs(sum(p["R"] for p in parts) - whole["R"]) <= 1e-14
     assert abs(sum(p["D"] for p in parts) - whole["D"]) <= 1e-14
+
+
+@pytest.mark.parametrize("name,s,t,eps", [("test_kite.txt", 0, 4, 0.0), ("COMP8-03.txt", 0, 7, 1e-7)])
+def test_child_feature_restatement_identities(name, s, t, eps):
+    """The learned order's child features restated on the CPU
+    (oracle/features_oracle.py) obey what their definition fixes."""
+    from features_oracle import child_feature_rows
+
+    og = OracleGraph.load(graph_path(name))
+    X, Y, nodes = child_feature_rows(og, s, t, eps)
+    assert len(X) == nodes - 1  # every node but the root is some item's child
+    # each edge is UNKNOWN, WORKING or FAILED in the child
+    assert np.abs(X[:, 1] + X[:, 2] + X[:, 3] - 1.0).max() < 1e-6
+    # a child's mass is its parent's times a probability: f0 <= f13
+    assert np.all(X[:, 0] <= X[:, 13] + 1e-6)
+    # the last child of an item: f4 = 1 and f14 = 0, and only then
+    last = np.abs(X[:, 4] - 1.0) < 1e-6
+    assert np.array_equal(last, np.abs(X[:, 14]) < 1e-9)
+    assert np.all((X[:, 12] == 0.0) | (X[:, 12] == 1.0)) and np.all(X[:, 15] == 1.0)
+    assert np.all((X[:, 9] >= 0.0) & (X[:, 9] <= 1.0))
+    # labels are log2 of integer subtree sizes
+    sizes = np.round(np.exp2(Y)).astype(np.int64)
+    assert np.allclose(np.exp2(Y), sizes, rtol=1e-12) and sizes.min() >= 1
