@@ -729,9 +729,12 @@ struct Expander {
   }
 
   /* find_planes() with the smallest LDS slice: adjacency derived on every
-   * use (no cache) and each node's BFS level kept as a byte (lvb: node u at
-   * (u>>2)*4*stride + (u&3) of this lane's slice); the parent of v is the
-   * lowest-index neighbour on the previous level, as in find(). */
+   * use (no cache) and each node's BFS parent kept as a byte (lvb: node u
+   * at (u>>2)*4*stride + (u&3) of this lane's slice).  The frontier is
+   * expanded in ascending node order and a node's parent is the frontier
+   * node that reaches it first, so the parent of v is the lowest-index
+   * neighbour on the previous level, as in find(); the walk back follows
+   * the parent bytes and derives no adjacency. */
   __device__ int find_lean(const u64 (&known)[W], const u64 (&up)[W], unsigned char* lvb) const {
     if (src == dst) {
       plen = 0;
@@ -740,13 +743,11 @@ struct Expander {
     u64 Dn[W];
 #pragma unroll
     for (int i = 0; i < W; ++i) Dn[i] = known[i] & ~up[i];
-    for (int i = 0; i < (n + 3) / 4; ++i) *(u32*)(lvb + i * 4 * stride) = 0xffffffffu;
-    auto setl = [&](int u, int l) { lvb[(u >> 2) * 4 * stride + (u & 3)] = (unsigned char)l; };
-    auto getl = [&](int u) { return (int)lvb[(u >> 2) * 4 * stride + (u & 3)]; };
+    auto setp = [&](int v, int u) { lvb[(v >> 2) * 4 * stride + (v & 3)] = (unsigned char)u; };
+    auto getp = [&](int v) { return (int)lvb[(v >> 2) * 4 * stride + (v & 3)]; };
     const NS dbit = (NS)1 << dst;
     NS vis = (NS)1 << src;
     NS F = vis;
-    setl(src, 0);
     int L = 0;
     for (;;) {
       NS nx = 0;
@@ -754,35 +755,24 @@ struct Expander {
       while (f) {
         const int u = ctzs<NS>(f);
         f &= f - 1;
-        nx |= adj_of(u, Dn);
+        NS nb = adj_of(u, Dn) & ~(vis | nx); /* reached first from u */
+        nx |= nb;
+        while (nb) {
+          setp(ctzs<NS>(nb), u);
+          nb &= nb - 1;
+        }
         if (nx & dbit) break;
       }
-      nx &= ~vis;
       if (!nx) return -1;
       ++L;
       if (nx & dbit) break;
-      NS m = nx;
-      while (m) {
-        setl(ctzs<NS>(m), L);
-        m &= m - 1;
-      }
       vis |= nx;
       F = nx;
     }
     plen = L;
     int v = dst, k = 0;
     for (int l = L; l >= 1; --l) {
-      NS cand = adj_of(v, Dn);
-      int u = -1;
-      while (cand) {
-        const int c = ctzs<NS>(cand);
-        cand &= cand - 1;
-        if (getl(c) == l - 1) {
-          u = c;
-          break;
-        }
-      }
-      if (u < 0) return -2;
+      const int u = getp(v);
       const int e = eidm[u * n + v];
       if (!((pick<W>(known, e >> 6) >> (e & 63)) & 1ull)) uh[(k++) * stride] = (u16)e;
       v = u;

@@ -1022,10 +1022,7 @@ __device__ __forceinline__ void expand_level_items(const LevelArgs& a, const EX&
         accR.add(run); /* success term, main.cu:254-262 */
         ++c[C_PATHS];
       }
-    } else if (MODE == 0 && EX::kRowBytes > 0 && !settle) {
-      /* children with rows, every child queued: child-major (with settled
-       * children a live item queues one child on average and each lane
-       * writes its own, below) */
+    } else if (MODE == 0 && EX::kRowBytes > 0) { /* children with rows: child-major */
       /* `late` is wave-uniform: a late wave only carries, and has no children */
       if (!late)
         emit_children_wave<W, EX>(ex, out, a.stride, out_base, a.seg_cap, wbase, incl, stored, k,
@@ -1048,9 +1045,7 @@ __device__ __forceinline__ void expand_level_items(const LevelArgs& a, const EX&
         const u64 bit = 1ull << (e & 63);
         const double cm = run * ex.q[e];
         bool queue = true;
-        if (MODE == 0 && EX::kRowBytes > 0 && settle) {
-          queue = (lmask >> (k - 1 - j)) & 1ull; /* settle_children() counted the rest */
-        } else if (mass_on || settle) {
+        if (mass_on || settle) {
           if (cm < a.eps) { /* settled at creation: counted as a truncated node */
             accE.add(cm);
             ++c[C_NODES];
