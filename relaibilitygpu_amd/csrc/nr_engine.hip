@@ -2291,29 +2291,34 @@ int solve_batch(Workspace& ws, const nr_graph* const* gs, int count, const nr_co
       return NR_OK;
     };
     ChunkGraphs graphs;
+    bool use_graphs = !timing;
+    /* the chunk's graph, captured on first use; false (and eager launches
+     * from then on) if the runtime cannot capture or instantiate it */
+    auto chunk_graph = [&](int len, hipGraphExec_t& ge) -> bool {
+      if (ge) return true;
+      if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+      }
+      const int erc = enqueue_chunk(len);
+      hipGraph_t gr = nullptr;
+      const hipError_t ec = hipStreamEndCapture(st, &gr);
+      bool ok = erc == NR_OK && ec == hipSuccess && gr;
+      if (ok && hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0) != hipSuccess) {
+        ge = nullptr;
+        ok = false;
+      }
+      if (gr) (void)hipGraphDestroy(gr);
+      if (!ok) (void)hipGetLastError();
+      return ok;
+    };
     while (!finished) {
-      if (timing) {
-        if ((rc = enqueue_chunk(chunk))) return rc;
-      } else {
-        hipGraphExec_t& ge = chunk == 8 ? graphs.g8 : graphs.g16;
-        if (!ge) {
-          hipGraph_t gr = nullptr;
-          NR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-          const int erc = enqueue_chunk(chunk);
-          const hipError_t ec = hipStreamEndCapture(st, &gr);
-          if (erc) {
-            if (gr) (void)hipGraphDestroy(gr);
-            return erc;
-          }
-          if (ec != hipSuccess) return NR_ERR_DEVICE;
-          const hipError_t ei = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
-          (void)hipGraphDestroy(gr);
-          if (ei != hipSuccess) {
-            ge = nullptr;
-            return NR_ERR_DEVICE;
-          }
-        }
+      hipGraphExec_t& ge = chunk == 8 ? graphs.g8 : graphs.g16;
+      if (use_graphs && !chunk_graph(chunk, ge)) use_graphs = false;
+      if (use_graphs) {
         NR_HIP(hipGraphLaunch(ge, st));
+      } else if ((rc = enqueue_chunk(chunk))) {
+        return rc;
       }
       level += chunk;
       launches += chunk;
