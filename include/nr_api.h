@@ -432,6 +432,20 @@ int nr_log_subproblems(const nr_graph* g, const nr_config* cfg, long long max_re
                        unsigned long long seed, float* features, float* labels,
                        long long* n_out, nr_result* result);
 
+/* nr_log_subproblems with a second label per record: cond_rel[max_records]
+ * receives the child's conditional reliability, the success mass its
+ * subtree finds (main.cu:254-262 summed bottom-up over the subtree) divided
+ * by its own mass.  For an untruncated subtree this is P(src reaches dst |
+ * the child's WORKING and FAILED edges); under truncation it is the lower
+ * end of that probability's bracket.  Accumulated on the device in 62-bit
+ * fixed point relative to each record's mass (exactly repeatable; absolute
+ * error below depth * children * 2^-62).  The sample, features and size
+ * labels are those nr_log_subproblems returns for the same arguments.
+ * No counterpart in the reference (its tree is never logged). */
+int nr_log_subproblems_rel(const nr_graph* g, const nr_config* cfg, long long max_records,
+                           unsigned long long seed, float* features, float* labels,
+                           double* cond_rel, long long* n_out, nr_result* result);
+
 typedef struct nr_train_config {
   int epochs;         /* default 8 */
   int batch;          /* samples per Adam step, default 4096 */

@@ -22,13 +22,20 @@ path-rule tree with the oracle's own expansions (oracle_ffi.OracleGraph
 where u, w, f are the parent's UNKNOWN, WORKING and FAILED edge counts, E
 the edge count, n the node count and k the parent's UNKNOWN hops; each
 value is then rounded to bfloat16 (round to nearest even) as the log stores
-it.  The label of a child is log2 of its subtree's node count.
+it.  The label of a child is log2 of its subtree's node count; its second
+label (child_rows_with_reliability) is its conditional reliability: the
+success mass its subtree finds (main.cu:254-262, summed over the subtree)
+divided by its own mass, which for an untruncated subtree is the
+probability that src reaches dst given the child's WORKING and FAILED
+edges.
 
 Pinned by identities the definition fixes, checked on the restatement
 itself (tests/test_oracle.py): f1 + f2 + f3 = 1 (every edge is UNKNOWN,
 WORKING or FAILED in the child), f4 = 1 and f14 = 0 for the last child,
 f0 <= f13 (a child's mass is its parent's times a probability), one row per
-non-root node, and labels that are log2 of whole subtree sizes.
+non-root node, and labels that are log2 of whole subtree sizes.  The
+conditional reliabilities of an untruncated tree are pinned against brute
+force over the child's UNKNOWN edges (tests/test_oracle.py).
 """
 from __future__ import annotations
 
@@ -61,8 +68,17 @@ def _popcount(words) -> int:
 
 
 def child_feature_rows(og, src: int, dst: int, eps: float = 0.0):
-    """(features [m, 16] float32 before rounding, labels [m] float64) of
-    every child of the path-rule tree rooted at the all-UNKNOWN item."""
+    """(features [m, 16] float32 before rounding, labels [m] float64, node
+    count) of every child of the path-rule tree rooted at the all-UNKNOWN
+    item."""
+    X, Y, _, _, nodes = child_rows_with_reliability(og, src, dst, eps)
+    return X, Y, nodes
+
+
+def child_rows_with_reliability(og, src: int, dst: int, eps: float = 0.0):
+    """(features, log2-size labels, conditional reliabilities [m] float64,
+    the children's (known, up) planes [m, 2, W] uint64, node count) of every
+    child of the path-rule tree rooted at the all-UNKNOWN item."""
     rp, ci, ei, p = og.arrays()
     n = len(rp) - 1
     E = len(p)
@@ -90,13 +106,16 @@ def child_feature_rows(og, src: int, dst: int, eps: float = 0.0):
                 d += 1
         return d
 
-    rows, labels = [], []
+    rows, labels, conds, planes = [], [], [], []
 
     def walk(known, up, mass):
-        """Expands the item, records its children, returns its subtree size."""
+        """Expands the item, records its children, returns its subtree size
+        and the success mass its subtree finds."""
         k, succ, ck, cu, cm, path = og.expand(src, dst, known, up, mass)
-        if k <= 0:
-            return 1
+        if k < 0:
+            return 1, 0.0
+        if k == 0:
+            return 1, succ
         nk = _popcount(known)
         u_p, w_p, f_p = E - nk, _popcount(up), nk - _popcount(up)
         hops = [int(e) for e in path if not ((int(known[e >> 6]) >> (e & 63)) & 1)]
@@ -136,14 +155,19 @@ def child_feature_rows(og, src: int, dst: int, eps: float = 0.0):
                  np.float32(1.0)]
             rows.append(np.array(f, np.float32))
             labels.append(0.0)
+            conds.append(0.0)
+            planes.append(np.stack([ck[t], cu[t]]))
             lp_prefix = np.float32(lp_prefix + lp32[e])
-        total = 1
+        total, found = 1, succ
         for t in range(k):
-            sz = walk(ck[t], cu[t], float(cm[t])) if cm[t] >= eps else 1
+            sz, r = walk(ck[t], cu[t], float(cm[t])) if cm[t] >= eps else (1, 0.0)
             labels[first + t] = math.log2(sz)
+            conds[first + t] = r / float(cm[t]) if cm[t] > 0.0 else 0.0
             total += sz
-        return total
+            found += r
+        return total, found
 
     z = np.zeros(W, np.uint64)
-    nodes = walk(z, z, 1.0)
-    return np.array(rows, np.float32).reshape(-1, N_FEATURES), np.array(labels), nodes
+    nodes, _ = walk(z, z, 1.0)
+    return (np.array(rows, np.float32).reshape(-1, N_FEATURES), np.array(labels),
+            np.array(conds), np.array(planes, np.uint64).reshape(-1, 2, W), nodes)

@@ -201,6 +201,9 @@ def lib() -> ctypes.CDLL:
     fp = P(ctypes.c_float)
     L.nr_log_subproblems.argtypes = [vp, P(Config), ctypes.c_longlong, ctypes.c_ulonglong, fp,
                                      fp, P(ctypes.c_longlong), P(Result)]
+    L.nr_log_subproblems_rel.argtypes = [vp, P(Config), ctypes.c_longlong, ctypes.c_ulonglong,
+                                         fp, fp, P(ctypes.c_double), P(ctypes.c_longlong),
+                                         P(Result)]
     L.nr_train_config_init.argtypes = [P(TrainConfig)]
     L.nr_mlp_init.argtypes = [ctypes.c_ulonglong, fp]
     L.nr_mlp_grad.argtypes = [fp, fp, ctypes.c_longlong, fp, fp, P(ctypes.c_double), ctypes.c_int]
@@ -492,6 +495,25 @@ def log_subproblems(graph: Graph, max_records: int, seed: int = 1,
                                     _ptr(X, ctypes.c_float), _ptr(Y, ctypes.c_float),
                                     ctypes.byref(n), ctypes.byref(res)), "nr_log_subproblems")
     return X[: n.value], Y[: n.value], res.as_dict()
+
+
+def log_subproblems_rel(graph: Graph, max_records: int, seed: int = 1,
+                        config: Config | None = None, **kw):
+    """Tree log with both labels (nr_log_subproblems_rel): (features [m,16],
+    log2 subtree nodes [m], conditional reliabilities [m] float64, result
+    dict); the sample is the one log_subproblems returns."""
+    cfg = config if config is not None else make_config(**kw)
+    X = np.zeros((max(max_records, 1), MLP_FEATURES), np.float32)
+    Y = np.zeros(max(max_records, 1), np.float32)
+    C = np.zeros(max(max_records, 1), np.float64)
+    n = ctypes.c_longlong()
+    res = Result()
+    _check(lib().nr_log_subproblems_rel(graph.handle, ctypes.byref(cfg), int(max_records),
+                                        int(seed), _ptr(X, ctypes.c_float),
+                                        _ptr(Y, ctypes.c_float), _ptr(C, ctypes.c_double),
+                                        ctypes.byref(n), ctypes.byref(res)),
+           "nr_log_subproblems_rel")
+    return X[: n.value], Y[: n.value], C[: n.value], res.as_dict()
 
 
 def log_choices(graph: Graph, max_records: int, seed: int = 1,

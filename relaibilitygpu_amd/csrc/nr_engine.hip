@@ -1139,6 +1139,8 @@ struct LevelExtra {
   u32* log_level = nullptr;
   u64* log_count = nullptr;
   u64 log_cap = 0;
+  double* log_mass = nullptr; /* conditional-reliability labels (optional) */
+  double* log_succ = nullptr;
   const MlpDev* mlp = nullptr;
   const MlpDev* chooser = nullptr; /* NR_RULE_LEARNED: the path chooser */
   float* log_choice = nullptr;     /* tree log of the choices (mode 1) */
@@ -1530,6 +1532,8 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
     a.log_level = x->log_level;
     a.log_count = x->log_count;
     a.log_cap = x->log_cap;
+    a.log_mass = x->log_mass;
+    a.log_succ = x->log_succ;
   }
   if (mode == 2) a.mlp = x->mlp;
   if (x) {
@@ -2478,24 +2482,50 @@ __global__ void __launch_bounds__(256) k_log_bounds(const u32* level, u64 n, u64
     if (i == 0 || level[i] != level[i - 1]) first[level[i]] = i;
 }
 
+/* Conditional reliability of record r once its subtree is accumulated: its
+ * own success mass over its mass, plus its children's shares, which acc[r]
+ * holds in fixed point (units of 2^-62 of r's mass).  Integer sums do not
+ * depend on the order the device adds in, so the labels repeat exactly;
+ * each share is truncated by at most 2^-62, so the label's error is below
+ * depth * children * 2^-62. */
+__device__ inline double log_cond(const double* mass, const double* succ, const u64* acc,
+                                  u64 r) {
+  const double m = mass[r];
+  if (!(m > 0.0)) return 0.0;
+  const double c = succ[r] / m + (double)acc[r] * 0x1p-62;
+  return c < 1.0 ? c : 1.0;
+}
+
 /* one depth, deepest first: each child's finished subtree size is added to
- * its parent's (the root has no record) */
+ * its parent's (the root has no record); with acc, the child's conditional
+ * reliability times its share of the parent's mass is added to the
+ * parent's fixed-point sum */
 __global__ void __launch_bounds__(256) k_log_accumulate(const u32* parent, u64* size, u64 r0,
-                                                        u64 r1) {
+                                                        u64 r1, const double* mass,
+                                                        const double* succ, u64* acc) {
   for (u64 r = r0 + (u64)blockIdx.x * blockDim.x + threadIdx.x; r < r1;
        r += (u64)gridDim.x * blockDim.x) {
     const u32 p = parent[r];
-    if (p != 0xffffffffu) atomicAdd((unsigned long long*)&size[p], (unsigned long long)size[r]);
+    if (p == 0xffffffffu) continue;
+    atomicAdd((unsigned long long*)&size[p], (unsigned long long)size[r]);
+    if (acc && mass[p] > 0.0) {
+      const double x = log_cond(mass, succ, acc, r) * (mass[r] / mass[p]);
+      const u64 v = (u64)((x < 1.0 ? x : 1.0) * 0x1p62);
+      if (v) atomicAdd((unsigned long long*)&acc[p], (unsigned long long)v);
+    }
   }
 }
 
 __global__ void __launch_bounds__(256) k_log_gather(const float* feat, const u64* size,
-                                                    const u64* idx, u64 m, float* X, float* Y) {
+                                                    const u64* idx, u64 m, float* X, float* Y,
+                                                    const double* mass, const double* succ,
+                                                    const u64* acc, double* C) {
   for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < m;
        i += (u64)gridDim.x * blockDim.x) {
     const u64 r = idx[i];
     for (int k = 0; k < kFeat; ++k) X[i * kFeat + k] = feat[r * kFeat + k];
     Y[i] = (float)log2((double)size[r]);
+    if (C) C[i] = log_cond(mass, succ, acc, r);
   }
 }
 
@@ -2537,10 +2567,10 @@ __global__ void __launch_bounds__(256) k_log_select(const float* feat, const u64
  * nodes of the record. */
 int log_subproblems(Workspace& ws, const nr_graph* g, const nr_config* cfg, long long max_out,
                     unsigned long long seed, float* X, float* Y, long long* n_out,
-                    nr_result* res, int choices = 0) {
+                    nr_result* res, int choices = 0, double* C = nullptr) {
   size_t free_b = 0, total_b = 0;
   NR_HIP(hipMemGetInfo(&free_b, &total_b));
-  const size_t rec_bytes = kFeat * 4 * (choices ? 2 : 1) + 4 + 4 + 8;
+  const size_t rec_bytes = kFeat * 4 * (choices ? 2 : 1) + 4 + 4 + 8 + (C ? 24 : 0);
   const u64 log_cap = std::min<u64>((u64)(free_b / 4 / rec_bytes), 1ull << 31);
   DevMem mem;
   LevelExtra x;
@@ -2556,6 +2586,14 @@ int log_subproblems(Workspace& ws, const nr_graph* g, const nr_config* cfg, long
   if (choices) {
     if ((rc = mem.alloc(&x.log_choice, (size_t)log_cap * kChoiceFeat))) return rc;
     NR_HIP(hipMemsetAsync(x.log_choice, 0, sizeof(float) * kChoiceFeat * log_cap, st));
+  }
+  u64* dacc = nullptr;
+  if (C) {
+    if ((rc = mem.alloc(&x.log_mass, log_cap)) || (rc = mem.alloc(&x.log_succ, log_cap)) ||
+        (rc = mem.alloc(&dacc, log_cap)))
+      return rc;
+    NR_HIP(hipMemsetAsync(x.log_succ, 0, sizeof(double) * log_cap, st));
+    NR_HIP(hipMemsetAsync(dacc, 0, sizeof(u64) * log_cap, st));
   }
   NR_HIP(hipMemsetAsync(x.log_count, 0, sizeof(u64), st));
   nr_config c2 = *cfg;
@@ -2586,7 +2624,8 @@ int log_subproblems(Workspace& ws, const nr_graph* g, const nr_config* cfg, long
         if (first[e] < r1 && first[e] >= r0) r1 = first[e];
       if (r0 >= r1) continue;
       hipLaunchKernelGGL(k_log_accumulate, dim3(1024), dim3(256), 0, st,
-                         (const u32*)x.log_parent, dsize, r0, r1);
+                         (const u32*)x.log_parent, dsize, r0, r1, (const double*)x.log_mass,
+                         (const double*)x.log_succ, dacc);
     }
   }
   /* the sample: records whose seeded content hash falls below a threshold
@@ -2634,15 +2673,18 @@ int log_subproblems(Workspace& ws, const nr_graph* g, const nr_config* cfg, long
   if (m) {
     u64* didx;
     float *dX, *dY;
+    double* dC = nullptr;
     if ((rc = mem.alloc(&didx, m)) || (rc = mem.alloc(&dX, (size_t)m * kFeat)) ||
-        (rc = mem.alloc(&dY, m)))
+        (rc = mem.alloc(&dY, m)) || (C && (rc = mem.alloc(&dC, m))))
       return rc;
     NR_HIP(hipMemcpyAsync(didx, idx.data(), 8 * m, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_log_gather, dim3(1024), dim3(256), 0, st,
                        (const float*)(choices ? x.log_choice : x.log_feat), (const u64*)dsize,
-                       (const u64*)didx, m, dX, dY);
+                       (const u64*)didx, m, dX, dY, (const double*)x.log_mass,
+                       (const double*)x.log_succ, (const u64*)dacc, dC);
     NR_HIP(hipMemcpyAsync(X, dX, sizeof(float) * m * kFeat, hipMemcpyDeviceToHost, st));
     NR_HIP(hipMemcpyAsync(Y, dY, sizeof(float) * m, hipMemcpyDeviceToHost, st));
+    if (C) NR_HIP(hipMemcpyAsync(C, dC, sizeof(double) * m, hipMemcpyDeviceToHost, st));
     NR_HIP(hipStreamSynchronize(st));
   }
   *n_out = (long long)m;
@@ -2982,6 +3024,28 @@ int nr_log_subproblems(const nr_graph* g, const nr_config* cfg, long long max_re
     nr::WsLock L(cfg->device);
     if (L.rc) return L.rc;
     return nr::log_subproblems(*L.ws, g, cfg, max_records, seed, features, labels, n_out, result);
+  });
+}
+
+int nr_log_subproblems_rel(const nr_graph* g, const nr_config* cfg, long long max_records,
+                           unsigned long long seed, float* features, float* labels,
+                           double* cond_rel, long long* n_out, nr_result* result) {
+  return nr::guarded([&]() -> int {
+    if (!g || !cfg || max_records < 0 || !n_out || !result ||
+        (max_records > 0 && (!features || !labels || !cond_rel)))
+      return NR_ERR_ARG;
+    if (cfg->queue_capacity < 0 || cfg->truncation_eps < 0 || cfg->sweep_period < 0)
+      return NR_ERR_ARG;
+    if (cfg->rule != NR_RULE_PATH && cfg->rule != NR_RULE_ALTERNATE) return NR_ERR_UNSUPPORTED;
+    nr::Variant var;
+    int rc = nr::pick_variant(g, var);
+    if (rc) return rc;
+    if (!var.level_log) return NR_ERR_UNSUPPORTED;
+    nr::WsLock L(cfg->device);
+    if (L.rc) return L.rc;
+    double dummy = 0.0;
+    return nr::log_subproblems(*L.ws, g, cfg, max_records, seed, features, labels, n_out, result,
+                               0, cond_rel ? cond_rel : &dummy);
   });
 }
 

@@ -123,6 +123,11 @@ struct LevelArgs {
   u32* log_level;     /* [cap] tree depth of the child */
   u64* log_count;
   u64 log_cap;
+  /* optional (null: not logged): the child's FP64 mass, and the success
+   * mass its own expansion finds (zero-filled beforehand; a dead or
+   * truncated record keeps zero) */
+  double* log_mass;
+  double* log_succ;
   /* learned order */
   const MlpDev* mlp;
   /* learned path choice: the chooser's exact inference form; in a tree log
@@ -1078,6 +1083,7 @@ __device__ __forceinline__ void expand_level_items(const LevelArgs& a, const EX&
                 for (int x = 0; x < kFeat; ++x) dstf[x] = bf16_value(bf16_bits(f[x]));
                 a.log_parent[rid] = tag_in;
                 a.log_level[rid] = (u32)level + 1u;
+                if (a.log_mass) a.log_mass[rid] = cm;
               }
             }
           } else { /* segment full: main.cu:245-252 */
@@ -1095,6 +1101,7 @@ __device__ __forceinline__ void expand_level_items(const LevelArgs& a, const EX&
       }
       accR.add(run); /* success term, main.cu:254-262 */
       ++c[C_PATHS];
+      if (MODE == 1 && a.log_succ && tag_in < a.log_cap) a.log_succ[tag_in] = run;
     }
   }
 

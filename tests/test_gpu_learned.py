@@ -206,3 +206,59 @@ def test_tree_log_features_match_restatement(name, s, t, eps, cap):
     Xo, Yo, nodes = child_feature_rows(OracleGraph.load(graph_path(name)), s, t, eps)
     assert res["nodes"] == nodes and len(Yg) == len(Yo) == nodes - 1
     _feature_multiset_equal(Xg, Yg, Xo, Yo)
+
+
+@pytest.mark.parametrize("name,s,t,eps,cap", [("COMP8-09.txt", 0, 7, 0.0, 1 << 16),
+                                              ("COMP8-03.txt", 0, 7, 1e-7, 1 << 18),
+                                              ("test_kite.txt", 0, 4, 0.0, 1 << 16)])
+def test_tree_log_conditional_reliability(name, s, t, eps, cap):
+    """Second label of the tree log (nr_log_subproblems_rel): every child's
+    conditional reliability against the restatement's (subtree success mass
+    over the child's mass, FP64), within 1e-13; the sample, features and
+    size labels equal nr_log_subproblems'."""
+    from features_oracle import bf16, child_rows_with_reliability
+
+    print(f"case tree-log conditional reliability {name} eps={eps}", flush=True)
+    g = nr.Graph.load(graph_path(name), s, t)
+    Xg, Yg, Cg, res = nr.log_subproblems_rel(g, 1 << 20, seed=3, truncation_eps=eps,
+                                             queue_capacity=cap)
+    X1, Y1, _ = nr.log_subproblems(g, 1 << 20, seed=3, truncation_eps=eps, queue_capacity=cap)
+    assert np.array_equal(Xg, X1) and np.array_equal(Yg, Y1)
+    Xo, Yo, Co, _, nodes = child_rows_with_reliability(OracleGraph.load(graph_path(name)), s, t,
+                                                       eps)
+    assert res["nodes"] == nodes and len(Cg) == len(Co) == nodes - 1
+    assert np.all((Cg >= 0.0) & (Cg <= 1.0))
+    exact = [1, 2, 3, 4, 5, 6, 7, 10, 11, 12, 14, 15]
+    Xo = bf16(Xo)
+
+    def groups(X, Y, C):
+        d = {}
+        for x, y, c in zip(X, Y, C):
+            d.setdefault((tuple(x[exact].view(np.uint32).tolist()), round(float(y), 4)),
+                         []).append(float(c))
+        return {k: np.sort(v) for k, v in d.items()}
+
+    gg, go = groups(Xg, Yg, Cg), groups(Xo, Yo, Co)
+    assert gg.keys() == go.keys()
+    worst = max(float(np.abs(gg[k] - go[k]).max()) for k in gg)
+    assert worst <= 1e-13, worst
+
+
+def test_tree_log_conditional_reliability_k10_repeats():
+    """K10 in full (14.9 M subproblems): the fixed-point labels repeat bit
+    for bit and lie in [0, 1]; leaves carry exactly 0 or 1; the sample,
+    features and size labels equal the plain log's."""
+    print("case tree-log conditional reliability K10 repeat", flush=True)
+    g = nr.Graph.load(graph_path("COMP10-05.txt"), 0, 9)
+    kw = dict(truncation_eps=0.0, queue_capacity=1 << 22)
+    X1, Y1, C1, r1 = nr.log_subproblems_rel(g, 50000, seed=7, **kw)
+    X2, Y2, C2, r2 = nr.log_subproblems_rel(g, 50000, seed=7, **kw)
+    X0, Y0, _ = nr.log_subproblems(g, 50000, seed=7, **kw)
+    assert len(C1) == 50000 and r1["nodes"] == 14866064
+    assert np.array_equal(C1, C2) and np.array_equal(X1, X0) and np.array_equal(Y1, Y0)
+    assert np.all((C1 >= 0.0) & (C1 <= 1.0))
+    # a leaf (subtree of one node) is a success path or a dead end: its
+    # conditional reliability is exactly 1 or 0
+    leaf = Y1 == 0.0
+    assert leaf.any() and np.all((C1[leaf] == 0.0) | (C1[leaf] == 1.0))
+

@@ -834,3 +834,66 @@ def test_child_feature_restatement_identities(name, s, t, eps):
     # labels are log2 of integer subtree sizes
     sizes = np.round(np.exp2(Y)).astype(np.int64)
     assert np.allclose(np.exp2(Y), sizes, rtol=1e-12) and sizes.min() >= 1
+
+
+def _brute_conditional(rp, ci, ei, p, src, dst, known, up):
+    """P(src reaches dst | the WORKING and FAILED edges of (known, up)):
+    every assignment of the UNKNOWN edges, vectorised over assignments,
+    reachability by relaxing every edge n times."""
+    n, E = len(rp) - 1, len(p)
+    bit = lambda planes, e: (int(planes[e >> 6]) >> (e & 63)) & 1  # noqa: E731
+    ends = {}
+    for u in range(n):
+        for j in range(rp[u], rp[u + 1]):
+            ends[int(ei[j])] = (u, int(ci[j]))
+    unknown = [e for e in range(E) if not bit(known, e)]
+    A = 1 << len(unknown)
+    on = np.zeros((A, E), bool)
+    prob = np.ones(A)
+    assign = (np.arange(A)[:, None] >> np.arange(len(unknown))[None, :]) & 1
+    for e in range(E):
+        if bit(known, e):
+            on[:, e] = bool(bit(up, e))
+    for i, e in enumerate(unknown):
+        on[:, e] = assign[:, i] == 1
+        prob *= np.where(on[:, e], p[e], 1.0 - p[e])
+    reach = np.zeros((A, n), bool)
+    reach[:, src] = True
+    for _ in range(n):
+        for e, (a, b) in ends.items():
+            w = on[:, e]
+            ra, rb = reach[:, a].copy(), reach[:, b].copy()
+            reach[:, b] |= ra & w
+            reach[:, a] |= rb & w
+    return float(prob[reach[:, dst]].sum())
+
+
+@pytest.mark.parametrize("name,s,t,max_unknown,limit", [
+    ("test_kite.txt", 0, 4, 64, None), ("cycle4.txt", 0, 2, 64, None),
+    ("test_bridge.txt", 0, 3, 64, None), ("COMP8-09.txt", 0, 7, 11, 400)])
+def test_child_conditional_reliability_brute_force(name, s, t, max_unknown, limit):
+    """The tree log's second label (a child's subtree success mass over its
+    own mass) equals brute force over the child's UNKNOWN edges."""
+    from features_oracle import child_rows_with_reliability
+
+    og = load(name)
+    rp, ci, ei, p = og.arrays()
+    X, Y, C, planes, nodes = child_rows_with_reliability(og, s, t, 0.0)
+    assert len(C) == nodes - 1 and np.all((C >= 0.0) & (C <= 1.0 + 1e-15))
+    E = len(p)
+    picked = [i for i in range(len(C))
+              if E - sum(bin(int(w)).count("1") for w in planes[i, 0]) <= max_unknown]
+    if limit:
+        picked = picked[:: max(1, len(picked) // limit)]
+    assert len(picked) >= min(len(C), 20)
+    for i in picked:
+        bf = _brute_conditional(rp, ci, ei, p, s, t, planes[i, 0], planes[i, 1])
+        assert abs(C[i] - bf) <= 1e-12, (i, C[i], bf)
+    # the root: its own success mass plus its children's conditional
+    # reliabilities weighted by their masses is R
+    z = np.zeros(og.mask_words, np.uint64)
+    k, succ, ck, cu, cm, _ = og.expand(s, t, z, z, 1.0)
+    first = [i for i in range(len(C)) if np.array_equal(planes[i, 0], ck[0])][0]
+    total = succ + sum(C[first + j] * cm[j] for j in range(k))
+    assert np.array_equal(planes[first:first + k, 0], ck)
+    assert abs(total - og.solve(s, t)["R"]) <= 1e-13
