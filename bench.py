@@ -64,6 +64,39 @@ def traffic_from_profiles(kernel: str, workload: str):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def copy_peak_child(mib: int = 1024, reps: int = 10) -> float:
+    """HBM bandwidth a plain device-to-device copy reaches on this GPU (read
+    + write bytes over the median of reps copies, HIP events); runs in a
+    process of its own (torch's HIP runtime does not share a process with
+    the engine's)."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    ms = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        e1.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    return 2 * n * 4 / (statistics.median(ms) / 1e3) / 1e9
+
+
+def copy_peak_gbs():
+    """The measured copy peak (copy_peak_child in a child process), or None
+    if the child fails."""
+    try:
+        out = subprocess.run([sys.executable, os.path.abspath(__file__), "--copy-peak"],
+                             capture_output=True, text=True, timeout=300, check=True).stdout
+        return float(json.loads(out.strip().splitlines()[-1])["copy_peak_gbs"])
+    except (subprocess.SubprocessError, ValueError, KeyError, IndexError, OSError):
+        return None
+
+
 def valu_roofline(kernel: str, workload: str, subproblems_per_s: float):
     """Instruction roofline of the expand kernel: VALU wave instructions per
     subproblem (rocprofv3 SQ_INSTS_VALU over whole solves, committed under
@@ -413,7 +446,12 @@ def main() -> int:
                     help="time budget of the grid / Barabasi-Albert certified-gap leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
+    ap.add_argument("--copy-peak", action="store_true",
+                    help="only measure the device copy bandwidth (GB/s)")
     args = ap.parse_args()
+    if args.copy_peak:
+        print(json.dumps({"copy_peak_gbs": copy_peak_child()}))
+        return 0
     if args.copy_probe:
         import torch
 
@@ -438,6 +476,9 @@ def main() -> int:
     if "batch" in wl:
         b = run_batch(args, wl)
         insts = b.pop("_insts")
+        cp = copy_peak_gbs()
+        b["roofline"]["copy_peak"] = cp
+        b["roofline"]["frac_of_copy_peak"] = b["roofline"]["achieved"] / cp if cp else None
         line = {"metric": METRIC, "value": b.pop("value"), "unit": "subproblems/s",
                 "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
                 "ms_per_step": b.pop("ms_per_step"), "higher_is_better": True,
@@ -475,6 +516,7 @@ def main() -> int:
     achieved = nodes * len(timed) * bytes_per_sub / (kernel_ms / 1e3) / 1e9
     kernel = wl.get("kernel", "k_expand_level")
     traffic = traffic_from_profiles(kernel, args.workload)
+    copy_peak = copy_peak_gbs()
     line = {
         "metric": METRIC,
         "value": value,
@@ -510,6 +552,8 @@ def main() -> int:
             "bytes_per_subproblem": bytes_per_sub,
             "avg_launch_ms": kernel_ms / launches,
             "launches_per_step": launches / len(timed),
+            "copy_peak": copy_peak,
+            "frac_of_copy_peak": achieved / copy_peak if copy_peak else None,
         },
         "roofline_valu": valu_roofline(kernel, args.workload, value),
         "gpu_elapsed_ms_per_step": sum(r["elapsed_ms"] for r in results) / args.steps,
@@ -524,6 +568,9 @@ def main() -> int:
         bw = WORKLOADS["er4096"]
         b = run_batch(args, bw, steps=3, warmup=1)
         insts = b.pop("_insts")
+        b["roofline"]["copy_peak"] = copy_peak
+        b["roofline"]["frac_of_copy_peak"] = (b["roofline"]["achieved"] / copy_peak
+                                              if copy_peak else None)
         if not args.no_cpu:
             b["cpu_baseline"] = cpu_baseline_batch(insts, bw["eps"], args.cpu_sample)
         b["unit"] = "subproblems/s"
