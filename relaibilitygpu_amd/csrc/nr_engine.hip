@@ -1404,10 +1404,10 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
   if (rc) return rc;
   const int mode = x ? x->mode : 0;
   /* the plain path rule on a row-form graph whose rows would exceed 64 bytes
-   * per item runs with items that carry none (measured on MI355X, level
-   * order: K10, 40 B of rows, 1.33 ms with rows vs 1.34 ms without; K15,
-   * 60 B, 19.8 vs 20.4 ms; K25, 100 B, 54.9 vs 30.6 ms), unless the caller
-   * asks for the row-carrying kernel */
+   * per item runs with items that carry none, unless the caller asks for
+   * the row-carrying kernel (measured on MI355X, level order: K8 0.207 ms
+   * with rows vs 0.241 ms without, K10 0.758 vs 1.24 ms, K15 at 1e-12 8.8
+   * vs 10.3 ms; K25, 100 B of rows, 54.9 vs 30.6 ms) */
   if (mode == 0 && cfg->rule == NR_RULE_PATH && g->terms.empty() && !var.nw &&
       (g->n * var.ns_bytes > 64 || (cfg->flags & NR_FLAG_LEAN)) && !(cfg->flags & NR_FLAG_ROWS)) {
     var = variant_lean(mask_words(g), var.ns_bytes);
