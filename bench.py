@@ -448,6 +448,8 @@ def main() -> int:
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     ap.add_argument("--copy-peak", action="store_true",
                     help="only measure the device copy bandwidth (GB/s)")
+    ap.add_argument("--no-copy-peak", action="store_true",
+                    help="skip the copy-bandwidth measurement (profiled runs)")
     args = ap.parse_args()
     if args.copy_peak:
         print(json.dumps({"copy_peak_gbs": copy_peak_child()}))
@@ -476,7 +478,7 @@ def main() -> int:
     if "batch" in wl:
         b = run_batch(args, wl)
         insts = b.pop("_insts")
-        cp = copy_peak_gbs()
+        cp = None if args.no_copy_peak else copy_peak_gbs()
         b["roofline"]["copy_peak"] = cp
         b["roofline"]["frac_of_copy_peak"] = b["roofline"]["achieved"] / cp if cp else None
         line = {"metric": METRIC, "value": b.pop("value"), "unit": "subproblems/s",
@@ -516,7 +518,7 @@ def main() -> int:
     achieved = nodes * len(timed) * bytes_per_sub / (kernel_ms / 1e3) / 1e9
     kernel = wl.get("kernel", "k_expand_level")
     traffic = traffic_from_profiles(kernel, args.workload)
-    copy_peak = copy_peak_gbs()
+    copy_peak = None if args.no_copy_peak else copy_peak_gbs()
     line = {
         "metric": METRIC,
         "value": value,
