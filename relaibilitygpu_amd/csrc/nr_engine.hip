@@ -1851,6 +1851,16 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
      * first): carrying the rest then costs at most three times the useful
      * work, against ~pool/1M times with a fixed-size round */
     const u64 target0 = std::max<u64>(1ull << 20, (u64)grid * block);
+    /* max-mass order (MODE 4): a round that leaves much of the pool
+     * unexpanded is split first (k_class_split), so its waves expand only */
+    const void* split_fn = nullptr;
+    if (mode == 0) {
+      const KernelGetter sg = split_kernel(W, var.lean ? 0 : var.ns_bytes);
+      if (sg) {
+        split_fn = (const void*)sg();
+        if ((rc = ws.allow_lds(split_fn, 64 * 1024))) return rc;
+      }
+    }
     while (pool > 0) {
       const u64 target = std::max<u64>(target0, pool / 4);
       /* trim: when the pool exceeds the capacity, give up its lightest
@@ -1880,7 +1890,21 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
       a.drop_class = drop;
       /* buffer parity counts this call's launches (seeds are in buffer 0) */
       const int parity = (int)(launches_here & 1);
-      if ((rc = launch_level(parity, (int)rounds))) return rc;
+      /* a quarter or more of the pool carried or given up: the split pass
+       * leaves the round only items to expand (a wave whose lanes mostly
+       * carry would run its few expansions at the cost of a full wave) */
+      const bool split = split_fn && (pool - cum) * 4 >= pool;
+      if (split) {
+        int sp_parity = parity;
+        void* sargs[] = {&a, (void*)&sp_parity};
+        NR_HIP(hipLaunchKernel(split_fn, dim3(grid), dim3(block), sargs, lay.total_bytes, st));
+        a.append = 1;
+        a.thr_class = kClasses - 1;
+        a.drop_class = kClasses;
+      }
+      rc = launch_level(parity, (int)rounds);
+      a.append = 0;
+      if (rc) return rc;
       void* hargs[] = {&a};
       NR_HIP(hipLaunchKernel((const void*)&k_hist_total, dim3(kClasses), dim3(256), hargs, 0, st));
       NR_HIP(hipMemcpyAsync(ws.hctl, a.ctl, kCtlCopyBytes, hipMemcpyDeviceToHost, st));

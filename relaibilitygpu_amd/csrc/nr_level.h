@@ -12,6 +12,7 @@
 
 #include <cstddef>
 #include <cstring>
+#include <type_traits>
 
 #include "nr_device.h"
 #include "nr_learn.h"
@@ -116,6 +117,9 @@ struct LevelArgs {
   int mass_mode;
   int thr_class;
   int drop_class;
+  /* class rounds after k_class_split: the output segments and histograms
+   * already hold the carried items; the launch appends children to them */
+  int append;
   /* tree log (level order): one record per child, written by its parent */
   u32* tag[2];        /* per slot: record id (log) or class (learned order) */
   float* log_feat;    /* [cap][kFeat] child features (bf16-rounded) */
@@ -812,7 +816,7 @@ __device__ __forceinline__ void expand_level_items(const LevelArgs& a, const EX&
       load_item<W>(in, a.stride, slot_in, known, up, mass);
       /* level order: the rows travel with the planes (one memory round) */
       if (MODE == 0 && !late) ex.load(in, a.stride, slot_in);
-      if (MODE != 0) tag_in = a.tag[parity][slot_in];
+      if (MODE == 1 || MODE == 2) tag_in = a.tag[parity][slot_in];
       const int cl = !mass_on ? 0 : MODE == 2 ? (int)tag_in : mass_class(mass);
       if (late) {
         carry = true;
@@ -1168,11 +1172,11 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
     }
   }
   if (n_items == 0) {
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && !a.append) {
       a.seg_count[parity ^ 1][blockIdx.x] = 0;
       if (a.settle) a.seg_settle[parity ^ 1][blockIdx.x] = 0;
     }
-    if (mass_on)
+    if (mass_on && !a.append)
       for (int i = threadIdx.x; i < kClasses; i += blockDim.x)
         a.hist[(u64)blockIdx.x * kClasses + i] = 0;
     if (a.fin_on_empty && !mass_on && blockIdx.x < kFinMax) {
@@ -1189,14 +1193,15 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
   u32* s_hist = part + blockDim.x; /* [kClasses], class rounds only */
   u32* s_set = s_hist + kClasses;  /* children settled at creation */
   if (threadIdx.x == 0) {
-    *s_cnt = 0;
+    *s_cnt = a.append ? a.seg_count[parity ^ 1][blockIdx.x] : 0u;
     *s_set = 0;
   }
   /* per-wave feature tile [64][kFeat] bf16 (learned order), 16-byte aligned */
   u16* tile = (u16*)(((size_t)(s_set + 1) + 15) & ~(size_t)15) +
               (threadIdx.x >> 6) * 64 * kFeat;
   if (mass_on)
-    for (int i = threadIdx.x; i < kClasses; i += blockDim.x) s_hist[i] = 0;
+    for (int i = threadIdx.x; i < kClasses; i += blockDim.x)
+      s_hist[i] = a.append ? a.hist[(u64)blockIdx.x * kClasses + i] : 0u;
   __syncthreads();
 
   EX ex;
@@ -1238,6 +1243,114 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
     const u32 n = *s_cnt;
     a.seg_count[parity ^ 1][blockIdx.x] = n < a.seg_cap ? n : a.seg_cap;
     if (a.settle) a.seg_settle[parity ^ 1][blockIdx.x] = *s_set;
+  }
+}
+
+/* Class rounds (max-mass order): the pool in buffer `parity` split before a
+ * round in which many items are not expanded.  Workgroup w takes segment w:
+ * an item of class >= drop_class is given up to ESP (counted folded, as the
+ * round itself would), an item of a class in (thr_class, drop_class) is
+ * carried to segment w of the output buffer, and an item to expand is moved
+ * to the front of its own segment, in order.  The round's expand launch
+ * (append) then finds only items to expand, so every lane of a wave expands
+ * one, and appends the children after the carried items.  Items, children
+ * and counts are those of the round without the split; only which wave
+ * expands which item changes.  Rows (RowNS: the row form's node-set type,
+ * void for items without rows) move with their items through the lane's
+ * LDS row slice. */
+template <int W, typename RowNS>
+__global__ void __launch_bounds__(256) k_class_split(LevelArgs a, int parity) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr bool kRows = !std::is_void<RowNS>::value;
+  using NS = typename std::conditional<kRows, RowNS, u32>::type;
+  u32* s_w = (u32*)(smem + a.lay.off_extra); /* [4] expand, [4] carry counts per wave */
+  u32* s_hist = s_w + 8;                     /* [kClasses] carried items per class */
+  u32* part = s_hist + kClasses;             /* block_flush scratch */
+  const u32 pre = prefetch_slot(a);
+  for (int i = threadIdx.x; i < kClasses; i += blockDim.x) s_hist[i] = 0;
+  const QBuf in = a.buf[parity];
+  const QBuf out = a.buf[parity ^ 1];
+  const u32 cnt = a.seg_count[parity][blockIdx.x];
+  const u32 n = cnt < a.seg_cap ? cnt : a.seg_cap;
+  const u64 base = (u64)blockIdx.x * a.seg_cap;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int n_w = (blockDim.x + 63) >> 6;
+  const int nn = a.g.n;
+  NS* my = (NS*)(smem + a.lay.off_rows) + threadIdx.x; /* this lane's row slice */
+  const u64 below = (1ull << lane) - 1ull;
+  Acc2 accR, accE, accD;
+  accR.init();
+  accE.init();
+  accD.init();
+  u32 c[C_N] = {0, 0, 0, 0, 0, 0, 0, 0};
+  u32 keep = 0, carried = 0; /* workgroup-uniform */
+  __syncthreads();
+  for (u32 c0 = 0; c0 < n; c0 += blockDim.x) {
+    const u32 i = c0 + threadIdx.x;
+    u64 known[W], up[W];
+    double mass = 0.0;
+    int role = 0, cl = 0; /* 1 expand, 2 carry, 3 given up */
+    if (i < n) {
+      load_item<W>(in, a.stride, base + i, known, up, mass);
+      /* the planes in registers before the barrier below: a use makes the
+       * loads complete here, whatever the barrier waits for */
+#pragma unroll
+      for (int w = 0; w < W; ++w) asm volatile("" ::"v"(known[w]), "v"(up[w]));
+      cl = mass_class(mass);
+      role = cl >= a.drop_class ? 3 : cl > a.thr_class ? 2 : 1;
+      if (role == 3) { /* queue over capacity: given up to ESP */
+        accE.add(mass);
+        ++c[C_FOLD];
+      } else if (kRows) {
+        const NS* r = (const NS*)in.rows + base + i;
+        for (int u = 0; u < nn; ++u) my[u * blockDim.x] = r[(u64)u * a.stride];
+      }
+    }
+    const u64 be = __ballot(role == 1), bc = __ballot(role == 2);
+    if (lane == 0) {
+      s_w[wave] = (u32)__popcll(be);
+      s_w[4 + wave] = (u32)__popcll(bc);
+    }
+    /* every item of the chunk is read (the barrier waits for the loads)
+     * before any is written: an item moves to a slot at or below its own,
+     * possibly one another lane of the chunk reads */
+    __syncthreads();
+    u32 pe = keep, pc = carried, te = 0, tc = 0;
+    for (int w = 0; w < n_w; ++w) {
+      const u32 xe = s_w[w], xc = s_w[4 + w];
+      if (w < wave) {
+        pe += xe;
+        pc += xc;
+      }
+      te += xe;
+      tc += xc;
+    }
+    if (role == 1) {
+      const u64 d = base + pe + (u32)__popcll(be & below);
+      store_item<W>(in, a.stride, d, known, up, mass);
+      if (kRows) {
+        NS* r = (NS*)in.rows + d;
+        for (int u = 0; u < nn; ++u) r[(u64)u * a.stride] = my[u * blockDim.x];
+      }
+    } else if (role == 2) {
+      const u64 d = base + pc + (u32)__popcll(bc & below);
+      store_item<W>(out, a.stride, d, known, up, mass);
+      if (kRows) {
+        NS* r = (NS*)out.rows + d;
+        for (int u = 0; u < nn; ++u) r[(u64)u * a.stride] = my[u * blockDim.x];
+      }
+      atomicAdd(&s_hist[cl], 1u);
+    }
+    keep += te;
+    carried += tc;
+    __syncthreads(); /* s_w and the row slices are reused by the next chunk */
+  }
+  block_flush(a, accR, accE, accD, c, (unsigned char*)part, &pre);
+  for (int i = threadIdx.x; i < kClasses; i += blockDim.x)
+    a.hist[(u64)blockIdx.x * kClasses + i] = s_hist[i];
+  if (threadIdx.x == 0) {
+    a.seg_count[parity][blockIdx.x] = keep;
+    a.seg_count[parity ^ 1][blockIdx.x] = carried;
   }
 }
 
@@ -1628,6 +1741,7 @@ struct Variant {
   int choose;                          /* the learned chooser picks the sweep */
   KernelGetter ramp;                   /* k_ramp_levels of this form */
   int settle; /* the level kernel can settle dead children at creation */
+  KernelGetter split; /* k_class_split for this form's items (class rounds) */
 };
 
 /* The alternating rule with the learned sweep choice (ChooseAltExpander, or
@@ -1638,6 +1752,10 @@ Variant variant_choose_csr(int W, int NW);
 
 /* MODE 4 kernel of a variant (nr_mass.hip), filled into v. */
 void add_mass_kernels(Variant& v, int W);
+
+/* k_class_split for items of W plane words carrying rows of row_bytes per
+ * node (0: no rows), nr_mass.hip. */
+KernelGetter split_kernel(int W, int row_bytes);
 
 /* Lean row form (plain path rule, two terminals) for W words, node sets of
  * NS bytes (nr_lean.hip). */

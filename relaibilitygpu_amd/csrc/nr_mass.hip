@@ -31,6 +31,28 @@ template <int W> using Csr1 = CsrExpander<W, 1>;
 template <int W> using Csr2 = CsrExpander<W, 2>;
 template <int W> using Csr4 = CsrExpander<W, 4>;
 
+template <int W, typename RowNS>
+struct KernSplit {
+  static void* split() { return (void*)&k_class_split<W, RowNS>; }
+};
+
+template <typename RowNS>
+static KernelGetter split_for_w(int W) {
+  switch (W) {
+#define NR_W(X) \
+  case X: return &KernSplit<X, RowNS>::split;
+    NR_W(1) NR_W(2) NR_W(3) NR_W(4) NR_W(5) NR_W(6) NR_W(7) NR_W(8)
+#undef NR_W
+    default: return nullptr;
+  }
+}
+
+KernelGetter split_kernel(int W, int row_bytes) {
+  return row_bytes == 4   ? split_for_w<u32>(W)
+         : row_bytes == 8 ? split_for_w<u64>(W)
+                          : split_for_w<void>(W);
+}
+
 void add_mass_kernels(Variant& v, int W) {
   if (v.nw == 1)
     v.level_mass = mass_for_w<Csr1>(W);
