@@ -1851,11 +1851,12 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
      * first): carrying the rest then costs at most three times the useful
      * work, against ~pool/1M times with a fixed-size round */
     const u64 target0 = std::max<u64>(1ull << 20, (u64)grid * block);
-    /* max-mass order (MODE 4): a round that leaves much of the pool
-     * unexpanded is split first (k_class_split), so its waves expand only */
+    /* a round that leaves much of the pool unexpanded is split first
+     * (k_class_split), so its waves expand only; learned order classes by
+     * the items' tags */
     const void* split_fn = nullptr;
-    if (mode == 0) {
-      const KernelGetter sg = split_kernel(W, var.lean ? 0 : var.ns_bytes);
+    if (mode == 0 || mode == 2) {
+      const KernelGetter sg = split_kernel(W, var.lean ? 0 : var.ns_bytes, mode == 2);
       if (sg) {
         split_fn = (const void*)sg();
         if ((rc = ws.allow_lds(split_fn, 64 * 1024))) return rc;

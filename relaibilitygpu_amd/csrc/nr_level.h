@@ -1246,8 +1246,9 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
   }
 }
 
-/* Class rounds (max-mass order): the pool in buffer `parity` split before a
- * round in which many items are not expanded.  Workgroup w takes segment w:
+/* Class rounds (max-mass order, or learned order with kTag: the class is
+ * the item's tag): the pool in buffer `parity` split before a round in
+ * which many items are not expanded.  Workgroup w takes segment w:
  * an item of class >= drop_class is given up to ESP (counted folded, as the
  * round itself would), an item of a class in (thr_class, drop_class) is
  * carried to segment w of the output buffer, and an item to expand is moved
@@ -1258,7 +1259,7 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
  * expands which item changes.  Rows (RowNS: the row form's node-set type,
  * void for items without rows) move with their items through the lane's
  * LDS row slice. */
-template <int W, typename RowNS>
+template <int W, typename RowNS, bool kTag>
 __global__ void __launch_bounds__(256) k_class_split(LevelArgs a, int parity) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr bool kRows = !std::is_void<RowNS>::value;
@@ -1290,13 +1291,15 @@ __global__ void __launch_bounds__(256) k_class_split(LevelArgs a, int parity) {
     u64 known[W], up[W];
     double mass = 0.0;
     int role = 0, cl = 0; /* 1 expand, 2 carry, 3 given up */
+    u32 tag = 0;
     if (i < n) {
       load_item<W>(in, a.stride, base + i, known, up, mass);
+      if (kTag) tag = a.tag[parity][base + i];
       /* the planes in registers before the barrier below: a use makes the
        * loads complete here, whatever the barrier waits for */
 #pragma unroll
       for (int w = 0; w < W; ++w) asm volatile("" ::"v"(known[w]), "v"(up[w]));
-      cl = mass_class(mass);
+      cl = kTag ? (int)tag : mass_class(mass);
       role = cl >= a.drop_class ? 3 : cl > a.thr_class ? 2 : 1;
       if (role == 3) { /* queue over capacity: given up to ESP */
         accE.add(mass);
@@ -1328,6 +1331,7 @@ __global__ void __launch_bounds__(256) k_class_split(LevelArgs a, int parity) {
     if (role == 1) {
       const u64 d = base + pe + (u32)__popcll(be & below);
       store_item<W>(in, a.stride, d, known, up, mass);
+      if (kTag) a.tag[parity][d] = tag;
       if (kRows) {
         NS* r = (NS*)in.rows + d;
         for (int u = 0; u < nn; ++u) r[(u64)u * a.stride] = my[u * blockDim.x];
@@ -1335,6 +1339,7 @@ __global__ void __launch_bounds__(256) k_class_split(LevelArgs a, int parity) {
     } else if (role == 2) {
       const u64 d = base + pc + (u32)__popcll(bc & below);
       store_item<W>(out, a.stride, d, known, up, mass);
+      if (kTag) a.tag[parity ^ 1][d] = tag;
       if (kRows) {
         NS* r = (NS*)out.rows + d;
         for (int u = 0; u < nn; ++u) r[(u64)u * a.stride] = my[u * blockDim.x];
@@ -1754,8 +1759,9 @@ Variant variant_choose_csr(int W, int NW);
 void add_mass_kernels(Variant& v, int W);
 
 /* k_class_split for items of W plane words carrying rows of row_bytes per
- * node (0: no rows), nr_mass.hip. */
-KernelGetter split_kernel(int W, int row_bytes);
+ * node (0: no rows), classed by mass or (tag_class) by their tag,
+ * nr_mass.hip. */
+KernelGetter split_kernel(int W, int row_bytes, bool tag_class);
 
 /* Lean row form (plain path rule, two terminals) for W words, node sets of
  * NS bytes (nr_lean.hip). */

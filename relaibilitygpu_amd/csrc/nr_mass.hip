@@ -31,26 +31,30 @@ template <int W> using Csr1 = CsrExpander<W, 1>;
 template <int W> using Csr2 = CsrExpander<W, 2>;
 template <int W> using Csr4 = CsrExpander<W, 4>;
 
-template <int W, typename RowNS>
+template <int W, typename RowNS, bool kTag>
 struct KernSplit {
-  static void* split() { return (void*)&k_class_split<W, RowNS>; }
+  static void* split() { return (void*)&k_class_split<W, RowNS, kTag>; }
 };
 
-template <typename RowNS>
+template <typename RowNS, bool kTag>
 static KernelGetter split_for_w(int W) {
   switch (W) {
 #define NR_W(X) \
-  case X: return &KernSplit<X, RowNS>::split;
+  case X: return &KernSplit<X, RowNS, kTag>::split;
     NR_W(1) NR_W(2) NR_W(3) NR_W(4) NR_W(5) NR_W(6) NR_W(7) NR_W(8)
 #undef NR_W
     default: return nullptr;
   }
 }
 
-KernelGetter split_kernel(int W, int row_bytes) {
-  return row_bytes == 4   ? split_for_w<u32>(W)
-         : row_bytes == 8 ? split_for_w<u64>(W)
-                          : split_for_w<void>(W);
+KernelGetter split_kernel(int W, int row_bytes, bool tag_class) {
+  if (tag_class)
+    return row_bytes == 4   ? split_for_w<u32, true>(W)
+           : row_bytes == 8 ? split_for_w<u64, true>(W)
+                            : split_for_w<void, true>(W);
+  return row_bytes == 4   ? split_for_w<u32, false>(W)
+         : row_bytes == 8 ? split_for_w<u64, false>(W)
+                          : split_for_w<void, false>(W);
 }
 
 void add_mass_kernels(Variant& v, int W) {

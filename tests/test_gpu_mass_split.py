@@ -78,3 +78,26 @@ def test_capacity_binding_brackets(cap):
     assert m["R"] <= ref["R"] + ref["ESP"] + 1e-12
     assert ref["R"] <= m["R"] + m["ESP"] + 1e-12
 
+
+
+@pytest.mark.parametrize("case", ["K10", "grid"])
+def test_learned_order_rounds(case):
+    """Learned order: class rounds on the scorer's classes (the items' tags),
+    split by tag; a random scorer still builds level order's tree."""
+    import numpy as np
+
+    print(f"case learned order, random scorer, {case}", flush=True)
+    rng = np.random.default_rng(5)
+    m = nr.Model(rng.normal(0, 0.3, nr.MLP_PARAMS).astype(np.float32), alpha=1.0)
+    if case == "K10":
+        g = nr.Graph.load(graph_path("COMP10-05.txt"), 0, 9)
+        kw = dict(truncation_eps=0.0, queue_capacity=1 << 23)
+    else:
+        g = learn.grid(10, 10, 0x6121D)
+        kw = dict(truncation_eps=1e-9, queue_capacity=96777216)
+    lv = nr.solve(g, order="level", **kw)
+    r = nr.solve_learned(g, m, **kw)
+    for key in ("paths", "nodes", "dead", "truncated"):
+        assert lv[key] == r[key], (key, lv[key], r[key])
+    for key in ("R", "ESP", "D"):
+        assert close(lv[key], r[key]), (key, lv[key], r[key])
