@@ -63,13 +63,15 @@ def _mlp_forward(params: np.ndarray, X: np.ndarray) -> np.ndarray:
 
 
 # ------------------------------------------------------------ order scorer
-def train_order_scorer(alpha: float = 0.5, device: int = 0):
-    """The order scorer on children logged from K10 solves of the
-    alternating rule (delta 0, 1e-6, 1e-9): (parameters, summary)."""
+def train_order_scorer(alpha: float = 0.5, device: int = 0, records: int = 2000000):
+    """The order scorer on `records` children logged from K10 solves of the
+    alternating rule, the rule it orders (up to 200 000 each at delta 1e-6
+    and 1e-9, the rest from the full tree): (parameters, summary)."""
     t0 = time.perf_counter()
     k10 = complete(10)
     Xs, Ys = [], []
-    for eps, m in ((0.0, 1600000), (1e-6, 200000), (1e-9, 200000)):
+    for eps in (1e-6, 1e-9, 0.0):
+        m = 200000 if eps > 0 else records - sum(len(y) for y in Ys)
         X, Y, _ = log_subproblems(k10, m, seed=7, truncation_eps=eps, queue_capacity=1 << 22,
                                   rule="alternate", device=device)
         Xs.append(X)
