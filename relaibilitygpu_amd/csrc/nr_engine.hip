@@ -3068,9 +3068,9 @@ int nr_log_subproblems_rel(const nr_graph* g, const nr_config* cfg, long long ma
     if (!var.level_log) return NR_ERR_UNSUPPORTED;
     nr::WsLock L(cfg->device);
     if (L.rc) return L.rc;
-    double dummy = 0.0;
+    /* cond_rel is null only with max_records == 0: no labels to return */
     return nr::log_subproblems(*L.ws, g, cfg, max_records, seed, features, labels, n_out, result,
-                               0, cond_rel ? cond_rel : &dummy);
+                               0, cond_rel);
   });
 }
 
