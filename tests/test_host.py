@@ -192,3 +192,28 @@ def test_cli_rejects_unknown_rule_or_order():
         r = subprocess.run([nr.CLI_PATH, graph_path("test_simple.txt"), "0", "3", flag, value],
                            capture_output=True, text=True)
         assert r.returncode == 1 and "Usage:" in r.stderr and value in r.stderr, (flag, value)
+
+
+def test_log_rel_argument_checks():
+    """nr_log_subproblems_rel rejects missing outputs and bad configs
+    before it touches a device (NR_ERR_ARG = 4, NR_ERR_UNSUPPORTED for a
+    rule whose tree is not logged)."""
+    import ctypes
+
+    lib = nr.lib()
+    g = nr.Graph.load(graph_path("COMP8-09.txt"), 0, 7)
+    cfg = nr.make_config(truncation_eps=0.0, queue_capacity=1 << 16)
+    X = (ctypes.c_float * 16)()
+    Y = (ctypes.c_float * 1)()
+    C = (ctypes.c_double * 1)()
+    n = ctypes.c_longlong()
+    res = nr.Result()
+    f = lib.nr_log_subproblems_rel
+    assert f(None, ctypes.byref(cfg), 1, 1, X, Y, C, ctypes.byref(n), ctypes.byref(res)) == 4
+    assert f(g.handle, ctypes.byref(cfg), 1, 1, X, Y, None, ctypes.byref(n), ctypes.byref(res)) == 4
+    assert f(g.handle, ctypes.byref(cfg), -1, 1, X, Y, C, ctypes.byref(n), ctypes.byref(res)) == 4
+    assert f(g.handle, ctypes.byref(cfg), 1, 1, X, Y, C, None, ctypes.byref(res)) == 4
+    bad = nr.make_config(truncation_eps=-1.0)
+    assert f(g.handle, ctypes.byref(bad), 1, 1, X, Y, C, ctypes.byref(n), ctypes.byref(res)) == 4
+    fork = nr.make_config(rule="forkbest")
+    assert f(g.handle, ctypes.byref(fork), 1, 1, X, Y, C, ctypes.byref(n), ctypes.byref(res)) != 0
