@@ -95,12 +95,13 @@ typedef struct nr_config {
                                workgroup runs alone, every workgroup expands
                                its own slice's subtrees level after level
                                (no launch per level) until the levels hold
-                               about this many items; 0 = 512 per compute
-                               unit, < 0 = one launch per level from
-                               there.  Same results either way: a ramp that
-                               overflows a workgroup's region is discarded and
-                               the solve runs again without it
-                               (nr_result.rerun) */
+                               about this many items; 0 = 1024 per compute
+                               unit (four workgroups per unit; at most a
+                               quarter of the workgroups' region slots),
+                               < 0 = one launch per level from there.  Same
+                               results either way: a ramp that overflows a
+                               workgroup's region is discarded and the solve
+                               runs again without it (nr_result.rerun) */
 } nr_config;
 
 /* Decomposition rule (nr_config.rule).  Both are exact; they build

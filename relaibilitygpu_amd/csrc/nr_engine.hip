@@ -1744,14 +1744,21 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
     void* sargs[] = {&a, &mi, &ml};
     int r = timed_launch((const void*)var.small(), 1, sargs);
     if (r || !ramp || !var.ramp || grid < 2 || cfg->ramp_items < 0) return r;
-    /* one workgroup per CU (each runs its levels alone on its CU: one wave
-     * step of a level then takes about as long as in k_small_levels), each
-     * with the segments 1.. of the grid split evenly; by default the ramp
-     * ends near 512 items per CU (K10 on MI355X: 0.746 ms per solve against
-     * 0.756-0.768 ms with 1024 and 0.80 ms without the ramp) */
-    const int groups = std::min(grid - 1, cus);
+    /* four workgroups per CU, each running its own subtrees level after
+     * level with the segments 1.. of the grid split evenly; by default the
+     * ramp ends near 1024 items per CU (MI355X: K10 0.745-0.754 ms per
+     * solve against 0.762 ms with one workgroup per CU and 512 items, 0.80
+     * ms without the ramp; the alternating rule on K15 at 1e-12 2.62 against
+     * 2.70 ms; K25 at 1e-12 within 0.5%) */
+    const int groups = std::min(grid - 1, 4 * cus);
     int span = (grid - 1) / groups;
-    u32 target = cfg->ramp_items > 0 ? (u32)cfg->ramp_items : 512u * (u32)cus;
+    /* by default the last ramp level's children fit the workgroups'
+     * regions with room to spare (a region that fills restarts the solve
+     * without the ramp); a requested target is taken as it is */
+    u64 target64 = cfg->ramp_items > 0 ? (u64)cfg->ramp_items
+                                       : std::min<u64>(1024ull * (u64)cus,
+                                                       (u64)groups * (u64)span * seg_cap / 4);
+    u32 target = (u32)std::min<u64>(target64, 0xffffffffull);
     void* rargs[] = {&a, &span, &target};
     if ((r = timed_launch((const void*)var.ramp(), (unsigned)groups, rargs))) return r;
     ramp_used = true;
