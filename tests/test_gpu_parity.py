@@ -217,14 +217,16 @@ def test_mass_order_capacity_bracket(expected, cap):
     assert abs(r["R"] + r["D"] + r["ESP"] - 1.0) <= 1e-13
 
 
+@pytest.mark.parametrize("budget", [0.01, 1e-6])
 @pytest.mark.parametrize("order", ["level", "mass"])
-def test_time_budget_bracket(expected, order):
-    """A 10 ms budget on K15 (delta 1e-15, a run of several hundred ms): the
-    queue is folded into ESP and the bracket R <= R_true <= R + ESP holds."""
-    print(f"case deadline K15 {order}", flush=True)
+def test_time_budget_bracket(expected, order, budget):
+    """A 10 ms budget on K15 (delta 1e-15, a run of several hundred ms), and
+    a 1 us one that expires before the first level: the queue is folded
+    into ESP and the bracket R <= R_true <= R + ESP holds."""
+    print(f"case deadline K15 {order} budget {budget}", flush=True)
     g = nr.Graph.load(graph_path("COMP15-05.txt"), 0, 14)
     r = nr.solve(g, truncation_eps=1e-15, queue_capacity=1 << 24, order=order,
-                 time_budget_s=0.01)
+                 time_budget_s=budget)
     Rt = closed_form(expected, 15, "0.5")
     assert r["deadline_hit"] == 1
     assert r["R"] <= Rt <= r["R"] + r["ESP"] + 1e-14
