@@ -101,3 +101,13 @@ def test_learned_order_rounds(case):
         assert lv[key] == r[key], (key, lv[key], r[key])
     for key in ("R", "ESP", "D"):
         assert close(lv[key], r[key]), (key, lv[key], r[key])
+
+
+@pytest.mark.parametrize("grid,block", [(1, 256), (7, 64), (300, 128), (5000, 256)])
+def test_launch_shapes(grid, block):
+    """Split and round launches at any grid and block size: one workgroup
+    (one segment), odd grids, 64-thread blocks (one wave), more workgroups
+    than the device holds at once."""
+    print(f"case launch shape grid {grid} block {block}", flush=True)
+    g = nr.Graph.load(graph_path("COMP9.txt"), 0, 8)
+    same_tree(g, truncation_eps=1e-7, queue_capacity=1 << 22, grid_blocks=grid, block_size=block)
