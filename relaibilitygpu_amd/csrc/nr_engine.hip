@@ -282,8 +282,6 @@ __device__ inline void fx_flush(u64* acc, u64 l2, u64 l1, u64 l0) {
   if (hi) atomicAdd((unsigned long long*)&acc[1], (unsigned long long)hi);
 }
 
-__device__ inline u64 wave_sum_if(u64 x, u64 any) { return any ? wave_sum_u64(x) : 0ull; }
-
 /* Per-instance description (device). */
 struct InstDesc {
   int n, e, src, dst;
@@ -571,31 +569,48 @@ __global__ void __launch_bounds__(256) k_expand_batch(BatchArgs a, int parity) {
       fx_limbs_add(r2, r1, r0, run); /* success term, main.cu:254-262 */
       paths += 1;
     }
-    /* flush: one group of lanes per distinct instance in the wave; the
-     * group's lowest lane adds the group's sums to that instance */
-    u64 pending = __ballot(active);
-    const u64 any_e = __ballot((e2 | e1 | e0) != 0), any_d = __ballot((d2 | d1 | d0) != 0);
-    while (pending) {
-      const int leader = __builtin_ctzll(pending);
-      const u32 lid = __shfl(id, leader, 64);
-      const bool mine = active && id == lid;
-      pending &= ~__ballot(mine);
-      const u64 gc = wave_sum_u64(mine ? cnt : 0ull);
-      const u64 gp = wave_sum_u64(mine ? paths : 0ull);
-      const u64 g_r2 = wave_sum_u64(mine ? r2 : 0ull), g_r1 = wave_sum_u64(mine ? r1 : 0ull),
-                g_r0 = wave_sum_u64(mine ? r0 : 0ull);
-      const u64 g_e2 = wave_sum_if(mine ? e2 : 0ull, any_e),
-                g_e1 = wave_sum_if(mine ? e1 : 0ull, any_e),
-                g_e0 = wave_sum_if(mine ? e0 : 0ull, any_e);
-      const u64 g_d2 = wave_sum_if(mine ? d2 : 0ull, any_d),
-                g_d1 = wave_sum_if(mine ? d1 : 0ull, any_d),
-                g_d0 = wave_sum_if(mine ? d0 : 0ull, any_d);
-      /* levels: this item's, and its settled children's below it */
-      u32 gdepth = mine ? depth + (settled_any ? 2u : 1u) : 0u;
+    /* flush: lanes of one instance form runs (a parent's children are
+     * adjacent); a segmented inclusive scan sums every run at once and the
+     * run's last lane adds its sums to the instance (integer sums: the
+     * grouping does not change them) */
+    {
+      const u32 myid = active ? id : 0xffffffffu;
+      const u32 prev_id = __shfl_up(myid, 1, 64);
+      const u32 next_id = __shfl_down(myid, 1, 64);
+      const bool head = lane == 0 || prev_id != myid;
+      const bool tail = lane == 63 || next_id != myid;
+      const u64 heads = __ballot(head);
+      const u64 upto = lane == 63 ? heads : heads & ((2ull << lane) - 1ull);
+      const int seg0 = 63 - __builtin_clzll(upto);
+      const bool any_e = __ballot((e2 | e1 | e0) != 0) != 0;
+      const bool any_d = __ballot((d2 | d1 | d0) != 0) != 0;
+      u64 gc = cnt, gp = paths, g_r2 = r2, g_r1 = r1, g_r0 = r0;
+      u64 g_e2 = e2, g_e1 = e1, g_e0 = e0, g_d2 = d2, g_d1 = d1, g_d0 = d0;
+      u32 gdepth = active ? depth + (settled_any ? 2u : 1u) : 0u;
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) gdepth = max(gdepth, (u32)__shfl_xor(gdepth, off, 64));
-      if (lane == leader) {
-        u64* res = a.res + (u64)lid * B_N;
+      for (int off = 1; off < 64; off <<= 1) {
+        const bool take = lane - off >= seg0;
+        u64 o;
+        o = __shfl_up(gc, off, 64); if (take) gc += o;
+        o = __shfl_up(gp, off, 64); if (take) gp += o;
+        o = __shfl_up(g_r2, off, 64); if (take) g_r2 += o;
+        o = __shfl_up(g_r1, off, 64); if (take) g_r1 += o;
+        o = __shfl_up(g_r0, off, 64); if (take) g_r0 += o;
+        if (any_e) {
+          o = __shfl_up(g_e2, off, 64); if (take) g_e2 += o;
+          o = __shfl_up(g_e1, off, 64); if (take) g_e1 += o;
+          o = __shfl_up(g_e0, off, 64); if (take) g_e0 += o;
+        }
+        if (any_d) {
+          o = __shfl_up(g_d2, off, 64); if (take) g_d2 += o;
+          o = __shfl_up(g_d1, off, 64); if (take) g_d1 += o;
+          o = __shfl_up(g_d0, off, 64); if (take) g_d0 += o;
+        }
+        const u32 od = __shfl_up(gdepth, off, 64);
+        if (take) gdepth = max(gdepth, od);
+      }
+      if (tail && active) {
+        u64* res = a.res + (u64)id * B_N;
         const u64 c_nodes = gc & 0xffff, c_tr = (gc >> 16) & 0xffff, c_dead = (gc >> 32) & 0xffff,
                   c_over = gc >> 48;
         atomicAdd((unsigned long long*)&res[B_NODES], (unsigned long long)c_nodes);
@@ -605,8 +620,8 @@ __global__ void __launch_bounds__(256) k_expand_batch(BatchArgs a, int parity) {
         if (c_over) atomicAdd((unsigned long long*)&res[B_OVER], (unsigned long long)c_over);
         if (gp) atomicAdd((unsigned long long*)&res[B_PATHS], (unsigned long long)gp);
         fx_flush(&res[B_R], g_r2, g_r1, g_r0);
-        fx_flush(&res[B_E], g_e2, g_e1, g_e0);
-        fx_flush(&res[B_D], g_d2, g_d1, g_d0);
+        if (any_e) fx_flush(&res[B_E], g_e2, g_e1, g_e0);
+        if (any_d) fx_flush(&res[B_D], g_d2, g_d1, g_d0);
       }
     }
   }
