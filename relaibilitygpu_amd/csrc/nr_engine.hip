@@ -2563,6 +2563,14 @@ __global__ void __launch_bounds__(256) k_log_accumulate(const u32* parent, u64* 
   }
 }
 
+/* conditional reliabilities of the records ids[0 .. m-1] */
+__global__ void __launch_bounds__(256) k_log_cond(const u64* ids, u64 m, const double* mass,
+                                                  const double* succ, const u64* acc, double* C) {
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (u64)gridDim.x * blockDim.x)
+    C[i] = log_cond(mass, succ, acc, ids[i]);
+}
+
 __global__ void __launch_bounds__(256) k_log_gather(const float* feat, const u64* size,
                                                     const u64* idx, u64 m, float* X, float* Y,
                                                     const double* mass, const double* succ,
@@ -2707,10 +2715,26 @@ int log_subproblems(Workspace& ws, const nr_graph* g, const nr_config* cfg, long
     NR_HIP(hipMemcpyAsync(keys.data(), dkeys, 8 * found, hipMemcpyDeviceToHost, st));
     NR_HIP(hipMemcpyAsync(ids.data(), dids, 8 * found, hipMemcpyDeviceToHost, st));
     NR_HIP(hipStreamSynchronize(st));
+    /* records of equal content (features and size) share a hash and are
+     * interchangeable, except for their conditional reliabilities: when
+     * those are returned they order equal hashes before the record id
+     * (the order the device wrote them in), so the labels repeat too */
+    std::vector<u64> cbits(C ? found : 0, 0);
+    if (C && found) {
+      double* dcf;
+      if ((rc = mem.alloc(&dcf, found))) return rc;
+      hipLaunchKernelGGL(k_log_cond, dim3(1024), dim3(256), 0, st, (const u64*)dids, found,
+                         (const double*)x.log_mass, (const double*)x.log_succ, (const u64*)dacc,
+                         dcf);
+      NR_HIP(hipMemcpyAsync(cbits.data(), dcf, 8 * found, hipMemcpyDeviceToHost, st));
+      NR_HIP(hipStreamSynchronize(st));
+    }
     std::vector<u64> order(found);
     for (u64 i = 0; i < found; ++i) order[i] = i;
     std::sort(order.begin(), order.end(), [&](u64 x1, u64 x2) {
-      return keys[x1] != keys[x2] ? keys[x1] < keys[x2] : ids[x1] < ids[x2];
+      if (keys[x1] != keys[x2]) return keys[x1] < keys[x2];
+      if (C && cbits[x1] != cbits[x2]) return cbits[x1] < cbits[x2];
+      return ids[x1] < ids[x2];
     });
     const u64 m0 = std::min<u64>(want, found);
     idx.resize((size_t)m0);

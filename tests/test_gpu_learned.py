@@ -262,3 +262,22 @@ def test_tree_log_conditional_reliability_k10_repeats():
     leaf = Y1 == 0.0
     assert leaf.any() and np.all((C1[leaf] == 0.0) | (C1[leaf] == 1.0))
 
+
+
+def test_tree_log_conditional_reliability_alternating_rule():
+    """The alternating rule's tree log carries the second label too (K9 in
+    full): the sample and size labels equal nr_log_subproblems', the labels
+    lie in [0, 1], leaves carry exactly 0 or 1, and a second run repeats
+    them bit for bit."""
+    print("case tree-log conditional reliability, alternating rule, K9", flush=True)
+    g = nr.Graph.load(graph_path("COMP9.txt"), 0, 8)
+    kw = dict(truncation_eps=0.0, queue_capacity=1 << 22, rule="alternate", sweep_period=2)
+    X1, Y1, C1, r1 = nr.log_subproblems_rel(g, 200000, seed=11, **kw)
+    X2, Y2, C2, _ = nr.log_subproblems_rel(g, 200000, seed=11, **kw)
+    X0, Y0, _ = nr.log_subproblems(g, 200000, seed=11, **kw)
+    assert len(C1) == min(200000, r1["nodes"] - 1)
+    assert np.array_equal(X1, X0) and np.array_equal(Y1, Y0) and np.array_equal(C1, C2)
+    assert np.all((C1 >= 0.0) & (C1 <= 1.0))
+    leaf = Y1 == 0.0
+    assert leaf.any() and np.all((C1[leaf] == 0.0) | (C1[leaf] == 1.0))
+    assert 0.0 < C1.mean() < 1.0
