@@ -178,7 +178,9 @@ typedef struct nr_result {
   int blocks_per_cu;            /* occupancy reported by the runtime */
   int compute_units;
   int rerun;                    /* nr_solve_batch: 1 if the instance overflowed
-                                   the shared queue and was solved again alone;
+                                   the shared queue and was solved again alone
+                                   (same capacity: overflowed > 0 if it
+                                   overflows even alone);
                                    nr_solve: 1 if the ramp (ramp_items)
                                    overflowed and the solve ran again without
                                    it */
