@@ -74,3 +74,64 @@ def test_random_graphs_against_oracle(seed):
             assert r["R"] <= o["R"] + o["ESP"] + 1e-12 and o["R"] <= r["R"] + r["ESP"] + 1e-12
         assert abs(r["R"] + r["D"] + r["ESP"] - 1.0) <= 1e-12
         done += 1
+
+
+@pytest.mark.parametrize("seed", [21, 22])
+def test_random_csr_digraphs_and_choosers(seed):
+    """Directed graphs (10-120 nodes), undirected graphs of 65-250 nodes (the
+    CSR form) and the learned path choice with random chooser weights, in
+    level and max-mass order, against the oracle (its learned mode takes the
+    same integer chooser)."""
+    import numpy as np
+
+    from oracle_ffi import MODE_LEARNED
+
+    rng = random.Random(seed)
+    done = 0
+    while done < 50:
+        kind = rng.choice(["digraph", "big_und", "choice_small", "choice_csr"])
+        eps = rng.choice([1e-5, 1e-6, 1e-7])
+        kw = dict(truncation_eps=eps, queue_capacity=rng.choice([1 << 18, 1 << 22]),
+                  order=rng.choice(["level", "mass"]))
+        model = None
+        if kind == "digraph":
+            n, arcs, probs = wl.random_digraph(rng.randint(10, 120), rng.uniform(2.0, 4.0), 0.6,
+                                               0.97, rng.randint(0, 10**6))
+            rp, ci, ei = wl.to_csr_directed(n, arcs)
+            if len(arcs) > 512:
+                continue
+        else:
+            n = rng.randint(65, 250) if kind != "choice_small" else rng.randint(8, 40)
+            d = rng.uniform(2.2, 4.0)
+            edges = {(u, v) for u in range(n) for v in range(u + 1, n) if rng.random() < d / (n - 1)}
+            edges |= {(u, u + 1) for u in range(n - 1) if rng.random() < 0.7}
+            edges = sorted(edges)
+            if len(edges) > 512:
+                continue
+            probs = [rng.uniform(0.7, 0.99) for _ in edges]
+            rp, ci, ei = wl.to_csr(n, edges)
+            if kind.startswith("choice"):
+                p = np.random.default_rng(rng.randint(0, 10**6)).normal(
+                    0.0, 0.3, nr.MLP_PARAMS).astype(np.float32)
+                model = nr.Model(None, chooser=p, margin=rng.choice([0.0, 0.5]))
+                kw.update(rule="learned", sweep_period=rng.choice([1, 2, 3]))
+        og = OracleGraph.from_csr(rp, ci, ei, probs)
+        if model is not None:
+            o = og.solve(0, n - 1, mode=MODE_LEARNED, order=ORDER_DFS, threads=8, trunc_eps=eps,
+                         sweep_period=kw["sweep_period"], chooser=model.chooser_weights(),
+                         time_budget_s=5.0)
+        else:
+            o = og.solve(0, n - 1, order=ORDER_DFS, threads=8, trunc_eps=eps, time_budget_s=5.0)
+        if o["deadline_hit"] or o["nodes"] > 3e6:
+            continue
+        print(f"case seed {seed} #{done} {kind} n={n} {kw}", flush=True)
+        g = nr.Graph.from_csr(rp, ci, ei, probs, 0, n - 1)
+        r = nr.solve_learned(g, model, **kw) if model is not None else nr.solve(g, **kw)
+        if r["overflowed"] == 0 and r["folded"] == 0:
+            for key in ("paths", "nodes", "dead", "truncated"):
+                assert r[key] == o[key], (key, r[key], o[key])
+            assert close(r["R"], o["R"], 1e-11) and close(r["ESP"], o["ESP"], 1e-10)
+        else:
+            assert r["R"] <= o["R"] + o["ESP"] + 1e-12 and o["R"] <= r["R"] + r["ESP"] + 1e-12
+        assert abs(r["R"] + r["D"] + r["ESP"] - 1.0) <= 1e-12
+        done += 1
