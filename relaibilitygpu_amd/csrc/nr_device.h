@@ -288,9 +288,29 @@ struct Expander {
     pivot = g.pivot;
     lk = g.lk;
   }
+  /* Queue form of the rows: with 32-bit node sets and n <= 16 nodes, rows
+   * 2j and 2j+1 share the 32-bit word j (low and high half), so an item
+   * carries ceil(n/2) words instead of n; otherwise one NS word per node.
+   * The choice is uniform over the launch (n is the graph's). */
+  __device__ bool packed() const { return sizeof(NS) == 4 && n <= 16; }
   /* the item's rows into this lane's slice: 16 loads in flight before the
    * first LDS store waits (one memory round for n <= 16) */
   __device__ void load(const QBuf& in, u64 qstride, u64 slot) const {
+    if (packed()) {
+      const u32* win = (const u32*)in.rows + slot;
+      const int nw = (n + 1) >> 1;
+      u32 v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (j < nw) v[j] = win[(u64)j * qstride];
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (j < nw) {
+          rows[(2 * j) * stride] = (NS)(v[j] & 0xffffu);
+          if (2 * j + 1 < n) rows[(2 * j + 1) * stride] = (NS)(v[j] >> 16);
+        }
+      return;
+    }
     const NS* rin = (const NS*)in.rows + slot;
     for (int u0 = 0; u0 < n; u0 += 16) {
       NS v[16];
@@ -424,6 +444,15 @@ struct Expander {
     return k;
   }
   __device__ void store_child(const QBuf& out, u64 qstride, u64 slot, int e) const {
+    if (packed()) {
+      u32* wout = (u32*)out.rows + slot;
+      for (int u = 0; u < n; u += 2) {
+        u32 v = (u32)child_row(u, e);
+        if (u + 1 < n) v |= (u32)child_row(u + 1, e) << 16;
+        wout[(u64)(u >> 1) * qstride] = v;
+      }
+      return;
+    }
     NS* rout = (NS*)out.rows + slot;
     for (int u = 0; u < n; ++u) rout[(u64)u * qstride] = child_row(u, e);
   }
@@ -530,6 +559,18 @@ struct Expander {
     const u32 x = ep[e];
     const int a0 = (int)(x & 0xffffu), b0 = (int)(x >> 16);
     const NS* r = rows + delta;
+    if (packed()) {
+      u32* wout = (u32*)out.rows + slot;
+      for (int u = 0; u < n; u += 2) {
+        u32 lo = (u32)r[u * stride], hi = u + 1 < n ? (u32)r[(u + 1) * stride] : 0u;
+        if (u == a0) lo &= ~(1u << b0);
+        if (u == b0) lo &= ~(1u << a0);
+        if (u + 1 == a0) hi &= ~(1u << b0);
+        if (u + 1 == b0) hi &= ~(1u << a0);
+        wout[(u64)(u >> 1) * qstride] = lo | (hi << 16);
+      }
+      return;
+    }
     NS* rout = (NS*)out.rows + slot;
     for (int u = 0; u < n; ++u) {
       NS v = r[u * stride];
@@ -541,6 +582,14 @@ struct Expander {
 
   /* rows of a child that keeps the parent's allowed edges */
   __device__ void store_rows(const QBuf& out, u64 qstride, u64 slot) const {
+    if (packed()) {
+      u32* wout = (u32*)out.rows + slot;
+      for (int u = 0; u < n; u += 2) {
+        const u32 hi = u + 1 < n ? (u32)rows[(u + 1) * stride] : 0u;
+        wout[(u64)(u >> 1) * qstride] = (u32)rows[u * stride] | (hi << 16);
+      }
+      return;
+    }
     NS* rout = (NS*)out.rows + slot;
     for (int u = 0; u < n; ++u) rout[(u64)u * qstride] = rows[u * stride];
   }
@@ -556,7 +605,8 @@ struct Expander {
                         u64 qstride) const {
     const NS* rin = (const NS*)in.rows + slot_in;
     NS* rout = (NS*)out.rows + slot_out;
-    for (int u = 0; u < n; ++u) rout[(u64)u * qstride] = rin[(u64)u * qstride];
+    const int nw = packed() ? (n + 1) >> 1 : n; /* packed: u32 words, NS is u32 */
+    for (int u = 0; u < nw; ++u) rout[(u64)u * qstride] = rin[(u64)u * qstride];
   }
 
   /* Allowed neighbours of u (edges not FAILED): the static neighbour set

@@ -154,7 +154,11 @@ __global__ void __launch_bounds__(256) k_init(LevelArgs a, Root root, int W, int
       a.buf[0].up[(u64)w * a.stride] = root.up[w];
     }
     for (int u = 0; u < n && ns_bytes; ++u) {
-      if (ns_bytes == 4)
+      if (ns_bytes == 4 && n <= 16) /* two 16-bit rows per word (Expander::packed) */
+        ((u32*)a.buf[0].rows)[(u64)(u >> 1) * a.stride] =
+            (u & 1) ? ((u32*)a.buf[0].rows)[(u64)(u >> 1) * a.stride] | ((u32)root.rows[u] << 16)
+                    : (u32)root.rows[u];
+      else if (ns_bytes == 4)
         ((u32*)a.buf[0].rows)[(u64)u * a.stride] = (u32)root.rows[u];
       else
         ((u64*)a.buf[0].rows)[(u64)u * a.stride] = root.rows[u];

@@ -1276,7 +1276,9 @@ __global__ void __launch_bounds__(256) k_class_split(LevelArgs a, int parity) {
   const u64 base = (u64)blockIdx.x * a.seg_cap;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int n_w = (blockDim.x + 63) >> 6;
-  const int nn = a.g.n;
+  /* queue words of an item's rows: two 16-bit rows per 32-bit word when
+   * n <= 16 (Expander::packed), else one per node */
+  const int nn = sizeof(NS) == 4 && a.g.n <= 16 ? (a.g.n + 1) >> 1 : a.g.n;
   NS* my = (NS*)(smem + a.lay.off_rows) + threadIdx.x; /* this lane's row slice */
   const u64 below = (1ull << lane) - 1ull;
   Acc2 accR, accE, accD;
