@@ -91,7 +91,8 @@ inline Layout make_layout(int n, int e, int w, int block, int ns_bytes,
   L.off_extra = take(extra_bytes, 16);
   L.off_rows = take(ns_bytes * n * block, 16);
   L.off_lvl = take(ns_bytes * n * block, 16);
-  L.off_uh = take(2 * n * block, 16);
+  /* UNKNOWN hops of a path (or of a terminal tree's edges): at most n - 1 */
+  L.off_uh = take(2 * (n > 1 ? n - 1 : 1) * block, 16);
   L.total_bytes = (o + 15) / 16 * 16;
   return L;
 }

@@ -1462,16 +1462,19 @@ int solve_level(Workspace& ws, const nr_graph* g, const nr_config* cfg, nr_resul
                                                         : var.level;
   if (!getter) return NR_ERR_UNSUPPORTED;
   const void* kfun = (const void*)getter();
+  /* the level kernel's LDS class counts: class rounds and the learned order */
+  const int hist_words = (mass_order || mode == 2) ? kClasses : 0;
   int block =
-      choose_block(var, g, W, cfg->block_size, 4 * (1027 + 256 + kClasses) + tile_bytes);
+      choose_block(var, g, W, cfg->block_size, 4 * (1027 + 256 + hist_words) + tile_bytes);
   if (block <= 0) return NR_ERR_UNSUPPORTED;
-  Layout lay0 = layout_for(var, g, W, block, 4 * (block + kClasses) + tile_bytes);
+  Layout lay0 = layout_for(var, g, W, block, 4 * (block + hist_words) + tile_bytes);
   if ((rc = ws.allow_lds(kfun, 64 * 1024))) return rc;
   int per_cu0 = 0;
   if ((rc = ws.blocks_per_cu(kfun, block, lay0.total_bytes, &per_cu0))) return rc;
   if (per_cu0 < 1) return NR_ERR_UNSUPPORTED;
   int grid = cfg->grid_blocks > 0 ? cfg->grid_blocks : per_cu0 * cus;
-  const int extra = 4 * (grid + 3 + block + kClasses) + tile_bytes;
+  /* k_ramp_levels keeps 4 + 84 + kRampMax + 1 words there whatever the grid */
+  const int extra = 4 * std::max(grid + 3 + block + hist_words, 4 + 84 + kRampMax + 1) + tile_bytes;
   Layout lay = layout_for(var, g, W, block, extra);
   if (lay.total_bytes > 64 * 1024) return NR_ERR_UNSUPPORTED;
   int per_cu = 0;

@@ -1190,8 +1190,10 @@ __global__ void __launch_bounds__(256) k_expand_level(LevelArgs a, int parity, i
     }
     return;
   }
-  u32* s_hist = part + blockDim.x; /* [kClasses], class rounds only */
-  u32* s_set = s_hist + kClasses;  /* children settled at creation */
+  /* [kClasses] class counts: class rounds and the learned order only (the
+   * host sizes the LDS area without them otherwise) */
+  u32* s_hist = part + blockDim.x;
+  u32* s_set = s_hist + (mass_on || MODE == 2 ? kClasses : 0); /* children settled at creation */
   if (threadIdx.x == 0) {
     *s_cnt = a.append ? a.seg_count[parity ^ 1][blockIdx.x] : 0u;
     *s_set = 0;
